@@ -1,0 +1,124 @@
+/*
+ * pmsz.h -- C-ABI of libpmsz.so, the MI355X (gfx950) extrema + Morse-Smale
+ * segmentation correction engine.  Plain C types only.
+ *
+ * The reference exposes no library interface: its correction entry point is the
+ * internal template c_loops<T>(...) (pMSz.cu:1872-1878, 35 positional arguments
+ * plus globals) driven by main (pMSz.cu:2596-3188).  Each function below names
+ * the reference code it replaces.
+ *
+ * Conventions
+ *   - dtype 0 = float32, 1 = float64; arrays are C order with x fastest
+ *     (index = x + nx*(y + ny*z)), as the reference's raw reader (pMSz.cu:289-309).
+ *   - "d_" pointers are device pointers (HIP), "h_" pointers host pointers.
+ *   - Return value 0 = success, negative = error (pmsz_strerror); the reference
+ *     prints and calls MPI_Abort instead (pMSz.cu:1899-1910).
+ *   - The library owns its scratch state (codes, delta counters, marks, work
+ *     lists), allocated on first use per device and reused; the caller owns f / g.
+ *   - Calls are serialised per process by an internal lock and run on the
+ *     library's own stream; they return when the result is complete.
+ */
+#ifndef PMSZ_H
+#define PMSZ_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum { PMSZ_F32 = 0, PMSZ_F64 = 1 } pmsz_dtype;
+/* lazy ghost sync (pMSz.cu) or per-iteration sync (sync_pMSz.cu) */
+typedef enum { PMSZ_LAZY = 0, PMSZ_SYNC = 1 } pmsz_sync_mode;
+/* clamp of the edit primitive: the reference's g = f - eb (pMSz.cu:636) in the
+ * value type, or that value moved up by ulps until |f - g| <= eb holds exactly */
+typedef enum { PMSZ_CLAMP_REFERENCE = 0, PMSZ_CLAMP_BOUND_SAFE = 1 } pmsz_clamp_mode;
+/* ghost cells exchanged between subdomains: the reference's faces + 4 edges
+ * (pMSz.cu:394-488) or all 14 stencil directions */
+typedef enum { PMSZ_GHOST_REFERENCE = 0, PMSZ_GHOST_FULL = 1 } pmsz_ghost_policy;
+
+enum {
+  PMSZ_OK = 0,
+  PMSZ_EINVAL = -1,        /* bad parameter, NaN / sentinel value in a field */
+  PMSZ_ENOMEM = -2,        /* device allocation failed */
+  PMSZ_NOT_CONVERGED = -3, /* max_sweeps reached; stats are filled */
+  PMSZ_EDEVICE = -4,       /* HIP runtime error / no device */
+  PMSZ_ETOOBIG = -5,       /* more voxels than the index width supports */
+  PMSZ_EBUSY = -6,         /* batch / server capacity exhausted */
+};
+
+typedef struct {
+  int dtype;             /* pmsz_dtype */
+  int64_t nx, ny, nz;    /* field extent (argv[1] "path,W,H,D", pMSz.cu:2648-2661) */
+  double abs_eb;         /* absolute error bound (argv[2], pMSz.cu:2628) */
+  int q;                 /* decrements before clamping; 5 in the reference (pMSz.cu:2630) */
+  int sub[3];            /* subdomain grid (argv[4..6] PX PY PZ, pMSz.cu:2631-2633) */
+  int mode;              /* pmsz_sync_mode */
+  int max_sweeps;        /* cap on sweeps / outer iterations; the reference has none */
+  int clamp_mode;        /* pmsz_clamp_mode */
+  int ghost_policy;      /* pmsz_ghost_policy */
+} pmsz_params;
+
+typedef struct {
+  int64_t edit_sweeps;     /* detect+edit sweeps of the reference schedule (summed over subdomains) */
+  int64_t check_sweeps;    /* checkFinal sweeps of the reference schedule */
+  int64_t outer_iters;     /* outer iterations = ghost syncs */
+  int64_t ite;             /* the reference's "iteration is:" counter (pMSz.cu:2073) */
+  int64_t touched_voxels;  /* voxels edited at least once (final delta counter > 0) */
+  int64_t clamped_voxels;  /* voxels clamped (final delta counter == q) */
+  int64_t total_targets;   /* edits applied, summed over sweeps */
+  uint32_t final_cnt[3];   /* counters of the last check: false max, false min, direction */
+  int status;              /* PMSZ_OK or PMSZ_NOT_CONVERGED */
+  int64_t dense_passes;    /* full-volume kernel passes actually run */
+  int64_t sparse_voxels;   /* voxels re-examined by work-list sweeps */
+  double t_total_s;        /* device time of the correction (events on the library stream) */
+  double t_dense_s;        /* device time of the full first pass (labeling + first sweep) */
+  double bytes_algorithmic;/* algorithmic HBM bytes of the passes actually run */
+} pmsz_stats;
+
+/* Parameter defaults: q = 5, sub = 1,1,1, lazy, max_sweeps = 100000,
+ * bound-safe clamp, full ghost policy. */
+void pmsz_default_params(pmsz_params *p);
+
+/* Correct g in place so that the extrema and both steepest-gradient labelings of
+ * g equal those of f, every edit staying within abs_eb.  Replaces c_loops
+ * (pMSz.cu:1872-2074 lazy / sync_pMSz.cu:2035-2204 sync) together with the
+ * buffers main sets up for it (pMSz.cu:2833-2900).  d_dc (nullable, N bytes)
+ * receives the final delta counters (delta_counter, pMSz.cu:173). */
+int pmsz_correct(const pmsz_params *p, const void *d_f, void *d_g, uint8_t *d_dc,
+                 pmsz_stats *stats);
+
+/* The same with host arrays: copies in, corrects, copies the result back
+ * (the H2D / D2H copies of main, pMSz.cu:2870-2900, 2971). */
+int pmsz_correct_host(const pmsz_params *p, const void *h_f, void *h_g, uint8_t *h_dc,
+                      pmsz_stats *stats);
+
+/* Original classification codes of a field (iscriticle data_type=1,
+ * pMSz.cu:649-784): per voxel, low nibble = rank of the steepest-ascent
+ * candidate, high nibble = rank of the steepest-descent candidate, where the 15
+ * candidates (the voxel and its 14 stencil neighbours, pMSz.cu:186-193) are
+ * ranked 0..14 by linear index and rank 7 is the voxel itself (an extremum). */
+int pmsz_codes(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_field,
+               uint8_t *d_codes);
+
+/* Path-compressed ascending / descending Morse-Smale labels: the linear index of
+ * the maximum / minimum each voxel's steepest path ends at.  Replaces
+ * PathCompression (pMSz.cu:255-271) over iscriticle's pointers
+ * (preserveMSS=1, pMSz.cu:739-760).  N must be < 2^31. */
+int pmsz_labels(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_field,
+                int32_t *d_asc, int32_t *d_desc);
+
+/* Per-voxel delta symbols and the edited-voxel count (extract_edits_kernel,
+ * pMSz.cu:2090-2138): symbol q where g is the clamp value, round((g0-g)/delta)
+ * where g < g0, else 0. */
+int pmsz_edit_symbols(const pmsz_params *p, const void *d_f, const void *d_g0,
+                      const void *d_g, uint8_t *d_symbols, int64_t *edited);
+
+const char *pmsz_strerror(int code);
+/* Library version string. */
+const char *pmsz_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

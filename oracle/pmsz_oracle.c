@@ -1,0 +1,178 @@
+/*
+ * pmsz_oracle.c -- CPU restatement of pMSz's correction.  TEST INFRASTRUCTURE
+ * ONLY: loaded by the tests, the smoke check and the benchmark's CPU-baseline
+ * leg as the checker; the product never links or calls it.
+ *
+ * Reference lines restated (file:line in the pMSz sources):
+ *   stencil + order ........ pMSz.cu:186-193, 610-620
+ *   original labeling ...... pMSz.cu:649-784
+ *   detection / targets .... pMSz.cu:786-898
+ *   edit primitive ......... pMSz.cu:623-645
+ *   loops .................. pMSz.cu:1872-2074 (lazy), sync_pMSz.cu:2102-2200 (sync)
+ *   ghost values / sync .... pMSz.cu:320-608, 909-1272
+ *   path compression ....... pMSz.cu:255-271
+ *   edit symbols ........... pMSz.cu:2090-2138
+ *   block split ............ pMSz.cu:2690-2711
+ *
+ * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
+ */
+#include "pmsz_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* The 14 stencil offsets (dx, dy, dz) in the reference's order (pMSz.cu:186-193):
+ * the 6 faces, then the Freudenthal diagonals split along (-1, 1, 1).  Pairs
+ * (2k, 2k+1) are opposite directions. */
+static const int ORC_DIRS[14][3] = {
+    {1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0}, {0, 0, 1},   {0, 0, -1},  {-1, 1, 0},
+    {1, -1, 0}, {-1, 0, 1}, {1, 0, -1}, {0, 1, 1},  {0, -1, -1}, {-1, 1, 1},  {1, -1, -1}};
+
+/* Missing-neighbour sentinel (INVALID_VAL, pMSz.cu:215), compared in the value
+ * type of the field. */
+#define ORC_INVALID 1e30
+
+void orc_split(int64_t n, int parts, int coord, int64_t *local, int64_t *offset) {
+  int64_t base = n / parts, rem = n % parts;
+  *local = base + (coord < rem ? 1 : 0);
+  *offset = (int64_t)coord * base + (coord < rem ? coord : rem);
+}
+
+/* |a - b| <= eb over the reals.  s = fl(a - b) with its exact rounding error e
+ * (Knuth's TwoSum), so a - b = s + e exactly; then decide by cases, using that
+ * |s| - eb is exact when eb/2 <= |s| <= 2 eb (Sterbenz). */
+int orc_within(double a, double b, double eb) {
+  double s = a - b;
+  double bb = s - a;
+  double e = (a - (s - bb)) + (-b - bb);
+  if (s < 0) { s = -s; e = -e; }
+  if (s < eb) {
+    /* s <= pred(eb), and |e| <= ulp(s)/2 < eb - s */
+    return 1;
+  }
+  if (s > 2 * eb) return 0;
+  double r = s - eb; /* exact: eb <= s <= 2 eb */
+  return r <= -e;
+}
+
+static int orc_ghost_dir_exchanged(int policy, int d) {
+  if (policy == ORC_GHOST_FULL) return 1;
+  return d < 10; /* faces and the four edges of pMSz.cu:394-488 */
+}
+
+static int orc_dir_of(int64_t PX, int64_t PY, int64_t i, int64_t j) {
+  if (i == j) return 14;
+  for (int d = 0; d < 14; d++) {
+    int64_t off = ORC_DIRS[d][0] + PX * (ORC_DIRS[d][1] + PY * ORC_DIRS[d][2]);
+    if (i + off == j) return d;
+  }
+  return 15; /* unreachable for a valid pointer */
+}
+
+static int64_t orc_neighbor_global(int64_t nx, int64_t ny, int64_t i, int d) {
+  if (d >= 14) return i;
+  return i + ORC_DIRS[d][0] + nx * (ORC_DIRS[d][1] + ny * ORC_DIRS[d][2]);
+}
+
+/* Root of every pointer chain (PathCompression, pMSz.cu:255-271).  The reference
+ * chases each chain independently; walking a chain and then rewriting it gives
+ * the same roots. */
+static void orc_compress_pointers(int64_t *lab, int64_t n) {
+  for (int64_t i = 0; i < n; i++) {
+    int64_t r = i;
+    while (lab[r] != r) r = lab[r];
+    int64_t c = i;
+    while (lab[c] != r) {
+      int64_t nx = lab[c];
+      lab[c] = r;
+      c = nx;
+    }
+  }
+}
+
+static float next_up_f32(float c) { return nextafterf(c, INFINITY); }
+static double next_up_f64(double c) { return nextafter(c, INFINITY); }
+static float round_half_away_f32(float r) { return roundf(r); }
+static double round_half_away_f64(double r) { return round(r); }
+
+#define T float
+#define SFX(x) x##_f32
+#include "oracle_impl.inc"
+#undef T
+#undef SFX
+
+#define T double
+#define SFX(x) x##_f64
+#include "oracle_impl.inc"
+#undef T
+#undef SFX
+
+static int orc_check(const orc_params *p) {
+  if (!p) return ORC_EINVAL;
+  if (p->dtype != ORC_F32 && p->dtype != ORC_F64) return ORC_EINVAL;
+  if (p->nx < 1 || p->ny < 1 || p->nz < 1) return ORC_EINVAL;
+  if (p->nx * p->ny * p->nz < 2) return ORC_EINVAL;
+  if (!(p->abs_eb > 0) || isinf(p->abs_eb)) return ORC_EINVAL;
+  if (p->q < 1 || p->q > 255) return ORC_EINVAL;
+  for (int a = 0; a < 3; a++)
+    if (p->sub[a] < 1) return ORC_EINVAL;
+  if (p->sub[0] > p->nx || p->sub[1] > p->ny || p->sub[2] > p->nz) return ORC_EINVAL;
+  if (p->max_sweeps < 1) return ORC_EINVAL;
+  return ORC_OK;
+}
+
+int orc_codes(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *field,
+              uint8_t *codes) {
+  if (nx < 1 || ny < 1 || nz < 1) return ORC_EINVAL;
+  if (dtype == ORC_F32) return codes_f32(nx, ny, nz, (const float *)field, codes);
+  if (dtype == ORC_F64) return codes_f64(nx, ny, nz, (const double *)field, codes);
+  return ORC_EINVAL;
+}
+
+int orc_labels(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *field,
+               int64_t *asc, int64_t *desc) {
+  if (nx < 1 || ny < 1 || nz < 1) return ORC_EINVAL;
+  if (dtype == ORC_F32) return labels_f32(nx, ny, nz, (const float *)field, asc, desc);
+  if (dtype == ORC_F64) return labels_f64(nx, ny, nz, (const double *)field, asc, desc);
+  return ORC_EINVAL;
+}
+
+int orc_sweep(const orc_params *p, const void *f, void *g, uint8_t *dc, int edit,
+              uint32_t counts[3], int64_t *ntargets, uint8_t *target_mask) {
+  int rc = orc_check(p);
+  if (rc) return rc;
+  if (p->dtype == ORC_F32)
+    return sweep_single_f32(p, (const float *)f, (float *)g, dc, edit, counts, ntargets,
+                            target_mask);
+  return sweep_single_f64(p, (const double *)f, (double *)g, dc, edit, counts, ntargets,
+                          target_mask);
+}
+
+int orc_correct(const orc_params *p, const void *f, void *g, uint8_t *dc, orc_stats *st,
+                orc_trace *trace) {
+  int rc = orc_check(p);
+  if (rc) return rc;
+  orc_stats tmp;
+  if (!st) st = &tmp;
+  if (trace) trace->n = 0;
+  if (p->dtype == ORC_F32) {
+    rc = correct_f32(p, (const float *)f, (float *)g, dc, st, trace);
+  } else {
+    rc = correct_f64(p, (const double *)f, (double *)g, dc, st, trace);
+  }
+  return rc;
+}
+
+int orc_edit_symbols(const orc_params *p, const void *f, const void *g0, const void *g,
+                     uint8_t *symbols, int64_t *edited) {
+  int rc = orc_check(p);
+  if (rc) return rc;
+  if (p->dtype == ORC_F32)
+    edit_symbols_f32(p, (const float *)f, (const float *)g0, (const float *)g, symbols,
+                     edited);
+  else
+    edit_symbols_f64(p, (const double *)f, (const double *)g0, (const double *)g, symbols,
+                     edited);
+  return ORC_OK;
+}

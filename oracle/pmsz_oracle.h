@@ -1,0 +1,114 @@
+/*
+ * pmsz_oracle.h -- CPU restatement of pMSz's extrema + Morse-Smale segmentation
+ * correction.  TEST INFRASTRUCTURE ONLY: the test suite, the smoke check and the
+ * CPU-baseline leg of the benchmark load this library as the checker.  The product
+ * (libpmsz.so) never links, loads or calls it.
+ *
+ * Every function restates the reference algorithm under *snapshot semantics*:
+ * within one sweep every classification reads the start-of-sweep field and every
+ * target receives exactly one edit, evaluated on its start-of-sweep (f, g, dc).
+ * The reference kernel (pMSz.cu:786-898) reads neighbours while other threads of
+ * the same launch lower them, so its own output is schedule dependent; the
+ * snapshot order is the deterministic restatement both this oracle and the HIP
+ * kernels implement.  Its per-sweep edit dedup mirrors the atomicCAS on edits[]
+ * at pMSz.cu:795.
+ *
+ * Layout: every subdomain is held as the reference holds one rank's block, a
+ * padded (lx+2)(ly+2)(lz+2) array whose 1-voxel halo holds ghost values or the
+ * missing-neighbour sentinel 1e30 (pMSz.cu:215, 2709-2711).  One subdomain
+ * (sub = 1,1,1) is the single-rank case of the reference.
+ */
+#ifndef PMSZ_ORACLE_H
+#define PMSZ_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_F32 = 0, ORC_F64 = 1 };
+enum { ORC_LAZY = 0, ORC_SYNC = 1 };
+enum { ORC_CLAMP_REFERENCE = 0, ORC_CLAMP_BOUND_SAFE = 1 };
+enum { ORC_GHOST_REFERENCE = 0, ORC_GHOST_FULL = 1 };
+
+enum {
+  ORC_OK = 0,
+  ORC_EINVAL = -1,
+  ORC_ENOMEM = -2,
+  ORC_NOT_CONVERGED = -3,
+};
+
+typedef struct {
+  int dtype;            /* ORC_F32 / ORC_F64 */
+  int64_t nx, ny, nz;   /* x fastest */
+  double abs_eb;        /* absolute error bound (pMSz.cu:2628) */
+  int q;                /* decrements before clamping, 5 in the reference (pMSz.cu:2630) */
+  int sub[3];           /* subdomain grid PX PY PZ (pMSz.cu:2631-2633) */
+  int mode;             /* ORC_LAZY (pMSz.cu) or ORC_SYNC (sync_pMSz.cu) */
+  int max_sweeps;       /* cap on edit sweeps; the reference has none */
+  int clamp_mode;       /* ORC_CLAMP_* */
+  int ghost_policy;     /* ORC_GHOST_* */
+} orc_params;
+
+typedef struct {
+  int64_t edit_sweeps;      /* dense detect+edit sweeps (summed over subdomains' max) */
+  int64_t check_sweeps;     /* checkFinal sweeps */
+  int64_t outer_iters;      /* outer iterations (ghost syncs) */
+  int64_t ite;              /* the reference's printed "iteration is:" counter (max over ranks) */
+  int64_t edited_voxels;    /* h_count of extract_edits_kernel (pMSz.cu:2116-2119) */
+  int64_t clamped_voxels;   /* voxels whose final dc == q */
+  int64_t total_targets;    /* sum over sweeps of distinct targets edited */
+  uint32_t final_cnt[3];    /* counters of the last check sweep: max, min, dir */
+  int status;
+} orc_stats;
+
+/* Per-sweep trace (optional): the counters and the number of distinct targets of
+ * every edit sweep, in execution order (subdomain-major within an outer iteration). */
+typedef struct {
+  int64_t cap;              /* capacity of the arrays below */
+  int64_t n;                /* entries written */
+  uint32_t *cnt;            /* 3 per entry */
+  int64_t *targets;         /* 1 per entry */
+} orc_trace;
+
+/* Original classification codes of a whole field: for every voxel the stencil
+ * direction (0..13) of its steepest-ascent neighbour in the low nibble and of its
+ * steepest-descent neighbour in the high nibble, 14 meaning the voxel itself
+ * (a maximum / minimum).  Restates iscriticle data_type=1, pMSz.cu:649-784. */
+int orc_codes(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *field,
+              uint8_t *codes);
+
+/* Path-compressed ascending / descending Morse-Smale labels (global linear index of
+ * the extremum each voxel flows to).  Restates PathCompression, pMSz.cu:255-271,
+ * over the pointers of iscriticle(preserveMSS=1), pMSz.cu:739-760. */
+int orc_labels(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *field,
+               int64_t *asc, int64_t *desc);
+
+/* One snapshot sweep on a single domain: classify g, count, and (edit=1) apply
+ * one edit to every target.  counts[3] = max, min, dir; *ntargets = distinct
+ * targets; target_mask (nullable, N bytes) marks them.  dc is updated in place. */
+int orc_sweep(const orc_params *p, const void *f, void *g, uint8_t *dc, int edit,
+              uint32_t counts[3], int64_t *ntargets, uint8_t *target_mask);
+
+/* The full correction: g holds the decompressed field on entry and the corrected
+ * field on exit; dc (nullable) receives the final per-voxel delta counters.
+ * Restates c_loops, pMSz.cu:1872-2074 (lazy) and sync_pMSz.cu:2102-2200 (sync). */
+int orc_correct(const orc_params *p, const void *f, void *g, uint8_t *dc,
+                orc_stats *st, orc_trace *trace);
+
+/* Per-voxel delta symbols and the edited-voxel count of a correction
+ * (extract_edits_kernel, pMSz.cu:2090-2138). */
+int orc_edit_symbols(const orc_params *p, const void *f, const void *g0,
+                     const void *g, uint8_t *symbols, int64_t *edited);
+
+/* Balanced block split of one dimension (pMSz.cu:2690-2706). */
+void orc_split(int64_t n, int parts, int coord, int64_t *local, int64_t *offset);
+
+/* The exact bound predicate |a - b| <= eb over the reals (a, b, eb doubles). */
+int orc_within(double a, double b, double eb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,197 @@
+"""ctypes view of the CPU oracle (libpmsz_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` and by
+the CPU-baseline leg of ``bench.py`` as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libpmsz_oracle.so")
+
+LAZY, SYNC = 0, 1
+CLAMP_REFERENCE, CLAMP_BOUND_SAFE = 0, 1
+GHOST_REFERENCE, GHOST_FULL = 0, 1
+NOT_CONVERGED = -3
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("dtype", ctypes.c_int),
+        ("nx", ctypes.c_int64),
+        ("ny", ctypes.c_int64),
+        ("nz", ctypes.c_int64),
+        ("abs_eb", ctypes.c_double),
+        ("q", ctypes.c_int),
+        ("sub", ctypes.c_int * 3),
+        ("mode", ctypes.c_int),
+        ("max_sweeps", ctypes.c_int),
+        ("clamp_mode", ctypes.c_int),
+        ("ghost_policy", ctypes.c_int),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("edit_sweeps", ctypes.c_int64),
+        ("check_sweeps", ctypes.c_int64),
+        ("outer_iters", ctypes.c_int64),
+        ("ite", ctypes.c_int64),
+        ("edited_voxels", ctypes.c_int64),
+        ("clamped_voxels", ctypes.c_int64),
+        ("total_targets", ctypes.c_int64),
+        ("final_cnt", ctypes.c_uint32 * 3),
+        ("status", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "final_cnt"}
+        d["final_cnt"] = list(self.final_cnt)
+        return d
+
+
+class Trace(ctypes.Structure):
+    _fields_ = [
+        ("cap", ctypes.c_int64),
+        ("n", ctypes.c_int64),
+        ("cnt", ctypes.POINTER(ctypes.c_uint32)),
+        ("targets", ctypes.POINTER(ctypes.c_int64)),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
+        L.orc_codes.argtypes = [i32, i64, i64, i64, vp, vp]
+        L.orc_labels.argtypes = [i32, i64, i64, i64, vp, vp, vp]
+        L.orc_sweep.argtypes = [ctypes.POINTER(Params), vp, vp, vp, i32, vp, vp, vp]
+        L.orc_correct.argtypes = [ctypes.POINTER(Params), vp, vp, vp, ctypes.POINTER(Stats),
+                                  ctypes.POINTER(Trace)]
+        L.orc_edit_symbols.argtypes = [ctypes.POINTER(Params), vp, vp, vp, vp, vp]
+        L.orc_within.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.orc_within.restype = i32
+        L.orc_split.argtypes = [i64, i32, i32, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.orc_split.restype = None
+        _lib = L
+    return _lib
+
+
+def _dtype_code(a):
+    if a.dtype == np.float32:
+        return 0
+    if a.dtype == np.float64:
+        return 1
+    raise TypeError(a.dtype)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def make_params(shape, dtype, eb, q=5, sub=(1, 1, 1), mode=LAZY, max_sweeps=100000,
+                clamp_mode=CLAMP_BOUND_SAFE, ghost_policy=GHOST_FULL):
+    """shape is (nz, ny, nx) -- numpy C order with x fastest."""
+    nz, ny, nx = shape
+    p = Params()
+    p.dtype = 0 if np.dtype(dtype) == np.float32 else 1
+    p.nx, p.ny, p.nz = nx, ny, nz
+    p.abs_eb = float(eb)
+    p.q = q
+    p.sub[0], p.sub[1], p.sub[2] = sub
+    p.mode = mode
+    p.max_sweeps = max_sweeps
+    p.clamp_mode = clamp_mode
+    p.ghost_policy = ghost_policy
+    return p
+
+
+def codes(field):
+    field = np.ascontiguousarray(field)
+    nz, ny, nx = field.shape
+    out = np.empty(field.shape, np.uint8)
+    rc = lib().orc_codes(_dtype_code(field), nx, ny, nz, _ptr(field), _ptr(out))
+    assert rc == 0, rc
+    return out
+
+
+def labels(field):
+    field = np.ascontiguousarray(field)
+    nz, ny, nx = field.shape
+    asc = np.empty(field.shape, np.int64)
+    desc = np.empty(field.shape, np.int64)
+    rc = lib().orc_labels(_dtype_code(field), nx, ny, nz, _ptr(field), _ptr(asc), _ptr(desc))
+    assert rc == 0, rc
+    return asc, desc
+
+
+def sweep(f, g, dc, eb, edit=True, **kw):
+    """One snapshot sweep on a single domain; returns (g', dc', counts, ntargets, mask)."""
+    f = np.ascontiguousarray(f)
+    g = np.array(g, dtype=f.dtype, copy=True)
+    dc = np.array(dc, dtype=np.uint8, copy=True)
+    p = make_params(f.shape, f.dtype, eb, **kw)
+    counts = np.zeros(3, np.uint32)
+    nt = ctypes.c_int64(0)
+    mask = np.zeros(f.shape, np.uint8)
+    rc = lib().orc_sweep(ctypes.byref(p), _ptr(f), _ptr(g), _ptr(dc), 1 if edit else 0,
+                         _ptr(counts), ctypes.cast(ctypes.byref(nt), ctypes.c_void_p), _ptr(mask))
+    assert rc == 0, rc
+    return g, dc, counts, nt.value, mask
+
+
+def correct(f, g0, eb, trace_cap=0, **kw):
+    """Full correction; returns (g, dc, stats dict, trace or None, rc)."""
+    f = np.ascontiguousarray(f)
+    g = np.array(g0, dtype=f.dtype, copy=True)
+    dc = np.zeros(f.shape, np.uint8)
+    p = make_params(f.shape, f.dtype, eb, **kw)
+    st = Stats()
+    tr = None
+    trp = ctypes.POINTER(Trace)()
+    if trace_cap:
+        cnt = np.zeros(3 * trace_cap, np.uint32)
+        tg = np.zeros(trace_cap, np.int64)
+        tr = Trace(trace_cap, 0, cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                   tg.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        trp = ctypes.pointer(tr)
+    rc = lib().orc_correct(ctypes.byref(p), _ptr(f), _ptr(g), _ptr(dc), ctypes.byref(st), trp)
+    trace = None
+    if tr is not None:
+        trace = {"cnt": cnt[: 3 * tr.n].reshape(-1, 3).copy(), "targets": tg[: tr.n].copy()}
+    return g, dc, st.as_dict(), trace, rc
+
+
+def edit_symbols(f, g0, g, eb, **kw):
+    p = make_params(f.shape, f.dtype, eb, **kw)
+    sym = np.zeros(f.shape, np.uint8)
+    ed = ctypes.c_int64(0)
+    rc = lib().orc_edit_symbols(ctypes.byref(p), _ptr(f), _ptr(g0), _ptr(g), _ptr(sym),
+                                ctypes.cast(ctypes.byref(ed), ctypes.c_void_p))
+    assert rc == 0, rc
+    return sym, ed.value
+
+
+def within(a, b, eb):
+    return bool(lib().orc_within(float(a), float(b), float(eb)))
+
+
+def split(n, parts, coord):
+    lo, off = ctypes.c_int64(0), ctypes.c_int64(0)
+    lib().orc_split(n, parts, coord, ctypes.byref(lo), ctypes.byref(off))
+    return lo.value, off.value

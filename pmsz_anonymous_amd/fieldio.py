@@ -1,0 +1,130 @@
+"""Python view of libpmsz_io.so: synthetic Perlin fields, the error-bounded codecs
+that produce the decompressed field, and raw I/O.  Front end of the correction
+path (the reference's reader / compressor adapters), not part of the hot path."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libpmsz_io.so")
+_lib = None
+
+EUNAVAILABLE = -6
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(
+                f"{_LIB_PATH} is not built; run __graft_entry__.build() or "
+                "python -m pmsz_anonymous_amd.build")
+        L = ctypes.CDLL(_LIB_PATH)
+        vp, i64, i32, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+        L.pmsz_io_perlin.argtypes = [i32, i64, i64, i64, ctypes.c_uint64, dbl, i32, dbl, vp]
+        L.pmsz_io_range.argtypes = [i32, i64, vp, ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
+        L.pmsz_io_szlike.argtypes = [i32, i64, i64, i64, vp, dbl, vp, ctypes.POINTER(i64),
+                                     ctypes.POINTER(dbl)]
+        L.pmsz_io_zfp_available.argtypes = []
+        L.pmsz_io_zfp.argtypes = [i32, i64, i64, i64, vp, dbl, vp, ctypes.POINTER(i64)]
+        L.pmsz_io_max_abs_err.argtypes = [i32, i64, vp, vp, dbl, ctypes.POINTER(dbl),
+                                          ctypes.POINTER(i64)]
+        L.pmsz_io_read_raw.argtypes = [ctypes.c_char_p, i32, i64, vp]
+        L.pmsz_io_write_raw.argtypes = [ctypes.c_char_p, i32, i64, vp]
+        _lib = L
+    return _lib
+
+
+def _dt(dtype):
+    dtype = np.dtype(dtype)
+    if dtype == np.float32:
+        return 0
+    if dtype == np.float64:
+        return 1
+    raise TypeError(f"unsupported dtype {dtype}")
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def perlin(shape, seed, freq, octaves, persistence=0.5, dtype=np.float32):
+    """Fractal Perlin noise of shape (nz, ny, nx), values about [-1, 1]."""
+    nz, ny, nx = shape
+    out = np.empty(shape, dtype)
+    rc = lib().pmsz_io_perlin(_dt(dtype), nx, ny, nz, seed, freq, octaves, persistence, _p(out))
+    if rc:
+        raise ValueError(f"pmsz_io_perlin failed ({rc})")
+    return out
+
+
+def value_range(a):
+    a = np.ascontiguousarray(a)
+    lo, hi = ctypes.c_double(), ctypes.c_double()
+    rc = lib().pmsz_io_range(_dt(a.dtype), a.size, _p(a), ctypes.byref(lo), ctypes.byref(hi))
+    if rc:
+        raise ValueError(rc)
+    return lo.value, hi.value
+
+
+def szlike(f, abs_eb):
+    """Decompressed field of the SZ-style codec; returns (g0, info)."""
+    f = np.ascontiguousarray(f)
+    nz, ny, nx = f.shape
+    g0 = np.empty_like(f)
+    un, eb_bytes = ctypes.c_int64(), ctypes.c_double()
+    rc = lib().pmsz_io_szlike(_dt(f.dtype), nx, ny, nz, _p(f), abs_eb, _p(g0), ctypes.byref(un),
+                              ctypes.byref(eb_bytes))
+    if rc:
+        raise ValueError(f"pmsz_io_szlike failed ({rc})")
+    return g0, {"unpredictable": un.value, "est_bytes": eb_bytes.value}
+
+
+def zfp_available():
+    return bool(lib().pmsz_io_zfp_available())
+
+
+def zfp(f, tolerance):
+    """ZFP fixed-accuracy round trip; returns (g0, info).  Raises if libzfp is absent."""
+    f = np.ascontiguousarray(f)
+    nz, ny, nx = f.shape
+    g0 = np.empty_like(f)
+    nb = ctypes.c_int64()
+    rc = lib().pmsz_io_zfp(_dt(f.dtype), nx, ny, nz, _p(f), tolerance, _p(g0), ctypes.byref(nb))
+    if rc == EUNAVAILABLE:
+        raise RuntimeError("libzfp is not available")
+    if rc:
+        raise ValueError(f"pmsz_io_zfp failed ({rc})")
+    return g0, {"bytes": nb.value}
+
+
+def compress(f, abs_eb, compressor="sz3"):
+    """Decompressed field for the reference's compressor ids ('sz3' / 'zfp')."""
+    if compressor == "sz3":
+        return szlike(f, abs_eb)
+    if compressor == "zfp":
+        return zfp(f, abs_eb)
+    raise ValueError(f"unknown compressor {compressor!r}")
+
+
+def bound_violations(a, b, bound):
+    """(max |a-b|, number of voxels where |a-b| <= bound fails exactly)."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b, dtype=a.dtype)
+    m, v = ctypes.c_double(), ctypes.c_int64()
+    lib().pmsz_io_max_abs_err(_dt(a.dtype), a.size, _p(a), _p(b), bound, ctypes.byref(m),
+                              ctypes.byref(v))
+    return m.value, v.value
+
+
+def perlin_case(shape, seed, freq, octaves, eb_rel, compressor="sz3", dtype=np.float32,
+                persistence=0.5):
+    """The (f, g0, abs_eb) triple of a Perlin workload: abs_eb = eb_rel * value range."""
+    f = perlin(shape, seed, freq, octaves, persistence, dtype)
+    lo, hi = value_range(f)
+    eb = eb_rel * (hi - lo)
+    g0, _ = compress(f, eb, compressor)
+    return f, g0, eb
