@@ -1,0 +1,190 @@
+// pmsz_cli.cpp -- the command-line drop-in for the reference executable.
+//
+//   pmsz "<path>,<W>,<H>,<D>" <abs_eb> <sz3|zfp> <PX> <PY> <PZ> [options]
+//
+// The positional contract is the one main() parses (pMSz.cu:2625-2667): a raw
+// little-endian field (x fastest), the absolute error bound, the compressor id
+// and the subdomain grid.  The reference reads float64 only (pMSz.cu:2821);
+// --dtype f32 reads float32.  Output lines keep the reference's keys
+// ("iteration is:", pMSz.cu:2073; "overall edit ratio is: ... total_edited:",
+// pMSz.cu:3144).
+//
+// options: --dtype f32|f64   --mode lazy|sync   --max-sweeps N
+//          --clamp reference|bound-safe   --ghost reference|full   --q N
+//          --out <path> (corrected field, raw)   --symbols <path> (delta symbols)
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/pmsz.h"
+#include "../../include/pmsz_io.h"
+
+namespace {
+
+int usage() {
+  std::fprintf(stderr,
+               "usage: pmsz \"<path>,<W>,<H>,<D>\" <abs_eb> <sz3|zfp> <PX> <PY> <PZ> "
+               "[--dtype f32|f64] [--mode lazy|sync] [--max-sweeps N] "
+               "[--clamp reference|bound-safe] [--ghost reference|full] [--q N] "
+               "[--out path] [--symbols path]\n");
+  return 2;
+}
+
+bool parse_dims(const std::string &s, std::string &path, int64_t &w, int64_t &h, int64_t &d) {
+  size_t c1 = s.find(',');
+  if (c1 == std::string::npos) return false;
+  path = s.substr(0, c1);
+  std::istringstream iss(s.substr(c1 + 1));
+  char a, b;
+  if (!(iss >> w >> a >> h >> b >> d) || a != ',' || b != ',') return false;
+  return w > 0 && h > 0 && d > 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  if (argc < 7) return usage();
+  std::string path;
+  int64_t W = 0, H = 0, D = 0;
+  if (!parse_dims(argv[1], path, W, H, D)) {
+    std::fprintf(stderr, "Parsing error for dimensions\n");
+    return 1;
+  }
+  pmsz_params p;
+  pmsz_default_params(&p);
+  p.dtype = PMSZ_F64;
+  p.nx = W;
+  p.ny = H;
+  p.nz = D;
+  p.abs_eb = std::strtod(argv[2], nullptr);
+  std::string comp = argv[3];
+  p.sub[0] = std::atoi(argv[4]);
+  p.sub[1] = std::atoi(argv[5]);
+  p.sub[2] = std::atoi(argv[6]);
+  std::string out_path, sym_path;
+  for (int i = 7; i < argc; i++) {
+    std::string o = argv[i];
+    auto val = [&](void) -> std::string {
+      if (i + 1 >= argc) {
+        usage();
+        std::exit(2);
+      }
+      return argv[++i];
+    };
+    if (o == "--dtype") {
+      std::string v = val();
+      p.dtype = v == "f32" ? PMSZ_F32 : PMSZ_F64;
+    } else if (o == "--mode") {
+      p.mode = val() == "sync" ? PMSZ_SYNC : PMSZ_LAZY;
+    } else if (o == "--max-sweeps") {
+      p.max_sweeps = std::atoi(val().c_str());
+    } else if (o == "--clamp") {
+      p.clamp_mode = val() == "reference" ? PMSZ_CLAMP_REFERENCE : PMSZ_CLAMP_BOUND_SAFE;
+    } else if (o == "--ghost") {
+      p.ghost_policy = val() == "reference" ? PMSZ_GHOST_REFERENCE : PMSZ_GHOST_FULL;
+    } else if (o == "--q") {
+      p.q = std::atoi(val().c_str());
+    } else if (o == "--out") {
+      out_path = val();
+    } else if (o == "--symbols") {
+      sym_path = val();
+    } else {
+      return usage();
+    }
+  }
+  if (comp != "sz3" && comp != "zfp") {
+    std::fprintf(stderr, "unknown compressor %s (sz3|zfp)\n", comp.c_str());
+    return 1;
+  }
+  const int64_t n = W * H * D;
+  const size_t es = p.dtype == PMSZ_F32 ? 4 : 8;
+  std::vector<unsigned char> f(n * es), g0(n * es), g(n * es);
+  if (pmsz_io_read_raw(path.c_str(), p.dtype, n, f.data()) != PMSZ_IO_OK) {
+    std::fprintf(stderr, "cannot read %lld values from %s\n", (long long)n, path.c_str());
+    return 1;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  int rc;
+  int64_t cbytes = 0;
+  double est = 0;
+  if (comp == "sz3") {
+    int64_t un = 0;
+    rc = pmsz_io_szlike(p.dtype, W, H, D, f.data(), p.abs_eb, g0.data(), &un, &est);
+    cbytes = (int64_t)est;
+  } else {
+    rc = pmsz_io_zfp(p.dtype, W, H, D, f.data(), p.abs_eb, g0.data(), &cbytes);
+  }
+  if (rc != PMSZ_IO_OK) {
+    std::fprintf(stderr, "compressor %s failed (%d)\n", comp.c_str(), rc);
+    return 1;
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  std::memcpy(g.data(), g0.data(), g.size());
+  void *df = nullptr, *dg0 = nullptr, *dg = nullptr;
+  uint8_t *dsym = nullptr;
+  if (hipMalloc(&df, f.size()) != hipSuccess || hipMalloc(&dg0, g.size()) != hipSuccess ||
+      hipMalloc(&dg, g.size()) != hipSuccess || hipMalloc((void **)&dsym, n) != hipSuccess) {
+    std::fprintf(stderr, "device allocation failed\n");
+    return 1;
+  }
+  if (hipMemcpy(df, f.data(), f.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dg0, g0.data(), g0.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dg, g0.data(), g0.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    std::fprintf(stderr, "host to device copy failed\n");
+    return 1;
+  }
+  std::printf("topology preserving started!\n");
+  pmsz_stats st;
+  auto t2 = std::chrono::steady_clock::now();
+  rc = pmsz_correct(&p, df, dg, nullptr, &st);
+  auto t3 = std::chrono::steady_clock::now();
+  if (rc != PMSZ_OK && rc != PMSZ_NOT_CONVERGED) {
+    std::fprintf(stderr, "pmsz_correct: %s (%d)\n", pmsz_strerror(rc), rc);
+    return 1;
+  }
+  std::printf("iteration is: %lld\n", (long long)st.ite);
+  int64_t edited = 0;
+  if (pmsz_edit_symbols(&p, df, dg0, dg, dsym, &edited) != PMSZ_OK) {
+    std::fprintf(stderr, "edit extraction failed\n");
+    return 1;
+  }
+  if (hipMemcpy(g.data(), dg, g.size(), hipMemcpyDeviceToHost) != hipSuccess) return 1;
+  if (!out_path.empty() && pmsz_io_write_raw(out_path.c_str(), p.dtype, n, g.data()) != 0) {
+    std::fprintf(stderr, "cannot write %s\n", out_path.c_str());
+    return 1;
+  }
+  if (!sym_path.empty()) {
+    std::vector<uint8_t> sym(n);
+    if (hipMemcpy(sym.data(), dsym, n, hipMemcpyDeviceToHost) != hipSuccess) return 1;
+    FILE *fp = std::fopen(sym_path.c_str(), "wb");
+    if (!fp || std::fwrite(sym.data(), 1, n, fp) != (size_t)n) return 1;
+    std::fclose(fp);
+  }
+  double max_err = 0;
+  int64_t viol = 0;
+  pmsz_io_max_abs_err(p.dtype, n, f.data(), g.data(), p.abs_eb, &max_err, &viol);
+  double secs = std::chrono::duration<double>(t3 - t2).count();
+  std::printf("compression time: %.6f s (%s, %lld bytes)\n",
+              std::chrono::duration<double>(t1 - t0).count(), comp.c_str(), (long long)cbytes);
+  std::printf("comfputational overhead is: %.6f\n", secs);
+  std::printf("sweeps: %lld edit, %lld check, outer iterations %lld, status %s\n",
+              (long long)st.edit_sweeps, (long long)st.check_sweeps, (long long)st.outer_iters,
+              rc == PMSZ_OK ? "converged" : "NOT CONVERGED");
+  std::printf("max abs error: %.17g (bound %.17g, violations %lld)\n", max_err, p.abs_eb,
+              (long long)viol);
+  std::printf("overall edit ratio is: %.17g total_edited:%lld\n", (double)edited / (double)n,
+              (long long)edited);
+  std::printf("throughput: %.3f GVoxels/s (device %.3f ms)\n", n / secs / 1e9,
+              st.t_total_s * 1e3);
+  (void)hipFree(df);
+  (void)hipFree(dg0);
+  (void)hipFree(dg);
+  (void)hipFree(dsym);
+  return rc == PMSZ_OK ? 0 : 3;
+}
