@@ -114,6 +114,12 @@ int pmsz_labels(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_fie
 int pmsz_edit_symbols(const pmsz_params *p, const void *d_f, const void *d_g0,
                       const void *d_g, uint8_t *d_symbols, int64_t *edited);
 
+/* Average device time of the fused first pass (original labeling + first
+ * detection sweep) over `iters` launches, timed with HIP events on the library
+ * stream.  Measurement aid for the roofline report; it leaves no result. */
+int pmsz_time_first_pass(const pmsz_params *p, const void *d_f, const void *d_g, int iters,
+                         double *avg_seconds);
+
 const char *pmsz_strerror(int code);
 /* Library version string. */
 const char *pmsz_version(void);

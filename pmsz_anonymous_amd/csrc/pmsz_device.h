@@ -1,0 +1,230 @@
+// pmsz_device.h -- device-side building blocks shared by the correction kernels:
+// grid geometry, per-sweep counters, the candidate scans, the detection rules and
+// block-aggregated work-list appends.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pmsz_common.h"
+
+namespace pmsz {
+
+struct Grid3 {
+  int64_t nx, ny, nz;
+  int64_t sxy;  // nx*ny
+  int64_t n;    // nx*ny*nz
+};
+
+// Next-sweep decision written by the apply of sweep j into its counter block.
+enum : uint32_t {
+  kModeUnset = 0,
+  kModeDense = 1,   // detection j+1 runs over every voxel (tile kernel)
+  kModeSparse = 2,  // detection j+1 runs over the work list
+  kModeTail = 3,    // the single-workgroup tail kernel takes over at sweep j
+  kModeDone = 4,    // finished (converged or capped)
+};
+
+// Counters of detection j and of the apply that follows it (one 64-byte block per
+// sweep, in a ring).  cnt: false max / false min / direction (count_f_max,
+// count_f_min, count_f_dir, pMSz.cu:174); ntargets: entries of tlist; nwork: entries
+// of wlist queued by apply j; mode: the decision of apply j.
+struct Counters {
+  uint32_t cnt[3];
+  uint32_t ntargets;
+  uint32_t nwork;
+  uint32_t mode;
+  uint32_t err;
+  uint32_t pad[9];
+};
+static_assert(sizeof(Counters) == 64, "Counters layout");
+
+// Run-wide state, written only by kernels that end a phase.
+struct RunState {
+  uint32_t tail_j;     // sweep whose apply the tail kernel starts with (0: none)
+  uint32_t done;       // 0 running, 1 converged, 2 sweep cap reached
+  uint32_t last_j;     // detections performed when done
+  uint32_t err;        // bit 0: NaN or sentinel value in an input
+  unsigned long long total_targets;
+  unsigned long long touched;
+  unsigned long long clamped;
+  unsigned long long sparse_voxels;
+  unsigned long long dense_passes;
+  unsigned long long pad[3];
+};
+
+constexpr int kCtrRing = 256;  // counter blocks kept; consecutive sweeps use distinct slots
+
+__device__ __forceinline__ int64_t rank_off(int r, const Grid3 &d) {
+  return (int64_t)rank_dx(r) + d.nx * (int64_t)rank_dy(r) + d.sxy * (int64_t)rank_dz(r);
+}
+
+// Bit r set iff candidate r of (x, y, z) lies in the domain; out-of-domain
+// neighbours are the reference's INVALID halo (pMSz.cu:709).
+__device__ __forceinline__ uint32_t valid_mask(int64_t x, int64_t y, int64_t z, const Grid3 &d) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int r = 0; r < kCand; r++) {
+    int64_t a = x + rank_dx(r), b = y + rank_dy(r), c = z + rank_dz(r);
+    bool ok = a >= 0 && a < d.nx && b >= 0 && b < d.ny && c >= 0 && c < d.nz;
+    m |= (ok ? 1u : 0u) << r;
+  }
+  return m;
+}
+
+// Steepest ascent / descent candidate over all 15 candidates.  Visiting ranks in
+// increasing order, ">=" keeps the highest rank among equal maxima and "<" the
+// lowest rank among equal minima: the (value, index) order of pMSz.cu:610-620.
+template <typename T>
+__device__ __forceinline__ void scan_full(const T (&c)[kCand], int &asc, int &desc) {
+  T bv = c[0], sv = c[0];
+  int br = 0, sr = 0;
+#pragma unroll
+  for (int r = 1; r < kCand; r++) {
+    bool ge = c[r] >= bv;
+    bv = ge ? c[r] : bv;
+    br = ge ? r : br;
+    bool lt = c[r] < sv;
+    sv = lt ? c[r] : sv;
+    sr = lt ? r : sr;
+  }
+  asc = br;
+  desc = sr;
+}
+
+// The same over the valid candidates only (domain boundary).
+template <typename T>
+__device__ __forceinline__ void scan_masked(const T (&c)[kCand], uint32_t valid, int &asc,
+                                            int &desc) {
+  T bv = c[kSelf], sv = c[kSelf];
+  int br = kSelf, sr = kSelf;
+#pragma unroll
+  for (int r = 0; r < kCand; r++) {
+    if (r == kSelf || !((valid >> r) & 1u)) continue;
+    bool gt = c[r] > bv || (c[r] == bv && r > br);
+    bv = gt ? c[r] : bv;
+    br = gt ? r : br;
+    bool lt = c[r] < sv || (c[r] == sv && r < sr);
+    sv = lt ? c[r] : sv;
+    sr = lt ? r : sr;
+  }
+  asc = br;
+  desc = sr;
+}
+
+// Detection at one voxel from its original codes (oa, od) and the current
+// steepest candidates (da, dd): iscriticle data_type=0, pMSz.cu:786-898.
+// Returns the number of target ranks written to t[], counters in c[].
+__device__ __forceinline__ int detect_targets(int oa, int od, int da, int dd, int (&t)[4],
+                                              uint32_t (&c)[3]) {
+  bool omax = oa == kSelf;
+  bool omin = od == kSelf && !omax;  // a voxel that is both counts as a maximum (pMSz.cu:750)
+  bool dmax = da == kSelf, dmin = dd == kSelf;
+  bool fm = dmax != omax, fn = dmin != omin;
+  int nt = 0;
+  if (fm) {
+    c[0]++;
+    t[nt++] = omax ? da : kSelf;  // lower the new top neighbour, or the false maximum
+  }
+  if (fn) {
+    c[1]++;
+    t[nt++] = omin ? kSelf : od;  // lower the lost minimum, or the true descent neighbour
+  }
+  if (fm || fn) {
+    c[2]++;
+  } else {
+    if (da != oa) {
+      c[2]++;
+      t[nt++] = da;  // the wrong steepest-ascent neighbour is too high
+    }
+    if (dd != od) {
+      c[2]++;
+      t[nt++] = od;  // the true steepest-descent neighbour is not low enough
+    }
+  }
+  return nt;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Set the mark bit of v; true if it was clear (the caller then owns v).
+__device__ __forceinline__ bool claim(uint32_t *marks, uint32_t v) {
+  uint32_t bit = 1u << (v & 31u);
+  uint32_t old = atomicOr(&marks[v >> 5], bit);
+  return (old & bit) == 0;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  return x;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Block-wide exclusive scan of per-thread counts with one global atomicAdd on
+// `counter` for the block's total; returns this thread's first list position.
+// All threads of the block must call it (it holds two barriers).
+template <int kBlock>
+__device__ __forceinline__ uint32_t block_reserve(uint32_t n, uint32_t *counter) {
+  constexpr int kWaves = kBlock / 64;
+  __shared__ uint32_t s_off[kWaves];
+  __shared__ uint32_t s_base;
+  const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
+  uint32_t incl = wave_incl_scan(n);
+  if (lane == 63) s_off[wave] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int w = 0; w < kWaves; w++) {
+      uint32_t v = s_off[w];
+      s_off[w] = run;
+      run += v;
+    }
+    s_base = run ? atomicAdd(counter, run) : 0u;
+  }
+  __syncthreads();
+  uint32_t pos = s_base + s_off[wave] + incl - n;
+  __syncthreads();
+  return pos;
+}
+
+// Block-wide sum of three counters added to dst[0..2] with one atomic each.
+template <int kBlock>
+__device__ __forceinline__ void block_add3(const uint32_t (&c)[3], uint32_t *dst) {
+  constexpr int kWaves = kBlock / 64;
+  __shared__ uint32_t s_c[3][kWaves];
+  const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    uint32_t s = wave_sum(c[k]);
+    if (lane == 0) s_c[k][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    uint32_t s = 0;
+    for (int w = 0; w < kWaves; w++) s += s_c[threadIdx.x][w];
+    if (s) atomicAdd(&dst[threadIdx.x], s);
+  }
+  __syncthreads();
+}
+
+// Bits [p, p+32) of the 96-bit little-endian value (w0, w1, w2); bits outside
+// [0, 96) read as zero.  p in [-31, 95].
+__device__ __forceinline__ uint32_t bits32_at(uint32_t w0, uint32_t w1, uint32_t w2, int p) {
+  if (p < 0) return w0 << (-p);
+  if (p >= 96) return 0u;
+  int q = p >> 5, r = p & 31;
+  uint32_t lo = q == 0 ? w0 : (q == 1 ? w1 : w2);
+  uint32_t hi = q == 0 ? w1 : (q == 1 ? w2 : 0u);
+  return r ? ((lo >> r) | (hi << (32 - r))) : lo;
+}
+
+}  // namespace pmsz

@@ -10,20 +10,26 @@
 //                           (edits[] + atomicCAS, pMSz.cu:795) as one bit per voxel
 //   wmark       u32[N/32]   voxels queued for the next sweep
 //   tlist/wlist u32[N]      the same two sets as compact work lists
-//   ctr         Counters[]  one block of counters per sweep
+//   ring        Counters[]  ring of per-sweep counter blocks
+//   run         RunState    phase hand-over between kernels, run totals
 //
 // Snapshot semantics: a sweep detects on the start-of-sweep g, writing only
-// marks, lists and counters; a separate launch then applies one edit per target
-// (applyDeltaBuffer_local, pMSz.cu:623-645).  The classification of a voxel
-// depends on g over its closed stencil neighbourhood only, so after the first
-// sweep only the neighbourhoods of the previous sweep's targets can change
-// (the stencil is symmetric).  Sweep 1 is a full pass fused with the original
-// labeling; every later sweep examines exactly that work list and produces the
-// same counters and targets as a full sweep would.
+// marks, lists and counters; a separate phase then applies one edit per target
+// (applyDeltaBuffer_local, pMSz.cu:623-645).
+//
+// Schedule.  Sweep 1 is one full pass fused with the original labeling.  After
+// that, the apply of sweep j decides on the device how sweep j+1 detects:
+//   - over every voxel (tile kernel) while the targets are dense,
+//   - over the closed stencil neighbourhoods of sweep j's targets otherwise: a
+//     voxel's classification depends only on g over its closed neighbourhood and
+//     the stencil is symmetric, so no other voxel can change -- the counters and
+//     targets equal those of a full sweep;
+//   - in a single-workgroup tail kernel once the targets are few, where a sweep
+//     costs two workgroup barriers instead of two kernel launches.
+// The host enqueues the whole chain without waiting and synchronises once.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -31,187 +37,68 @@
 #include <vector>
 
 #include "../../include/pmsz.h"
-#include "pmsz_common.h"
+#include "pmsz_device.h"
 
 namespace pmsz {
 
-struct Grid3 {
-  int64_t nx, ny, nz;
-  int64_t sxy;  // nx*ny
-  int64_t n;    // nx*ny*nz
-};
-
-// Counters of one sweep.  cnt: false max / false min / direction (count_f_max,
-// count_f_min, count_f_dir, pMSz.cu:174); ntargets: entries of tlist; nwork:
-// entries of wlist queued by this sweep's apply; err bit 0: NaN or the sentinel
-// value in an input field; touched / clamped: voxels whose delta counter left 0 /
-// reached q in this sweep's apply.
-struct Counters {
-  uint32_t cnt[3];
-  uint32_t ntargets;
-  uint32_t nwork;
-  uint32_t err;
-  uint32_t touched;
-  uint32_t clamped;
-};
-
-constexpr int kTX = 64;
+constexpr int kTX = 64;  // dense tile: 64 x 4 voxels per plane, 8 planes
 constexpr int kTY = 4;
 constexpr int kTZ = 8;
 constexpr int kThreads = kTX * kTY;
-constexpr int kCtrCap = 1024;
+constexpr int kPX = kTX + 2, kPY = kTY + 2, kPZ = kTZ + 2;  // tile + halo
+constexpr int kRows = kPY * kPZ;                              // bitmap rows of 66 cells
+constexpr int kApplyThreads = 256;
+constexpr int kTailThreads = 1024;
+constexpr uint32_t kTailTargets = 2048;  // the tail kernel takes over at or below this
 
-__device__ __forceinline__ int64_t rank_off(int r, const Grid3 &d) {
-  return (int64_t)rank_dx(r) + d.nx * (int64_t)rank_dy(r) + d.sxy * (int64_t)rank_dz(r);
-}
-
-// Validity of the 15 candidates of (x, y, z): bit r set iff candidate r lies in
-// the domain (out-of-domain neighbours are the reference's INVALID halo).
-__device__ __forceinline__ uint32_t valid_mask(int64_t x, int64_t y, int64_t z, const Grid3 &d) {
-  uint32_t m = 0;
-#pragma unroll
-  for (int r = 0; r < kCand; r++) {
-    int64_t a = x + rank_dx(r), b = y + rank_dy(r), c = z + rank_dz(r);
-    bool ok = a >= 0 && a < d.nx && b >= 0 && b < d.ny && c >= 0 && c < d.nz;
-    m |= (ok ? 1u : 0u) << r;
-  }
-  return m;
-}
-
-// Steepest ascent / descent candidate over all 15 candidates.  Visiting ranks in
-// increasing order, ">=" keeps the highest rank among equal maxima and "<" the
-// lowest rank among equal minima: the (value, index) order of pMSz.cu:610-620.
-template <typename T>
-__device__ __forceinline__ void scan_full(const T (&c)[kCand], int &asc, int &desc) {
-  T bv = c[0], sv = c[0];
-  int br = 0, sr = 0;
-#pragma unroll
-  for (int r = 1; r < kCand; r++) {
-    bool ge = c[r] >= bv;
-    bv = ge ? c[r] : bv;
-    br = ge ? r : br;
-    bool lt = c[r] < sv;
-    sv = lt ? c[r] : sv;
-    sr = lt ? r : sr;
-  }
-  asc = br;
-  desc = sr;
-}
-
-// The same over the valid candidates only (domain boundary).
-template <typename T>
-__device__ __forceinline__ void scan_masked(const T (&c)[kCand], uint32_t valid, int &asc,
-                                            int &desc) {
-  T bv = c[kSelf], sv = c[kSelf];
-  int br = kSelf, sr = kSelf;
-#pragma unroll
-  for (int r = 0; r < kCand; r++) {
-    if (r == kSelf || !((valid >> r) & 1u)) continue;
-    bool gt = c[r] > bv || (c[r] == bv && r > br);
-    bv = gt ? c[r] : bv;
-    br = gt ? r : br;
-    bool lt = c[r] < sv || (c[r] == sv && r < sr);
-    sv = lt ? c[r] : sv;
-    sr = lt ? r : sr;
-  }
-  asc = br;
-  desc = sr;
-}
-
-__device__ __forceinline__ uint32_t lane_prefix(uint64_t m) {
-  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
-// Wave-aggregated append: every lane of the wave must call it.
-__device__ __forceinline__ void wave_append(bool has, uint32_t val, uint32_t *counter,
-                                            uint32_t *list) {
-  uint64_t m = __ballot(has);
-  if (m == 0) return;
-  uint32_t base = 0;
-  int leader = __ffsll((long long)m) - 1;
-  if ((int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-  base = __shfl(base, leader);
-  if (has) list[base + lane_prefix(m)] = val;
-}
-
-// Set the mark bit of v; true if it was clear (the caller then owns v).
-__device__ __forceinline__ bool claim(uint32_t *marks, uint32_t v) {
-  uint32_t bit = 1u << (v & 31u);
-  uint32_t old = atomicOr(&marks[v >> 5], bit);
-  return (old & bit) == 0;
-}
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
-// Detection at one voxel from its original codes (oa, od) and the current
-// steepest candidates (da, dd): iscriticle data_type=0, pMSz.cu:786-898.
-// Returns up to 4 target ranks in t[], counters in c[].
-__device__ __forceinline__ int detect_targets(int oa, int od, int da, int dd, int (&t)[4],
-                                              uint32_t (&c)[3]) {
-  bool omax = oa == kSelf;
-  bool omin = od == kSelf && !omax;  // a voxel that is both counts as a maximum (pMSz.cu:750)
-  bool dmax = da == kSelf, dmin = dd == kSelf;
-  bool fm = dmax != omax, fn = dmin != omin;
-  int nt = 0;
-  if (fm) {
-    c[0]++;
-    t[nt++] = omax ? da : kSelf;  // lower the new top neighbour, or the false maximum
-  }
-  if (fn) {
-    c[1]++;
-    t[nt++] = omin ? kSelf : od;  // lower the lost minimum, or the true descent neighbour
-  }
-  if (fm || fn) {
-    c[2]++;
-  } else {
-    if (da != oa) {
-      c[2]++;
-      t[nt++] = da;  // the wrong steepest-ascent neighbour is too high
-    }
-    if (dd != od) {
-      c[2]++;
-      t[nt++] = od;  // the true steepest-descent neighbour is not low enough
-    }
-  }
-  return nt;
-}
+struct Sched {
+  int mode;  // PMSZ_LAZY / PMSZ_SYNC
+  uint32_t max_sweeps;
+  uint32_t dense_min_work;  // 15 * targets at or above this -> next detection dense
+};
 
 // ---------------------------------------------------------------------------
-// Full pass: original labeling of f (codes) and, with kDetect, the first
-// detection sweep on g.  One block = a 64 x 4 column of voxels walking 8
-// planes in z; the three planes a voxel's stencil touches are staged in LDS
-// with a one-voxel halo.
-template <typename T, bool kDetect>
-__global__ __launch_bounds__(kThreads) void k_dense_pass(
+// Tile kernel: with kLabel the original labeling of f (codes), with kDetect a
+// detection sweep on g (sweep 1 when both are set; a later dense sweep reading
+// the codes when only kDetect is set).  One block = 64 x 4 voxels walking 8
+// planes; the three planes a voxel's stencil touches are staged in LDS with a
+// one-voxel halo.  Targets are collected in an LDS bitmap over the tile and its
+// halo and flushed once per tile as row masks (one global atomicOr per 32 cells,
+// one list reservation per block).
+template <typename T, bool kLabel, bool kDetect>
+__global__ __launch_bounds__(kThreads) void k_tiles(
     const T *__restrict__ f, const T *__restrict__ g, uint8_t *__restrict__ code,
     uint8_t *__restrict__ dc, uint32_t *__restrict__ tmark, uint32_t *__restrict__ tlist,
-    Counters *__restrict__ ctr, Grid3 d) {
-  constexpr int PX = kTX + 2, PY = kTY + 2;
-  __shared__ T sf[3][PY][PX];
-  __shared__ T sg[kDetect ? 3 : 1][PY][PX];
+    Counters *__restrict__ ring, uint32_t jslot_prev, uint32_t jslot, RunState *run, Grid3 d) {
+  if (!kLabel) {
+    if (ring[jslot_prev].mode != kModeDense) return;  // uniform
+  }
+  constexpr int kFields = (kLabel ? 1 : 0) + (kDetect ? 1 : 0);
+  __shared__ T sv[kFields][3][kPY][kPX];
+  __shared__ uint32_t sbits[kDetect ? kRows : 1][3];
   const int tid = threadIdx.x;
   const int tx = tid % kTX, ty = tid / kTX;
   const int64_t bx = (int64_t)blockIdx.x * kTX, by = (int64_t)blockIdx.y * kTY,
                 bz = (int64_t)blockIdx.z * kTZ;
   const int64_t x = bx + tx, y = by + ty;
+  Counters *out = kDetect ? &ring[jslot] : nullptr;
+  constexpr int kF = 0, kG = kLabel ? 1 : 0;
+
+  if (kDetect)
+    for (int e = tid; e < kRows * 3; e += kThreads) (&sbits[0][0])[e] = 0;
 
   auto load_plane = [&](int64_t pz, int slot) {
-    for (int e = tid; e < PX * PY; e += kThreads) {
-      int lx = e % PX, ly = e / PX;
+    for (int e = tid; e < kPX * kPY; e += kThreads) {
+      int lx = e % kPX, ly = e / kPX;
       int64_t gx = bx + lx - 1, gy = by + ly - 1;
       T vf = 0, vg = 0;
       if (pz >= 0 && pz < d.nz && gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny) {
         int64_t gi = gx + d.nx * gy + d.sxy * pz;
-        vf = f[gi];
+        if (kLabel) vf = f[gi];
         if (kDetect) vg = g[gi];
       }
-      sf[slot][ly][lx] = vf;
-      if (kDetect) sg[slot][ly][lx] = vg;
+      if (kLabel) sv[kF][slot][ly][lx] = vf;
+      if (kDetect) sv[kG][slot][ly][lx] = vg;
     }
   };
 
@@ -225,155 +112,331 @@ __global__ __launch_bounds__(kThreads) void k_dense_pass(
     load_plane(z + 1, (k + 2) % 3);
     __syncthreads();
     const int slot[3] = {k % 3, (k + 1) % 3, (k + 2) % 3};
-    const bool active = x < d.nx && y < d.ny;
-    int tr[4];
-    int nt = 0;
-    int64_t i = 0;
-    if (active) {
-      i = x + d.nx * y + d.sxy * z;
-      T cf[kCand];
-#pragma unroll
-      for (int r = 0; r < kCand; r++)
-        cf[r] = sf[slot[rank_dz(r) + 1]][ty + 1 + rank_dy(r)][tx + 1 + rank_dx(r)];
+    if (x < d.nx && y < d.ny) {
+      const int64_t i = x + d.nx * y + d.sxy * z;
       const bool interior =
           x > 0 && x < d.nx - 1 && y > 0 && y < d.ny - 1 && z > 0 && z < d.nz - 1;
       const uint32_t vm = interior ? 0x7fffu : valid_mask(x, y, z, d);
       int oa, od;
-      if (interior) scan_full(cf, oa, od);
-      else scan_masked(cf, vm, oa, od);
-      code[i] = (uint8_t)(oa | (od << 4));
-      T fv = cf[kSelf];
-      if (!(fv == fv) || fv == sentinel<T>()) lerr |= 1u;
+      if (kLabel) {
+        T cf[kCand];
+#pragma unroll
+        for (int r = 0; r < kCand; r++)
+          cf[r] = sv[kF][slot[rank_dz(r) + 1]][ty + 1 + rank_dy(r)][tx + 1 + rank_dx(r)];
+        if (interior) scan_full(cf, oa, od);
+        else scan_masked(cf, vm, oa, od);
+        code[i] = (uint8_t)(oa | (od << 4));
+        T fv = cf[kSelf];
+        if (!(fv == fv) || fv == sentinel<T>()) lerr |= 1u;
+        if (kDetect) dc[i] = 0;
+      } else {
+        uint8_t c = code[i];
+        oa = c & 15;
+        od = c >> 4;
+      }
       if (kDetect) {
-        dc[i] = 0;
         T cg[kCand];
 #pragma unroll
         for (int r = 0; r < kCand; r++)
-          cg[r] = sg[slot[rank_dz(r) + 1]][ty + 1 + rank_dy(r)][tx + 1 + rank_dx(r)];
+          cg[r] = sv[kG][slot[rank_dz(r) + 1]][ty + 1 + rank_dy(r)][tx + 1 + rank_dx(r)];
         T gv = cg[kSelf];
-        if (!(gv == gv) || gv == sentinel<T>()) lerr |= 1u;
+        if (kLabel && (!(gv == gv) || gv == sentinel<T>())) lerr |= 1u;
         int da, dd;
         if (interior) scan_full(cg, da, dd);
         else scan_masked(cg, vm, da, dd);
-        nt = detect_targets(oa, od, da, dd, tr, lc);
-      }
-    }
-    if (kDetect) {
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        bool has = j < nt;
-        uint32_t tv = 0;
-        if (has) {
-          tv = (uint32_t)(i + rank_off(tr[j], d));
-          has = claim(tmark, tv);
+        int tr[4];
+        int nt = detect_targets(oa, od, da, dd, tr, lc);
+        for (int q = 0; q < nt; q++) {
+          int r = tr[q];
+          int lx = tx + 1 + rank_dx(r), ly = ty + 1 + rank_dy(r), lz = k + 1 + rank_dz(r);
+          atomicOr(&sbits[lz * kPY + ly][lx >> 5], 1u << (lx & 31));
         }
-        wave_append(has, tv, &ctr->ntargets, tlist);
       }
     }
     __syncthreads();
   }
-  if (kDetect) {
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      uint32_t s = wave_sum(lc[c]);
-      if (__lane_id() == 0 && s) atomicAdd(&ctr->cnt[c], s);
+  if (kLabel && lerr) atomicOr(&run->err, 1u);
+  if (!kDetect) return;
+  block_add3<kThreads>(lc, out->cnt);
+  // flush: (row, word) items, 60 rows x 4 global words
+  uint32_t claimed = 0;
+  int64_t wbase = 0;
+  if (tid < kRows * 4) {
+    const int row = tid >> 2, kw = tid & 3;
+    const int ly = row % kPY, lz = row / kPY;
+    const int64_t gy = by + ly - 1, gz = bz + lz - 1;
+    if (gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz) {
+      const int64_t i0 = (bx - 1) + d.nx * gy + d.sxy * gz;  // cell lx = 0 (may be -1)
+      const int64_t w0 = i0 >> 5;                              // floor
+      const int s = (int)(i0 - (w0 << 5));
+      uint32_t m = bits32_at(sbits[row][0], sbits[row][1], sbits[row][2], 32 * kw - s);
+      if (m) {
+        wbase = w0 + kw;
+        uint32_t old = atomicOr(&tmark[wbase], m);
+        claimed = m & ~old;
+      }
     }
   }
-  uint64_t em = __ballot(lerr != 0);
-  if (em && __lane_id() == 0) atomicOr(&ctr->err, 1u);
+  uint32_t pos = block_reserve<kThreads>(__popc(claimed), &out->ntargets);
+  while (claimed) {
+    int b = __ffs(claimed) - 1;
+    claimed &= claimed - 1;
+    tlist[pos++] = (uint32_t)(wbase * 32 + b);
+  }
+  if (tid == 0 && !kLabel) atomicAdd(&run->dense_passes, 1ull);
 }
 
-// Apply one edit to every target of the sweep, then queue the closed stencil
-// neighbourhood of each target for the next sweep.
+// ---------------------------------------------------------------------------
+// Shared pieces of the apply phase.
+
+// Decision of the apply of sweep j (identical in every thread): the mode of
+// detection j+1, or kModeTail, or kModeDone with *done = 1 (converged) / 2 (cap).
+__device__ __forceinline__ uint32_t decide(const Counters &cj, uint32_t j, const Sched &s,
+                                           bool tail_allowed, uint32_t *done) {
+  bool zero = cj.cnt[0] == 0 && cj.cnt[1] == 0 && cj.cnt[2] == 0;
+  *done = 0;
+  if (zero) {
+    *done = 1;
+    return kModeDone;
+  }
+  if (s.mode == PMSZ_SYNC && j > s.max_sweeps) {
+    *done = 2;
+    return kModeDone;
+  }
+  if (tail_allowed && cj.ntargets <= kTailTargets) return kModeTail;
+  if (s.mode == PMSZ_LAZY && j >= s.max_sweeps) {
+    *done = 2;  // the edits of sweep j still land (they belong to sweep j)
+    return kModeDone;
+  }
+  return (uint64_t)cj.ntargets * kCand >= s.dense_min_work ? kModeDense : kModeSparse;
+}
+
+// One edit (applyDeltaBuffer_local) on target t, counting first edits / clamps.
 template <typename T>
-__global__ __launch_bounds__(256) void k_apply(T *__restrict__ g, const T *__restrict__ f,
-                                               uint8_t *__restrict__ dc,
-                                               uint32_t *__restrict__ tmark,
-                                               const uint32_t *__restrict__ tlist,
-                                               uint32_t ntargets, uint32_t *__restrict__ wmark,
-                                               uint32_t *__restrict__ wlist,
-                                               Counters *__restrict__ ctr, Grid3 d,
-                                               EditParams ep) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool act = i < ntargets;
-  uint32_t t = 0;
-  int64_t x = 0, y = 0, z = 0;
-  bool first = false, clamped = false;
-  if (act) {
-    t = tlist[i];
-    T gv = g[t];
-    uint8_t c0 = dc[t], c = c0;
-    edit_value<T>(f[t], gv, c, ep);
-    g[t] = gv;
-    dc[t] = c;
-    first = c0 == 0;
-    clamped = c == ep.q && c0 != ep.q;
-    atomicAnd(&tmark[t >> 5], ~(1u << (t & 31u)));
-    x = (int64_t)t % d.nx;
-    y = ((int64_t)t / d.nx) % d.ny;
-    z = (int64_t)t / d.sxy;
-  }
-  uint64_t mf = __ballot(first), mc = __ballot(clamped);
-  if (__lane_id() == 0) {
-    if (mf) atomicAdd(&ctr->touched, (uint32_t)__popcll(mf));
-    if (mc) atomicAdd(&ctr->clamped, (uint32_t)__popcll(mc));
-  }
+__device__ __forceinline__ void apply_one(T *g, const T *f, uint8_t *dc, uint32_t *tmark,
+                                          uint32_t t, const EditParams &ep, uint32_t &touched,
+                                          uint32_t &clamped) {
+  T gv = g[t];
+  uint8_t c0 = dc[t], c = c0;
+  edit_value<T>(f[t], gv, c, ep);
+  g[t] = gv;
+  dc[t] = c;
+  touched += c0 == 0;
+  clamped += (c == ep.q && c0 != ep.q);
+  atomicAnd(&tmark[t >> 5], ~(1u << (t & 31u)));
+}
+
+// Claims the closed stencil neighbourhood of t in wmark; returns the new voxels.
+__device__ __forceinline__ int queue_neighbourhood(uint32_t t, uint32_t *wmark, const Grid3 &d,
+                                                   uint32_t (&u)[kCand]) {
+  int64_t x = (int64_t)t % d.nx, y = ((int64_t)t / d.nx) % d.ny, z = (int64_t)t / d.sxy;
+  int n = 0;
 #pragma unroll
   for (int r = 0; r < kCand; r++) {
     int64_t a = x + rank_dx(r), b = y + rank_dy(r), c = z + rank_dz(r);
-    bool has = act && a >= 0 && a < d.nx && b >= 0 && b < d.ny && c >= 0 && c < d.nz;
-    uint32_t u = 0;
-    if (has) {
-      u = (uint32_t)((int64_t)t + rank_off(r, d));
-      has = claim(wmark, u);
+    if (a >= 0 && a < d.nx && b >= 0 && b < d.ny && c >= 0 && c < d.nz) {
+      uint32_t v = (uint32_t)((int64_t)t + rank_off(r, d));
+      if (claim(wmark, v)) u[n++] = v;
     }
-    wave_append(has, u, &ctr->nwork, wlist);
   }
+  return n;
 }
 
-// A detection sweep over the work list queued by the previous apply.
+// Detection at one listed voxel (global loads); returns the claimed targets.
 template <typename T>
-__global__ __launch_bounds__(256) void k_detect_sparse(
+__device__ __forceinline__ int detect_one(const T *g, const uint8_t *code, uint32_t *tmark,
+                                          uint32_t v, const Grid3 &d, uint32_t (&lc)[3],
+                                          uint32_t (&out)[4]) {
+  int64_t x = (int64_t)v % d.nx, y = ((int64_t)v / d.nx) % d.ny, z = (int64_t)v / d.sxy;
+  uint32_t vm = valid_mask(x, y, z, d);
+  T cg[kCand];
+#pragma unroll
+  for (int r = 0; r < kCand; r++)
+    cg[r] = ((vm >> r) & 1u) ? g[(int64_t)v + rank_off(r, d)] : (T)0;
+  int da, dd;
+  if (vm == 0x7fffu) scan_full(cg, da, dd);
+  else scan_masked(cg, vm, da, dd);
+  uint8_t oc = code[v];
+  int tr[4];
+  int nt = detect_targets(oc & 15, oc >> 4, da, dd, tr, lc);
+  int n = 0;
+  for (int q = 0; q < nt; q++) {
+    uint32_t t = (uint32_t)((int64_t)v + rank_off(tr[q], d));
+    if (claim(tmark, t)) out[n++] = t;
+  }
+  return n;
+}
+
+__device__ __forceinline__ void zero_counters(Counters *c) {
+  uint32_t *w = reinterpret_cast<uint32_t *>(c);
+  for (int k = 0; k < (int)(sizeof(Counters) / 4); k++) w[k] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Apply of sweep j (grid-stride over its targets), queueing the next work list
+// when detection j+1 is sparse.
+template <typename T>
+__global__ __launch_bounds__(kApplyThreads) void k_apply(
+    T *__restrict__ g, const T *__restrict__ f, uint8_t *__restrict__ dc,
+    uint32_t *__restrict__ tmark, const uint32_t *__restrict__ tlist,
+    uint32_t *__restrict__ wmark, uint32_t *__restrict__ wlist, Counters *__restrict__ ring,
+    RunState *run, uint32_t j, Grid3 d, EditParams ep, Sched s) {
+  Counters &prev = ring[(j - 1) % kCtrRing];
+  Counters &cj = ring[j % kCtrRing];
+  const uint32_t pm = prev.mode;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (pm == kModeTail || pm == kModeDone || pm == kModeUnset) {
+    if (lead) cj.mode = pm == kModeUnset ? kModeDone : pm;
+    return;
+  }
+  uint32_t done;
+  const uint32_t mode = decide(cj, j, s, true, &done);
+  if (lead) {
+    cj.mode = mode;
+    if (mode == kModeTail) run->tail_j = j;
+    if (done) {
+      run->done = done;
+      run->last_j = j;
+    }
+  }
+  if (mode == kModeTail) return;
+  if (mode == kModeDone && (done == 1 || s.mode == PMSZ_SYNC)) return;
+  if (lead) zero_counters(&ring[(j + 1) % kCtrRing]);
+  const uint32_t nt = cj.ntargets;
+  const bool queue = mode == kModeSparse;
+  uint32_t touched = 0, clamped = 0;
+  const uint32_t stride = gridDim.x * kApplyThreads;
+  for (uint32_t base = blockIdx.x * kApplyThreads; base < nt; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    uint32_t u[kCand];
+    int n = 0;
+    if (i < nt) {
+      const uint32_t t = tlist[i];
+      apply_one<T>(g, f, dc, tmark, t, ep, touched, clamped);
+      if (queue) n = queue_neighbourhood(t, wmark, d, u);
+    }
+    if (queue) {
+      uint32_t pos = block_reserve<kApplyThreads>((uint32_t)n, &cj.nwork);
+      for (int q = 0; q < n; q++) wlist[pos + q] = u[q];
+    }
+  }
+  uint32_t ts = wave_sum(touched), cs = wave_sum(clamped);
+  if (lane_id() == 0) {
+    if (ts) atomicAdd(&run->touched, (unsigned long long)ts);
+    if (cs) atomicAdd(&run->clamped, (unsigned long long)cs);
+  }
+  if (lead) atomicAdd(&run->total_targets, (unsigned long long)nt);
+}
+
+// Sparse detection j+1 over the work list queued by apply j (grid-stride).
+template <typename T>
+__global__ __launch_bounds__(kApplyThreads) void k_detect_list(
     const T *__restrict__ g, const uint8_t *__restrict__ code, uint32_t *__restrict__ tmark,
     uint32_t *__restrict__ tlist, uint32_t *__restrict__ wmark,
-    const uint32_t *__restrict__ wlist, const Counters *__restrict__ prev,
-    Counters *__restrict__ ctr, Grid3 d) {
-  const uint32_t nwork = prev->nwork;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool act = i < nwork;
+    const uint32_t *__restrict__ wlist, Counters *__restrict__ ring, RunState *run, uint32_t j,
+    Grid3 d) {
+  const Counters &cj = ring[j % kCtrRing];
+  if (cj.mode != kModeSparse) return;
+  Counters &cn = ring[(j + 1) % kCtrRing];
+  const uint32_t nw = cj.nwork;
   uint32_t lc[3] = {0, 0, 0};
-  int tr[4];
-  int nt = 0;
-  uint32_t v = 0;
-  if (act) {
-    v = wlist[i];
-    atomicAnd(&wmark[v >> 5], ~(1u << (v & 31u)));
-    int64_t x = (int64_t)v % d.nx, y = ((int64_t)v / d.nx) % d.ny, z = (int64_t)v / d.sxy;
-    uint32_t vm = valid_mask(x, y, z, d);
-    T cg[kCand];
-#pragma unroll
-    for (int r = 0; r < kCand; r++)
-      cg[r] = ((vm >> r) & 1u) ? g[(int64_t)v + rank_off(r, d)] : (T)0;
-    int da, dd;
-    if (vm == 0x7fffu) scan_full(cg, da, dd);
-    else scan_masked(cg, vm, da, dd);
-    uint8_t oc = code[v];
-    nt = detect_targets(oc & 15, oc >> 4, da, dd, tr, lc);
-  }
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    bool has = j < nt;
-    uint32_t tv = 0;
-    if (has) {
-      tv = (uint32_t)((int64_t)v + rank_off(tr[j], d));
-      has = claim(tmark, tv);
+  const uint32_t stride = gridDim.x * kApplyThreads;
+  for (uint32_t base = blockIdx.x * kApplyThreads; base < nw; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    uint32_t out[4];
+    int n = 0;
+    if (i < nw) {
+      const uint32_t v = wlist[i];
+      atomicAnd(&wmark[v >> 5], ~(1u << (v & 31u)));
+      n = detect_one<T>(g, code, tmark, v, d, lc, out);
     }
-    wave_append(has, tv, &ctr->ntargets, tlist);
+    uint32_t pos = block_reserve<kApplyThreads>((uint32_t)n, &cn.ntargets);
+    for (int q = 0; q < n; q++) tlist[pos + q] = out[q];
   }
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    uint32_t s = wave_sum(lc[c]);
-    if (__lane_id() == 0 && s) atomicAdd(&ctr->cnt[c], s);
+  block_add3<kApplyThreads>(lc, cn.cnt);
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    atomicAdd(&run->sparse_voxels, (unsigned long long)nw);
+}
+
+// Single-workgroup tail: runs apply / detect sweeps to the end once the target
+// sets are small, with workgroup barriers between the phases.
+template <typename T>
+__global__ __launch_bounds__(kTailThreads) void k_tail(
+    T *__restrict__ g, const T *__restrict__ f, uint8_t *__restrict__ dc,
+    const uint8_t *__restrict__ code, uint32_t *__restrict__ tmark, uint32_t *__restrict__ tlist,
+    uint32_t *__restrict__ wmark, uint32_t *__restrict__ wlist, Counters *__restrict__ ring,
+    RunState *run, Grid3 d, EditParams ep, Sched s) {
+  __shared__ uint32_t s_ctl[2];
+  uint32_t j = run->tail_j;
+  if (j == 0 || run->done) return;
+  const bool lead = threadIdx.x == 0;
+  unsigned long long tt = 0, sv = 0;
+  uint32_t touched = 0, clamped = 0;
+  while (true) {
+    Counters &cj = ring[j % kCtrRing];
+    if (lead) {
+      uint32_t done;
+      const uint32_t mode = decide(cj, j, s, false, &done);
+      s_ctl[0] = mode;
+      s_ctl[1] = done;
+      cj.mode = mode == kModeDone ? kModeDone : kModeTail;
+      if (done) {
+        run->done = done;
+        run->last_j = j;
+      }
+    }
+    __syncthreads();
+    const uint32_t mode = s_ctl[0], done = s_ctl[1];
+    if (done == 1 || (done == 2 && s.mode == PMSZ_SYNC)) break;
+    if (lead) zero_counters(&ring[(j + 1) % kCtrRing]);
+    // apply sweep j, queueing the next work list unless this was the last sweep
+    const uint32_t nt = cj.ntargets;
+    const bool queue = mode != kModeDone;
+    for (uint32_t base = 0; base < nt; base += kTailThreads) {
+      const uint32_t i = base + threadIdx.x;
+      uint32_t u[kCand];
+      int n = 0;
+      if (i < nt) {
+        const uint32_t t = tlist[i];
+        apply_one<T>(g, f, dc, tmark, t, ep, touched, clamped);
+        if (queue) n = queue_neighbourhood(t, wmark, d, u);
+      }
+      if (queue) {
+        uint32_t pos = block_reserve<kTailThreads>((uint32_t)n, &cj.nwork);
+        for (int q = 0; q < n; q++) wlist[pos + q] = u[q];
+      }
+    }
+    tt += nt;
+    __syncthreads();
+    if (!queue) break;  // lazy sweep cap: sweep j's edits landed, no detection j+1
+    // detection j+1 over the work list
+    Counters &cn = ring[(j + 1) % kCtrRing];
+    const uint32_t nw = cj.nwork;
+    uint32_t lc[3] = {0, 0, 0};
+    for (uint32_t base = 0; base < nw; base += kTailThreads) {
+      const uint32_t i = base + threadIdx.x;
+      uint32_t out[4];
+      int n = 0;
+      if (i < nw) {
+        const uint32_t v = wlist[i];
+        atomicAnd(&wmark[v >> 5], ~(1u << (v & 31u)));
+        n = detect_one<T>(g, code, tmark, v, d, lc, out);
+      }
+      uint32_t pos = block_reserve<kTailThreads>((uint32_t)n, &cn.ntargets);
+      for (int q = 0; q < n; q++) tlist[pos + q] = out[q];
+    }
+    block_add3<kTailThreads>(lc, cn.cnt);
+    sv += nw;
+    j++;
+  }
+  uint32_t ts = wave_sum(touched), cs = wave_sum(clamped);
+  if (lane_id() == 0) {
+    if (ts) atomicAdd(&run->touched, (unsigned long long)ts);
+    if (cs) atomicAdd(&run->clamped, (unsigned long long)cs);
+  }
+  if (lead) {
+    atomicAdd(&run->total_targets, tt);
+    atomicAdd(&run->sparse_voxels, sv);
   }
 }
 
@@ -434,31 +497,34 @@ __global__ void k_edit_symbols(const T *__restrict__ f, const T *__restrict__ g0
 
 struct Ctx {
   int device = -1;
+  int cus = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   int64_t cap = 0;  // voxels
   uint8_t *code = nullptr, *dc = nullptr;
   uint32_t *tmark = nullptr, *wmark = nullptr, *tlist = nullptr, *wlist = nullptr;
-  Counters *ctr = nullptr;
-  Counters *h_ctr = nullptr;  // pinned mirror of two consecutive Counters blocks
+  Counters *ring = nullptr;
+  RunState *run = nullptr;
+  Counters *h_ring = nullptr;  // pinned mirror of the ring
+  RunState *h_run = nullptr;   // pinned mirror of the run state
 };
 
 std::mutex g_mu;
 Ctx g_ctx;
 
-#define PMSZ_HIP(call)                  \
-  do {                                  \
-    hipError_t e_ = (call);             \
-    if (e_ != hipSuccess) {             \
-      last_hip_error() = e_;            \
-      return PMSZ_EDEVICE;              \
-    }                                   \
-  } while (0)
-
 hipError_t &last_hip_error() {
   static hipError_t e = hipSuccess;
   return e;
 }
+
+#define PMSZ_HIP(call)       \
+  do {                       \
+    hipError_t e_ = (call);  \
+    if (e_ != hipSuccess) {  \
+      last_hip_error() = e_; \
+      return PMSZ_EDEVICE;   \
+    }                        \
+  } while (0)
 
 void release(Ctx &c) {
   (void)hipFree(c.code);
@@ -479,12 +545,16 @@ int ensure(Ctx &c, int64_t n) {
     // a new device: the state of the previous one is left to process exit
     c = Ctx();
     c.device = dev;
+    PMSZ_HIP(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, dev));
     PMSZ_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     PMSZ_HIP(hipEventCreate(&c.ev0));
     PMSZ_HIP(hipEventCreate(&c.ev1));
     PMSZ_HIP(hipEventCreate(&c.ev2));
-    PMSZ_HIP(hipMalloc(&c.ctr, sizeof(Counters) * kCtrCap));
-    PMSZ_HIP(hipHostMalloc(&c.h_ctr, 2 * sizeof(Counters), hipHostMallocDefault));
+    PMSZ_HIP(hipEventCreate(&c.ev3));
+    PMSZ_HIP(hipMalloc(&c.ring, sizeof(Counters) * kCtrRing));
+    PMSZ_HIP(hipMalloc(&c.run, sizeof(RunState)));
+    PMSZ_HIP(hipHostMalloc(&c.h_ring, sizeof(Counters) * kCtrRing, hipHostMallocDefault));
+    PMSZ_HIP(hipHostMalloc(&c.h_run, sizeof(RunState), hipHostMallocDefault));
   }
   if (n <= c.cap) return PMSZ_OK;
   release(c);
@@ -529,18 +599,26 @@ Grid3 grid_of(int64_t nx, int64_t ny, int64_t nz) {
   return d;
 }
 
-dim3 dense_grid(const Grid3 &d) {
+dim3 tile_grid(const Grid3 &d) {
   return dim3((unsigned)((d.nx + kTX - 1) / kTX), (unsigned)((d.ny + kTY - 1) / kTY),
               (unsigned)((d.nz + kTZ - 1) / kTZ));
 }
 
-// Copies counter blocks k (and k+1 when two = true) into the pinned mirror.
-int read_counters(Ctx &c, int k, bool two = false) {
-  PMSZ_HIP(hipMemcpyAsync(c.h_ctr, c.ctr + k, (two ? 2 : 1) * sizeof(Counters),
-                          hipMemcpyDeviceToHost, c.stream));
-  PMSZ_HIP(hipStreamSynchronize(c.stream));
-  return PMSZ_OK;
+int64_t tile_count(const Grid3 &d) {
+  dim3 t = tile_grid(d);
+  return (int64_t)t.x * t.y * t.z;
 }
+
+EditParams edit_params(const pmsz_params *p) {
+  EditParams ep;
+  ep.eb = p->abs_eb;
+  ep.delta_d = p->abs_eb / p->q;
+  ep.q = p->q;
+  ep.clamp_mode = p->clamp_mode;
+  return ep;
+}
+
+constexpr uint32_t kChain = 8;  // apply/detect steps enqueued before each host check
 
 template <typename T>
 int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_out,
@@ -548,128 +626,125 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
   const Grid3 d = grid_of(p->nx, p->ny, p->nz);
   int rc = ensure(c, d.n);
   if (rc) return rc;
-  EditParams ep;
-  ep.eb = p->abs_eb;
-  ep.delta_d = p->abs_eb / p->q;
-  ep.q = p->q;
-  ep.clamp_mode = p->clamp_mode;
+  const EditParams ep = edit_params(p);
+  Sched s;
+  s.mode = p->mode;
+  s.max_sweeps = (uint32_t)p->max_sweeps;
+  s.dense_min_work = (uint32_t)std::min<int64_t>(std::max<int64_t>(d.n / 16, 1), 0xffffffffLL);
   const int64_t words = (d.n + 31) / 32 + 1;
+  const unsigned grid_list = (unsigned)std::max(1, c.cus * 4);
   std::memset(st, 0, sizeof(*st));
+
   PMSZ_HIP(hipEventRecord(c.ev0, c.stream));
   PMSZ_HIP(hipMemsetAsync(c.tmark, 0, sizeof(uint32_t) * words, c.stream));
   PMSZ_HIP(hipMemsetAsync(c.wmark, 0, sizeof(uint32_t) * words, c.stream));
-  PMSZ_HIP(hipMemsetAsync(c.ctr, 0, sizeof(Counters) * kCtrCap, c.stream));
+  PMSZ_HIP(hipMemsetAsync(c.ring, 0, sizeof(Counters) * kCtrRing, c.stream));
+  PMSZ_HIP(hipMemsetAsync(c.run, 0, sizeof(RunState), c.stream));
+  // apply(1) reads slot 0's mode: mark detection 1 as a running sweep
+  static const uint32_t kRunning = kModeDense;
+  PMSZ_HIP(hipMemcpyAsync(&c.ring[0].mode, &kRunning, sizeof(uint32_t), hipMemcpyHostToDevice,
+                          c.stream));
   PMSZ_HIP(hipEventRecord(c.ev2, c.stream));
-  k_dense_pass<T, true><<<dense_grid(d), kThreads, 0, c.stream>>>(f, g, c.code, c.dc, c.tmark,
-                                                                    c.tlist, c.ctr, d);
+  k_tiles<T, true, true><<<tile_grid(d), kThreads, 0, c.stream>>>(
+      f, g, c.code, c.dc, c.tmark, c.tlist, c.ring, 0, 1, c.run, d);
   PMSZ_HIP(hipGetLastError());
-  PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
-  st->dense_passes = 1;
-  double bytes = (double)d.n * (2.0 * sizeof(T) + 2.0);
-  rc = read_counters(c, 0);
-  if (rc) return rc;
-  if (c.h_ctr->err) return PMSZ_EINVAL;
-  {
-    float dms = 0;
-    PMSZ_HIP(hipEventElapsedTime(&dms, c.ev2, c.ev1));
-    st->t_dense_s = dms * 1e-3;
-  }
-  int k = 0;        // counters slot of the running sweep
-  int64_t j = 1;    // detections done
-  int64_t applies = 0;
-  int status = PMSZ_OK;
-  uint32_t cnt[3] = {c.h_ctr->cnt[0], c.h_ctr->cnt[1], c.h_ctr->cnt[2]};
-  uint32_t nt = c.h_ctr->ntargets;
-  bool pending = false;  // an apply whose counters were not read back yet
+  PMSZ_HIP(hipEventRecord(c.ev3, c.stream));
+
+  uint32_t j0 = 1;
   while (true) {
-    bool zero = cnt[0] == 0 && cnt[1] == 0 && cnt[2] == 0;
-    if (p->mode == PMSZ_LAZY) {
-      if (zero) break;
-      if (j >= p->max_sweeps) {
-        // the sweep's edits still land (they belong to sweep j)
-        status = PMSZ_NOT_CONVERGED;
-      }
-    } else {
-      if (applies >= p->max_sweeps) {
-        status = PMSZ_NOT_CONVERGED;
-        break;
-      }
+    for (uint32_t j = j0; j < j0 + kChain; j++) {
+      k_apply<T><<<grid_list, kApplyThreads, 0, c.stream>>>(g, f, c.dc, c.tmark, c.tlist,
+                                                            c.wmark, c.wlist, c.ring, c.run, j,
+                                                            d, ep, s);
+      k_tiles<T, false, true><<<tile_grid(d), kThreads, 0, c.stream>>>(
+          nullptr, g, c.code, nullptr, c.tmark, c.tlist, c.ring, j % kCtrRing,
+          (j + 1) % kCtrRing, c.run, d);
+      k_detect_list<T><<<grid_list, kApplyThreads, 0, c.stream>>>(
+          g, c.code, c.tmark, c.tlist, c.wmark, c.wlist, c.ring, c.run, j, d);
     }
-    // apply sweep j's targets and queue the next work list
-    if (nt > 0) {
-      unsigned blocks = (nt + 255) / 256;
-      k_apply<T><<<blocks, 256, 0, c.stream>>>(g, f, c.dc, c.tmark, c.tlist, nt, c.wmark,
-                                               c.wlist, c.ctr + k, d, ep);
-      PMSZ_HIP(hipGetLastError());
-      bytes += (double)nt * (3.0 * sizeof(T) + 2.0);
-      st->total_targets += nt;
-    }
-    applies++;
-    pending = nt > 0;
-    if (status != PMSZ_OK) break;
-    if (p->mode == PMSZ_SYNC && zero) break;
-    // next detection over the work list
-    int kn = k + 1;
-    if (kn >= kCtrCap) {
-      PMSZ_HIP(hipMemcpyAsync(c.ctr, c.ctr + k, sizeof(Counters), hipMemcpyDeviceToDevice,
-                              c.stream));
-      PMSZ_HIP(hipMemsetAsync(c.ctr + 1, 0, sizeof(Counters) * (kCtrCap - 1), c.stream));
-      k = 0;
-      kn = 1;
-    }
-    int64_t upper = std::min<int64_t>((int64_t)nt * kCand, d.n);
-    if (upper > 0) {
-      unsigned blocks = (unsigned)((upper + 255) / 256);
-      k_detect_sparse<T><<<blocks, 256, 0, c.stream>>>(g, c.code, c.tmark, c.tlist, c.wmark,
-                                                       c.wlist, c.ctr + k, c.ctr + kn, d);
-      PMSZ_HIP(hipGetLastError());
-    }
-    // slot k holds the apply's work-list size, slot kn the new sweep's counters
-    rc = read_counters(c, k, true);
-    if (rc) return rc;
-    pending = false;
-    st->sparse_voxels += c.h_ctr[0].nwork;
-    st->touched_voxels += c.h_ctr[0].touched;
-    st->clamped_voxels += c.h_ctr[0].clamped;
-    bytes += (double)c.h_ctr[0].nwork * (sizeof(T) + 1.0);
-    k = kn;
-    j++;
-    cnt[0] = c.h_ctr[1].cnt[0];
-    cnt[1] = c.h_ctr[1].cnt[1];
-    cnt[2] = c.h_ctr[1].cnt[2];
-    nt = c.h_ctr[1].ntargets;
-    if (p->mode == PMSZ_SYNC && cnt[0] == 0 && cnt[1] == 0 && cnt[2] == 0) break;
-  }
-  if (pending) {
-    rc = read_counters(c, k);
-    if (rc) return rc;
-    st->touched_voxels += c.h_ctr[0].touched;
-    st->clamped_voxels += c.h_ctr[0].clamped;
+    k_tail<T><<<1, kTailThreads, 0, c.stream>>>(g, f, c.dc, c.code, c.tmark, c.tlist, c.wmark,
+                                                c.wlist, c.ring, c.run, d, ep, s);
+    PMSZ_HIP(hipGetLastError());
+    PMSZ_HIP(hipMemcpyAsync(c.h_run, c.run, sizeof(RunState), hipMemcpyDeviceToHost, c.stream));
+    PMSZ_HIP(hipStreamSynchronize(c.stream));
+    if (c.h_run->err) return PMSZ_EINVAL;
+    if (c.h_run->done) break;
+    // the chain ended before the tail took over: continue it where it stopped
+    j0 += kChain;
   }
   PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
+  PMSZ_HIP(hipMemcpyAsync(c.h_ring, c.ring, sizeof(Counters) * kCtrRing, hipMemcpyDeviceToHost,
+                          c.stream));
   if (dc_out)
     PMSZ_HIP(hipMemcpyAsync(dc_out, c.dc, (size_t)d.n, hipMemcpyDeviceToDevice, c.stream));
   PMSZ_HIP(hipStreamSynchronize(c.stream));
-  float ms = 0;
+  float ms = 0, dms = 0;
   PMSZ_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  PMSZ_HIP(hipEventElapsedTime(&dms, c.ev2, c.ev3));
   st->t_total_s = ms * 1e-3;
+  st->t_dense_s = dms * 1e-3;
+
+  const RunState &r = *c.h_run;
+  const uint32_t m = r.last_j;
+  const Counters &cm = c.h_ring[m % kCtrRing];
+  const bool conv = r.done == 1;
+  const int status = conv ? PMSZ_OK : PMSZ_NOT_CONVERGED;
   st->status = status;
-  st->final_cnt[0] = cnt[0];
-  st->final_cnt[1] = cnt[1];
-  st->final_cnt[2] = cnt[2];
+  st->final_cnt[0] = cm.cnt[0];
+  st->final_cnt[1] = cm.cnt[1];
+  st->final_cnt[2] = cm.cnt[2];
+  st->total_targets = (int64_t)r.total_targets;
+  st->touched_voxels = (int64_t)r.touched;
+  st->clamped_voxels = (int64_t)r.clamped;
+  st->sparse_voxels = (int64_t)r.sparse_voxels;
+  const int64_t extra_dense = (int64_t)r.dense_passes / std::max<int64_t>(1, tile_count(d));
+  st->dense_passes = 1 + extra_dense;
   if (p->mode == PMSZ_LAZY) {
-    st->edit_sweeps = j;
-    st->check_sweeps = status == PMSZ_OK ? 1 : 0;
-    st->outer_iters = status == PMSZ_OK ? 1 : 0;
-    st->ite = j + 1;
+    st->edit_sweeps = m;
+    st->check_sweeps = conv ? 1 : 0;
+    st->outer_iters = conv ? 1 : 0;
+    st->ite = m + 1;
   } else {
+    int64_t applies = conv ? std::max<int64_t>((int64_t)m - 1, 1) : (int64_t)m - 1;
     st->edit_sweeps = applies;
     st->check_sweeps = applies;
     st->outer_iters = applies;
     st->ite = 2 * applies;
   }
-  st->bytes_algorithmic = bytes;
+  // first pass: f, g read, code, dc written; later dense sweeps: g, code read;
+  // edits: f, g, dc read and g, dc written; listed voxels: g and code read
+  st->bytes_algorithmic = (double)d.n * (2.0 * sizeof(T) + 2.0) +
+                          (double)d.n * (sizeof(T) + 1.0) * (double)extra_dense +
+                          (double)st->total_targets * (3.0 * sizeof(T) + 2.0) +
+                          (double)st->sparse_voxels * (sizeof(T) + 1.0);
   return status;
+}
+
+// Timing of the fused first pass alone (events around each launch), for the
+// roofline report.
+template <typename T>
+int time_first_pass(Ctx &c, const pmsz_params *p, const T *f, const T *g, int iters,
+                    double *avg_s) {
+  const Grid3 d = grid_of(p->nx, p->ny, p->nz);
+  int rc = ensure(c, d.n);
+  if (rc) return rc;
+  const int64_t words = (d.n + 31) / 32 + 1;
+  double tot = 0;
+  for (int it = 0; it < iters; it++) {
+    PMSZ_HIP(hipMemsetAsync(c.tmark, 0, sizeof(uint32_t) * words, c.stream));
+    PMSZ_HIP(hipMemsetAsync(c.ring, 0, sizeof(Counters) * 2, c.stream));
+    PMSZ_HIP(hipEventRecord(c.ev0, c.stream));
+    k_tiles<T, true, true><<<tile_grid(d), kThreads, 0, c.stream>>>(
+        f, g, c.code, c.dc, c.tmark, c.tlist, c.ring, 0, 1, c.run, d);
+    PMSZ_HIP(hipGetLastError());
+    PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
+    PMSZ_HIP(hipStreamSynchronize(c.stream));
+    float ms = 0;
+    PMSZ_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    tot += ms * 1e-3;
+  }
+  *avg_s = tot / std::max(1, iters);
+  return PMSZ_OK;
 }
 
 }  // namespace pmsz
@@ -701,6 +776,19 @@ int pmsz_correct(const pmsz_params *p, const void *d_f, void *d_g, uint8_t *d_dc
   if (p->dtype == PMSZ_F32)
     return correct_single<float>(g_ctx, p, (const float *)d_f, (float *)d_g, d_dc, stats);
   return correct_single<double>(g_ctx, p, (const double *)d_f, (double *)d_g, d_dc, stats);
+}
+
+int pmsz_time_first_pass(const pmsz_params *p, const void *d_f, const void *d_g, int iters,
+                         double *avg_seconds) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (!d_f || !d_g || !avg_seconds || iters < 1) return PMSZ_EINVAL;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (p->dtype == PMSZ_F32)
+    return time_first_pass<float>(g_ctx, p, (const float *)d_f, (const float *)d_g, iters,
+                                  avg_seconds);
+  return time_first_pass<double>(g_ctx, p, (const double *)d_f, (const double *)d_g, iters,
+                                 avg_seconds);
 }
 
 int pmsz_correct_host(const pmsz_params *p, const void *h_f, void *h_g, uint8_t *h_dc,
@@ -750,17 +838,19 @@ int pmsz_codes(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_fiel
   const Grid3 d = grid_of(nx, ny, nz);
   rc = ensure(c, 1);
   if (rc) return rc;
-  PMSZ_HIP(hipMemsetAsync(c.ctr, 0, sizeof(Counters), c.stream));
+  PMSZ_HIP(hipMemsetAsync(c.run, 0, sizeof(RunState), c.stream));
   if (dtype == PMSZ_F32)
-    k_dense_pass<float, false><<<dense_grid(d), kThreads, 0, c.stream>>>(
-        (const float *)d_field, nullptr, d_codes, nullptr, nullptr, nullptr, c.ctr, d);
+    k_tiles<float, true, false><<<tile_grid(d), kThreads, 0, c.stream>>>(
+        (const float *)d_field, nullptr, d_codes, nullptr, nullptr, nullptr, c.ring, 0, 0, c.run,
+        d);
   else
-    k_dense_pass<double, false><<<dense_grid(d), kThreads, 0, c.stream>>>(
-        (const double *)d_field, nullptr, d_codes, nullptr, nullptr, nullptr, c.ctr, d);
+    k_tiles<double, true, false><<<tile_grid(d), kThreads, 0, c.stream>>>(
+        (const double *)d_field, nullptr, d_codes, nullptr, nullptr, nullptr, c.ring, 0, 0,
+        c.run, d);
   PMSZ_HIP(hipGetLastError());
-  rc = read_counters(c, 0);
-  if (rc) return rc;
-  return c.h_ctr->err ? PMSZ_EINVAL : PMSZ_OK;
+  PMSZ_HIP(hipMemcpyAsync(c.h_run, c.run, sizeof(RunState), hipMemcpyDeviceToHost, c.stream));
+  PMSZ_HIP(hipStreamSynchronize(c.stream));
+  return c.h_run->err ? PMSZ_EINVAL : PMSZ_OK;
 }
 
 int pmsz_labels(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_field,
@@ -803,12 +893,8 @@ int pmsz_edit_symbols(const pmsz_params *p, const void *d_f, const void *d_g0, c
   rc = ensure(c, 1);
   if (rc) return rc;
   int64_t n = p->nx * p->ny * p->nz;
-  EditParams ep;
-  ep.eb = p->abs_eb;
-  ep.delta_d = p->abs_eb / p->q;
-  ep.q = p->q;
-  ep.clamp_mode = p->clamp_mode;
-  unsigned long long *dcount = (unsigned long long *)c.ctr;
+  const EditParams ep = edit_params(p);
+  unsigned long long *dcount = &c.run->total_targets;
   PMSZ_HIP(hipMemsetAsync(dcount, 0, sizeof(unsigned long long), c.stream));
   unsigned blocks = (unsigned)((n + 255) / 256);
   if (p->dtype == PMSZ_F32)
@@ -840,6 +926,6 @@ const char *pmsz_strerror(int code) {
   }
 }
 
-const char *pmsz_version(void) { return "pmsz-mi355x 0.1"; }
+const char *pmsz_version(void) { return "pmsz-mi355x 0.2"; }
 
 }  // extern "C"

@@ -27,7 +27,7 @@ OK, EINVAL, ENOMEM, NOT_CONVERGED, EDEVICE, ETOOBIG, EBUSY = 0, -1, -2, -3, -4, 
 
 EXPORTED = [
     "pmsz_default_params", "pmsz_correct", "pmsz_correct_host", "pmsz_codes", "pmsz_labels",
-    "pmsz_edit_symbols", "pmsz_strerror", "pmsz_version",
+    "pmsz_edit_symbols", "pmsz_time_first_pass", "pmsz_strerror", "pmsz_version",
 ]
 
 
@@ -102,6 +102,7 @@ def lib():
         L.pmsz_codes.argtypes = [i32, i64, i64, i64, vp, vp]
         L.pmsz_labels.argtypes = [i32, i64, i64, i64, vp, vp, vp]
         L.pmsz_edit_symbols.argtypes = [P, vp, vp, vp, vp, ctypes.POINTER(i64)]
+        L.pmsz_time_first_pass.argtypes = [P, vp, vp, i32, ctypes.POINTER(ctypes.c_double)]
         L.pmsz_strerror.argtypes = [i32]
         L.pmsz_strerror.restype = ctypes.c_char_p
         L.pmsz_version.argtypes = []
@@ -229,6 +230,18 @@ def edit_symbols(f, g0, g, abs_eb, **kw):
                                  ctypes.c_void_p(sym.data_ptr()), ctypes.byref(ed))
     _check(rc, "pmsz_edit_symbols")
     return sym, ed.value
+
+
+def time_first_pass(f, g, abs_eb, iters=5, **kw):
+    """Average seconds of the fused first pass on CUDA tensors f, g (g is not modified)."""
+    import torch
+    p = make_params(f.shape, f.dtype, abs_eb, **kw)
+    out = ctypes.c_double(0)
+    torch.cuda.current_stream().synchronize()
+    rc = lib().pmsz_time_first_pass(ctypes.byref(p), ctypes.c_void_p(f.data_ptr()),
+                                    ctypes.c_void_p(g.data_ptr()), int(iters), ctypes.byref(out))
+    _check(rc, "pmsz_time_first_pass")
+    return out.value
 
 
 def version():
