@@ -74,6 +74,9 @@ typedef struct {
   double t_total_s;        /* device time of the correction (events on the library stream) */
   double t_dense_s;        /* device time of the full first pass (labeling + first sweep) */
   double bytes_algorithmic;/* algorithmic HBM bytes of the passes actually run */
+  int64_t trace_n;         /* sweeps recorded below (the first min(64, sweeps)) */
+  uint32_t trace_cnt[64][3];   /* per detection: false max, false min, direction counters */
+  uint32_t trace_targets[64];  /* per detection: distinct targets */
 } pmsz_stats;
 
 /* Parameter defaults: q = 5, sub = 1,1,1, lazy, max_sweeps = 100000,

@@ -41,12 +41,16 @@
 
 namespace pmsz {
 
-constexpr int kTX = 64;  // dense tile: 64 x 4 voxels per plane, 8 planes
-constexpr int kTY = 4;
-constexpr int kTZ = 8;
-constexpr int kThreads = kTX * kTY;
+constexpr int kTX = 64;  // dense tile: 64 x 8 voxels per plane, 16 planes
+constexpr int kTY = 8;
+constexpr int kTZ = 16;
+constexpr int kThreads = 256;  // each thread classifies 2 rows of the tile
 constexpr int kPX = kTX + 2, kPY = kTY + 2, kPZ = kTZ + 2;  // tile + halo
+constexpr int kPlane = kPX * kPY;                             // cells of one staged plane
+constexpr int kEPT = (kPlane + kThreads - 1) / kThreads;      // staged cells per thread
+constexpr int kSlots = 4;                                     // LDS plane ring
 constexpr int kRows = kPY * kPZ;                              // bitmap rows of 66 cells
+constexpr int kFlushPT = (kRows * 4 + kThreads - 1) / kThreads;
 constexpr int kApplyThreads = 256;
 constexpr int kTailThreads = 1024;
 constexpr uint32_t kTailTargets = 2048;  // the tail kernel takes over at or below this
@@ -60,11 +64,17 @@ struct Sched {
 // ---------------------------------------------------------------------------
 // Tile kernel: with kLabel the original labeling of f (codes), with kDetect a
 // detection sweep on g (sweep 1 when both are set; a later dense sweep reading
-// the codes when only kDetect is set).  One block = 64 x 4 voxels walking 8
-// planes; the three planes a voxel's stencil touches are staged in LDS with a
-// one-voxel halo.  Targets are collected in an LDS bitmap over the tile and its
-// halo and flushed once per tile as row masks (one global atomicOr per 32 cells,
-// one list reservation per block).
+// the codes when only kDetect is set).
+//
+// Persistent blocks walk 64 x 8 x 16 tiles.  Blocks are dealt to the 8 XCDs
+// round-robin by workgroup id, so block b takes tiles from the b % 8 -th
+// contiguous eighth of the tile space: z- and y-neighbouring tiles, whose halos
+// overlap, then share one XCD's L2.  A tile walks its planes through a ring of
+// four LDS slots: while plane z is classified, plane z+2 is already in LDS and
+// plane z+3 is in flight to registers, so one barrier per plane suffices and the
+// loads overlap the compute.  Targets are collected in an LDS bitmap over the
+// tile and its halo and flushed once per tile as row masks (one global atomicOr
+// per 32 cells, one list reservation per block).
 template <typename T, bool kLabel, bool kDetect>
 __global__ __launch_bounds__(kThreads) void k_tiles(
     const T *__restrict__ f, const T *__restrict__ g, uint8_t *__restrict__ code,
@@ -74,116 +84,166 @@ __global__ __launch_bounds__(kThreads) void k_tiles(
     if (ring[jslot_prev].mode != kModeDense) return;  // uniform
   }
   constexpr int kFields = (kLabel ? 1 : 0) + (kDetect ? 1 : 0);
-  __shared__ T sv[kFields][3][kPY][kPX];
+  constexpr int kF = 0, kG = kLabel ? 1 : 0;
+  __shared__ T sv[kFields][kSlots][kPlane];
   __shared__ uint32_t sbits[kDetect ? kRows : 1][3];
   const int tid = threadIdx.x;
-  const int tx = tid % kTX, ty = tid / kTX;
-  const int64_t bx = (int64_t)blockIdx.x * kTX, by = (int64_t)blockIdx.y * kTY,
-                bz = (int64_t)blockIdx.z * kTZ;
-  const int64_t x = bx + tx, y = by + ty;
+  const int tx = tid % kTX, ty0 = tid / kTX;  // rows ty0 and ty0 + kTY / 2
+  const int64_t ntx = (d.nx + kTX - 1) / kTX, nty = (d.ny + kTY - 1) / kTY,
+                ntz = (d.nz + kTZ - 1) / kTZ;
+  const int64_t ntiles = ntx * nty * ntz;
+  const int64_t per_xcd_blocks = gridDim.x / 8;
+  const int64_t chunk = (ntiles + 7) / 8;
+  const int64_t t_begin = (int64_t)(blockIdx.x % 8) * chunk;
+  const int64_t t_end = t_begin + chunk < ntiles ? t_begin + chunk : ntiles;
   Counters *out = kDetect ? &ring[jslot] : nullptr;
-  constexpr int kF = 0, kG = kLabel ? 1 : 0;
-
-  if (kDetect)
-    for (int e = tid; e < kRows * 3; e += kThreads) (&sbits[0][0])[e] = 0;
-
-  auto load_plane = [&](int64_t pz, int slot) {
-    for (int e = tid; e < kPX * kPY; e += kThreads) {
-      int lx = e % kPX, ly = e / kPX;
-      int64_t gx = bx + lx - 1, gy = by + ly - 1;
-      T vf = 0, vg = 0;
-      if (pz >= 0 && pz < d.nz && gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny) {
-        int64_t gi = gx + d.nx * gy + d.sxy * pz;
-        if (kLabel) vf = f[gi];
-        if (kDetect) vg = g[gi];
-      }
-      if (kLabel) sv[kF][slot][ly][lx] = vf;
-      if (kDetect) sv[kG][slot][ly][lx] = vg;
-    }
-  };
-
-  load_plane(bz - 1, 0);
-  load_plane(bz, 1);
   uint32_t lc[3] = {0, 0, 0};
   uint32_t lerr = 0;
-  for (int k = 0; k < kTZ; k++) {
-    const int64_t z = bz + k;
-    if (z >= d.nz) break;  // uniform over the block
-    load_plane(z + 1, (k + 2) % 3);
-    __syncthreads();
-    const int slot[3] = {k % 3, (k + 1) % 3, (k + 2) % 3};
-    if (x < d.nx && y < d.ny) {
-      const int64_t i = x + d.nx * y + d.sxy * z;
-      const bool interior =
-          x > 0 && x < d.nx - 1 && y > 0 && y < d.ny - 1 && z > 0 && z < d.nz - 1;
-      const uint32_t vm = interior ? 0x7fffu : valid_mask(x, y, z, d);
-      int oa, od;
-      if (kLabel) {
-        T cf[kCand];
+
+  for (int64_t tile = t_begin + blockIdx.x / 8; tile < t_end; tile += per_xcd_blocks) {
+    const int64_t tzi = tile / (ntx * nty), rem = tile - tzi * ntx * nty;
+    const int64_t tyi = rem / ntx, txi = rem - tyi * ntx;
+    const int64_t bx = txi * kTX, by = tyi * kTY, bz = tzi * kTZ;
+    const int64_t x = bx + tx;
+
+    if (kDetect)
+      for (int e = tid; e < kRows * 3; e += kThreads) (&sbits[0][0])[e] = 0;
+
+    T pf[kEPT], pg[kEPT];
+    auto prefetch = [&](int64_t pz) {
 #pragma unroll
-        for (int r = 0; r < kCand; r++)
-          cf[r] = sv[kF][slot[rank_dz(r) + 1]][ty + 1 + rank_dy(r)][tx + 1 + rank_dx(r)];
-        if (interior) scan_full(cf, oa, od);
-        else scan_masked(cf, vm, oa, od);
-        code[i] = (uint8_t)(oa | (od << 4));
-        T fv = cf[kSelf];
-        if (!(fv == fv) || fv == sentinel<T>()) lerr |= 1u;
-        if (kDetect) dc[i] = 0;
-      } else {
-        uint8_t c = code[i];
-        oa = c & 15;
-        od = c >> 4;
+      for (int q = 0; q < kEPT; q++) {
+        const int e = tid + q * kThreads;
+        T vf = 0, vg = 0;
+        if (e < kPlane) {
+          const int lx = e % kPX, ly = e / kPX;
+          const int64_t gx = bx + lx - 1, gy = by + ly - 1;
+          if (pz >= 0 && pz < d.nz && gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny) {
+            const int64_t gi = gx + d.nx * gy + d.sxy * pz;
+            if (kLabel) vf = f[gi];
+            if (kDetect) vg = g[gi];
+          }
+        }
+        pf[q] = vf;
+        pg[q] = vg;
       }
-      if (kDetect) {
-        T cg[kCand];
+    };
+    auto store = [&](int slot) {
 #pragma unroll
-        for (int r = 0; r < kCand; r++)
-          cg[r] = sv[kG][slot[rank_dz(r) + 1]][ty + 1 + rank_dy(r)][tx + 1 + rank_dx(r)];
-        T gv = cg[kSelf];
-        if (kLabel && (!(gv == gv) || gv == sentinel<T>())) lerr |= 1u;
-        int da, dd;
-        if (interior) scan_full(cg, da, dd);
-        else scan_masked(cg, vm, da, dd);
-        int tr[4];
-        int nt = detect_targets(oa, od, da, dd, tr, lc);
-        for (int q = 0; q < nt; q++) {
-          int r = tr[q];
-          int lx = tx + 1 + rank_dx(r), ly = ty + 1 + rank_dy(r), lz = k + 1 + rank_dz(r);
-          atomicOr(&sbits[lz * kPY + ly][lx >> 5], 1u << (lx & 31));
+      for (int q = 0; q < kEPT; q++) {
+        const int e = tid + q * kThreads;
+        if (e < kPlane) {
+          if (kLabel) sv[kF][slot][e] = pf[q];
+          if (kDetect) sv[kG][slot][e] = pg[q];
+        }
+      }
+    };
+    prefetch(bz - 1);
+    store(0);
+    prefetch(bz);
+    store(1);
+    prefetch(bz + 1);
+    store(2);
+    prefetch(bz + 2);
+    __syncthreads();
+    const int kz = (int)((d.nz - bz) < kTZ ? (d.nz - bz) : kTZ);
+    for (int k = 0; k < kz; k++) {
+      const int64_t z = bz + k;
+      store((k + 3) % kSlots);  // plane z+2
+      prefetch(z + 3);
+      const int s0 = k % kSlots, s1 = (k + 1) % kSlots, s2 = (k + 2) % kSlots;
+#pragma unroll
+      for (int h = 0; h < kTY / (kThreads / kTX); h++) {
+        const int ty = ty0 + h * (kThreads / kTX);
+        const int64_t y = by + ty;
+        if (x < d.nx && y < d.ny) {
+          const int64_t i = x + d.nx * y + d.sxy * z;
+          const bool interior =
+              x > 0 && x < d.nx - 1 && y > 0 && y < d.ny - 1 && z > 0 && z < d.nz - 1;
+          const uint32_t vm = interior ? 0x7fffu : valid_mask(x, y, z, d);
+          int oa, od;
+          auto cell = [&](int r) {
+            const int sl = rank_dz(r) < 0 ? s0 : (rank_dz(r) == 0 ? s1 : s2);
+            return sl * kPlane + (ty + 1 + rank_dy(r)) * kPX + tx + 1 + rank_dx(r);
+          };
+          if (kLabel) {
+            T cf[kCand];
+#pragma unroll
+            for (int r = 0; r < kCand; r++) cf[r] = (&sv[kF][0][0])[cell(r)];
+            if (interior) scan_full(cf, oa, od);
+            else scan_masked(cf, vm, oa, od);
+            code[i] = (uint8_t)(oa | (od << 4));
+            const T fv = cf[kSelf];
+            if (!(fv == fv) || fv == sentinel<T>()) lerr |= 1u;
+            if (kDetect) dc[i] = 0;
+          } else {
+            const uint8_t c = code[i];
+            oa = c & 15;
+            od = c >> 4;
+          }
+          if (kDetect) {
+            T cg[kCand];
+#pragma unroll
+            for (int r = 0; r < kCand; r++) cg[r] = (&sv[kG][0][0])[cell(r)];
+            const T gv = cg[kSelf];
+            if (kLabel && (!(gv == gv) || gv == sentinel<T>())) lerr |= 1u;
+            int da, dd;
+            if (interior) scan_full(cg, da, dd);
+            else scan_masked(cg, vm, da, dd);
+            int tr[4];
+            const int nt = detect_targets(oa, od, da, dd, tr, lc);
+            for (int q = 0; q < nt; q++) {
+              const int r = tr[q];
+              const int lx = tx + 1 + rank_dx(r), ly = ty + 1 + rank_dy(r),
+                        lz = k + 1 + rank_dz(r);
+              atomicOr(&sbits[lz * kPY + ly][lx >> 5], 1u << (lx & 31));
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (kDetect) {
+      // flush: (row, word) items, kRows rows x 4 global words
+      uint32_t claimed[kFlushPT];
+      int64_t wbase[kFlushPT];
+      uint32_t n = 0;
+#pragma unroll
+      for (int q = 0; q < kFlushPT; q++) {
+        claimed[q] = 0;
+        wbase[q] = 0;
+        const int item = tid + q * kThreads;
+        if (item >= kRows * 4) continue;
+        const int row = item >> 2, kw = item & 3;
+        const int ly = row % kPY, lz = row / kPY;
+        const int64_t gy = by + ly - 1, gz = bz + lz - 1;
+        if (gy < 0 || gy >= d.ny || gz < 0 || gz >= d.nz) continue;
+        const int64_t i0 = (bx - 1) + d.nx * gy + d.sxy * gz;  // cell lx = 0 (may be -1)
+        const int64_t w0 = i0 >> 5;                              // floor
+        const int sh = (int)(i0 - (w0 << 5));
+        const uint32_t m = bits32_at(sbits[row][0], sbits[row][1], sbits[row][2], 32 * kw - sh);
+        if (m) {
+          wbase[q] = w0 + kw;
+          claimed[q] = m & ~atomicOr(&tmark[wbase[q]], m);
+          n += __popc(claimed[q]);
+        }
+      }
+      uint32_t pos = block_reserve<kThreads>(n, &out->ntargets);
+#pragma unroll
+      for (int q = 0; q < kFlushPT; q++) {
+        uint32_t cm = claimed[q];
+        while (cm) {
+          const int b = __ffs(cm) - 1;
+          cm &= cm - 1;
+          tlist[pos++] = (uint32_t)(wbase[q] * 32 + b);
         }
       }
     }
-    __syncthreads();
   }
   if (kLabel && lerr) atomicOr(&run->err, 1u);
   if (!kDetect) return;
   block_add3<kThreads>(lc, out->cnt);
-  // flush: (row, word) items, 60 rows x 4 global words
-  uint32_t claimed = 0;
-  int64_t wbase = 0;
-  if (tid < kRows * 4) {
-    const int row = tid >> 2, kw = tid & 3;
-    const int ly = row % kPY, lz = row / kPY;
-    const int64_t gy = by + ly - 1, gz = bz + lz - 1;
-    if (gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz) {
-      const int64_t i0 = (bx - 1) + d.nx * gy + d.sxy * gz;  // cell lx = 0 (may be -1)
-      const int64_t w0 = i0 >> 5;                              // floor
-      const int s = (int)(i0 - (w0 << 5));
-      uint32_t m = bits32_at(sbits[row][0], sbits[row][1], sbits[row][2], 32 * kw - s);
-      if (m) {
-        wbase = w0 + kw;
-        uint32_t old = atomicOr(&tmark[wbase], m);
-        claimed = m & ~old;
-      }
-    }
-  }
-  uint32_t pos = block_reserve<kThreads>(__popc(claimed), &out->ntargets);
-  while (claimed) {
-    int b = __ffs(claimed) - 1;
-    claimed &= claimed - 1;
-    tlist[pos++] = (uint32_t)(wbase * 32 + b);
-  }
-  if (tid == 0 && !kLabel) atomicAdd(&run->dense_passes, 1ull);
+  if (tid == 0 && blockIdx.x == 0 && !kLabel) atomicAdd(&run->dense_passes, 1ull);
 }
 
 // ---------------------------------------------------------------------------
@@ -223,21 +283,28 @@ __device__ __forceinline__ void apply_one(T *g, const T *f, uint8_t *dc, uint32_
   dc[t] = c;
   touched += c0 == 0;
   clamped += (c == ep.q && c0 != ep.q);
-  atomicAnd(&tmark[t >> 5], ~(1u << (t & 31u)));
+  // every set bit of tmark is a target of this sweep and is applied in this phase,
+  // so the word can be cleared with a plain store
+  tmark[t >> 5] = 0u;
 }
 
 // Claims the closed stencil neighbourhood of t in wmark; returns the new voxels.
 __device__ __forceinline__ int queue_neighbourhood(uint32_t t, uint32_t *wmark, const Grid3 &d,
                                                    uint32_t (&u)[kCand]) {
   int64_t x = (int64_t)t % d.nx, y = ((int64_t)t / d.nx) % d.ny, z = (int64_t)t / d.sxy;
+  const uint32_t vm = valid_mask(x, y, z, d);
+  // issue all claims before looking at any result, so the atomics overlap
+  uint32_t old[kCand];
+#pragma unroll
+  for (int r = 0; r < kCand; r++) {
+    const uint32_t v = (uint32_t)((int64_t)t + rank_off(r, d));
+    old[r] = ((vm >> r) & 1u) ? atomicOr(&wmark[v >> 5], 1u << (v & 31u)) : 0xffffffffu;
+  }
   int n = 0;
 #pragma unroll
   for (int r = 0; r < kCand; r++) {
-    int64_t a = x + rank_dx(r), b = y + rank_dy(r), c = z + rank_dz(r);
-    if (a >= 0 && a < d.nx && b >= 0 && b < d.ny && c >= 0 && c < d.nz) {
-      uint32_t v = (uint32_t)((int64_t)t + rank_off(r, d));
-      if (claim(wmark, v)) u[n++] = v;
-    }
+    const uint32_t v = (uint32_t)((int64_t)t + rank_off(r, d));
+    if (!((old[r] >> (v & 31u)) & 1u)) u[n++] = v;
   }
   return n;
 }
@@ -259,11 +326,14 @@ __device__ __forceinline__ int detect_one(const T *g, const uint8_t *code, uint3
   uint8_t oc = code[v];
   int tr[4];
   int nt = detect_targets(oc & 15, oc >> 4, da, dd, tr, lc);
-  int n = 0;
+  uint32_t tv[4], old[4];
   for (int q = 0; q < nt; q++) {
-    uint32_t t = (uint32_t)((int64_t)v + rank_off(tr[q], d));
-    if (claim(tmark, t)) out[n++] = t;
+    tv[q] = (uint32_t)((int64_t)v + rank_off(tr[q], d));
+    old[q] = atomicOr(&tmark[tv[q] >> 5], 1u << (tv[q] & 31u));
   }
+  int n = 0;
+  for (int q = 0; q < nt; q++)
+    if (!((old[q] >> (tv[q] & 31u)) & 1u)) out[n++] = tv[q];
   return n;
 }
 
@@ -347,7 +417,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
     int n = 0;
     if (i < nw) {
       const uint32_t v = wlist[i];
-      atomicAnd(&wmark[v >> 5], ~(1u << (v & 31u)));
+      wmark[v >> 5] = 0u;  // every set bit is a listed voxel of this sweep
       n = detect_one<T>(g, code, tmark, v, d, lc, out);
     }
     uint32_t pos = block_reserve<kApplyThreads>((uint32_t)n, &cn.ntargets);
@@ -599,14 +669,16 @@ Grid3 grid_of(int64_t nx, int64_t ny, int64_t nz) {
   return d;
 }
 
-dim3 tile_grid(const Grid3 &d) {
-  return dim3((unsigned)((d.nx + kTX - 1) / kTX), (unsigned)((d.ny + kTY - 1) / kTY),
-              (unsigned)((d.nz + kTZ - 1) / kTZ));
+int64_t tile_count(const Grid3 &d) {
+  return ((d.nx + kTX - 1) / kTX) * ((d.ny + kTY - 1) / kTY) * ((d.nz + kTZ - 1) / kTZ);
 }
 
-int64_t tile_count(const Grid3 &d) {
-  dim3 t = tile_grid(d);
-  return (int64_t)t.x * t.y * t.z;
+// Persistent grid of the tile kernel: a multiple of 8 (one share per XCD), at most
+// the resident capacity and no more than the tiles need.
+unsigned tile_blocks(const Ctx &c, const Grid3 &d) {
+  int64_t per_xcd = (tile_count(d) + 7) / 8;
+  int64_t cap = std::max<int64_t>(1, (int64_t)c.cus * 4 / 8);
+  return (unsigned)(8 * std::max<int64_t>(1, std::min(per_xcd, cap)));
 }
 
 EditParams edit_params(const pmsz_params *p) {
@@ -645,7 +717,7 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
   PMSZ_HIP(hipMemcpyAsync(&c.ring[0].mode, &kRunning, sizeof(uint32_t), hipMemcpyHostToDevice,
                           c.stream));
   PMSZ_HIP(hipEventRecord(c.ev2, c.stream));
-  k_tiles<T, true, true><<<tile_grid(d), kThreads, 0, c.stream>>>(
+  k_tiles<T, true, true><<<tile_blocks(c, d), kThreads, 0, c.stream>>>(
       f, g, c.code, c.dc, c.tmark, c.tlist, c.ring, 0, 1, c.run, d);
   PMSZ_HIP(hipGetLastError());
   PMSZ_HIP(hipEventRecord(c.ev3, c.stream));
@@ -656,7 +728,7 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
       k_apply<T><<<grid_list, kApplyThreads, 0, c.stream>>>(g, f, c.dc, c.tmark, c.tlist,
                                                             c.wmark, c.wlist, c.ring, c.run, j,
                                                             d, ep, s);
-      k_tiles<T, false, true><<<tile_grid(d), kThreads, 0, c.stream>>>(
+      k_tiles<T, false, true><<<tile_blocks(c, d), kThreads, 0, c.stream>>>(
           nullptr, g, c.code, nullptr, c.tmark, c.tlist, c.ring, j % kCtrRing,
           (j + 1) % kCtrRing, c.run, d);
       k_detect_list<T><<<grid_list, kApplyThreads, 0, c.stream>>>(
@@ -693,11 +765,19 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
   st->final_cnt[0] = cm.cnt[0];
   st->final_cnt[1] = cm.cnt[1];
   st->final_cnt[2] = cm.cnt[2];
+  if (m <= (uint32_t)kCtrRing) {
+    st->trace_n = std::min<int64_t>(m, 64);
+    for (int64_t k = 0; k < st->trace_n; k++) {
+      const Counters &ck = c.h_ring[(k + 1) % kCtrRing];
+      for (int q = 0; q < 3; q++) st->trace_cnt[k][q] = ck.cnt[q];
+      st->trace_targets[k] = ck.ntargets;
+    }
+  }
   st->total_targets = (int64_t)r.total_targets;
   st->touched_voxels = (int64_t)r.touched;
   st->clamped_voxels = (int64_t)r.clamped;
   st->sparse_voxels = (int64_t)r.sparse_voxels;
-  const int64_t extra_dense = (int64_t)r.dense_passes / std::max<int64_t>(1, tile_count(d));
+  const int64_t extra_dense = (int64_t)r.dense_passes;
   st->dense_passes = 1 + extra_dense;
   if (p->mode == PMSZ_LAZY) {
     st->edit_sweeps = m;
@@ -734,7 +814,7 @@ int time_first_pass(Ctx &c, const pmsz_params *p, const T *f, const T *g, int it
     PMSZ_HIP(hipMemsetAsync(c.tmark, 0, sizeof(uint32_t) * words, c.stream));
     PMSZ_HIP(hipMemsetAsync(c.ring, 0, sizeof(Counters) * 2, c.stream));
     PMSZ_HIP(hipEventRecord(c.ev0, c.stream));
-    k_tiles<T, true, true><<<tile_grid(d), kThreads, 0, c.stream>>>(
+    k_tiles<T, true, true><<<tile_blocks(c, d), kThreads, 0, c.stream>>>(
         f, g, c.code, c.dc, c.tmark, c.tlist, c.ring, 0, 1, c.run, d);
     PMSZ_HIP(hipGetLastError());
     PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
@@ -840,11 +920,11 @@ int pmsz_codes(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_fiel
   if (rc) return rc;
   PMSZ_HIP(hipMemsetAsync(c.run, 0, sizeof(RunState), c.stream));
   if (dtype == PMSZ_F32)
-    k_tiles<float, true, false><<<tile_grid(d), kThreads, 0, c.stream>>>(
+    k_tiles<float, true, false><<<tile_blocks(c, d), kThreads, 0, c.stream>>>(
         (const float *)d_field, nullptr, d_codes, nullptr, nullptr, nullptr, c.ring, 0, 0, c.run,
         d);
   else
-    k_tiles<double, true, false><<<tile_grid(d), kThreads, 0, c.stream>>>(
+    k_tiles<double, true, false><<<tile_blocks(c, d), kThreads, 0, c.stream>>>(
         (const double *)d_field, nullptr, d_codes, nullptr, nullptr, nullptr, c.ring, 0, 0,
         c.run, d);
   PMSZ_HIP(hipGetLastError());

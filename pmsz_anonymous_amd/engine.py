@@ -70,11 +70,18 @@ class Stats(ctypes.Structure):
         ("t_total_s", ctypes.c_double),
         ("t_dense_s", ctypes.c_double),
         ("bytes_algorithmic", ctypes.c_double),
+        ("trace_n", ctypes.c_int64),
+        ("trace_cnt", (ctypes.c_uint32 * 3) * 64),
+        ("trace_targets", ctypes.c_uint32 * 64),
     ]
 
     def as_dict(self):
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "final_cnt"}
+        skip = ("final_cnt", "trace_cnt", "trace_targets")
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in skip}
         d["final_cnt"] = list(self.final_cnt)
+        n = int(self.trace_n)
+        d["trace_cnt"] = [list(self.trace_cnt[i]) for i in range(n)]
+        d["trace_targets"] = [int(self.trace_targets[i]) for i in range(n)]
         return d
 
 

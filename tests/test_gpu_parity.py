@@ -45,6 +45,12 @@ def _compare(f, g0, eb, dev, **kw):
               "clamped_voxels", "status", "final_cnt"):
         assert stg[k] == sto[k], (k, stg[k], sto[k])
     assert stg["touched_voxels"] == int((dco > 0).sum())
+    # per-sweep counters and target counts, sweep by sweep
+    n = stg["trace_n"]
+    assert n >= 1
+    m = min(n, len(tr["targets"]))
+    assert [list(c) for c in tr["cnt"][:m]] == stg["trace_cnt"][:m]
+    assert list(tr["targets"][:m]) == stg["trace_targets"][:m]
     return go, sto
 
 
