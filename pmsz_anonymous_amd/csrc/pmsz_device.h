@@ -71,6 +71,11 @@ __device__ __forceinline__ uint32_t valid_mask(int64_t x, int64_t y, int64_t z, 
   return m;
 }
 
+__device__ __forceinline__ uint32_t valid_mask(uint32_t x, uint32_t y, uint32_t z,
+                                               const Grid3 &d) {
+  return valid_mask((int64_t)x, (int64_t)y, (int64_t)z, d);
+}
+
 // Steepest ascent / descent candidate over all 15 candidates.  Visiting ranks in
 // increasing order, ">=" keeps the highest rank among equal maxima and "<" the
 // lowest rank among equal minima: the (value, index) order of pMSz.cu:610-620.
@@ -145,6 +150,14 @@ __device__ __forceinline__ int detect_targets(int oa, int od, int da, int dd, in
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Workgroup barrier ordering LDS only: global stores issued before it need not
+// complete, so loads in flight (prefetches) are not drained at the barrier.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 // Set the mark bit of v; true if it was clear (the caller then owns v).
 __device__ __forceinline__ bool claim(uint32_t *marks, uint32_t v) {

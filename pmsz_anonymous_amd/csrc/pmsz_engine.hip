@@ -48,6 +48,7 @@ constexpr int kThreads = 256;  // each thread classifies 2 rows of the tile
 constexpr int kPX = kTX + 2, kPY = kTY + 2, kPZ = kTZ + 2;  // tile + halo
 constexpr int kPlane = kPX * kPY;                             // cells of one staged plane
 constexpr int kEPT = (kPlane + kThreads - 1) / kThreads;      // staged cells per thread
+constexpr int kPlaneP = kEPT * kThreads;                       // padded slot size
 constexpr int kSlots = 4;                                     // LDS plane ring
 constexpr int kRows = kPY * kPZ;                              // bitmap rows of 66 cells
 constexpr int kFlushPT = (kRows * 4 + kThreads - 1) / kThreads;
@@ -85,81 +86,108 @@ __global__ __launch_bounds__(kThreads) void k_tiles(
   }
   constexpr int kFields = (kLabel ? 1 : 0) + (kDetect ? 1 : 0);
   constexpr int kF = 0, kG = kLabel ? 1 : 0;
-  __shared__ T sv[kFields][kSlots][kPlane];
+  constexpr int kRowsPT = kTY / (kThreads / kTX);  // tile rows per thread
+  __shared__ T sv[kFields][kSlots][kPlaneP];
   __shared__ uint32_t sbits[kDetect ? kRows : 1][3];
   const int tid = threadIdx.x;
-  const int tx = tid % kTX, ty0 = tid / kTX;  // rows ty0 and ty0 + kTY / 2
-  const int64_t ntx = (d.nx + kTX - 1) / kTX, nty = (d.ny + kTY - 1) / kTY,
-                ntz = (d.nz + kTZ - 1) / kTZ;
-  const int64_t ntiles = ntx * nty * ntz;
-  const int64_t per_xcd_blocks = gridDim.x / 8;
-  const int64_t chunk = (ntiles + 7) / 8;
-  const int64_t t_begin = (int64_t)(blockIdx.x % 8) * chunk;
-  const int64_t t_end = t_begin + chunk < ntiles ? t_begin + chunk : ntiles;
+  const int tx = tid % kTX, ty0 = tid / kTX;
+  // all voxel indices fit in 32 bits (checked on the host)
+  const uint32_t nx = (uint32_t)d.nx, ny = (uint32_t)d.ny, nz = (uint32_t)d.nz;
+  const uint32_t sxy = (uint32_t)d.sxy;
+  const uint32_t ntx = (nx + kTX - 1) / kTX, nty = (ny + kTY - 1) / kTY, ntz = (nz + kTZ - 1) / kTZ;
+  const uint32_t ntiles = ntx * nty * ntz;
+  const uint32_t per_xcd_blocks = gridDim.x / 8;
+  const uint32_t chunk = (ntiles + 7) / 8;
+  const uint32_t t_begin = (blockIdx.x % 8) * chunk;
+  const uint32_t t_end = t_begin + chunk < ntiles ? t_begin + chunk : ntiles;
   Counters *out = kDetect ? &ring[jslot] : nullptr;
   uint32_t lc[3] = {0, 0, 0};
   uint32_t lerr = 0;
 
-  for (int64_t tile = t_begin + blockIdx.x / 8; tile < t_end; tile += per_xcd_blocks) {
-    const int64_t tzi = tile / (ntx * nty), rem = tile - tzi * ntx * nty;
-    const int64_t tyi = rem / ntx, txi = rem - tyi * ntx;
-    const int64_t bx = txi * kTX, by = tyi * kTY, bz = tzi * kTZ;
-    const int64_t x = bx + tx;
+  for (uint32_t tile = t_begin + blockIdx.x / 8; tile < t_end; tile += per_xcd_blocks) {
+    const uint32_t tzi = tile / (ntx * nty), rem = tile - tzi * ntx * nty;
+    const uint32_t tyi = rem / ntx, txi = rem - tyi * ntx;
+    const uint32_t bx = txi * kTX, by = tyi * kTY, bz = tzi * kTZ;
+    const uint32_t x = bx + tx;
 
     if (kDetect)
       for (int e = tid; e < kRows * 3; e += kThreads) (&sbits[0][0])[e] = 0;
 
+    // staged cells of this thread: in-plane offset and in-plane validity, fixed
+    // for the whole tile; loads are unconditional (clamped to cell 0) so the
+    // compiler can issue them back to back
+    uint32_t soff[kEPT];
+    bool sok[kEPT];
+#pragma unroll
+    for (int q = 0; q < kEPT; q++) {
+      const int e = tid + q * kThreads;
+      const int lx = e % kPX, ly = e / kPX;
+      const int gx = (int)bx + lx - 1, gy = (int)by + ly - 1;
+      sok[q] = e < kPlane && gx >= 0 && gx < (int)nx && gy >= 0 && gy < (int)ny;
+      soff[q] = sok[q] ? (uint32_t)gx + nx * (uint32_t)gy : 0u;
+    }
+    // out-of-domain cells hold whatever cell 0 holds: no candidate reads them
+    // (interior voxels have every neighbour in the domain, boundary voxels use the
+    // validity mask), so the staged values need no select
     T pf[kEPT], pg[kEPT];
-    auto prefetch = [&](int64_t pz) {
+    auto prefetch = [&](int pz) {
+      const bool zok = pz >= 0 && pz < (int)nz;
+      const uint32_t zoff = zok ? sxy * (uint32_t)pz : 0u;
 #pragma unroll
       for (int q = 0; q < kEPT; q++) {
-        const int e = tid + q * kThreads;
-        T vf = 0, vg = 0;
-        if (e < kPlane) {
-          const int lx = e % kPX, ly = e / kPX;
-          const int64_t gx = bx + lx - 1, gy = by + ly - 1;
-          if (pz >= 0 && pz < d.nz && gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny) {
-            const int64_t gi = gx + d.nx * gy + d.sxy * pz;
-            if (kLabel) vf = f[gi];
-            if (kDetect) vg = g[gi];
-          }
-        }
-        pf[q] = vf;
-        pg[q] = vg;
+        const uint32_t gi = (zok && sok[q]) ? zoff + soff[q] : 0u;
+        if (kLabel) pf[q] = f[gi];
+        if (kDetect) pg[q] = g[gi];
       }
     };
     auto store = [&](int slot) {
 #pragma unroll
       for (int q = 0; q < kEPT; q++) {
-        const int e = tid + q * kThreads;
-        if (e < kPlane) {
-          if (kLabel) sv[kF][slot][e] = pf[q];
-          if (kDetect) sv[kG][slot][e] = pg[q];
-        }
+        const int e = tid + q * kThreads;  // slots are padded to kEPT * kThreads cells
+        if (kLabel) sv[kF][slot][e] = pf[q];
+        if (kDetect) sv[kG][slot][e] = pg[q];
       }
     };
-    prefetch(bz - 1);
+    // codes of this thread's voxels, one plane ahead (dense sweeps after the first)
+    uint8_t cpre[kRowsPT];
+    auto prefetch_codes = [&](int pz) {
+#pragma unroll
+      for (int h = 0; h < kRowsPT; h++) {
+        const uint32_t y = by + ty0 + h * (kThreads / kTX);
+        const bool ok = x < nx && y < ny && pz < (int)nz;
+        const uint8_t c = code[ok ? x + nx * y + sxy * (uint32_t)pz : 0u];
+        cpre[h] = c;
+      }
+    };
+    prefetch((int)bz - 1);
     store(0);
-    prefetch(bz);
+    prefetch((int)bz);
     store(1);
-    prefetch(bz + 1);
+    prefetch((int)bz + 1);
     store(2);
-    prefetch(bz + 2);
-    __syncthreads();
-    const int kz = (int)((d.nz - bz) < kTZ ? (d.nz - bz) : kTZ);
+    prefetch((int)bz + 2);
+    if (!kLabel) prefetch_codes((int)bz);
+    lds_barrier();
+    const int kz = (int)((nz - bz) < (uint32_t)kTZ ? (nz - bz) : (uint32_t)kTZ);
     for (int k = 0; k < kz; k++) {
-      const int64_t z = bz + k;
+      const uint32_t z = bz + k;
       store((k + 3) % kSlots);  // plane z+2
-      prefetch(z + 3);
+      prefetch((int)z + 3);
+      uint8_t ccur[kRowsPT];
+      if (!kLabel) {
+#pragma unroll
+        for (int h = 0; h < kRowsPT; h++) ccur[h] = cpre[h];
+        prefetch_codes((int)z + 1);
+      }
       const int s0 = k % kSlots, s1 = (k + 1) % kSlots, s2 = (k + 2) % kSlots;
 #pragma unroll
-      for (int h = 0; h < kTY / (kThreads / kTX); h++) {
+      for (int h = 0; h < kRowsPT; h++) {
         const int ty = ty0 + h * (kThreads / kTX);
-        const int64_t y = by + ty;
-        if (x < d.nx && y < d.ny) {
-          const int64_t i = x + d.nx * y + d.sxy * z;
+        const uint32_t y = by + ty;
+        if (x < nx && y < ny) {
+          const uint32_t i = x + nx * y + sxy * z;
           const bool interior =
-              x > 0 && x < d.nx - 1 && y > 0 && y < d.ny - 1 && z > 0 && z < d.nz - 1;
+              x > 0 && x + 1 < nx && y > 0 && y + 1 < ny && z > 0 && z + 1 < nz;
           const uint32_t vm = interior ? 0x7fffu : valid_mask(x, y, z, d);
           int oa, od;
           auto cell = [&](int r) {
@@ -177,9 +205,8 @@ __global__ __launch_bounds__(kThreads) void k_tiles(
             if (!(fv == fv) || fv == sentinel<T>()) lerr |= 1u;
             if (kDetect) dc[i] = 0;
           } else {
-            const uint8_t c = code[i];
-            oa = c & 15;
-            od = c >> 4;
+            oa = ccur[h] & 15;
+            od = ccur[h] >> 4;
           }
           if (kDetect) {
             T cg[kCand];
@@ -201,12 +228,12 @@ __global__ __launch_bounds__(kThreads) void k_tiles(
           }
         }
       }
-      __syncthreads();
+      lds_barrier();
     }
     if (kDetect) {
       // flush: (row, word) items, kRows rows x 4 global words
       uint32_t claimed[kFlushPT];
-      int64_t wbase[kFlushPT];
+      uint32_t wbase[kFlushPT];
       uint32_t n = 0;
 #pragma unroll
       for (int q = 0; q < kFlushPT; q++) {
@@ -216,14 +243,15 @@ __global__ __launch_bounds__(kThreads) void k_tiles(
         if (item >= kRows * 4) continue;
         const int row = item >> 2, kw = item & 3;
         const int ly = row % kPY, lz = row / kPY;
-        const int64_t gy = by + ly - 1, gz = bz + lz - 1;
-        if (gy < 0 || gy >= d.ny || gz < 0 || gz >= d.nz) continue;
-        const int64_t i0 = (bx - 1) + d.nx * gy + d.sxy * gz;  // cell lx = 0 (may be -1)
-        const int64_t w0 = i0 >> 5;                              // floor
+        const int gy = (int)by + ly - 1, gz = (int)bz + lz - 1;
+        if (gy < 0 || gy >= (int)ny || gz < 0 || gz >= (int)nz) continue;
+        // linear index of cell lx = 0 (may be -1 for the very first row)
+        const int64_t i0 = (int64_t)bx - 1 + (int64_t)nx * gy + (int64_t)sxy * gz;
+        const int64_t w0 = i0 >> 5;  // floor
         const int sh = (int)(i0 - (w0 << 5));
         const uint32_t m = bits32_at(sbits[row][0], sbits[row][1], sbits[row][2], 32 * kw - sh);
         if (m) {
-          wbase[q] = w0 + kw;
+          wbase[q] = (uint32_t)(w0 + kw);
           claimed[q] = m & ~atomicOr(&tmark[wbase[q]], m);
           n += __popc(claimed[q]);
         }
@@ -235,7 +263,7 @@ __global__ __launch_bounds__(kThreads) void k_tiles(
         while (cm) {
           const int b = __ffs(cm) - 1;
           cm &= cm - 1;
-          tlist[pos++] = (uint32_t)(wbase[q] * 32 + b);
+          tlist[pos++] = wbase[q] * 32u + (uint32_t)b;
         }
       }
     }
