@@ -192,7 +192,7 @@ __global__ __launch_bounds__(kThreads) void k_tiles(
           int oa, od;
           auto cell = [&](int r) {
             const int sl = rank_dz(r) < 0 ? s0 : (rank_dz(r) == 0 ? s1 : s2);
-            return sl * kPlane + (ty + 1 + rank_dy(r)) * kPX + tx + 1 + rank_dx(r);
+            return sl * kPlaneP + (ty + 1 + rank_dy(r)) * kPX + tx + 1 + rank_dx(r);
           };
           if (kLabel) {
             T cf[kCand];
