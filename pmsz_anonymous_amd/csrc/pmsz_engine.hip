@@ -87,7 +87,8 @@ __global__ __launch_bounds__(kThreads) void k_tiles(
   constexpr int kFields = (kLabel ? 1 : 0) + (kDetect ? 1 : 0);
   constexpr int kF = 0, kG = kLabel ? 1 : 0;
   constexpr int kRowsPT = kTY / (kThreads / kTX);  // tile rows per thread
-  __shared__ T sv[kFields][kSlots][kPlaneP];
+  constexpr bool kKeys = sizeof(T) == 4;  // float32: keys, float64: rank scan
+  __shared__ double sv[kFields][kSlots][kPlaneP];
   __shared__ uint32_t sbits[kDetect ? kRows : 1][3];
   const int tid = threadIdx.x;
   const int tx = tid % kTX, ty0 = tid / kTX;
@@ -130,12 +131,16 @@ __global__ __launch_bounds__(kThreads) void k_tiles(
     // (interior voxels have every neighbour in the domain, boundary voxels use the
     // validity mask), so the staged values need no select
     T pf[kEPT], pg[kEPT];
+    uint32_t pok = 0;  // bit q: staged cell q lies in the domain
     auto prefetch = [&](int pz) {
       const bool zok = pz >= 0 && pz < (int)nz;
       const uint32_t zoff = zok ? sxy * (uint32_t)pz : 0u;
+      pok = 0;
 #pragma unroll
       for (int q = 0; q < kEPT; q++) {
-        const uint32_t gi = (zok && sok[q]) ? zoff + soff[q] : 0u;
+        const bool ok = zok && sok[q];
+        pok |= (ok ? 1u : 0u) << q;
+        const uint32_t gi = ok ? zoff + soff[q] : 0u;
         if (kLabel) pf[q] = f[gi];
         if (kDetect) pg[q] = g[gi];
       }
@@ -144,8 +149,16 @@ __global__ __launch_bounds__(kThreads) void k_tiles(
 #pragma unroll
       for (int q = 0; q < kEPT; q++) {
         const int e = tid + q * kThreads;  // slots are padded to kEPT * kThreads cells
-        if (kLabel) sv[kF][slot][e] = pf[q];
-        if (kDetect) sv[kG][slot][e] = pg[q];
+        if (kLabel) {
+          const T v = pf[q];
+          if (((pok >> q) & 1u) && (!(v == v) || v == sentinel<T>())) lerr |= 1u;
+          sv[kF][slot][e] = stage_value(v);
+        }
+        if (kDetect) {
+          const T v = pg[q];
+          if (kLabel && ((pok >> q) & 1u) && (!(v == v) || v == sentinel<T>())) lerr |= 1u;
+          sv[kG][slot][e] = stage_value(v);
+        }
       }
     };
     // codes of this thread's voxels, one plane ahead (dense sweeps after the first)
@@ -194,29 +207,29 @@ __global__ __launch_bounds__(kThreads) void k_tiles(
             const int sl = rank_dz(r) < 0 ? s0 : (rank_dz(r) == 0 ? s1 : s2);
             return sl * kPlaneP + (ty + 1 + rank_dy(r)) * kPX + tx + 1 + rank_dx(r);
           };
-          if (kLabel) {
-            T cf[kCand];
+          auto classify = [&](int field, int &a, int &dsc) {
+            double st[kCand];
 #pragma unroll
-            for (int r = 0; r < kCand; r++) cf[r] = (&sv[kF][0][0])[cell(r)];
-            if (interior) scan_full(cf, oa, od);
-            else scan_masked(cf, vm, oa, od);
+            for (int r = 0; r < kCand; r++) st[r] = (&sv[field][0][0])[cell(r)];
+            if (kKeys) {
+              classify_keys(st, interior, vm, a, dsc);
+            } else if (interior) {
+              scan_full(st, a, dsc);
+            } else {
+              scan_masked(st, vm, a, dsc);
+            }
+          };
+          if (kLabel) {
+            classify(kF, oa, od);
             code[i] = (uint8_t)(oa | (od << 4));
-            const T fv = cf[kSelf];
-            if (!(fv == fv) || fv == sentinel<T>()) lerr |= 1u;
             if (kDetect) dc[i] = 0;
           } else {
             oa = ccur[h] & 15;
             od = ccur[h] >> 4;
           }
           if (kDetect) {
-            T cg[kCand];
-#pragma unroll
-            for (int r = 0; r < kCand; r++) cg[r] = (&sv[kG][0][0])[cell(r)];
-            const T gv = cg[kSelf];
-            if (kLabel && (!(gv == gv) || gv == sentinel<T>())) lerr |= 1u;
             int da, dd;
-            if (interior) scan_full(cg, da, dd);
-            else scan_masked(cg, vm, da, dd);
+            classify(kG, da, dd);
             int tr[4];
             const int nt = detect_targets(oa, od, da, dd, tr, lc);
             for (int q = 0; q < nt; q++) {
