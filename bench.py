@@ -123,6 +123,8 @@ def main():
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                        "p99": float(np.percentile(lat, 99) * 1e3)},
         "correction": {"edit_sweeps": s0["edit_sweeps"], "total_targets": s0["total_targets"],
+                       "targets_per_sweep": s0["trace_targets"],
+                       "dense_passes": s0["dense_passes"],
                        "sparse_voxels": s0["sparse_voxels"], "topology_preserved": bool(ok),
                        "device_ms": s0["t_total_s"] * 1e3},
         "roofline": {

@@ -1,0 +1,401 @@
+// pmsz_dense.h -- the full-volume kernel of the correction: original labeling
+// of f and/or one detection sweep over g, for every voxel.
+//
+// Layout of the work: a wave owns 62 consecutive x (lanes 1..62; lanes 0 and 63
+// carry the x-1 / x+1 halo), kColRows consecutive y, and walks z.  No LDS and no
+// barriers: a lane loads its x column of kColRows+2 rows per plane from HBM
+// (coalesced across the wave), the x-1 / x+1 neighbours arrive through DPP wave
+// shifts, and z is a register pipeline.
+//
+// The 15 candidates of voxel (x, y, z) -- itself and its 14 stencil neighbours,
+// ranked by linear index (pmsz_common.h) -- split by plane: ranks 0..3 lie in
+// plane z-1, 4..10 in plane z, 11..14 in plane z+1.  When plane p arrives, each
+// row gets three partial extremes from it: "lo" (p as the z-1 plane of voxel
+// p+1), "mid" (p as the voxel's own plane) and "hi" (p as the z+1 plane of voxel
+// p-1).  Voxel p-1 is then complete: extreme(lo(p-2), mid(p-1), hi(p)).
+// Every cell is loaded once per pass.
+//
+// Candidate order.  float32: the staged key 2^52 + 16 * ord(v) plus the rank is
+// an exact integer in a double whose numeric order is the (value, index) order
+// of islarger_shared / isless_shared (pMSz.cu:610-620); max / min are single
+// v_max_f64 / v_min_f64 and the winner's rank is the key's low 4 bits.
+// float64: (value, rank) pairs combined in increasing rank order, the later one
+// winning ties for the maximum and the earlier one for the minimum.
+// Out-of-domain cells (the reference's INVALID halo, pMSz.cu:709) are quiet NaN:
+// maxnum / minnum and the pair combines skip them.
+//
+// Targets (with kDetect) go to per-lane bit masks over (dx, row) for the three
+// planes a voxel's targets can lie in; a completed plane is exchanged with the
+// neighbouring lanes, turned into one 64-lane mask per row by ballots and OR-ed
+// into the target bitmap with one atomic per 32 cells; cells whose bit was new
+// are appended to the target list with one reservation per wave.
+#pragma once
+#include "pmsz_device.h"
+
+namespace pmsz {
+
+constexpr int kColOwned = 62;  // x owned by a wave (64 lanes minus the two halo lanes)
+constexpr int kColRows = 4;    // y rows per wave
+constexpr int kColH = kColRows + 2;
+constexpr int kColZ = 64;      // planes per work unit
+constexpr int kColWaves = 4;   // waves per block
+constexpr int kColThreads = 64 * kColWaves;
+
+// Value of lane i-1 / lane i+1 (lane 0 / lane 63 receive 0).
+__device__ __forceinline__ uint32_t from_left_u(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t from_right_u(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ double from_left(double v) {
+  return __hiloint2double((int)from_left_u((uint32_t)__double2hiint(v)),
+                          (int)from_left_u((uint32_t)__double2loint(v)));
+}
+__device__ __forceinline__ double from_right(double v) {
+  return __hiloint2double((int)from_right_u((uint32_t)__double2hiint(v)),
+                          (int)from_right_u((uint32_t)__double2loint(v)));
+}
+
+// Offsets of rank r packed for runtime lookup: (dx+1) | (dy+1) << 2, 4 bits per rank.
+__host__ __device__ constexpr uint64_t rank_xy_table() {
+  uint64_t t = 0;
+  for (int r = 0; r < kCand; r++)
+    t |= (uint64_t)((rank_dx(r) + 1) | ((rank_dy(r) + 1) << 2)) << (4 * r);
+  return t;
+}
+
+// Running extremes over candidates visited in increasing rank order.
+template <typename T>
+struct Ext;
+
+template <>
+struct Ext<float> {  // staged keys
+  double mx, mn;
+  __device__ __forceinline__ void first(double k, int r) { mx = mn = k + (double)r; }
+  __device__ __forceinline__ void add(double k, int r) {
+    const double v = k + (double)r;
+    mx = fmax(mx, v);
+    mn = fmin(mn, v);
+  }
+  __device__ __forceinline__ void merge(const Ext &o) {  // o holds higher ranks
+    mx = fmax(mx, o.mx);
+    mn = fmin(mn, o.mn);
+  }
+  __device__ __forceinline__ int asc() const { return __double2loint(mx) & 15; }
+  __device__ __forceinline__ int desc() const { return __double2loint(mn) & 15; }
+  static __device__ __forceinline__ double stage(float v) { return stage_value(v); }
+};
+
+template <>
+struct Ext<double> {  // (value, rank) pairs
+  double mx, mn;
+  int rmx, rmn;
+  __device__ __forceinline__ void first(double v, int r) {
+    mx = mn = v;
+    rmx = rmn = r;
+  }
+  // b comes later in rank order: it wins a tie for the maximum, loses it for
+  // the minimum; NaN (invalid) never wins against a valid value
+  __device__ __forceinline__ void put(double v, int r) {
+    const bool anan = mx != mx;
+    const bool tmx = v >= mx || anan;
+    mx = tmx ? v : mx;
+    rmx = tmx ? r : rmx;
+    const bool tmn = v < mn || anan;
+    mn = tmn ? v : mn;
+    rmn = tmn ? r : rmn;
+  }
+  __device__ __forceinline__ void add(double v, int r) { put(v, r); }
+  __device__ __forceinline__ void merge(const Ext &o) {
+    const bool anan = mx != mx;
+    const bool tmx = o.mx >= mx || anan;
+    mx = tmx ? o.mx : mx;
+    rmx = tmx ? o.rmx : rmx;
+    const bool tmn = o.mn < mn || anan;
+    mn = tmn ? o.mn : mn;
+    rmn = tmn ? o.rmn : rmn;
+  }
+  __device__ __forceinline__ int asc() const { return rmx; }
+  __device__ __forceinline__ int desc() const { return rmn; }
+  static __device__ __forceinline__ double stage(double v) { return v; }
+};
+
+// One plane's staged column of one field, with the x-1 / x+1 neighbours.
+struct ColPlane {
+  double k[kColH], l[kColH], r[kColH];
+};
+
+__device__ __forceinline__ void shift_plane(ColPlane &c) {
+#pragma unroll
+  for (int h = 0; h < kColH; h++) {
+    c.l[h] = from_left(c.k[h]);
+    c.r[h] = from_right(c.k[h]);
+  }
+}
+
+// Partial extremes contributed by plane p to voxel row h (1..kColRows).
+template <typename T>
+__device__ __forceinline__ void partial_lo(const ColPlane &c, int h, Ext<T> &e) {
+  e.first(c.k[h - 1], 0);   // (x,   y-1, z-1)
+  e.add(c.r[h - 1], 1);     // (x+1, y-1, z-1)
+  e.add(c.k[h], 2);         // (x,   y,   z-1)
+  e.add(c.r[h], 3);         // (x+1, y,   z-1)
+}
+template <typename T>
+__device__ __forceinline__ void partial_mid(const ColPlane &c, int h, Ext<T> &e) {
+  e.first(c.k[h - 1], 4);   // (x,   y-1)
+  e.add(c.r[h - 1], 5);     // (x+1, y-1)
+  e.add(c.l[h], 6);         // (x-1, y)
+  e.add(c.k[h], 7);         // the voxel
+  e.add(c.r[h], 8);         // (x+1, y)
+  e.add(c.l[h + 1], 9);     // (x-1, y+1)
+  e.add(c.k[h + 1], 10);    // (x,   y+1)
+}
+template <typename T>
+__device__ __forceinline__ void partial_hi(const ColPlane &c, int h, Ext<T> &e) {
+  e.first(c.l[h], 11);      // (x-1, y,   z+1)
+  e.add(c.k[h], 12);        // (x,   y,   z+1)
+  e.add(c.l[h + 1], 13);    // (x-1, y+1, z+1)
+  e.add(c.k[h + 1], 14);    // (x,   y+1, z+1)
+}
+
+// Flush the completed target plane `pz` of this wave's column block: `m` holds,
+// per lane, bits (dx+1)*kColH + row for targets at (x+dx, y0-1+row, pz).
+__device__ __forceinline__ void flush_targets(uint32_t m, int64_t x0m1, int64_t y0m1, int64_t pz,
+                                              const Grid3 &d, uint32_t *tmark, uint32_t *tlist,
+                                              uint32_t *ntargets) {
+  const uint32_t lane = lane_id();
+  const uint32_t dxm = m & ((1u << kColH) - 1);                 // dx = -1
+  const uint32_t dx0 = (m >> kColH) & ((1u << kColH) - 1);      // dx =  0
+  const uint32_t dxp = (m >> (2 * kColH)) & ((1u << kColH) - 1);  // dx = +1
+  const uint32_t col = dx0 | from_left_u(dxp) | from_right_u(dxm);
+  uint64_t rowmask[kColH];
+  uint64_t any = 0;
+#pragma unroll
+  for (int h = 0; h < kColH; h++) {
+    rowmask[h] = __ballot((col >> h) & 1u);
+    any |= rowmask[h];
+  }
+  if (any == 0) return;  // uniform
+  // lanes 0..3*kColH-1: one (row, word) each
+  uint32_t claimed = 0;
+  int64_t wbase = 0;
+  if (lane < 3u * kColH) {
+    const int h = (int)lane / 3, kw = (int)lane % 3;
+    uint64_t b = 0;
+#pragma unroll
+    for (int q = 0; q < kColH; q++) b = q == h ? rowmask[q] : b;
+    const int64_t y = y0m1 + h;
+    if (b != 0 && y >= 0 && y < d.ny && pz >= 0 && pz < d.nz) {
+      const int64_t i0 = x0m1 + d.nx * y + d.sxy * pz;  // lane 0's cell (may be -1)
+      const int64_t w0 = i0 >> 5;                        // floor
+      const int s = (int)(i0 - (w0 << 5));
+      const uint64_t lo = b << s;
+      const uint64_t hi = s ? (b >> (64 - s)) : 0ull;
+      const uint32_t wm = kw == 0 ? (uint32_t)lo : (kw == 1 ? (uint32_t)(lo >> 32) : (uint32_t)hi);
+      if (wm) {
+        wbase = w0 + kw;
+        claimed = wm & ~atomicOr(&tmark[wbase], wm);
+      }
+    }
+  }
+  const uint32_t n = __popc(claimed);
+  const uint32_t incl = wave_incl_scan(n);
+  const uint32_t total = __shfl(incl, 63);
+  uint32_t base = 0;
+  if (lane == 0 && total) base = atomicAdd(ntargets, total);
+  base = __shfl(base, 0);
+  uint32_t pos = base + incl - n;
+  while (claimed) {
+    const int bit = __ffs(claimed) - 1;
+    claimed &= claimed - 1;
+    tlist[pos++] = (uint32_t)(wbase * 32 + bit);
+  }
+}
+
+template <typename T, bool kLabel, bool kDetect>
+__global__ __launch_bounds__(kColThreads) void k_columns(
+    const T *__restrict__ f, const T *__restrict__ g, uint8_t *__restrict__ code,
+    uint8_t *__restrict__ dc, uint32_t *__restrict__ tmark, uint32_t *__restrict__ tlist,
+    Counters *__restrict__ ring, uint32_t jslot_prev, uint32_t jslot, RunState *run, Grid3 d) {
+  if (!kLabel) {
+    if (ring[jslot_prev].mode != kModeDense) return;  // uniform
+  }
+  constexpr uint64_t kXY = rank_xy_table();
+  const double kNaN = __builtin_nan("");
+  const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
+  const uint32_t nx = (uint32_t)d.nx, ny = (uint32_t)d.ny, nz = (uint32_t)d.nz;
+  const uint32_t sxy = (uint32_t)d.sxy;
+  const uint32_t nux = (nx + kColOwned - 1) / kColOwned, nuy = (ny + kColRows - 1) / kColRows,
+                 nuz = (nz + kColZ - 1) / kColZ;
+  const uint32_t nunits = nux * nuy * nuz;
+  // XCD-aware split: blocks are dealt to the 8 XCDs by workgroup id; each XCD
+  // takes one contiguous eighth of the units, so neighbouring units share an L2
+  const uint32_t waves_per_xcd = (gridDim.x / 8) * kColWaves;
+  const uint32_t chunk = (nunits + 7) / 8;
+  const uint32_t u_begin = (blockIdx.x % 8) * chunk;
+  const uint32_t u_end = u_begin + chunk < nunits ? u_begin + chunk : nunits;
+  Counters *out = kDetect ? &ring[jslot] : nullptr;
+  uint32_t lc[3] = {0, 0, 0};
+  uint32_t lerr = 0;
+
+  for (uint32_t unit = u_begin + (blockIdx.x / 8) * kColWaves + wave; unit < u_end;
+       unit += waves_per_xcd) {
+    const uint32_t uz = unit / (nux * nuy), rem = unit - uz * nux * nuy;
+    const uint32_t uy = rem / nux, ux = rem - uy * nux;
+    const int x = (int)(ux * kColOwned) - 1 + (int)lane;  // this lane's x
+    const int y0m1 = (int)(uy * kColRows) - 1;             // y of loaded row 0
+    const int zs = (int)(uz * kColZ);
+    const int ze = zs + kColZ < (int)nz ? zs + kColZ : (int)nz;
+    const bool xok = x >= 0 && x < (int)nx;
+    const bool owned = lane >= 1 && lane <= (uint32_t)kColOwned && xok;
+    // in-plane offsets of the loaded rows, fixed for the unit
+    uint32_t roff[kColH];
+    bool rok[kColH];
+#pragma unroll
+    for (int h = 0; h < kColH; h++) {
+      const int y = y0m1 + h;
+      rok[h] = xok && y >= 0 && y < (int)ny;
+      roff[h] = rok[h] ? (uint32_t)x + nx * (uint32_t)y : 0u;
+    }
+    // raw values of the next plane, in flight
+    T nf[kColH], ng[kColH];
+    auto fetch = [&](int pz) {
+      const bool zok = pz >= 0 && pz < (int)nz;
+      const uint32_t zoff = zok ? sxy * (uint32_t)pz : 0u;
+#pragma unroll
+      for (int h = 0; h < kColH; h++) {
+        const uint32_t gi = (zok && rok[h]) ? zoff + roff[h] : 0u;
+        if (kLabel) nf[h] = f[gi];
+        if (kDetect) ng[h] = g[gi];
+      }
+    };
+    auto stage = [&](int pz, const T (&raw)[kColH], ColPlane &c, bool check) {
+      const bool zok = pz >= 0 && pz < (int)nz;
+#pragma unroll
+      for (int h = 0; h < kColH; h++) {
+        const bool ok = zok && rok[h];
+        const T v = raw[h];
+        if (check && ok && (!(v == v) || v == sentinel<T>())) lerr |= 1u;
+        c.k[h] = ok ? Ext<T>::stage(v) : kNaN;
+      }
+      shift_plane(c);
+    };
+    uint8_t ccode[kColRows], ncode[kColRows];  // codes of the plane to emit / of the next
+    auto fetch_codes = [&](int pz) {
+#pragma unroll
+      for (int h = 0; h < kColRows; h++) {
+        const int y = y0m1 + 1 + h;
+        const bool ok = owned && y < (int)ny && pz >= 0 && pz < (int)nz;
+        ncode[h] = code[ok ? (uint32_t)x + nx * (uint32_t)y + sxy * (uint32_t)pz : 0u];
+      }
+    };
+
+    Ext<T> lo_f[kColRows], part_f[kColRows], lo_g[kColRows], part_g[kColRows];
+#pragma unroll
+    for (int h = 0; h < kColRows; h++) {
+      lo_f[h].first(kNaN, 0);
+      lo_g[h].first(kNaN, 0);
+    }
+    uint32_t acc0 = 0, acc1 = 0, acc2 = 0;  // target masks of planes e-1, e, e+1
+    fetch(zs - 1);
+    for (int p = zs - 1; p <= ze; p++) {
+      ColPlane cf, cg;
+      if (kLabel) stage(p, nf, cf, true);
+      if (kDetect) stage(p, ng, cg, kLabel);
+      if (!kLabel) {
+#pragma unroll
+        for (int h = 0; h < kColRows; h++) ccode[h] = ncode[h];
+      }
+      if (p < ze) {
+        fetch(p + 1);
+        if (!kLabel) fetch_codes(p);
+      }
+      const int e = p - 1;  // plane completed at this step
+      const bool emit = e >= zs;
+#pragma unroll
+      for (int h = 1; h <= kColRows; h++) {
+        Ext<T> hf, hg, mf, mg, lf, lg;
+        if (emit) {
+          if (kLabel) {
+            partial_hi<T>(cf, h, hf);
+            part_f[h - 1].merge(hf);
+          }
+          if (kDetect) {
+            partial_hi<T>(cg, h, hg);
+            part_g[h - 1].merge(hg);
+          }
+          const int y = y0m1 + h;
+          if (owned && y < (int)ny) {
+            const uint32_t i = (uint32_t)x + nx * (uint32_t)y + sxy * (uint32_t)e;
+            int oa, od;
+            if (kLabel) {
+              oa = part_f[h - 1].asc();
+              od = part_f[h - 1].desc();
+              code[i] = (uint8_t)(oa | (od << 4));
+              if (kDetect) dc[i] = 0;
+            } else {
+              oa = ccode[h - 1] & 15;
+              od = ccode[h - 1] >> 4;
+            }
+            if (kDetect) {
+              int tr[4];
+              const int nt =
+                  detect_targets(oa, od, part_g[h - 1].asc(), part_g[h - 1].desc(), tr, lc);
+              for (int q = 0; q < nt; q++) {
+                const int r = tr[q];
+                const uint32_t xy = (uint32_t)(kXY >> (4 * r)) & 15u;
+                const uint32_t bit = (xy & 3u) * kColH + (uint32_t)h + (xy >> 2) - 1u;
+                const uint32_t mb = 1u << bit;
+                if (r < 4) acc0 |= mb;
+                else if (r < 11) acc1 |= mb;
+                else acc2 |= mb;
+              }
+            }
+          }
+        }
+        // partials of plane p for the voxels of planes p and p+1
+        if (kLabel) {
+          partial_mid<T>(cf, h, mf);
+          partial_lo<T>(cf, h, lf);
+          part_f[h - 1] = lo_f[h - 1];
+          part_f[h - 1].merge(mf);
+          lo_f[h - 1] = lf;
+        }
+        if (kDetect) {
+          partial_mid<T>(cg, h, mg);
+          partial_lo<T>(cg, h, lg);
+          part_g[h - 1] = lo_g[h - 1];
+          part_g[h - 1].merge(mg);
+          lo_g[h - 1] = lg;
+        }
+      }
+      if (kDetect && emit) {
+        // plane e-1 can receive no more targets
+        flush_targets(acc0, (int64_t)x - (int64_t)lane, y0m1, (int64_t)e - 1, d, tmark, tlist,
+                      &out->ntargets);
+        acc0 = acc1;
+        acc1 = acc2;
+        acc2 = 0;
+      }
+    }
+    if (kDetect) {
+      flush_targets(acc0, (int64_t)x - (int64_t)lane, y0m1, (int64_t)ze - 1, d, tmark, tlist,
+                    &out->ntargets);
+      flush_targets(acc1, (int64_t)x - (int64_t)lane, y0m1, (int64_t)ze, d, tmark, tlist,
+                    &out->ntargets);
+    }
+  }
+  const uint64_t em = __ballot(lerr != 0);
+  if (kLabel && em && lane == 0) atomicOr(&run->err, 1u);
+  if (!kDetect) return;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const uint32_t s = wave_sum(lc[c]);
+    if (lane == 0 && s) atomicAdd(&out->cnt[c], s);
+  }
+  if (threadIdx.x == 0 && blockIdx.x == 0 && !kLabel) atomicAdd(&run->dense_passes, 1ull);
+}
+
+}  // namespace pmsz

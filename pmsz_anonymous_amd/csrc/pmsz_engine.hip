@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "../../include/pmsz.h"
+#include "pmsz_dense.h"
 #include "pmsz_device.h"
 
 namespace pmsz {
@@ -722,6 +723,17 @@ unsigned tile_blocks(const Ctx &c, const Grid3 &d) {
   return (unsigned)(8 * std::max<int64_t>(1, std::min(per_xcd, cap)));
 }
 
+// Persistent grid of the column kernel: one wave per work unit up to about the
+// resident capacity, a multiple of 8 blocks (one share per XCD).
+unsigned col_blocks(const Ctx &c, const Grid3 &d) {
+  const int64_t units = ((d.nx + kColOwned - 1) / kColOwned) * ((d.ny + kColRows - 1) / kColRows) *
+                        ((d.nz + kColZ - 1) / kColZ);
+  const int64_t blocks = (units + kColWaves - 1) / kColWaves;
+  const int64_t per_xcd = (blocks + 7) / 8;
+  const int64_t cap = std::max<int64_t>(1, (int64_t)c.cus * 8 / 8);
+  return (unsigned)(8 * std::max<int64_t>(1, std::min(per_xcd, cap)));
+}
+
 EditParams edit_params(const pmsz_params *p) {
   EditParams ep;
   ep.eb = p->abs_eb;
@@ -758,7 +770,7 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
   PMSZ_HIP(hipMemcpyAsync(&c.ring[0].mode, &kRunning, sizeof(uint32_t), hipMemcpyHostToDevice,
                           c.stream));
   PMSZ_HIP(hipEventRecord(c.ev2, c.stream));
-  k_tiles<T, true, true><<<tile_blocks(c, d), kThreads, 0, c.stream>>>(
+  k_columns<T, true, true><<<col_blocks(c, d), kColThreads, 0, c.stream>>>(
       f, g, c.code, c.dc, c.tmark, c.tlist, c.ring, 0, 1, c.run, d);
   PMSZ_HIP(hipGetLastError());
   PMSZ_HIP(hipEventRecord(c.ev3, c.stream));
@@ -769,7 +781,7 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
       k_apply<T><<<grid_list, kApplyThreads, 0, c.stream>>>(g, f, c.dc, c.tmark, c.tlist,
                                                             c.wmark, c.wlist, c.ring, c.run, j,
                                                             d, ep, s);
-      k_tiles<T, false, true><<<tile_blocks(c, d), kThreads, 0, c.stream>>>(
+      k_columns<T, false, true><<<col_blocks(c, d), kColThreads, 0, c.stream>>>(
           nullptr, g, c.code, nullptr, c.tmark, c.tlist, c.ring, j % kCtrRing,
           (j + 1) % kCtrRing, c.run, d);
       k_detect_list<T><<<grid_list, kApplyThreads, 0, c.stream>>>(
@@ -855,7 +867,7 @@ int time_first_pass(Ctx &c, const pmsz_params *p, const T *f, const T *g, int it
     PMSZ_HIP(hipMemsetAsync(c.tmark, 0, sizeof(uint32_t) * words, c.stream));
     PMSZ_HIP(hipMemsetAsync(c.ring, 0, sizeof(Counters) * 2, c.stream));
     PMSZ_HIP(hipEventRecord(c.ev0, c.stream));
-    k_tiles<T, true, true><<<tile_blocks(c, d), kThreads, 0, c.stream>>>(
+    k_columns<T, true, true><<<col_blocks(c, d), kColThreads, 0, c.stream>>>(
         f, g, c.code, c.dc, c.tmark, c.tlist, c.ring, 0, 1, c.run, d);
     PMSZ_HIP(hipGetLastError());
     PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
@@ -961,11 +973,11 @@ int pmsz_codes(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_fiel
   if (rc) return rc;
   PMSZ_HIP(hipMemsetAsync(c.run, 0, sizeof(RunState), c.stream));
   if (dtype == PMSZ_F32)
-    k_tiles<float, true, false><<<tile_blocks(c, d), kThreads, 0, c.stream>>>(
+    k_columns<float, true, false><<<col_blocks(c, d), kColThreads, 0, c.stream>>>(
         (const float *)d_field, nullptr, d_codes, nullptr, nullptr, nullptr, c.ring, 0, 0, c.run,
         d);
   else
-    k_tiles<double, true, false><<<tile_blocks(c, d), kThreads, 0, c.stream>>>(
+    k_columns<double, true, false><<<col_blocks(c, d), kColThreads, 0, c.stream>>>(
         (const double *)d_field, nullptr, d_codes, nullptr, nullptr, nullptr, c.ring, 0, 0,
         c.run, d);
   PMSZ_HIP(hipGetLastError());
