@@ -161,13 +161,15 @@ __device__ __forceinline__ void partial_hi(const ColPlane &c, int h, Ext<T> &e) 
 }
 
 // Flush the completed target plane `pz` of this wave's column block: `m` holds,
-// per lane, bits (dx+1)*kColH + row for targets at (x+dx, y0-1+row, pz).
+// per lane, bits (dx+1)*kColH + row for targets at (x+dx, y0-1+row, pz).  The
+// row masks are OR-ed into the bitmap without waiting for the old words (the
+// target list is compacted from the bitmap afterwards), so no wave stalls on
+// the atomics.
 __device__ __forceinline__ void flush_targets(uint32_t m, int64_t x0m1, int64_t y0m1, int64_t pz,
-                                              const Grid3 &d, uint32_t *tmark, uint32_t *tlist,
-                                              uint32_t *ntargets) {
+                                              const Grid3 &d, uint32_t *tmark) {
   const uint32_t lane = lane_id();
-  const uint32_t dxm = m & ((1u << kColH) - 1);                 // dx = -1
-  const uint32_t dx0 = (m >> kColH) & ((1u << kColH) - 1);      // dx =  0
+  const uint32_t dxm = m & ((1u << kColH) - 1);                   // dx = -1
+  const uint32_t dx0 = (m >> kColH) & ((1u << kColH) - 1);        // dx =  0
   const uint32_t dxp = (m >> (2 * kColH)) & ((1u << kColH) - 1);  // dx = +1
   const uint32_t col = dx0 | from_left_u(dxp) | from_right_u(dxm);
   uint64_t rowmask[kColH];
@@ -179,8 +181,6 @@ __device__ __forceinline__ void flush_targets(uint32_t m, int64_t x0m1, int64_t 
   }
   if (any == 0) return;  // uniform
   // lanes 0..3*kColH-1: one (row, word) each
-  uint32_t claimed = 0;
-  int64_t wbase = 0;
   if (lane < 3u * kColH) {
     const int h = (int)lane / 3, kw = (int)lane % 3;
     uint64_t b = 0;
@@ -194,23 +194,44 @@ __device__ __forceinline__ void flush_targets(uint32_t m, int64_t x0m1, int64_t 
       const uint64_t lo = b << s;
       const uint64_t hi = s ? (b >> (64 - s)) : 0ull;
       const uint32_t wm = kw == 0 ? (uint32_t)lo : (kw == 1 ? (uint32_t)(lo >> 32) : (uint32_t)hi);
-      if (wm) {
-        wbase = w0 + kw;
-        claimed = wm & ~atomicOr(&tmark[wbase], wm);
-      }
+      if (wm) __hip_atomic_fetch_or(&tmark[w0 + kw], wm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
-  const uint32_t n = __popc(claimed);
-  const uint32_t incl = wave_incl_scan(n);
-  const uint32_t total = __shfl(incl, 63);
-  uint32_t base = 0;
-  if (lane == 0 && total) base = atomicAdd(ntargets, total);
-  base = __shfl(base, 0);
-  uint32_t pos = base + incl - n;
-  while (claimed) {
-    const int bit = __ffs(claimed) - 1;
-    claimed &= claimed - 1;
-    tlist[pos++] = (uint32_t)(wbase * 32 + bit);
+}
+
+// Target list from the target bitmap after a full-volume detection: every set
+// bit is one distinct target.  Each block takes kCompactWords words per step and
+// reserves its list range with one atomic.
+constexpr int kCompactThreads = 256;
+constexpr int kCompactPT = 8;  // words per thread per step
+__global__ __launch_bounds__(kCompactThreads) void k_compact(const uint32_t *__restrict__ tmark,
+                                                             uint32_t *__restrict__ tlist,
+                                                             Counters *__restrict__ ring,
+                                                             uint32_t jslot_prev, uint32_t jslot,
+                                                             bool always, uint32_t nwords) {
+  if (!always && ring[jslot_prev].mode != kModeDense) return;  // uniform
+  uint32_t *counter = &ring[jslot].ntargets;
+  const uint32_t step = gridDim.x * kCompactThreads * kCompactPT;
+  for (uint32_t base = blockIdx.x * kCompactThreads * kCompactPT; base < nwords; base += step) {
+    uint32_t w[kCompactPT];
+    uint32_t n = 0;
+#pragma unroll
+    for (int q = 0; q < kCompactPT; q++) {
+      const uint32_t i = base + (uint32_t)q * kCompactThreads + threadIdx.x;
+      w[q] = i < nwords ? tmark[i] : 0u;
+      n += __popc(w[q]);
+    }
+    uint32_t pos = block_reserve<kCompactThreads>(n, counter);
+#pragma unroll
+    for (int q = 0; q < kCompactPT; q++) {
+      const uint32_t i = base + (uint32_t)q * kCompactThreads + threadIdx.x;
+      uint32_t m = w[q];
+      while (m) {
+        const int b = __ffs(m) - 1;
+        m &= m - 1;
+        tlist[pos++] = i * 32u + (uint32_t)b;
+      }
+    }
   }
 }
 
@@ -373,18 +394,15 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
       }
       if (kDetect && emit) {
         // plane e-1 can receive no more targets
-        flush_targets(acc0, (int64_t)x - (int64_t)lane, y0m1, (int64_t)e - 1, d, tmark, tlist,
-                      &out->ntargets);
+        flush_targets(acc0, (int64_t)x - (int64_t)lane, y0m1, (int64_t)e - 1, d, tmark);
         acc0 = acc1;
         acc1 = acc2;
         acc2 = 0;
       }
     }
     if (kDetect) {
-      flush_targets(acc0, (int64_t)x - (int64_t)lane, y0m1, (int64_t)ze - 1, d, tmark, tlist,
-                    &out->ntargets);
-      flush_targets(acc1, (int64_t)x - (int64_t)lane, y0m1, (int64_t)ze, d, tmark, tlist,
-                    &out->ntargets);
+      flush_targets(acc0, (int64_t)x - (int64_t)lane, y0m1, (int64_t)ze - 1, d, tmark);
+      flush_targets(acc1, (int64_t)x - (int64_t)lane, y0m1, (int64_t)ze, d, tmark);
     }
   }
   const uint64_t em = __ballot(lerr != 0);

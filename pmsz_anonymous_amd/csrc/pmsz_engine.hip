@@ -723,6 +723,11 @@ unsigned tile_blocks(const Ctx &c, const Grid3 &d) {
   return (unsigned)(8 * std::max<int64_t>(1, std::min(per_xcd, cap)));
 }
 
+unsigned compact_blocks(const Ctx &c, int64_t words) {
+  const int64_t per = (int64_t)kCompactThreads * kCompactPT;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((words + per - 1) / per, c.cus * 4));
+}
+
 // Persistent grid of the column kernel: one wave per work unit up to about the
 // resident capacity, a multiple of 8 blocks (one share per XCD).
 unsigned col_blocks(const Ctx &c, const Grid3 &d) {
@@ -774,6 +779,8 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
       f, g, c.code, c.dc, c.tmark, c.tlist, c.ring, 0, 1, c.run, d);
   PMSZ_HIP(hipGetLastError());
   PMSZ_HIP(hipEventRecord(c.ev3, c.stream));
+  k_compact<<<compact_blocks(c, words), kCompactThreads, 0, c.stream>>>(c.tmark, c.tlist, c.ring,
+                                                                        0, 1, true, (uint32_t)words);
 
   uint32_t j0 = 1;
   while (true) {
@@ -784,6 +791,8 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
       k_columns<T, false, true><<<col_blocks(c, d), kColThreads, 0, c.stream>>>(
           nullptr, g, c.code, nullptr, c.tmark, c.tlist, c.ring, j % kCtrRing,
           (j + 1) % kCtrRing, c.run, d);
+      k_compact<<<compact_blocks(c, words), kCompactThreads, 0, c.stream>>>(
+          c.tmark, c.tlist, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false, (uint32_t)words);
       k_detect_list<T><<<grid_list, kApplyThreads, 0, c.stream>>>(
           g, c.code, c.tmark, c.tlist, c.wmark, c.wlist, c.ring, c.run, j, d);
     }
@@ -870,6 +879,8 @@ int time_first_pass(Ctx &c, const pmsz_params *p, const T *f, const T *g, int it
     k_columns<T, true, true><<<col_blocks(c, d), kColThreads, 0, c.stream>>>(
         f, g, c.code, c.dc, c.tmark, c.tlist, c.ring, 0, 1, c.run, d);
     PMSZ_HIP(hipGetLastError());
+    k_compact<<<compact_blocks(c, words), kCompactThreads, 0, c.stream>>>(
+        c.tmark, c.tlist, c.ring, 0, 1, true, (uint32_t)words);
     PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
     PMSZ_HIP(hipStreamSynchronize(c.stream));
     float ms = 0;
