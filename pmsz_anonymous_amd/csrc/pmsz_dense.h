@@ -161,41 +161,24 @@ __device__ __forceinline__ void partial_hi(const ColPlane &c, int h, Ext<T> &e) 
 }
 
 // Flush the completed target plane `pz` of this wave's column block: `m` holds,
-// per lane, bits (dx+1)*kColH + row for targets at (x+dx, y0-1+row, pz).  The
-// row masks are OR-ed into the bitmap without waiting for the old words (the
-// target list is compacted from the bitmap afterwards), so no wave stalls on
-// the atomics.
-__device__ __forceinline__ void flush_targets(uint32_t m, int64_t x0m1, int64_t y0m1, int64_t pz,
+// per lane, bits (dx+1)*kColH + row for targets at (x+dx, y0-1+row, pz).  After
+// the exchange with the neighbouring lanes each lane holds the target bits of its
+// own x and ORs each one into the bitmap without waiting for the old word (the
+// target list is compacted from the bitmap afterwards).
+__device__ __forceinline__ void flush_targets(uint32_t m, int64_t xl, int64_t y0m1, int64_t pz,
                                               const Grid3 &d, uint32_t *tmark) {
-  const uint32_t lane = lane_id();
   const uint32_t dxm = m & ((1u << kColH) - 1);                   // dx = -1
   const uint32_t dx0 = (m >> kColH) & ((1u << kColH) - 1);        // dx =  0
   const uint32_t dxp = (m >> (2 * kColH)) & ((1u << kColH) - 1);  // dx = +1
-  const uint32_t col = dx0 | from_left_u(dxp) | from_right_u(dxm);
-  uint64_t rowmask[kColH];
-  uint64_t any = 0;
-#pragma unroll
-  for (int h = 0; h < kColH; h++) {
-    rowmask[h] = __ballot((col >> h) & 1u);
-    any |= rowmask[h];
-  }
-  if (any == 0) return;  // uniform
-  // lanes 0..3*kColH-1: one (row, word) each
-  if (lane < 3u * kColH) {
-    const int h = (int)lane / 3, kw = (int)lane % 3;
-    uint64_t b = 0;
-#pragma unroll
-    for (int q = 0; q < kColH; q++) b = q == h ? rowmask[q] : b;
-    const int64_t y = y0m1 + h;
-    if (b != 0 && y >= 0 && y < d.ny && pz >= 0 && pz < d.nz) {
-      const int64_t i0 = x0m1 + d.nx * y + d.sxy * pz;  // lane 0's cell (may be -1)
-      const int64_t w0 = i0 >> 5;                        // floor
-      const int s = (int)(i0 - (w0 << 5));
-      const uint64_t lo = b << s;
-      const uint64_t hi = s ? (b >> (64 - s)) : 0ull;
-      const uint32_t wm = kw == 0 ? (uint32_t)lo : (kw == 1 ? (uint32_t)(lo >> 32) : (uint32_t)hi);
-      if (wm) __hip_atomic_fetch_or(&tmark[w0 + kw], wm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+  uint32_t col = dx0 | from_left_u(dxp) | from_right_u(dxm);
+  // a set bit is a target, so its cell lies in the domain
+  const int64_t base = xl + d.nx * y0m1 + d.sxy * pz;
+  while (col) {
+    const int h = __ffs(col) - 1;
+    col &= col - 1;
+    const uint64_t i = (uint64_t)(base + d.nx * h);
+    __hip_atomic_fetch_or(&tmark[i >> 5], 1u << (i & 31u), __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -280,26 +263,30 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
       rok[h] = xok && y >= 0 && y < (int)ny;
       roff[h] = rok[h] ? (uint32_t)x + nx * (uint32_t)y : 0u;
     }
-    // raw values of the next plane, in flight
+    // raw values of the next plane, in flight; the plane's validity is uniform
+    // over the wave, so an out-of-domain plane is a branch, not a select
     T nf[kColH], ng[kColH];
     auto fetch = [&](int pz) {
-      const bool zok = pz >= 0 && pz < (int)nz;
-      const uint32_t zoff = zok ? sxy * (uint32_t)pz : 0u;
+      if (pz < 0 || pz >= (int)nz) return;
+      const uint32_t zoff = sxy * (uint32_t)pz;
 #pragma unroll
       for (int h = 0; h < kColH; h++) {
-        const uint32_t gi = (zok && rok[h]) ? zoff + roff[h] : 0u;
+        const uint32_t gi = zoff + roff[h];
         if (kLabel) nf[h] = f[gi];
         if (kDetect) ng[h] = g[gi];
       }
     };
     auto stage = [&](int pz, const T (&raw)[kColH], ColPlane &c, bool check) {
-      const bool zok = pz >= 0 && pz < (int)nz;
+      if (pz < 0 || pz >= (int)nz) {
+#pragma unroll
+        for (int h = 0; h < kColH; h++) c.k[h] = c.l[h] = c.r[h] = kNaN;
+        return;
+      }
 #pragma unroll
       for (int h = 0; h < kColH; h++) {
-        const bool ok = zok && rok[h];
         const T v = raw[h];
-        if (check && ok && (!(v == v) || v == sentinel<T>())) lerr |= 1u;
-        c.k[h] = ok ? Ext<T>::stage(v) : kNaN;
+        if (check && rok[h] && (!(v == v) || v == sentinel<T>())) lerr |= 1u;
+        c.k[h] = rok[h] ? Ext<T>::stage(v) : kNaN;
       }
       shift_plane(c);
     };
@@ -394,15 +381,15 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
       }
       if (kDetect && emit) {
         // plane e-1 can receive no more targets
-        flush_targets(acc0, (int64_t)x - (int64_t)lane, y0m1, (int64_t)e - 1, d, tmark);
+        flush_targets(acc0, (int64_t)x, y0m1, (int64_t)e - 1, d, tmark);
         acc0 = acc1;
         acc1 = acc2;
         acc2 = 0;
       }
     }
     if (kDetect) {
-      flush_targets(acc0, (int64_t)x - (int64_t)lane, y0m1, (int64_t)ze - 1, d, tmark);
-      flush_targets(acc1, (int64_t)x - (int64_t)lane, y0m1, (int64_t)ze, d, tmark);
+      flush_targets(acc0, (int64_t)x, y0m1, (int64_t)ze - 1, d, tmark);
+      flush_targets(acc1, (int64_t)x, y0m1, (int64_t)ze, d, tmark);
     }
   }
   const uint64_t em = __ballot(lerr != 0);
