@@ -21,6 +21,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+# Algorithmic HBM bytes per voxel of the fused first pass (labeling + first
+# detection sweep): read f and g (4 B each for f32), write the code byte and
+# the cleared delta counter (1 B each).
+PASS1_BYTES_PER_VOXEL = 2 * 4 + 1 + 1
+# Committed rocprofv3 --pmc summary of the same kernel on the same workload;
+# its "traffic_bytes_per_launch" line is FETCH_SIZE + WRITE_SIZE, each scaled
+# by what the counter reads for a device copy of known size.
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_columns_pass1_512cube_f32_pmc.md")
 
 
 def log(*a):
@@ -47,6 +55,23 @@ def cpu_baseline(f, g0, eb, edge):
         "sample": f"first {edge}^3 block of the same field and eb, oracle correction, "
                   f"{dt:.2f} s, {st['edit_sweeps']} sweeps, rc={rc}",
     }
+
+
+def pmc_traffic(nvox):
+    """HBM bytes per launch of the first pass from the committed --pmc summary
+    (None when the summary is absent or was taken on another size)."""
+    try:
+        with open(PMC_SUMMARY) as fh:
+            kv = {}
+            for line in fh:
+                if "=" in line and line.lstrip().startswith("- "):
+                    k, v = line.lstrip()[2:].split("=", 1)
+                    kv[k.strip()] = v.strip().split()[0]
+        if int(kv.get("voxels", -1)) != nvox:
+            return None
+        return int(kv["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def main():
@@ -97,8 +122,9 @@ def main():
     ms = elapsed / args.steps * 1e3
     s0 = stats[-1]
     dense_s = float(np.mean([s["t_dense_s"] for s in stats]))
-    dense_bytes = nvox * (2 * 4 + 2)
+    dense_bytes = nvox * PASS1_BYTES_PER_VOXEL
     achieved = dense_bytes / dense_s / 1e9
+    traffic = pmc_traffic(nvox)
     # full check of one result against the original's codes (topology preserved)
     ok = torch.equal(engine.codes(tf), engine.codes(copies[-1]))
     res = {
@@ -128,13 +154,14 @@ def main():
                        "sparse_voxels": s0["sparse_voxels"], "topology_preserved": bool(ok),
                        "device_ms": s0["t_total_s"] * 1e3},
         "roofline": {
-            "kernel": "k_dense_pass<float,true> (labeling + first sweep)",
+            "kernel": "k_columns<float,true,true> (labeling + first detection sweep)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if traffic else None,
             "kernel_ms": dense_s * 1e3,
             "bytes_per_launch": dense_bytes,
         },

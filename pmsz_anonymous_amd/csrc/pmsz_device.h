@@ -9,11 +9,42 @@
 
 namespace pmsz {
 
+// Unsigned 32-bit division by an invariant divisor through one multiply-high
+// (Granlund & Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, the round-up variant valid for every 32-bit n).
+struct FastDiv {
+  uint32_t m;
+  int sh1, sh2;
+  __host__ void init(uint32_t d) {
+    int l = 0;
+    while (((uint64_t)1 << l) < d) l++;
+    m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << l) - d)) / d + 1);
+    sh1 = l < 1 ? l : 1;
+    sh2 = l - 1 > 0 ? l - 1 : 0;
+  }
+  __device__ __forceinline__ uint32_t div(uint32_t n) const {
+    const uint32_t t = __umulhi(m, n);
+    return (t + ((n - t) >> sh1)) >> sh2;
+  }
+};
+
 struct Grid3 {
   int64_t nx, ny, nz;
   int64_t sxy;  // nx*ny
   int64_t n;    // nx*ny*nz
+  FastDiv dnx, dsxy;  // division by nx and by nx*ny (32-bit voxel indices)
 };
+
+// (x, y, z) of a 32-bit voxel index.
+__device__ __forceinline__ void decompose(uint32_t v, const Grid3 &d, int64_t &x, int64_t &y,
+                                          int64_t &z) {
+  const uint32_t zz = d.dsxy.div(v);
+  const uint32_t r = v - zz * (uint32_t)d.sxy;
+  const uint32_t yy = d.dnx.div(r);
+  x = (int64_t)(r - yy * (uint32_t)d.nx);
+  y = (int64_t)yy;
+  z = (int64_t)zz;
+}
 
 // Next-sweep decision written by the apply of sweep j into its counter block.
 enum : uint32_t {

@@ -333,7 +333,8 @@ __device__ __forceinline__ void apply_one(T *g, const T *f, uint8_t *dc, uint32_
 // Claims the closed stencil neighbourhood of t in wmark; returns the new voxels.
 __device__ __forceinline__ int queue_neighbourhood(uint32_t t, uint32_t *wmark, const Grid3 &d,
                                                    uint32_t (&u)[kCand]) {
-  int64_t x = (int64_t)t % d.nx, y = ((int64_t)t / d.nx) % d.ny, z = (int64_t)t / d.sxy;
+  int64_t x, y, z;
+  decompose(t, d, x, y, z);
   const uint32_t vm = valid_mask(x, y, z, d);
   // issue all claims before looking at any result, so the atomics overlap
   uint32_t old[kCand];
@@ -356,7 +357,8 @@ template <typename T>
 __device__ __forceinline__ int detect_one(const T *g, const uint8_t *code, uint32_t *tmark,
                                           uint32_t v, const Grid3 &d, uint32_t (&lc)[3],
                                           uint32_t (&out)[4]) {
-  int64_t x = (int64_t)v % d.nx, y = ((int64_t)v / d.nx) % d.ny, z = (int64_t)v / d.sxy;
+  int64_t x, y, z;
+  decompose(v, d, x, y, z);
   uint32_t vm = valid_mask(x, y, z, d);
   T cg[kCand];
 #pragma unroll
@@ -708,6 +710,8 @@ Grid3 grid_of(int64_t nx, int64_t ny, int64_t nz) {
   d.nz = nz;
   d.sxy = nx * ny;
   d.n = nx * ny * nz;
+  d.dnx.init((uint32_t)std::min<int64_t>(nx, 0xffffffffLL));
+  d.dsxy.init((uint32_t)std::min<int64_t>(d.sxy, 0xffffffffLL));
   return d;
 }
 
