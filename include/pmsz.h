@@ -91,6 +91,22 @@ void pmsz_default_params(pmsz_params *p);
 int pmsz_correct(const pmsz_params *p, const void *d_f, void *d_g, uint8_t *d_dc,
                  pmsz_stats *stats);
 
+/* With sub = (PX, PY, PZ) != (1, 1, 1), pmsz_correct splits the field into that
+ * grid of blocks (pMSz.cu:2690-2711), one padded subdomain block each, and runs
+ * the reference's per-rank loops with the ghost synchronisation between them:
+ * the lazy outer loop (pMSz.cu:1943-2060) or the per-iteration one
+ * (sync_pMSz.cu:2102-2200), the initial ghost values of exchange_ghost_layers
+ * (pMSz.cu:320-608) and the delta-counter exchange (pMSz.cu:909-1272).  All
+ * blocks live in this GPU's memory. */
+
+/* Batch: n independent fields of one dtype, mode and max_sweeps (each with its
+ * own extent, bound, q and clamp mode), corrected together; field k's result and
+ * stats[k] equal those of pmsz_correct on that field alone.  The reference runs
+ * one field per process (main, pMSz.cu:2596-3188); this is its many-field form.
+ * d_dc may be NULL, as may any of its entries.  sub must be 1,1,1. */
+int pmsz_correct_batch(int n, const pmsz_params *params, const void *const *d_f,
+                       void *const *d_g, uint8_t *const *d_dc, pmsz_stats *stats);
+
 /* The same with host arrays: copies in, corrects, copies the result back
  * (the H2D / D2H copies of main, pMSz.cu:2870-2900, 2971). */
 int pmsz_correct_host(const pmsz_params *p, const void *h_f, void *h_g, uint8_t *h_dc,

@@ -7,6 +7,6 @@ Modules:
   build    in-tree build recipe of the native libraries
 """
 from . import engine, fieldio  # noqa: F401
-from .engine import PmszError, correct, codes, labels, edit_symbols  # noqa: F401
+from .engine import PmszError, correct, correct_batch, codes, labels, edit_symbols  # noqa: F401
 
 __all__ = ["engine", "fieldio", "PmszError", "correct", "codes", "labels", "edit_symbols"]

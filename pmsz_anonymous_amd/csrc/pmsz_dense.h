@@ -218,22 +218,37 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(const uint32_t *__r
   }
 }
 
-template <typename T, bool kLabel, bool kDetect>
+// Units (62 x 4 x 64 column blocks) of one domain's box.
+__host__ inline void col_units(int64_t bx, int64_t by, int64_t bz, uint32_t &nux, uint32_t &nuy,
+                               uint32_t &nuz) {
+  nux = (uint32_t)((bx + kColOwned - 1) / kColOwned);
+  nuy = (uint32_t)((by + kColRows - 1) / kColRows);
+  nuz = (uint32_t)((bz + kColZ - 1) / kColZ);
+}
+
+// The full-volume kernel.  With kLabel the original labeling of f (codes), with
+// kDetect a detection sweep on g (detection 1 when both are set; a later dense
+// detection reading the codes when only kDetect is set).  kMulti: the units of
+// all domains of the run, domain state in ds (par = parity of the detection).
+template <typename T, bool kLabel, bool kDetect, bool kMulti>
 __global__ __launch_bounds__(kColThreads) void k_columns(
-    const T *__restrict__ f, const T *__restrict__ g, uint8_t *__restrict__ code,
-    uint8_t *__restrict__ dc, uint32_t *__restrict__ tmark, uint32_t *__restrict__ tlist,
-    Counters *__restrict__ ring, uint32_t jslot_prev, uint32_t jslot, RunState *run, Grid3 d) {
+    Dom d0, const Dom *__restrict__ doms, uint32_t ndom, DomState *__restrict__ ds,
+    uint32_t *__restrict__ tmark, Counters *__restrict__ ring, uint32_t jslot_prev,
+    uint32_t jslot, uint32_t par, RunState *run) {
   if (!kLabel) {
     if (ring[jslot_prev].mode != kModeDense) return;  // uniform
   }
   constexpr uint64_t kXY = rank_xy_table();
   const double kNaN = __builtin_nan("");
-  const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
-  const uint32_t nx = (uint32_t)d.nx, ny = (uint32_t)d.ny, nz = (uint32_t)d.nz;
-  const uint32_t sxy = (uint32_t)d.sxy;
-  const uint32_t nux = (nx + kColOwned - 1) / kColOwned, nuy = (ny + kColRows - 1) / kColRows,
-                 nuz = (nz + kColZ - 1) / kColZ;
-  const uint32_t nunits = nux * nuy * nuz;
+  const uint32_t lane = lane_id();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  uint32_t nunits;
+  if (kMulti) {
+    const Dom &L = doms[ndom - 1];
+    nunits = L.unit_base + L.nux * L.nuy * L.nuz;
+  } else {
+    nunits = d0.nux * d0.nuy * d0.nuz;
+  }
   // XCD-aware split: blocks are dealt to the 8 XCDs by workgroup id; each XCD
   // takes one contiguous eighth of the units, so neighbouring units share an L2
   const uint32_t waves_per_xcd = (gridDim.x / 8) * kColWaves;
@@ -246,14 +261,40 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
 
   for (uint32_t unit = u_begin + (blockIdx.x / 8) * kColWaves + wave; unit < u_end;
        unit += waves_per_xcd) {
-    const uint32_t uz = unit / (nux * nuy), rem = unit - uz * nux * nuy;
-    const uint32_t uy = rem / nux, ux = rem - uy * nux;
-    const int x = (int)(ux * kColOwned) - 1 + (int)lane;  // this lane's x
-    const int y0m1 = (int)(uy * kColRows) - 1;             // y of loaded row 0
-    const int zs = (int)(uz * kColZ);
-    const int ze = zs + kColZ < (int)nz ? zs + kColZ : (int)nz;
+    uint32_t dk = 0;
+    if (kMulti) {  // last domain whose first unit is <= unit (uniform)
+      uint32_t a = 0, b = ndom;
+      while (b - a > 1) {
+        const uint32_t m = (a + b) / 2;
+        if (doms[m].unit_base <= unit) a = m;
+        else b = m;
+      }
+      dk = a;
+    }
+    const Dom &D = kMulti ? doms[dk] : d0;
+    bool emit = true;
+    if (kMulti && kDetect) {
+      const uint32_t fl = ds[dk].flags;
+      if (!(fl & kDomActive)) continue;
+      emit = (fl & kDomEmit) != 0;
+    }
+    const bool padded = kMulti && D.padded;
+    const T *__restrict__ f = (const T *)D.f;
+    const T *__restrict__ g = (const T *)D.g;
+    uint8_t *__restrict__ code = D.code;
+    uint8_t *__restrict__ dc = D.dc;
+    const uint32_t nx = (uint32_t)D.d.nx, ny = (uint32_t)D.d.ny, nz = (uint32_t)D.d.nz;
+    const uint32_t sxy = (uint32_t)D.d.sxy;
+    const uint32_t u = unit - D.unit_base;
+    const uint32_t uz = u / (D.nux * D.nuy), rem = u - uz * D.nux * D.nuy;
+    const uint32_t uy = rem / D.nux, ux = rem - uy * D.nux;
+    const int x = D.lo[0] + (int)(ux * kColOwned) - 1 + (int)lane;  // this lane's x
+    const int y0m1 = D.lo[1] + (int)(uy * kColRows) - 1;             // y of loaded row 0
+    const int zs = D.lo[2] + (int)(uz * kColZ);
+    const int ze = zs + kColZ < D.hi[2] ? zs + kColZ : D.hi[2];
     const bool xok = x >= 0 && x < (int)nx;
-    const bool owned = lane >= 1 && lane <= (uint32_t)kColOwned && xok;
+    const bool owned = lane >= 1 && lane <= (uint32_t)kColOwned && x < D.hi[0];
+    const int yhi = D.hi[1];
     // in-plane offsets of the loaded rows, fixed for the unit
     uint32_t roff[kColH];
     bool rok[kColH];
@@ -285,8 +326,13 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
 #pragma unroll
       for (int h = 0; h < kColH; h++) {
         const T v = raw[h];
-        if (check && rok[h] && (!(v == v) || v == sentinel<T>())) lerr |= 1u;
-        c.k[h] = rok[h] ? Ext<T>::stage(v) : kNaN;
+        bool ok = rok[h];
+        if (padded) {
+          ok = ok && v != sentinel<T>();  // a ghost cell that carries no value
+        } else if (check && ok && (!(v == v) || v == sentinel<T>())) {
+          lerr |= 1u;
+        }
+        c.k[h] = ok ? Ext<T>::stage(v) : kNaN;
       }
       shift_plane(c);
     };
@@ -295,7 +341,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
 #pragma unroll
       for (int h = 0; h < kColRows; h++) {
         const int y = y0m1 + 1 + h;
-        const bool ok = owned && y < (int)ny && pz >= 0 && pz < (int)nz;
+        const bool ok = owned && y < yhi && pz >= 0 && pz < (int)nz;
         ncode[h] = code[ok ? (uint32_t)x + nx * (uint32_t)y + sxy * (uint32_t)pz : 0u];
       }
     };
@@ -307,6 +353,8 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
       lo_g[h].first(kNaN, 0);
     }
     uint32_t acc0 = 0, acc1 = 0, acc2 = 0;  // target masks of planes e-1, e, e+1
+    uint32_t uc[3] = {0, 0, 0};              // counters of this unit
+    const int64_t gbase = (int64_t)D.gbase;
     fetch(zs - 1);
     for (int p = zs - 1; p <= ze; p++) {
       ColPlane cf, cg;
@@ -321,11 +369,11 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
         if (!kLabel) fetch_codes(p);
       }
       const int e = p - 1;  // plane completed at this step
-      const bool emit = e >= zs;
+      const bool emitp = e >= zs;
 #pragma unroll
       for (int h = 1; h <= kColRows; h++) {
         Ext<T> hf, hg, mf, mg, lf, lg;
-        if (emit) {
+        if (emitp) {
           if (kLabel) {
             partial_hi<T>(cf, h, hf);
             part_f[h - 1].merge(hf);
@@ -335,7 +383,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
             part_g[h - 1].merge(hg);
           }
           const int y = y0m1 + h;
-          if (owned && y < (int)ny) {
+          if (owned && y < yhi) {
             const uint32_t i = (uint32_t)x + nx * (uint32_t)y + sxy * (uint32_t)e;
             int oa, od;
             if (kLabel) {
@@ -350,15 +398,17 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
             if (kDetect) {
               int tr[4];
               const int nt =
-                  detect_targets(oa, od, part_g[h - 1].asc(), part_g[h - 1].desc(), tr, lc);
-              for (int q = 0; q < nt; q++) {
-                const int r = tr[q];
-                const uint32_t xy = (uint32_t)(kXY >> (4 * r)) & 15u;
-                const uint32_t bit = (xy & 3u) * kColH + (uint32_t)h + (xy >> 2) - 1u;
-                const uint32_t mb = 1u << bit;
-                if (r < 4) acc0 |= mb;
-                else if (r < 11) acc1 |= mb;
-                else acc2 |= mb;
+                  detect_targets(oa, od, part_g[h - 1].asc(), part_g[h - 1].desc(), tr, uc);
+              if (emit) {
+                for (int q = 0; q < nt; q++) {
+                  const int r = tr[q];
+                  const uint32_t xy = (uint32_t)(kXY >> (4 * r)) & 15u;
+                  const uint32_t bit = (xy & 3u) * kColH + (uint32_t)h + (xy >> 2) - 1u;
+                  const uint32_t mb = 1u << bit;
+                  if (r < 4) acc0 |= mb;
+                  else if (r < 11) acc1 |= mb;
+                  else acc2 |= mb;
+                }
               }
             }
           }
@@ -379,17 +429,26 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
           lo_g[h - 1] = lg;
         }
       }
-      if (kDetect && emit) {
+      if (kDetect && emitp) {
         // plane e-1 can receive no more targets
-        flush_targets(acc0, (int64_t)x, y0m1, (int64_t)e - 1, d, tmark);
+        flush_targets(acc0, gbase + x, y0m1, (int64_t)e - 1, D.d, tmark);
         acc0 = acc1;
         acc1 = acc2;
         acc2 = 0;
       }
     }
     if (kDetect) {
-      flush_targets(acc0, (int64_t)x, y0m1, (int64_t)ze - 1, d, tmark);
-      flush_targets(acc1, (int64_t)x, y0m1, (int64_t)ze, d, tmark);
+      flush_targets(acc0, gbase + x, y0m1, (int64_t)ze - 1, D.d, tmark);
+      flush_targets(acc1, gbase + x, y0m1, (int64_t)ze, D.d, tmark);
+      if (kMulti) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const uint32_t sm = wave_sum(uc[c]);
+          if (lane == 0 && sm) atomicAdd(&ds[dk].cnt[par][c], sm);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) lc[c] += uc[c];
     }
   }
   const uint64_t em = __ballot(lerr != 0);

@@ -46,6 +46,50 @@ __device__ __forceinline__ void decompose(uint32_t v, const Grid3 &d, int64_t &x
   z = (int64_t)zz;
 }
 
+// One domain of a run.  A single correction has one; a batch has one per field;
+// a subdomain run has one per padded block (core + 1-cell ghost halo, the
+// reference's padded subdomain, pMSz.cu:2709-2711).  Voxel ids used by the
+// bitmaps and lists are (domain << shift) | local index.
+struct Dom {
+  const void *f;      // original field of the stored block
+  void *g;            // working field of the stored block
+  uint8_t *code;      // original codes (library-owned)
+  uint8_t *dc;        // delta counters (library-owned)
+  uint8_t *snap;      // ghost-sync snapshot of dc (padded blocks only)
+  Grid3 d;            // extent of the stored block
+  int32_t lo[3], hi[3];  // box of classified voxels [lo, hi): the core
+  int32_t off[3];     // global coordinate of local cell lo (subdomains)
+  int32_t cc[3];      // coordinates in the subdomain grid
+  uint32_t unit_base; // first column unit of this domain
+  uint32_t nux, nuy, nuz;  // column units of the box
+  uint32_t shell_n;   // cells of the 2-layer shell (padded blocks)
+  uint32_t padded;    // 1: cells holding the sentinel are missing neighbours
+  uint32_t gbase;     // domain << shift
+  uint32_t pad_;
+  EditParams ep;      // bound, step, q and clamp mode of this domain's edits
+};
+
+// Per-domain state of a multi-domain run.
+enum : uint32_t {
+  kDomActive = 1,  // takes part in the next detection
+  kDomEmit = 2,    // its targets of that detection are edited (else it only counts)
+  kDomSend = 4,    // an edit touched its 2-layer boundary since the last ghost sync
+};
+enum : uint32_t { kDomRunning = 0, kDomConverged = 1, kDomCapped = 2 };
+
+struct DomState {
+  uint32_t cnt[2][3];   // counters of the detections of parity 0 / 1
+  uint32_t flags;
+  uint32_t sweeps;      // edit sweeps counted for this domain
+  uint32_t status;
+  uint32_t last_cnt[3]; // counters of its last detection
+  uint32_t last_j;      // index of its last detection
+  uint32_t pad_;
+  unsigned long long targets;  // edits applied
+  unsigned long long touched;  // voxels with dc > 0 at the end (core)
+  unsigned long long clamped;  // voxels with dc == q at the end (core)
+};
+
 // Next-sweep decision written by the apply of sweep j into its counter block.
 enum : uint32_t {
   kModeUnset = 0,
@@ -65,7 +109,9 @@ struct Counters {
   uint32_t nwork;
   uint32_t mode;
   uint32_t err;
-  uint32_t pad[9];
+  uint32_t nactive;  // multi-domain: domains taking part in detection j+1
+  uint32_t ncapped;  // multi-domain: domains stopped by the sweep cap
+  uint32_t pad[7];
 };
 static_assert(sizeof(Counters) == 64, "Counters layout");
 
@@ -302,6 +348,105 @@ __device__ __forceinline__ void block_add3(const uint32_t (&c)[3], uint32_t *dst
     if (s) atomicAdd(&dst[threadIdx.x], s);
   }
   __syncthreads();
+}
+
+// Bit r set iff candidate r of (x, y, z) lies in the box [lo, hi).
+__device__ __forceinline__ uint32_t box_mask(int64_t x, int64_t y, int64_t z, const int32_t *lo,
+                                             const int32_t *hi) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int r = 0; r < kCand; r++) {
+    int64_t a = x + rank_dx(r), b = y + rank_dy(r), c = z + rank_dz(r);
+    bool ok = a >= lo[0] && a < hi[0] && b >= lo[1] && b < hi[1] && c >= lo[2] && c < hi[2];
+    m |= (ok ? 1u : 0u) << r;
+  }
+  return m;
+}
+
+// Domain of a voxel id and its local index.
+template <bool kMulti>
+__device__ __forceinline__ const Dom &dom_of(uint32_t gid, const Dom &d0, const Dom *doms,
+                                             uint32_t shift, uint32_t &k, uint32_t &local) {
+  if (!kMulti) {
+    k = 0;
+    local = gid;
+    return d0;
+  }
+  k = gid >> shift;
+  local = gid & ((1u << shift) - 1u);
+  return doms[k];
+}
+
+// Adds three per-lane counters of lanes with `has` to ds[dom].cnt[par], one atomic
+// per distinct domain in the wave.  Every lane of the wave must call it.
+__device__ __forceinline__ void dom_add3(bool has, uint32_t dom, const uint32_t (&c)[3],
+                                         DomState *ds, uint32_t par) {
+  const uint32_t lane = lane_id();
+  uint64_t pending = __ballot(has && (c[0] | c[1] | c[2]) != 0);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t dl = (uint32_t)__shfl((int)dom, leader);
+    const bool mine = ((pending >> lane) & 1ull) && dom == dl;
+    const uint64_t m = __ballot(mine);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const uint32_t s = wave_sum(mine ? c[q] : 0u);
+      if (lane == (uint32_t)leader && s) atomicAdd(&ds[dl].cnt[par][q], s);
+    }
+    pending &= ~m;
+  }
+}
+
+// Adds a per-lane count to ds[dom].targets, one atomic per distinct domain.
+__device__ __forceinline__ void dom_add_targets(bool has, uint32_t dom, DomState *ds) {
+  const uint32_t lane = lane_id();
+  uint64_t pending = __ballot(has);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t dl = (uint32_t)__shfl((int)dom, leader);
+    const uint64_t m = __ballot(((pending >> lane) & 1ull) && dom == dl);
+    if (lane == (uint32_t)leader) atomicAdd(&ds[dl].targets, (unsigned long long)__popcll(m));
+    pending &= ~m;
+  }
+}
+
+// Cells of the 2-layer shell of a P0 x P1 x P2 block (every cell with a
+// coordinate in {0, 1, P-2, P-1}); the whole block when an extent is below 4.
+__host__ __device__ inline uint32_t shell_count(uint32_t P0, uint32_t P1, uint32_t P2) {
+  if (P0 < 4 || P1 < 4 || P2 < 4) return P0 * P1 * P2;
+  return 4 * P0 * P1 + 4 * P0 * (P2 - 4) + 4 * (P1 - 4) * (P2 - 4);
+}
+__host__ __device__ inline void shell_cell(uint32_t i, uint32_t P0, uint32_t P1, uint32_t P2,
+                                           uint32_t &x, uint32_t &y, uint32_t &z) {
+  if (P0 < 4 || P1 < 4 || P2 < 4) {
+    x = i % P0;
+    y = (i / P0) % P1;
+    z = i / (P0 * P1);
+    return;
+  }
+  const uint32_t a = 4 * P0 * P1;
+  if (i < a) {  // planes z in {0, 1, P2-2, P2-1}
+    const uint32_t zi = i / (P0 * P1), r = i - zi * P0 * P1;
+    z = zi < 2 ? zi : P2 - 4 + zi;
+    y = r / P0;
+    x = r - y * P0;
+    return;
+  }
+  i -= a;
+  const uint32_t b = 4 * P0 * (P2 - 4);
+  if (i < b) {  // rows y in {0, 1, P1-2, P1-1} of planes 2..P2-3
+    const uint32_t yi = i / (P0 * (P2 - 4)), r = i - yi * P0 * (P2 - 4);
+    y = yi < 2 ? yi : P1 - 4 + yi;
+    z = 2 + r / P0;
+    x = r - (z - 2) * P0;
+    return;
+  }
+  i -= b;  // columns x in {0, 1, P0-2, P0-1} of rows 2..P1-3, planes 2..P2-3
+  const uint32_t c = (P1 - 4) * (P2 - 4);
+  const uint32_t xi = i / c, r = i - xi * c;
+  x = xi < 2 ? xi : P0 - 4 + xi;
+  z = 2 + r / (P1 - 4);
+  y = 2 + r - (z - 2) * (P1 - 4);
 }
 
 // Bits [p, p+32) of the 96-bit little-endian value (w0, w1, w2); bits outside

@@ -1,32 +1,44 @@
 // pmsz_engine.hip -- gfx950 kernels and the C-ABI (include/pmsz.h) of the
-// single-domain correction path.
+// correction path: one field, a batch of independent fields, or one field split
+// into padded subdomains with ghost synchronisation.
 //
-// Data in HBM (N = nx*ny*nz voxels, s = sizeof(T)):
-//   f, g        T[N]        caller-owned original / working field, x fastest
-//   code        u8[N]       original steepest-ascent rank (low nibble) and
-//                           steepest-descent rank (high nibble); rank 7 = the voxel
-//   dc          u8[N]       delta counters (delta_counter, pMSz.cu:173)
-//   tmark       u32[N/32]   targets of the running sweep: the per-sweep edit dedup
-//                           (edits[] + atomicCAS, pMSz.cu:795) as one bit per voxel
-//   wmark       u32[N/32]   voxels queued for the next sweep
-//   tlist/wlist u32[N]      the same two sets as compact work lists
-//   ring        Counters[]  ring of per-sweep counter blocks
-//   run         RunState    phase hand-over between kernels, run totals
+// Data in HBM, per domain (one field of a batch, or one padded subdomain block):
+//   f, g        T[n]       original / working field, x fastest (the caller's for
+//                          a field; library-owned padded blocks for subdomains)
+//   code        u8[n]      original steepest-ascent rank (low nibble) and
+//                          steepest-descent rank (high nibble); rank 7 = the voxel
+//   dc          u8[n]      delta counters (delta_counter, pMSz.cu:173)
+//   snap        u8[n]      subdomains: the dc of the 2-layer shell at a ghost sync
+// Per run:
+//   tmark       u32[]      targets of the running sweep, one bit per voxel id: the
+//                          per-sweep edit dedup (edits[] + atomicCAS, pMSz.cu:795)
+//   wmark       u32[]      voxels queued for the next sweep
+//   tlist/wlist u32[]      the same two sets as compact lists of voxel ids
+//   ring        Counters[] per-sweep counters and the schedule decisions
+//   run         RunState   phase hand-over between kernels, run totals
+//   doms, ds    Dom[], DomState[]  per-domain description and state
+// A voxel id is (domain << shift) | local index; a single field is domain 0 with
+// id = index.
 //
 // Snapshot semantics: a sweep detects on the start-of-sweep g, writing only
 // marks, lists and counters; a separate phase then applies one edit per target
 // (applyDeltaBuffer_local, pMSz.cu:623-645).
 //
-// Schedule.  Sweep 1 is one full pass fused with the original labeling.  After
-// that, the apply of sweep j decides on the device how sweep j+1 detects:
-//   - over every voxel (tile kernel) while the targets are dense,
+// Schedule.  Detection 1 is one full pass fused with the original labeling.
+// After that, the apply of sweep j decides on the device how detection j+1 runs:
+//   - over every voxel (column kernel) while the targets are dense,
 //   - over the closed stencil neighbourhoods of sweep j's targets otherwise: a
 //     voxel's classification depends only on g over its closed neighbourhood and
 //     the stencil is symmetric, so no other voxel can change -- the counters and
 //     targets equal those of a full sweep;
 //   - in a single-workgroup tail kernel once the targets are few, where a sweep
-//     costs two workgroup barriers instead of two kernel launches.
-// The host enqueues the whole chain without waiting and synchronises once.
+//     costs workgroup barriers instead of kernel launches.
+// The host enqueues the chain without waiting and synchronises once per chain.
+//
+// Several domains run in lockstep in the same launches.  Each keeps its own
+// counters and flags: it takes part in detections while its last detection
+// found violations, exactly as its own loop would (the domains are independent
+// between ghost syncs).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,257 +54,39 @@
 
 namespace pmsz {
 
-constexpr int kTX = 64;  // dense tile: 64 x 8 voxels per plane, 16 planes
-constexpr int kTY = 8;
-constexpr int kTZ = 16;
-constexpr int kThreads = 256;  // each thread classifies 2 rows of the tile
-constexpr int kPX = kTX + 2, kPY = kTY + 2, kPZ = kTZ + 2;  // tile + halo
-constexpr int kPlane = kPX * kPY;                             // cells of one staged plane
-constexpr int kEPT = (kPlane + kThreads - 1) / kThreads;      // staged cells per thread
-constexpr int kPlaneP = kEPT * kThreads;                       // padded slot size
-constexpr int kSlots = 4;                                     // LDS plane ring
-constexpr int kRows = kPY * kPZ;                              // bitmap rows of 66 cells
-constexpr int kFlushPT = (kRows * 4 + kThreads - 1) / kThreads;
 constexpr int kApplyThreads = 256;
 constexpr int kTailThreads = 1024;
 constexpr uint32_t kTailTargets = 2048;  // the tail kernel takes over at or below this
+constexpr int kSyncThreads = 256;
+constexpr uint32_t kMaxDomains = kTailThreads;  // domain bookkeeping: one thread each
 
 struct Sched {
   int mode;  // PMSZ_LAZY / PMSZ_SYNC
   uint32_t max_sweeps;
   uint32_t dense_min_work;  // 15 * targets at or above this -> next detection dense
+  uint32_t tail_ok;         // the single-workgroup tail may take over
 };
 
-// ---------------------------------------------------------------------------
-// Tile kernel: with kLabel the original labeling of f (codes), with kDetect a
-// detection sweep on g (sweep 1 when both are set; a later dense sweep reading
-// the codes when only kDetect is set).
-//
-// Persistent blocks walk 64 x 8 x 16 tiles.  Blocks are dealt to the 8 XCDs
-// round-robin by workgroup id, so block b takes tiles from the b % 8 -th
-// contiguous eighth of the tile space: z- and y-neighbouring tiles, whose halos
-// overlap, then share one XCD's L2.  A tile walks its planes through a ring of
-// four LDS slots: while plane z is classified, plane z+2 is already in LDS and
-// plane z+3 is in flight to registers, so one barrier per plane suffices and the
-// loads overlap the compute.  Targets are collected in an LDS bitmap over the
-// tile and its halo and flushed once per tile as row masks (one global atomicOr
-// per 32 cells, one list reservation per block).
-template <typename T, bool kLabel, bool kDetect>
-__global__ __launch_bounds__(kThreads) void k_tiles(
-    const T *__restrict__ f, const T *__restrict__ g, uint8_t *__restrict__ code,
-    uint8_t *__restrict__ dc, uint32_t *__restrict__ tmark, uint32_t *__restrict__ tlist,
-    Counters *__restrict__ ring, uint32_t jslot_prev, uint32_t jslot, RunState *run, Grid3 d) {
-  if (!kLabel) {
-    if (ring[jslot_prev].mode != kModeDense) return;  // uniform
-  }
-  constexpr int kFields = (kLabel ? 1 : 0) + (kDetect ? 1 : 0);
-  constexpr int kF = 0, kG = kLabel ? 1 : 0;
-  constexpr int kRowsPT = kTY / (kThreads / kTX);  // tile rows per thread
-  constexpr bool kKeys = sizeof(T) == 4;  // float32: keys, float64: rank scan
-  __shared__ double sv[kFields][kSlots][kPlaneP];
-  __shared__ uint32_t sbits[kDetect ? kRows : 1][3];
-  const int tid = threadIdx.x;
-  const int tx = tid % kTX, ty0 = tid / kTX;
-  // all voxel indices fit in 32 bits (checked on the host)
-  const uint32_t nx = (uint32_t)d.nx, ny = (uint32_t)d.ny, nz = (uint32_t)d.nz;
-  const uint32_t sxy = (uint32_t)d.sxy;
-  const uint32_t ntx = (nx + kTX - 1) / kTX, nty = (ny + kTY - 1) / kTY, ntz = (nz + kTZ - 1) / kTZ;
-  const uint32_t ntiles = ntx * nty * ntz;
-  const uint32_t per_xcd_blocks = gridDim.x / 8;
-  const uint32_t chunk = (ntiles + 7) / 8;
-  const uint32_t t_begin = (blockIdx.x % 8) * chunk;
-  const uint32_t t_end = t_begin + chunk < ntiles ? t_begin + chunk : ntiles;
-  Counters *out = kDetect ? &ring[jslot] : nullptr;
-  uint32_t lc[3] = {0, 0, 0};
-  uint32_t lerr = 0;
+// Stencil directions of the reference in its order d = 0..13 (directions_host,
+// pMSz.cu:186-193): component a of direction d.  The opposite of d is d ^ 1.
+__host__ __device__ constexpr int dir_comp(int d, int a) {
+  return a == 0 ? (d == 0 || d == 7 || d == 9 || d == 13 ? 1
+                   : d == 1 || d == 6 || d == 8 || d == 12 ? -1 : 0)
+       : a == 1 ? (d == 2 || d == 6 || d == 10 || d == 12 ? 1
+                   : d == 3 || d == 7 || d == 11 || d == 13 ? -1 : 0)
+                : (d == 4 || d == 8 || d == 10 || d == 12 ? 1
+                   : d == 5 || d == 9 || d == 11 || d == 13 ? -1 : 0);
+}
 
-  for (uint32_t tile = t_begin + blockIdx.x / 8; tile < t_end; tile += per_xcd_blocks) {
-    const uint32_t tzi = tile / (ntx * nty), rem = tile - tzi * ntx * nty;
-    const uint32_t tyi = rem / ntx, txi = rem - tyi * ntx;
-    const uint32_t bx = txi * kTX, by = tyi * kTY, bz = tzi * kTZ;
-    const uint32_t x = bx + tx;
-
-    if (kDetect)
-      for (int e = tid; e < kRows * 3; e += kThreads) (&sbits[0][0])[e] = 0;
-
-    // staged cells of this thread: in-plane offset and in-plane validity, fixed
-    // for the whole tile; loads are unconditional (clamped to cell 0) so the
-    // compiler can issue them back to back
-    uint32_t soff[kEPT];
-    bool sok[kEPT];
-#pragma unroll
-    for (int q = 0; q < kEPT; q++) {
-      const int e = tid + q * kThreads;
-      const int lx = e % kPX, ly = e / kPX;
-      const int gx = (int)bx + lx - 1, gy = (int)by + ly - 1;
-      sok[q] = e < kPlane && gx >= 0 && gx < (int)nx && gy >= 0 && gy < (int)ny;
-      soff[q] = sok[q] ? (uint32_t)gx + nx * (uint32_t)gy : 0u;
-    }
-    // out-of-domain cells hold whatever cell 0 holds: no candidate reads them
-    // (interior voxels have every neighbour in the domain, boundary voxels use the
-    // validity mask), so the staged values need no select
-    T pf[kEPT], pg[kEPT];
-    uint32_t pok = 0;  // bit q: staged cell q lies in the domain
-    auto prefetch = [&](int pz) {
-      const bool zok = pz >= 0 && pz < (int)nz;
-      const uint32_t zoff = zok ? sxy * (uint32_t)pz : 0u;
-      pok = 0;
-#pragma unroll
-      for (int q = 0; q < kEPT; q++) {
-        const bool ok = zok && sok[q];
-        pok |= (ok ? 1u : 0u) << q;
-        const uint32_t gi = ok ? zoff + soff[q] : 0u;
-        if (kLabel) pf[q] = f[gi];
-        if (kDetect) pg[q] = g[gi];
-      }
-    };
-    auto store = [&](int slot) {
-#pragma unroll
-      for (int q = 0; q < kEPT; q++) {
-        const int e = tid + q * kThreads;  // slots are padded to kEPT * kThreads cells
-        if (kLabel) {
-          const T v = pf[q];
-          if (((pok >> q) & 1u) && (!(v == v) || v == sentinel<T>())) lerr |= 1u;
-          sv[kF][slot][e] = stage_value(v);
-        }
-        if (kDetect) {
-          const T v = pg[q];
-          if (kLabel && ((pok >> q) & 1u) && (!(v == v) || v == sentinel<T>())) lerr |= 1u;
-          sv[kG][slot][e] = stage_value(v);
-        }
-      }
-    };
-    // codes of this thread's voxels, one plane ahead (dense sweeps after the first)
-    uint8_t cpre[kRowsPT];
-    auto prefetch_codes = [&](int pz) {
-#pragma unroll
-      for (int h = 0; h < kRowsPT; h++) {
-        const uint32_t y = by + ty0 + h * (kThreads / kTX);
-        const bool ok = x < nx && y < ny && pz < (int)nz;
-        const uint8_t c = code[ok ? x + nx * y + sxy * (uint32_t)pz : 0u];
-        cpre[h] = c;
-      }
-    };
-    prefetch((int)bz - 1);
-    store(0);
-    prefetch((int)bz);
-    store(1);
-    prefetch((int)bz + 1);
-    store(2);
-    prefetch((int)bz + 2);
-    if (!kLabel) prefetch_codes((int)bz);
-    lds_barrier();
-    const int kz = (int)((nz - bz) < (uint32_t)kTZ ? (nz - bz) : (uint32_t)kTZ);
-    for (int k = 0; k < kz; k++) {
-      const uint32_t z = bz + k;
-      store((k + 3) % kSlots);  // plane z+2
-      prefetch((int)z + 3);
-      uint8_t ccur[kRowsPT];
-      if (!kLabel) {
-#pragma unroll
-        for (int h = 0; h < kRowsPT; h++) ccur[h] = cpre[h];
-        prefetch_codes((int)z + 1);
-      }
-      const int s0 = k % kSlots, s1 = (k + 1) % kSlots, s2 = (k + 2) % kSlots;
-#pragma unroll
-      for (int h = 0; h < kRowsPT; h++) {
-        const int ty = ty0 + h * (kThreads / kTX);
-        const uint32_t y = by + ty;
-        if (x < nx && y < ny) {
-          const uint32_t i = x + nx * y + sxy * z;
-          const bool interior =
-              x > 0 && x + 1 < nx && y > 0 && y + 1 < ny && z > 0 && z + 1 < nz;
-          const uint32_t vm = interior ? 0x7fffu : valid_mask(x, y, z, d);
-          int oa, od;
-          auto cell = [&](int r) {
-            const int sl = rank_dz(r) < 0 ? s0 : (rank_dz(r) == 0 ? s1 : s2);
-            return sl * kPlaneP + (ty + 1 + rank_dy(r)) * kPX + tx + 1 + rank_dx(r);
-          };
-          auto classify = [&](int field, int &a, int &dsc) {
-            double st[kCand];
-#pragma unroll
-            for (int r = 0; r < kCand; r++) st[r] = (&sv[field][0][0])[cell(r)];
-            if (kKeys) {
-              classify_keys(st, interior, vm, a, dsc);
-            } else if (interior) {
-              scan_full(st, a, dsc);
-            } else {
-              scan_masked(st, vm, a, dsc);
-            }
-          };
-          if (kLabel) {
-            classify(kF, oa, od);
-            code[i] = (uint8_t)(oa | (od << 4));
-            if (kDetect) dc[i] = 0;
-          } else {
-            oa = ccur[h] & 15;
-            od = ccur[h] >> 4;
-          }
-          if (kDetect) {
-            int da, dd;
-            classify(kG, da, dd);
-            int tr[4];
-            const int nt = detect_targets(oa, od, da, dd, tr, lc);
-            for (int q = 0; q < nt; q++) {
-              const int r = tr[q];
-              const int lx = tx + 1 + rank_dx(r), ly = ty + 1 + rank_dy(r),
-                        lz = k + 1 + rank_dz(r);
-              atomicOr(&sbits[lz * kPY + ly][lx >> 5], 1u << (lx & 31));
-            }
-          }
-        }
-      }
-      lds_barrier();
-    }
-    if (kDetect) {
-      // flush: (row, word) items, kRows rows x 4 global words
-      uint32_t claimed[kFlushPT];
-      uint32_t wbase[kFlushPT];
-      uint32_t n = 0;
-#pragma unroll
-      for (int q = 0; q < kFlushPT; q++) {
-        claimed[q] = 0;
-        wbase[q] = 0;
-        const int item = tid + q * kThreads;
-        if (item >= kRows * 4) continue;
-        const int row = item >> 2, kw = item & 3;
-        const int ly = row % kPY, lz = row / kPY;
-        const int gy = (int)by + ly - 1, gz = (int)bz + lz - 1;
-        if (gy < 0 || gy >= (int)ny || gz < 0 || gz >= (int)nz) continue;
-        // linear index of cell lx = 0 (may be -1 for the very first row)
-        const int64_t i0 = (int64_t)bx - 1 + (int64_t)nx * gy + (int64_t)sxy * gz;
-        const int64_t w0 = i0 >> 5;  // floor
-        const int sh = (int)(i0 - (w0 << 5));
-        const uint32_t m = bits32_at(sbits[row][0], sbits[row][1], sbits[row][2], 32 * kw - sh);
-        if (m) {
-          wbase[q] = (uint32_t)(w0 + kw);
-          claimed[q] = m & ~atomicOr(&tmark[wbase[q]], m);
-          n += __popc(claimed[q]);
-        }
-      }
-      uint32_t pos = block_reserve<kThreads>(n, &out->ntargets);
-#pragma unroll
-      for (int q = 0; q < kFlushPT; q++) {
-        uint32_t cm = claimed[q];
-        while (cm) {
-          const int b = __ffs(cm) - 1;
-          cm &= cm - 1;
-          tlist[pos++] = wbase[q] * 32u + (uint32_t)b;
-        }
-      }
-    }
-  }
-  if (kLabel && lerr) atomicOr(&run->err, 1u);
-  if (!kDetect) return;
-  block_add3<kThreads>(lc, out->cnt);
-  if (tid == 0 && blockIdx.x == 0 && !kLabel) atomicAdd(&run->dense_passes, 1ull);
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
-// Shared pieces of the apply phase.
-
-// Decision of the apply of sweep j (identical in every thread): the mode of
+// Decisions of the apply of sweep j (identical in every thread): the mode of
 // detection j+1, or kModeTail, or kModeDone with *done = 1 (converged) / 2 (cap).
+
+// One domain: from the counters of detection j.
 __device__ __forceinline__ uint32_t decide(const Counters &cj, uint32_t j, const Sched &s,
                                            bool tail_allowed, uint32_t *done) {
   bool zero = cj.cnt[0] == 0 && cj.cnt[1] == 0 && cj.cnt[2] == 0;
@@ -313,10 +107,108 @@ __device__ __forceinline__ uint32_t decide(const Counters &cj, uint32_t j, const
   return (uint64_t)cj.ntargets * kCand >= s.dense_min_work ? kModeDense : kModeSparse;
 }
 
-// One edit (applyDeltaBuffer_local) on target t, counting first edits / clamps.
+// Several domains: from the domain bookkeeping after detection j (the caps are
+// per domain, applied there).  The targets of detection j are always applied:
+// a domain that must not edit them emitted none.
+__device__ __forceinline__ uint32_t decide_multi(const Counters &cj, const Sched &s,
+                                                 bool tail_allowed, uint32_t *done) {
+  *done = 0;
+  if (cj.nactive == 0) {
+    *done = cj.ncapped ? 2 : 1;
+    return kModeDone;
+  }
+  if (tail_allowed && cj.ntargets <= kTailTargets) return kModeTail;
+  return (uint64_t)cj.ntargets * kCand >= s.dense_min_work ? kModeDense : kModeSparse;
+}
+
+// ---------------------------------------------------------------------------
+// Domain bookkeeping after a detection (pMSz.cu:1943-1989 per subdomain / field,
+// sync_pMSz.cu:2102-2146 in sync mode).
+enum : uint32_t {
+  kOpDetect = 0,    // after a detection of the running loop
+  kOpPreCheck = 1,  // before the check detection that follows a ghost sync
+  kOpCheck = 2,     // after that check detection
+};
+
+__device__ __forceinline__ void dom_update(DomState &st, uint32_t jd, uint32_t op,
+                                           const Sched &s, uint32_t emit_ok, uint32_t &active,
+                                           uint32_t &capped) {
+  const uint32_t par = jd & 1u;
+  uint32_t fl = ld_agent(&st.flags);
+  if (op == kOpPreCheck) {
+    // every domain is checked; one whose loop may not run another sweep only counts
+    const bool emit = emit_ok && st.sweeps < s.max_sweeps;
+    fl = kDomActive | (emit ? kDomEmit : 0u);
+    for (int q = 0; q < 3; q++) st.cnt[par][q] = 0;
+  } else if (fl & kDomActive) {
+    uint32_t c[3];
+    for (int q = 0; q < 3; q++) c[q] = ld_agent(&st.cnt[par][q]);
+    const bool nonzero = (c[0] | c[1] | c[2]) != 0;
+    const bool emit = (fl & kDomEmit) != 0;
+    // an edit sweep of this domain: every emitting detection of its loop; after a
+    // ghost sync, the check doubles as the first sweep of the next inner loop
+    if (emit && (op == kOpDetect || nonzero)) st.sweeps++;
+    for (int q = 0; q < 3; q++) st.last_cnt[q] = c[q];
+    st.last_j = jd;
+    uint32_t nf = fl & kDomSend;
+    if (!nonzero) {
+      st.status = kDomConverged;
+    } else if (!emit) {
+      st.status = kDomCapped;
+    } else if (s.mode == PMSZ_LAZY && st.sweeps >= s.max_sweeps) {
+      st.status = kDomCapped;  // its edits of this sweep land; it sweeps no more
+    } else {
+      st.status = kDomRunning;
+      // sync mode: the detection after the last permitted edit sweep only counts
+      const bool next_emit = !(s.mode == PMSZ_SYNC && st.sweeps >= s.max_sweeps);
+      nf |= kDomActive | (next_emit ? kDomEmit : 0u);
+    }
+    fl = nf;
+  }
+  __hip_atomic_store(&st.flags, fl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (op != kOpPreCheck)
+    for (int q = 0; q < 3; q++) st.cnt[par ^ 1u][q] = 0;
+  active = (fl & kDomActive) ? 1u : 0u;
+  capped = st.status == kDomCapped ? 1u : 0u;
+}
+
+// Runs after detection jd when the apply of sweep jprev let it run (gate), or
+// unconditionally (gate = 0).
+__global__ __launch_bounds__(kMaxDomains) void k_domains(DomState *__restrict__ ds, uint32_t ndom,
+                                                         Counters *__restrict__ ring,
+                                                         uint32_t jprev, uint32_t gate,
+                                                         uint32_t jd, uint32_t op, Sched s,
+                                                         uint32_t emit_ok) {
+  if (gate) {
+    const uint32_t m = ring[jprev % kCtrRing].mode;
+    if (m != kModeDense && m != kModeSparse) return;
+  }
+  __shared__ uint32_t s_act, s_cap;
+  if (threadIdx.x == 0) {
+    s_act = 0;
+    s_cap = 0;
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < ndom; k += blockDim.x) {
+    uint32_t a, c;
+    dom_update(ds[k], jd, op, s, emit_ok, a, c);
+    if (a) atomicAdd(&s_act, 1u);
+    if (c) atomicAdd(&s_cap, 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && op != kOpPreCheck) {
+    ring[jd % kCtrRing].nactive = s_act;
+    ring[jd % kCtrRing].ncapped = s_cap;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Shared pieces of the apply and work-list phases.
+
+// One edit (applyDeltaBuffer_local) on local cell t, counting first edits / clamps.
 template <typename T>
-__device__ __forceinline__ void apply_one(T *g, const T *f, uint8_t *dc, uint32_t *tmark,
-                                          uint32_t t, const EditParams &ep, uint32_t &touched,
+__device__ __forceinline__ void apply_one(T *g, const T *f, uint8_t *dc, uint32_t t,
+                                          const EditParams &ep, uint32_t &touched,
                                           uint32_t &clamped) {
   T gv = g[t];
   uint8_t c0 = dc[t], c = c0;
@@ -325,54 +217,59 @@ __device__ __forceinline__ void apply_one(T *g, const T *f, uint8_t *dc, uint32_
   dc[t] = c;
   touched += c0 == 0;
   clamped += (c == ep.q && c0 != ep.q);
-  // every set bit of tmark is a target of this sweep and is applied in this phase,
-  // so the word can be cleared with a plain store
-  tmark[t >> 5] = 0u;
 }
 
-// Claims the closed stencil neighbourhood of t in wmark; returns the new voxels.
-__device__ __forceinline__ int queue_neighbourhood(uint32_t t, uint32_t *wmark, const Grid3 &d,
+// Claims the classified voxels of the closed stencil neighbourhood of local cell
+// t = (x, y, z) in wmark; returns their ids where the claim was new.
+__device__ __forceinline__ int queue_neighbourhood(uint32_t t, int64_t x, int64_t y, int64_t z,
+                                                   const Dom &D, uint32_t *wmark,
                                                    uint32_t (&u)[kCand]) {
-  int64_t x, y, z;
-  decompose(t, d, x, y, z);
-  const uint32_t vm = valid_mask(x, y, z, d);
+  const uint32_t vm = box_mask(x, y, z, D.lo, D.hi);
   // issue all claims before looking at any result, so the atomics overlap
   uint32_t old[kCand];
 #pragma unroll
   for (int r = 0; r < kCand; r++) {
-    const uint32_t v = (uint32_t)((int64_t)t + rank_off(r, d));
+    const uint32_t v = D.gbase + (uint32_t)((int64_t)t + rank_off(r, D.d));
     old[r] = ((vm >> r) & 1u) ? atomicOr(&wmark[v >> 5], 1u << (v & 31u)) : 0xffffffffu;
   }
   int n = 0;
 #pragma unroll
   for (int r = 0; r < kCand; r++) {
-    const uint32_t v = (uint32_t)((int64_t)t + rank_off(r, d));
+    const uint32_t v = D.gbase + (uint32_t)((int64_t)t + rank_off(r, D.d));
     if (!((old[r] >> (v & 31u)) & 1u)) u[n++] = v;
   }
   return n;
 }
 
-// Detection at one listed voxel (global loads); returns the claimed targets.
+// Detection at one listed voxel (global loads); returns the claimed targets.  In a
+// padded block a neighbour holding the sentinel is missing (pMSz.cu:709).
 template <typename T>
-__device__ __forceinline__ int detect_one(const T *g, const uint8_t *code, uint32_t *tmark,
-                                          uint32_t v, const Grid3 &d, uint32_t (&lc)[3],
+__device__ __forceinline__ int detect_one(const Dom &D, uint32_t *tmark, uint32_t local,
+                                          bool padded, bool emit, uint32_t (&lc)[3],
                                           uint32_t (&out)[4]) {
+  const T *g = (const T *)D.g;
   int64_t x, y, z;
-  decompose(v, d, x, y, z);
-  uint32_t vm = valid_mask(x, y, z, d);
+  decompose(local, D.d, x, y, z);
+  uint32_t vm = valid_mask(x, y, z, D.d);
   T cg[kCand];
 #pragma unroll
   for (int r = 0; r < kCand; r++)
-    cg[r] = ((vm >> r) & 1u) ? g[(int64_t)v + rank_off(r, d)] : (T)0;
+    cg[r] = ((vm >> r) & 1u) ? g[(int64_t)local + rank_off(r, D.d)] : (T)0;
+  if (padded) {
+#pragma unroll
+    for (int r = 0; r < kCand; r++)
+      if (cg[r] == sentinel<T>()) vm &= ~(1u << r);
+  }
   int da, dd;
   if (vm == 0x7fffu) scan_full(cg, da, dd);
   else scan_masked(cg, vm, da, dd);
-  uint8_t oc = code[v];
+  const uint8_t oc = D.code[local];
   int tr[4];
-  int nt = detect_targets(oc & 15, oc >> 4, da, dd, tr, lc);
+  const int nt = detect_targets(oc & 15, oc >> 4, da, dd, tr, lc);
+  if (!emit) return 0;
   uint32_t tv[4], old[4];
   for (int q = 0; q < nt; q++) {
-    tv[q] = (uint32_t)((int64_t)v + rank_off(tr[q], d));
+    tv[q] = D.gbase + (uint32_t)((int64_t)local + rank_off(tr[q], D.d));
     old[q] = atomicOr(&tmark[tv[q] >> 5], 1u << (tv[q] & 31u));
   }
   int n = 0;
@@ -386,15 +283,39 @@ __device__ __forceinline__ void zero_counters(Counters *c) {
   for (int k = 0; k < (int)(sizeof(Counters) / 4); k++) w[k] = 0;
 }
 
+// Apply of one target id (shared by the apply and tail kernels): the edit, the
+// mark clear, the ghost-sync flag and the queueing of the neighbourhood.
+template <typename T, bool kMulti>
+__device__ __forceinline__ int apply_target(uint32_t t, const Dom &d0, const Dom *doms,
+                                            uint32_t shift, DomState *ds, uint32_t *tmark,
+                                            uint32_t *wmark, bool queue, uint32_t &dk,
+                                            uint32_t &touched, uint32_t &clamped,
+                                            uint32_t (&u)[kCand]) {
+  uint32_t local;
+  const Dom &D = dom_of<kMulti>(t, d0, doms, shift, dk, local);
+  apply_one<T>((T *)D.g, (const T *)D.f, D.dc, local, D.ep, touched, clamped);
+  // every set bit of tmark is a target of this sweep and is applied in this phase,
+  // so the word can be cleared with a plain store
+  tmark[t >> 5] = 0u;
+  int64_t x, y, z;
+  decompose(local, D.d, x, y, z);
+  if (kMulti && D.padded) {
+    // send_flag: an edit within two layers of the padded boundary (pMSz.cu:642)
+    if (x <= 1 || x >= D.d.nx - 2 || y <= 1 || y >= D.d.ny - 2 || z <= 1 || z >= D.d.nz - 2)
+      if (!(ld_agent(&ds[dk].flags) & kDomSend)) atomicOr(&ds[dk].flags, kDomSend);
+  }
+  return queue ? queue_neighbourhood(local, x, y, z, D, wmark, u) : 0;
+}
+
 // ---------------------------------------------------------------------------
 // Apply of sweep j (grid-stride over its targets), queueing the next work list
 // when detection j+1 is sparse.
-template <typename T>
+template <typename T, bool kMulti>
 __global__ __launch_bounds__(kApplyThreads) void k_apply(
-    T *__restrict__ g, const T *__restrict__ f, uint8_t *__restrict__ dc,
+    Dom d0, const Dom *__restrict__ doms, uint32_t shift, DomState *__restrict__ ds,
     uint32_t *__restrict__ tmark, const uint32_t *__restrict__ tlist,
     uint32_t *__restrict__ wmark, uint32_t *__restrict__ wlist, Counters *__restrict__ ring,
-    RunState *run, uint32_t j, Grid3 d, EditParams ep, Sched s) {
+    RunState *run, uint32_t j, Sched s) {
   Counters &prev = ring[(j - 1) % kCtrRing];
   Counters &cj = ring[j % kCtrRing];
   const uint32_t pm = prev.mode;
@@ -404,7 +325,8 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
     return;
   }
   uint32_t done;
-  const uint32_t mode = decide(cj, j, s, true, &done);
+  const uint32_t mode =
+      kMulti ? decide_multi(cj, s, s.tail_ok != 0, &done) : decide(cj, j, s, true, &done);
   if (lead) {
     cj.mode = mode;
     if (mode == kModeTail) run->tail_j = j;
@@ -414,9 +336,10 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
     }
   }
   if (mode == kModeTail) return;
-  if (mode == kModeDone && (done == 1 || s.mode == PMSZ_SYNC)) return;
-  if (lead) zero_counters(&ring[(j + 1) % kCtrRing]);
+  if (!kMulti && mode == kModeDone && (done == 1 || s.mode == PMSZ_SYNC)) return;
   const uint32_t nt = cj.ntargets;
+  if (kMulti && mode == kModeDone && nt == 0) return;
+  if (lead && mode != kModeDone) zero_counters(&ring[(j + 1) % kCtrRing]);
   const bool queue = mode == kModeSparse;
   uint32_t touched = 0, clamped = 0;
   const uint32_t stride = gridDim.x * kApplyThreads;
@@ -424,46 +347,70 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
     const uint32_t i = base + threadIdx.x;
     uint32_t u[kCand];
     int n = 0;
-    if (i < nt) {
-      const uint32_t t = tlist[i];
-      apply_one<T>(g, f, dc, tmark, t, ep, touched, clamped);
-      if (queue) n = queue_neighbourhood(t, wmark, d, u);
-    }
+    uint32_t dk = 0;
+    const bool has = i < nt;
+    if (has)
+      n = apply_target<T, kMulti>(tlist[i], d0, doms, shift, ds, tmark, wmark, queue, dk,
+                                  touched, clamped, u);
+    if (kMulti) dom_add_targets(has, dk, ds);
     if (queue) {
       uint32_t pos = block_reserve<kApplyThreads>((uint32_t)n, &cj.nwork);
       for (int q = 0; q < n; q++) wlist[pos + q] = u[q];
     }
   }
-  uint32_t ts = wave_sum(touched), cs = wave_sum(clamped);
-  if (lane_id() == 0) {
-    if (ts) atomicAdd(&run->touched, (unsigned long long)ts);
-    if (cs) atomicAdd(&run->clamped, (unsigned long long)cs);
+  if (!kMulti) {
+    uint32_t ts = wave_sum(touched), cs = wave_sum(clamped);
+    if (lane_id() == 0) {
+      if (ts) atomicAdd(&run->touched, (unsigned long long)ts);
+      if (cs) atomicAdd(&run->clamped, (unsigned long long)cs);
+    }
   }
   if (lead) atomicAdd(&run->total_targets, (unsigned long long)nt);
 }
 
+// Work-list detection of one listed voxel id (shared by k_detect_list / k_tail).
+template <typename T, bool kMulti>
+__device__ __forceinline__ int detect_listed(uint32_t v, const Dom &d0, const Dom *doms,
+                                             uint32_t shift, const DomState *ds,
+                                             uint32_t *tmark, uint32_t &dk, bool &has,
+                                             uint32_t (&vc)[3], uint32_t (&out)[4]) {
+  uint32_t local;
+  const Dom &D = dom_of<kMulti>(v, d0, doms, shift, dk, local);
+  uint32_t fl = kDomActive | kDomEmit;
+  if (kMulti) fl = ds[dk].flags;
+  has = (fl & kDomActive) != 0;
+  if (!has) return 0;
+  return detect_one<T>(D, tmark, local, kMulti && D.padded, (fl & kDomEmit) != 0, vc, out);
+}
+
 // Sparse detection j+1 over the work list queued by apply j (grid-stride).
-template <typename T>
+template <typename T, bool kMulti>
 __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
-    const T *__restrict__ g, const uint8_t *__restrict__ code, uint32_t *__restrict__ tmark,
-    uint32_t *__restrict__ tlist, uint32_t *__restrict__ wmark,
-    const uint32_t *__restrict__ wlist, Counters *__restrict__ ring, RunState *run, uint32_t j,
-    Grid3 d) {
+    Dom d0, const Dom *__restrict__ doms, uint32_t shift, DomState *__restrict__ ds,
+    uint32_t *__restrict__ tmark, uint32_t *__restrict__ tlist, uint32_t *__restrict__ wmark,
+    const uint32_t *__restrict__ wlist, Counters *__restrict__ ring, RunState *run, uint32_t j) {
   const Counters &cj = ring[j % kCtrRing];
   if (cj.mode != kModeSparse) return;
   Counters &cn = ring[(j + 1) % kCtrRing];
+  const uint32_t par = (j + 1) & 1u;
   const uint32_t nw = cj.nwork;
   uint32_t lc[3] = {0, 0, 0};
   const uint32_t stride = gridDim.x * kApplyThreads;
   for (uint32_t base = blockIdx.x * kApplyThreads; base < nw; base += stride) {
     const uint32_t i = base + threadIdx.x;
     uint32_t out[4];
+    uint32_t vc[3] = {0, 0, 0};
+    uint32_t dk = 0;
+    bool has = false;
     int n = 0;
     if (i < nw) {
       const uint32_t v = wlist[i];
       wmark[v >> 5] = 0u;  // every set bit is a listed voxel of this sweep
-      n = detect_one<T>(g, code, tmark, v, d, lc, out);
+      n = detect_listed<T, kMulti>(v, d0, doms, shift, ds, tmark, dk, has, vc, out);
     }
+    if (kMulti) dom_add3(has, dk, vc, ds, par);
+#pragma unroll
+    for (int q = 0; q < 3; q++) lc[q] += vc[q];
     uint32_t pos = block_reserve<kApplyThreads>((uint32_t)n, &cn.ntargets);
     for (int q = 0; q < n; q++) tlist[pos + q] = out[q];
   }
@@ -474,13 +421,14 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
 
 // Single-workgroup tail: runs apply / detect sweeps to the end once the target
 // sets are small, with workgroup barriers between the phases.
-template <typename T>
+template <typename T, bool kMulti>
 __global__ __launch_bounds__(kTailThreads) void k_tail(
-    T *__restrict__ g, const T *__restrict__ f, uint8_t *__restrict__ dc,
-    const uint8_t *__restrict__ code, uint32_t *__restrict__ tmark, uint32_t *__restrict__ tlist,
+    Dom d0, const Dom *__restrict__ doms, uint32_t shift, uint32_t ndom,
+    DomState *__restrict__ ds, uint32_t *__restrict__ tmark, uint32_t *__restrict__ tlist,
     uint32_t *__restrict__ wmark, uint32_t *__restrict__ wlist, Counters *__restrict__ ring,
-    RunState *run, Grid3 d, EditParams ep, Sched s) {
+    RunState *run, Sched s) {
   __shared__ uint32_t s_ctl[2];
+  __shared__ uint32_t s_act, s_cap;
   uint32_t j = run->tail_j;
   if (j == 0 || run->done) return;
   const bool lead = threadIdx.x == 0;
@@ -490,7 +438,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
     Counters &cj = ring[j % kCtrRing];
     if (lead) {
       uint32_t done;
-      const uint32_t mode = decide(cj, j, s, false, &done);
+      const uint32_t mode =
+          kMulti ? decide_multi(cj, s, false, &done) : decide(cj, j, s, false, &done);
       s_ctl[0] = mode;
       s_ctl[1] = done;
       cj.mode = mode == kModeDone ? kModeDone : kModeTail;
@@ -501,20 +450,22 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
     }
     __syncthreads();
     const uint32_t mode = s_ctl[0], done = s_ctl[1];
-    if (done == 1 || (done == 2 && s.mode == PMSZ_SYNC)) break;
-    if (lead) zero_counters(&ring[(j + 1) % kCtrRing]);
+    if (!kMulti && (done == 1 || (done == 2 && s.mode == PMSZ_SYNC))) break;
+    const uint32_t nt = ld_agent(&cj.ntargets);
+    if (kMulti && mode == kModeDone && nt == 0) break;
+    if (lead && mode != kModeDone) zero_counters(&ring[(j + 1) % kCtrRing]);
     // apply sweep j, queueing the next work list unless this was the last sweep
-    const uint32_t nt = cj.ntargets;
     const bool queue = mode != kModeDone;
     for (uint32_t base = 0; base < nt; base += kTailThreads) {
       const uint32_t i = base + threadIdx.x;
       uint32_t u[kCand];
       int n = 0;
-      if (i < nt) {
-        const uint32_t t = tlist[i];
-        apply_one<T>(g, f, dc, tmark, t, ep, touched, clamped);
-        if (queue) n = queue_neighbourhood(t, wmark, d, u);
-      }
+      uint32_t dk = 0;
+      const bool has = i < nt;
+      if (has)
+        n = apply_target<T, kMulti>(tlist[i], d0, doms, shift, ds, tmark, wmark, queue, dk,
+                                    touched, clamped, u);
+      if (kMulti) dom_add_targets(has, dk, ds);
       if (queue) {
         uint32_t pos = block_reserve<kTailThreads>((uint32_t)n, &cj.nwork);
         for (int q = 0; q < n; q++) wlist[pos + q] = u[q];
@@ -522,35 +473,301 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
     }
     tt += nt;
     __syncthreads();
-    if (!queue) break;  // lazy sweep cap: sweep j's edits landed, no detection j+1
+    if (!queue) break;  // sweep cap: sweep j's edits landed, no detection j+1
     // detection j+1 over the work list
     Counters &cn = ring[(j + 1) % kCtrRing];
-    const uint32_t nw = cj.nwork;
+    const uint32_t par = (j + 1) & 1u;
+    const uint32_t nw = ld_agent(&cj.nwork);
     uint32_t lc[3] = {0, 0, 0};
     for (uint32_t base = 0; base < nw; base += kTailThreads) {
       const uint32_t i = base + threadIdx.x;
       uint32_t out[4];
+      uint32_t vc[3] = {0, 0, 0};
+      uint32_t dk = 0;
+      bool has = false;
       int n = 0;
       if (i < nw) {
         const uint32_t v = wlist[i];
         atomicAnd(&wmark[v >> 5], ~(1u << (v & 31u)));
-        n = detect_one<T>(g, code, tmark, v, d, lc, out);
+        n = detect_listed<T, kMulti>(v, d0, doms, shift, ds, tmark, dk, has, vc, out);
       }
+      if (kMulti) dom_add3(has, dk, vc, ds, par);
+#pragma unroll
+      for (int q = 0; q < 3; q++) lc[q] += vc[q];
       uint32_t pos = block_reserve<kTailThreads>((uint32_t)n, &cn.ntargets);
       for (int q = 0; q < n; q++) tlist[pos + q] = out[q];
     }
     block_add3<kTailThreads>(lc, cn.cnt);
     sv += nw;
+    if (kMulti) {
+      if (lead) {
+        s_act = 0;
+        s_cap = 0;
+      }
+      __syncthreads();
+      for (uint32_t k = threadIdx.x; k < ndom; k += kTailThreads) {
+        uint32_t a, c;
+        dom_update(ds[k], j + 1, kOpDetect, s, 1u, a, c);
+        if (a) atomicAdd(&s_act, 1u);
+        if (c) atomicAdd(&s_cap, 1u);
+      }
+      __syncthreads();
+      if (lead) {
+        cn.nactive = s_act;
+        cn.ncapped = s_cap;
+      }
+    }
     j++;
   }
-  uint32_t ts = wave_sum(touched), cs = wave_sum(clamped);
-  if (lane_id() == 0) {
-    if (ts) atomicAdd(&run->touched, (unsigned long long)ts);
-    if (cs) atomicAdd(&run->clamped, (unsigned long long)cs);
+  if (!kMulti) {
+    uint32_t ts = wave_sum(touched), cs = wave_sum(clamped);
+    if (lane_id() == 0) {
+      if (ts) atomicAdd(&run->touched, (unsigned long long)ts);
+      if (cs) atomicAdd(&run->clamped, (unsigned long long)cs);
+    }
   }
   if (lead) {
     atomicAdd(&run->total_targets, tt);
     atomicAdd(&run->sparse_voxels, sv);
+  }
+}
+
+// Hand-over to the check detection after a ghost sync: detection m+1 runs over
+// the work list that the sync queues into slot m.
+__global__ void k_reset_phase(Counters *ring, RunState *run, uint32_t m) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  Counters &cm = ring[m % kCtrRing];
+  cm.mode = kModeSparse;
+  cm.nwork = 0;
+  zero_counters(&ring[(m + 1) % kCtrRing]);
+  run->done = 0;
+  run->tail_j = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Subdomains: padded blocks, the delta-counter ghost sync, the gather.
+
+// Index of the stencil direction with components (ax, ay, az), -1 if none.
+__device__ __forceinline__ int dir_index(int ax, int ay, int az) {
+  int r = -1;
+#pragma unroll
+  for (int d = 0; d < 14; d++)
+    if (dir_comp(d, 0) == ax && dir_comp(d, 1) == ay && dir_comp(d, 2) == az) r = d;
+  return r;
+}
+
+__device__ __forceinline__ bool sub_exists(const int32_t *cc, int d, int sx, int sy, int sz) {
+  const int a = cc[0] + dir_comp(d, 0), b = cc[1] + dir_comp(d, 1), c = cc[2] + dir_comp(d, 2);
+  return a >= 0 && a < sx && b >= 0 && b < sy && c >= 0 && c < sz;
+}
+__device__ __forceinline__ int sub_of(const int32_t *cc, int d, int sx, int sy) {
+  return (cc[0] + dir_comp(d, 0)) + sx * ((cc[1] + dir_comp(d, 1)) + sy * (cc[2] + dir_comp(d, 2)));
+}
+
+// The padded blocks of every subdomain (blockIdx.y): the core of the global
+// field (read_subblock, pMSz.cu:274-313) and the initial ghost values
+// (exchange_ghost_layers, pMSz.cu:320-608).  A halo cell in an exchanged
+// stencil direction whose neighbour subdomain exists takes the neighbour's core
+// value; every other halo cell holds the sentinel.  The reference exchanges the
+// faces and the four edges d = 6..9 (pMSz.cu:394-488); the full policy all 14.
+template <typename T>
+__global__ __launch_bounds__(kSyncThreads) void k_scatter(const Dom *__restrict__ doms,
+                                                         const T *__restrict__ F,
+                                                         const T *__restrict__ G, int64_t gnx,
+                                                         int64_t gny, int sx, int sy, int sz,
+                                                         int policy, RunState *run) {
+  const Dom &D = doms[blockIdx.y];
+  T *fp = (T *)const_cast<void *>(D.f);
+  T *gp = (T *)D.g;
+  uint32_t err = 0;
+  const uint32_t n = (uint32_t)D.d.n;
+  for (uint32_t i = blockIdx.x * kSyncThreads + threadIdx.x; i < n;
+       i += gridDim.x * kSyncThreads) {
+    int64_t x, y, z;
+    decompose(i, D.d, x, y, z);
+    const int ax = x == 0 ? -1 : (x == D.d.nx - 1 ? 1 : 0);
+    const int ay = y == 0 ? -1 : (y == D.d.ny - 1 ? 1 : 0);
+    const int az = z == 0 ? -1 : (z == D.d.nz - 1 ? 1 : 0);
+    const bool core = ax == 0 && ay == 0 && az == 0;
+    bool take = core;
+    if (!core) {
+      const int d = dir_index(ax, ay, az);
+      take = d >= 0 && (policy == PMSZ_GHOST_FULL || d < 10) && sub_exists(D.cc, d, sx, sy, sz);
+    }
+    T fv = sentinel<T>(), gv = sentinel<T>();
+    if (take) {
+      const int64_t gx = x - 1 + D.off[0], gy = y - 1 + D.off[1], gz = z - 1 + D.off[2];
+      const int64_t gi = gx + gnx * (gy + gny * gz);
+      fv = F[gi];
+      gv = G[gi];
+      if (core && (!(fv == fv) || fv == sentinel<T>() || !(gv == gv) || gv == sentinel<T>()))
+        err = 1;
+    }
+    fp[i] = fv;
+    gp[i] = gv;
+    D.dc[i] = 0;
+  }
+  if (__ballot(err != 0) && lane_id() == 0) atomicOr(&run->err, 1u);
+}
+
+// Snapshot of the delta counters of every shell cell (the pack of
+// exchange_ghost_layers_cuda_aware, pMSz.cu:930-1001): a subdomain with no
+// boundary edit since the last sync sends nothing; cells outside the global
+// field or holding the sentinel send 0.  Runs before any unpack.
+template <typename T>
+__global__ __launch_bounds__(kSyncThreads) void k_pack(const Dom *__restrict__ doms,
+                                                      const DomState *__restrict__ ds,
+                                                      int64_t gnx, int64_t gny, int64_t gnz,
+                                                      const Counters *__restrict__ ring,
+                                                      uint32_t j, uint32_t gate) {
+  if (gate) {
+    const uint32_t m = ring[j % kCtrRing].mode;
+    if (m != kModeDense && m != kModeSparse) return;
+  }
+  const uint32_t k = blockIdx.y;
+  const Dom &D = doms[k];
+  const bool send = (ds[k].flags & kDomSend) != 0;
+  const T *g = (const T *)D.g;
+  const uint32_t P0 = (uint32_t)D.d.nx, P1 = (uint32_t)D.d.ny, P2 = (uint32_t)D.d.nz;
+  for (uint32_t i = blockIdx.x * kSyncThreads + threadIdx.x; i < D.shell_n;
+       i += gridDim.x * kSyncThreads) {
+    uint32_t x, y, z;
+    shell_cell(i, P0, P1, P2, x, y, z);
+    const uint32_t li = x + P0 * (y + P1 * z);
+    uint8_t v = 0;
+    if (send) {
+      const int64_t gx = (int64_t)x - 1 + D.off[0], gy = (int64_t)y - 1 + D.off[1],
+                    gz = (int64_t)z - 1 + D.off[2];
+      const bool in = gx >= 0 && gx < gnx && gy >= 0 && gy < gny && gz >= 0 && gz < gnz;
+      if (in && g[li] != sentinel<T>()) v = D.dc[li];
+    }
+    D.snap[li] = v;
+  }
+}
+
+// Receive side of the delta-counter sync (unpack, pMSz.cu:1015-1103): each shell
+// cell of subdomain blockIdx.y merges, in direction order d = 0..ndir-1, the
+// snapshot value the neighbour in direction d sent for the same global cell --
+// the cell's layer (0 or 1) in the receive region toward d is the same along
+// every nonzero component of d.  A larger value r replaces the counter and
+// lowers g by delta * (r - cur), or clamps when r reaches q.  Cells whose value
+// changed queue their classified neighbourhood for the next detection.
+template <typename T>
+__global__ __launch_bounds__(kSyncThreads) void k_unpack(const Dom *__restrict__ doms,
+                                                        DomState *__restrict__ ds, int sx,
+                                                        int sy, int sz, int ndir,
+                                                        uint32_t *__restrict__ wmark,
+                                                        uint32_t *__restrict__ wlist,
+                                                        Counters *__restrict__ ring, uint32_t j,
+                                                        uint32_t gate) {
+  Counters &cj = ring[j % kCtrRing];
+  const uint32_t mode = cj.mode;
+  if (gate && mode != kModeDense && mode != kModeSparse) return;
+  const bool queue = mode == kModeSparse;
+  const uint32_t k = blockIdx.y;
+  const Dom &D = doms[k];
+  T *g = (T *)D.g;
+  const T *f = (const T *)D.f;
+  const T delta = (T)D.ep.delta_d;
+  const int L0 = (int)D.d.nx - 2, L1 = (int)D.d.ny - 2, L2 = (int)D.d.nz - 2;
+  const uint32_t P0 = (uint32_t)D.d.nx, P1 = (uint32_t)D.d.ny, P2 = (uint32_t)D.d.nz;
+  for (uint32_t base = blockIdx.x * kSyncThreads; base < D.shell_n;
+       base += gridDim.x * kSyncThreads) {
+    const uint32_t i = base + threadIdx.x;
+    uint32_t u[kCand];
+    int n = 0;
+    if (i < D.shell_n) {
+      uint32_t x, y, z;
+      shell_cell(i, P0, P1, P2, x, y, z);
+      const uint32_t li = x + P0 * (y + P1 * z);
+      T gv = g[li];
+      if (gv != sentinel<T>()) {
+        uint8_t cur = D.dc[li];
+        bool changed = false;
+        const int c[3] = {(int)x, (int)y, (int)z};
+        const int L[3] = {L0, L1, L2};
+#pragma unroll
+        for (int d = 0; d < 14; d++) {
+          if (d >= ndir) continue;
+          bool in = true;
+          int layer = -1;
+#pragma unroll
+          for (int a = 0; a < 3; a++) {
+            const int dd = dir_comp(d, a);
+            if (dd == 0) {
+              in = in && c[a] >= 1 && c[a] <= L[a];
+            } else {
+              const int l = dd < 0 ? c[a] : L[a] + 1 - c[a];
+              in = in && (l == 0 || l == 1) && (layer < 0 || layer == l);
+              layer = l;
+            }
+          }
+          if (!in || !sub_exists(D.cc, d, sx, sy, sz)) continue;
+          const Dom &B = doms[sub_of(D.cc, d, sx, sy)];
+          const int64_t bx = (int64_t)x + D.off[0] - B.off[0];
+          const int64_t by = (int64_t)y + D.off[1] - B.off[1];
+          const int64_t bz = (int64_t)z + D.off[2] - B.off[2];
+          const uint8_t r = B.snap[bx + B.d.nx * (by + B.d.ny * bz)];
+          if (r == 0 || cur >= r) continue;
+          if (r < D.ep.q) {
+            const T step = delta * (T)(r - cur);
+            gv = gv - step;
+          } else {
+            gv = clamp_value<T>(f[li], D.ep);
+          }
+          cur = r;
+          changed = true;
+        }
+        if (changed) {
+          g[li] = gv;
+          D.dc[li] = cur;
+          if (queue) n = queue_neighbourhood(li, x, y, z, D, wmark, u);
+        }
+      }
+    }
+    if (queue) {
+      uint32_t pos = block_reserve<kSyncThreads>((uint32_t)n, &cj.nwork);
+      for (int q = 0; q < n; q++) wlist[pos + q] = u[q];
+    }
+  }
+  // send_flag = 0 after the exchange (pMSz.cu:2000-2007); the pack has read it
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAnd(&ds[k].flags, ~kDomSend);
+}
+
+// Per-domain end-of-run statistics over the classified box (touched: dc > 0,
+// clamped: dc == q) and, with G non-null, the gather of the cores of padded
+// blocks back into the global field and counters (the D2H copy, pMSz.cu:2971).
+template <typename T>
+__global__ __launch_bounds__(kSyncThreads) void k_finish(const Dom *__restrict__ doms,
+                                                        DomState *__restrict__ ds,
+                                                        T *__restrict__ G,
+                                                        uint8_t *__restrict__ DC, int64_t gnx,
+                                                        int64_t gny) {
+  const uint32_t k = blockIdx.y;
+  const Dom &D = doms[k];
+  const uint32_t bx = (uint32_t)(D.hi[0] - D.lo[0]), by = (uint32_t)(D.hi[1] - D.lo[1]),
+                 bz = (uint32_t)(D.hi[2] - D.lo[2]);
+  const uint32_t n = bx * by * bz;
+  uint32_t touched = 0, clamped = 0;
+  for (uint32_t i = blockIdx.x * kSyncThreads + threadIdx.x; i < n;
+       i += gridDim.x * kSyncThreads) {
+    const uint32_t x = i % bx, y = (i / bx) % by, z = i / (bx * by);
+    const int64_t li = (int64_t)(x + D.lo[0]) +
+                       D.d.nx * ((int64_t)(y + D.lo[1]) + D.d.ny * (int64_t)(z + D.lo[2]));
+    const uint8_t c = D.dc[li];
+    touched += c > 0;
+    clamped += c == D.ep.q;
+    if (G) {
+      const int64_t gi = (int64_t)(x + D.off[0]) + gnx * ((int64_t)(y + D.off[1]) +
+                                                          gny * (int64_t)(z + D.off[2]));
+      G[gi] = ((const T *)D.g)[li];
+      if (DC) DC[gi] = c;
+    }
+  }
+  const uint32_t ts = wave_sum(touched), cs = wave_sum(clamped);
+  if (lane_id() == 0) {
+    if (ts) atomicAdd(&ds[k].touched, (unsigned long long)ts);
+    if (cs) atomicAdd(&ds[k].clamped, (unsigned long long)cs);
   }
 }
 
@@ -609,23 +826,6 @@ __global__ void k_edit_symbols(const T *__restrict__ f, const T *__restrict__ g0
 // ---------------------------------------------------------------------------
 // Host side.
 
-struct Ctx {
-  int device = -1;
-  int cus = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
-  int64_t cap = 0;  // voxels
-  uint8_t *code = nullptr, *dc = nullptr;
-  uint32_t *tmark = nullptr, *wmark = nullptr, *tlist = nullptr, *wlist = nullptr;
-  Counters *ring = nullptr;
-  RunState *run = nullptr;
-  Counters *h_ring = nullptr;  // pinned mirror of the ring
-  RunState *h_run = nullptr;   // pinned mirror of the run state
-};
-
-std::mutex g_mu;
-Ctx g_ctx;
-
 hipError_t &last_hip_error() {
   static hipError_t e = hipSuccess;
   return e;
@@ -640,48 +840,74 @@ hipError_t &last_hip_error() {
     }                        \
   } while (0)
 
-void release(Ctx &c) {
-  (void)hipFree(c.code);
-  (void)hipFree(c.dc);
-  (void)hipFree(c.tmark);
-  (void)hipFree(c.wmark);
-  (void)hipFree(c.tlist);
-  (void)hipFree(c.wlist);
-  c.code = c.dc = nullptr;
-  c.tmark = c.wmark = c.tlist = c.wlist = nullptr;
-  c.cap = 0;
-}
+// A device buffer that only grows.
+template <typename E>
+struct Buf {
+  E *p = nullptr;
+  size_t cap = 0;  // elements
+  int ensure(size_t n) {
+    if (n <= cap) return PMSZ_OK;
+    (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(&p, sizeof(E) * std::max<size_t>(n, 1)) != hipSuccess) {
+      p = nullptr;
+      return PMSZ_ENOMEM;
+    }
+    cap = n;
+    return PMSZ_OK;
+  }
+};
 
-int ensure(Ctx &c, int64_t n) {
+struct Ctx {
+  int device = -1;
+  int cus = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  Buf<uint8_t> code, dc, snap;   // per stored cell
+  Buf<uint8_t> fpad, gpad;       // padded subdomain blocks (bytes)
+  Buf<uint32_t> tmark, wmark;    // one bit per voxel id
+  Buf<uint32_t> tlist, wlist;    // voxel ids
+  Buf<Dom> doms;
+  Buf<DomState> ds;
+  Counters *ring = nullptr;
+  RunState *run = nullptr;
+  Counters *h_ring = nullptr;  // pinned mirror of the ring
+  RunState *h_run = nullptr;   // pinned mirror of the run state
+  DomState *h_ds = nullptr;    // pinned mirror of the domain states
+  size_t h_ds_cap = 0;
+};
+
+std::mutex g_mu;
+Ctx g_ctx;
+
+int ctx_device(Ctx &c) {
   int dev = 0;
   PMSZ_HIP(hipGetDevice(&dev));
-  if (c.device != dev) {
-    // a new device: the state of the previous one is left to process exit
-    c = Ctx();
-    c.device = dev;
-    PMSZ_HIP(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, dev));
-    PMSZ_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-    PMSZ_HIP(hipEventCreate(&c.ev0));
-    PMSZ_HIP(hipEventCreate(&c.ev1));
-    PMSZ_HIP(hipEventCreate(&c.ev2));
-    PMSZ_HIP(hipEventCreate(&c.ev3));
-    PMSZ_HIP(hipMalloc(&c.ring, sizeof(Counters) * kCtrRing));
-    PMSZ_HIP(hipMalloc(&c.run, sizeof(RunState)));
-    PMSZ_HIP(hipHostMalloc(&c.h_ring, sizeof(Counters) * kCtrRing, hipHostMallocDefault));
-    PMSZ_HIP(hipHostMalloc(&c.h_run, sizeof(RunState), hipHostMallocDefault));
-  }
-  if (n <= c.cap) return PMSZ_OK;
-  release(c);
-  int64_t words = (n + 31) / 32 + 1;
-  if (hipMalloc(&c.code, (size_t)n) != hipSuccess || hipMalloc(&c.dc, (size_t)n) != hipSuccess ||
-      hipMalloc(&c.tmark, sizeof(uint32_t) * words) != hipSuccess ||
-      hipMalloc(&c.wmark, sizeof(uint32_t) * words) != hipSuccess ||
-      hipMalloc(&c.tlist, sizeof(uint32_t) * (size_t)n) != hipSuccess ||
-      hipMalloc(&c.wlist, sizeof(uint32_t) * (size_t)n) != hipSuccess) {
-    release(c);
-    return PMSZ_ENOMEM;
-  }
-  c.cap = n;
+  if (c.device == dev) return PMSZ_OK;
+  // a new device: the state of the previous one is left to process exit
+  c = Ctx();
+  c.device = dev;
+  PMSZ_HIP(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, dev));
+  PMSZ_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  PMSZ_HIP(hipEventCreate(&c.ev0));
+  PMSZ_HIP(hipEventCreate(&c.ev1));
+  PMSZ_HIP(hipEventCreate(&c.ev2));
+  PMSZ_HIP(hipEventCreate(&c.ev3));
+  PMSZ_HIP(hipMalloc(&c.ring, sizeof(Counters) * kCtrRing));
+  PMSZ_HIP(hipMalloc(&c.run, sizeof(RunState)));
+  PMSZ_HIP(hipHostMalloc(&c.h_ring, sizeof(Counters) * kCtrRing, hipHostMallocDefault));
+  PMSZ_HIP(hipHostMalloc(&c.h_run, sizeof(RunState), hipHostMallocDefault));
+  return PMSZ_OK;
+}
+
+int ensure_host_ds(Ctx &c, size_t n) {
+  if (n <= c.h_ds_cap) return PMSZ_OK;
+  if (c.h_ds) (void)hipHostFree(c.h_ds);
+  c.h_ds = nullptr;
+  c.h_ds_cap = 0;
+  PMSZ_HIP(hipHostMalloc(&c.h_ds, sizeof(DomState) * n, hipHostMallocDefault));
+  c.h_ds_cap = n;
   return PMSZ_OK;
 }
 
@@ -699,6 +925,7 @@ int check_params(const pmsz_params *p) {
   if (p->ghost_policy != 0 && p->ghost_policy != 1) return PMSZ_EINVAL;
   for (int a = 0; a < 3; a++)
     if (p->sub[a] < 1) return PMSZ_EINVAL;
+  if (p->sub[0] > p->nx || p->sub[1] > p->ny || p->sub[2] > p->nz) return PMSZ_EINVAL;
   if (n >= (int64_t)0xffffffffLL) return PMSZ_ETOOBIG;
   return PMSZ_OK;
 }
@@ -715,34 +942,6 @@ Grid3 grid_of(int64_t nx, int64_t ny, int64_t nz) {
   return d;
 }
 
-int64_t tile_count(const Grid3 &d) {
-  return ((d.nx + kTX - 1) / kTX) * ((d.ny + kTY - 1) / kTY) * ((d.nz + kTZ - 1) / kTZ);
-}
-
-// Persistent grid of the tile kernel: a multiple of 8 (one share per XCD), at most
-// the resident capacity and no more than the tiles need.
-unsigned tile_blocks(const Ctx &c, const Grid3 &d) {
-  int64_t per_xcd = (tile_count(d) + 7) / 8;
-  int64_t cap = std::max<int64_t>(1, (int64_t)c.cus * 4 / 8);
-  return (unsigned)(8 * std::max<int64_t>(1, std::min(per_xcd, cap)));
-}
-
-unsigned compact_blocks(const Ctx &c, int64_t words) {
-  const int64_t per = (int64_t)kCompactThreads * kCompactPT;
-  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((words + per - 1) / per, c.cus * 4));
-}
-
-// Persistent grid of the column kernel: one wave per work unit up to about the
-// resident capacity, a multiple of 8 blocks (one share per XCD).
-unsigned col_blocks(const Ctx &c, const Grid3 &d) {
-  const int64_t units = ((d.nx + kColOwned - 1) / kColOwned) * ((d.ny + kColRows - 1) / kColRows) *
-                        ((d.nz + kColZ - 1) / kColZ);
-  const int64_t blocks = (units + kColWaves - 1) / kColWaves;
-  const int64_t per_xcd = (blocks + 7) / 8;
-  const int64_t cap = std::max<int64_t>(1, (int64_t)c.cus * 8 / 8);
-  return (unsigned)(8 * std::max<int64_t>(1, std::min(per_xcd, cap)));
-}
-
 EditParams edit_params(const pmsz_params *p) {
   EditParams ep;
   ep.eb = p->abs_eb;
@@ -752,26 +951,103 @@ EditParams edit_params(const pmsz_params *p) {
   return ep;
 }
 
+// A domain whose classified box is the whole stored block (one field).
+Dom field_dom(const pmsz_params *p, const void *f, void *g, uint8_t *code, uint8_t *dc) {
+  Dom D;
+  std::memset(&D, 0, sizeof(D));
+  D.f = f;
+  D.g = g;
+  D.code = code;
+  D.dc = dc;
+  D.d = grid_of(p->nx, p->ny, p->nz);
+  D.hi[0] = (int32_t)p->nx;
+  D.hi[1] = (int32_t)p->ny;
+  D.hi[2] = (int32_t)p->nz;
+  col_units(p->nx, p->ny, p->nz, D.nux, D.nuy, D.nuz);
+  D.ep = edit_params(p);
+  return D;
+}
+
+uint32_t dom_units(const Dom &D) { return D.nux * D.nuy * D.nuz; }
+
+// Resident blocks per CU of a kernel (its register and LDS use), at least 1.
+template <typename K>
+int occupancy(K kernel, int threads) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0) != hipSuccess)
+    nb = 1;
+  return std::max(nb, 1);
+}
+
+// Persistent grid of the column kernel: one wave per unit up to the resident
+// capacity, a multiple of 8 blocks (one share per XCD).
+unsigned col_blocks(const Ctx &c, uint32_t units, int occ) {
+  const int64_t blocks = ((int64_t)units + kColWaves - 1) / kColWaves;
+  const int64_t per_xcd = (blocks + 7) / 8;
+  const int64_t cap = std::max<int64_t>(1, (int64_t)(c.cus / 8) * occ);
+  return (unsigned)(8 * std::max<int64_t>(1, std::min(per_xcd, cap)));
+}
+
+unsigned compact_blocks(const Ctx &c, int64_t words) {
+  const int64_t per = (int64_t)kCompactThreads * kCompactPT;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((words + per - 1) / per, c.cus * 4));
+}
+
 constexpr uint32_t kChain = 8;  // apply/detect steps enqueued before each host check
 
-template <typename T>
-int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_out,
-                   pmsz_stats *st) {
-  const Grid3 d = grid_of(p->nx, p->ny, p->nz);
-  int rc = ensure(c, d.n);
-  if (rc) return rc;
-  const EditParams ep = edit_params(p);
+// A run: its domains (device table in c.doms), the id layout and the schedule.
+struct Job {
+  std::vector<Dom> doms;
+  uint32_t shift = 32;     // id = (domain << shift) | local (multi-domain runs)
+  uint64_t words = 0;      // bitmap words
+  uint64_t list_cap = 0;   // list entries (distinct cells)
   Sched s;
-  s.mode = p->mode;
-  s.max_sweeps = (uint32_t)p->max_sweeps;
-  s.dense_min_work = (uint32_t)std::min<int64_t>(std::max<int64_t>(d.n / 16, 1), 0xffffffffLL);
-  const int64_t words = (d.n + 31) / 32 + 1;
-  const unsigned grid_list = (unsigned)std::max(1, c.cus * 4);
-  std::memset(st, 0, sizeof(*st));
+  bool subdomains = false; // padded blocks with ghost syncs
+  int sub[3] = {1, 1, 1};
+  int ndir = 14;
+  int64_t gn[3] = {0, 0, 0};
+};
+
+// What a run leaves for the statistics.
+struct Outcome {
+  uint32_t last_j = 0;     // index of the last detection
+  uint32_t done = 0;       // 1 converged, 2 capped
+  int64_t outer_done = 0;  // ghost syncs + checks completed (subdomains)
+  int64_t outer_started = 0;
+  float ms_total = 0, ms_first = 0;
+};
+
+// The chain of one run: detection 1 (fused with the labeling), then apply /
+// detect steps decided on the device, the tail, and -- for subdomains in lazy
+// mode -- a ghost sync and a check detection whenever every subdomain's loop has
+// ended.  In sync mode with subdomains the ghost sync follows every apply.
+template <typename T, bool kMulti>
+int drive(Ctx &c, Job &job, Outcome &out) {
+  const uint32_t ndom = (uint32_t)job.doms.size();
+  const Dom &d0 = job.doms[0];
+  const Dom *doms = kMulti ? c.doms.p : nullptr;
+  DomState *ds = kMulti ? c.ds.p : nullptr;
+  uint32_t nunits = 0;
+  for (const Dom &D : job.doms) nunits += dom_units(D);
+  static const int occ_first = occupancy(k_columns<T, true, true, kMulti>, kColThreads);
+  static const int occ_dense = occupancy(k_columns<T, false, true, kMulti>, kColThreads);
+  const unsigned g_first = col_blocks(c, nunits, occ_first);
+  const unsigned g_dense = col_blocks(c, nunits, occ_dense);
+  const unsigned g_list = (unsigned)std::max(1, c.cus * 4);
+  const unsigned g_compact = compact_blocks(c, (int64_t)job.words);
+  const bool sync_every = job.subdomains && job.s.mode == PMSZ_SYNC;
+  const bool sync_lazy = job.subdomains && job.s.mode == PMSZ_LAZY;
+  const int sx = job.sub[0], sy = job.sub[1], sz = job.sub[2];
+  uint32_t shell_max = 0;
+  for (const Dom &D : job.doms) shell_max = std::max(shell_max, D.shell_n);
+  const dim3 g_shell(
+      (unsigned)std::max<uint32_t>(1, std::min<uint32_t>((shell_max + kSyncThreads - 1) / kSyncThreads,
+                                                         (uint32_t)c.cus * 4)),
+      ndom);
 
   PMSZ_HIP(hipEventRecord(c.ev0, c.stream));
-  PMSZ_HIP(hipMemsetAsync(c.tmark, 0, sizeof(uint32_t) * words, c.stream));
-  PMSZ_HIP(hipMemsetAsync(c.wmark, 0, sizeof(uint32_t) * words, c.stream));
+  PMSZ_HIP(hipMemsetAsync(c.tmark.p, 0, sizeof(uint32_t) * job.words, c.stream));
+  PMSZ_HIP(hipMemsetAsync(c.wmark.p, 0, sizeof(uint32_t) * job.words, c.stream));
   PMSZ_HIP(hipMemsetAsync(c.ring, 0, sizeof(Counters) * kCtrRing, c.stream));
   PMSZ_HIP(hipMemsetAsync(c.run, 0, sizeof(RunState), c.stream));
   // apply(1) reads slot 0's mode: mark detection 1 as a running sweep
@@ -779,58 +1055,180 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
   PMSZ_HIP(hipMemcpyAsync(&c.ring[0].mode, &kRunning, sizeof(uint32_t), hipMemcpyHostToDevice,
                           c.stream));
   PMSZ_HIP(hipEventRecord(c.ev2, c.stream));
-  k_columns<T, true, true><<<col_blocks(c, d), kColThreads, 0, c.stream>>>(
-      f, g, c.code, c.dc, c.tmark, c.tlist, c.ring, 0, 1, c.run, d);
+  k_columns<T, true, true, kMulti><<<g_first, kColThreads, 0, c.stream>>>(
+      d0, doms, ndom, ds, c.tmark.p, c.ring, 0, 1, 1u, c.run);
   PMSZ_HIP(hipGetLastError());
   PMSZ_HIP(hipEventRecord(c.ev3, c.stream));
-  k_compact<<<compact_blocks(c, words), kCompactThreads, 0, c.stream>>>(c.tmark, c.tlist, c.ring,
-                                                                        0, 1, true, (uint32_t)words);
+  k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(c.tmark.p, c.tlist.p, c.ring, 0, 1,
+                                                         true, (uint32_t)job.words);
+  if (kMulti)
+    k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, 0, 0, 1, kOpDetect, job.s, 1);
 
   uint32_t j0 = 1;
+  out.outer_started = 1;
   while (true) {
     for (uint32_t j = j0; j < j0 + kChain; j++) {
-      k_apply<T><<<grid_list, kApplyThreads, 0, c.stream>>>(g, f, c.dc, c.tmark, c.tlist,
-                                                            c.wmark, c.wlist, c.ring, c.run, j,
-                                                            d, ep, s);
-      k_columns<T, false, true><<<col_blocks(c, d), kColThreads, 0, c.stream>>>(
-          nullptr, g, c.code, nullptr, c.tmark, c.tlist, c.ring, j % kCtrRing,
-          (j + 1) % kCtrRing, c.run, d);
-      k_compact<<<compact_blocks(c, words), kCompactThreads, 0, c.stream>>>(
-          c.tmark, c.tlist, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false, (uint32_t)words);
-      k_detect_list<T><<<grid_list, kApplyThreads, 0, c.stream>>>(
-          g, c.code, c.tmark, c.tlist, c.wmark, c.wlist, c.ring, c.run, j, d);
+      k_apply<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
+          d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j,
+          job.s);
+      if (sync_every) {
+        k_pack<T><<<g_shell, kSyncThreads, 0, c.stream>>>(doms, ds, job.gn[0], job.gn[1],
+                                                          job.gn[2], c.ring, j, 1);
+        k_unpack<T><<<g_shell, kSyncThreads, 0, c.stream>>>(doms, ds, sx, sy, sz, job.ndir,
+                                                            c.wmark.p, c.wlist.p, c.ring, j, 1);
+      }
+      k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
+          d0, doms, ndom, ds, c.tmark.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, (j + 1) & 1u,
+          c.run);
+      k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
+          c.tmark.p, c.tlist.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false,
+          (uint32_t)job.words);
+      k_detect_list<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
+          d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j);
+      if (kMulti)
+        k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, j, 1, j + 1, kOpDetect,
+                                                   job.s, 1);
     }
-    k_tail<T><<<1, kTailThreads, 0, c.stream>>>(g, f, c.dc, c.code, c.tmark, c.tlist, c.wmark,
-                                                c.wlist, c.ring, c.run, d, ep, s);
+    if (job.s.tail_ok)
+      k_tail<T, kMulti><<<1, kTailThreads, 0, c.stream>>>(d0, doms, job.shift, ndom, ds,
+                                                          c.tmark.p, c.tlist.p, c.wmark.p,
+                                                          c.wlist.p, c.ring, c.run, job.s);
     PMSZ_HIP(hipGetLastError());
     PMSZ_HIP(hipMemcpyAsync(c.h_run, c.run, sizeof(RunState), hipMemcpyDeviceToHost, c.stream));
     PMSZ_HIP(hipStreamSynchronize(c.stream));
     if (c.h_run->err) return PMSZ_EINVAL;
-    if (c.h_run->done) break;
-    // the chain ended before the tail took over: continue it where it stopped
-    j0 += kChain;
+    if (!c.h_run->done) {
+      // the chain ended before the tail took over: continue it where it stopped
+      j0 += kChain;
+      continue;
+    }
+    const uint32_t m = c.h_run->last_j;
+    if (!(sync_lazy && c.h_run->done == 1)) {
+      out.last_j = m;
+      out.done = c.h_run->done;
+      break;
+    }
+    // every subdomain's inner loop has ended: ghost sync, then the check sweep
+    // (pMSz.cu:1991-2060), which doubles as the first sweep of the next loops
+    out.outer_done++;
+    const uint32_t emit_ok = out.outer_done < (int64_t)job.s.max_sweeps ? 1u : 0u;
+    k_reset_phase<<<1, 64, 0, c.stream>>>(c.ring, c.run, m);
+    k_pack<T><<<g_shell, kSyncThreads, 0, c.stream>>>(doms, ds, job.gn[0], job.gn[1], job.gn[2],
+                                                      c.ring, m, 0);
+    k_unpack<T><<<g_shell, kSyncThreads, 0, c.stream>>>(doms, ds, sx, sy, sz, job.ndir,
+                                                        c.wmark.p, c.wlist.p, c.ring, m, 0);
+    k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, 0, 0, m + 1, kOpPreCheck, job.s,
+                                               emit_ok);
+    k_detect_list<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
+        d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, m);
+    k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, 0, 0, m + 1, kOpCheck, job.s, 1);
+    PMSZ_HIP(hipGetLastError());
+    PMSZ_HIP(hipMemcpyAsync(c.h_ring + ((m + 1) % kCtrRing), c.ring + ((m + 1) % kCtrRing),
+                            sizeof(Counters), hipMemcpyDeviceToHost, c.stream));
+    PMSZ_HIP(hipStreamSynchronize(c.stream));
+    const Counters &cc = c.h_ring[(m + 1) % kCtrRing];
+    if (cc.nactive == 0) {
+      out.last_j = m + 1;
+      out.done = cc.ncapped ? 2 : 1;
+      break;
+    }
+    out.outer_started++;
+    j0 = m + 1;
   }
   PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
   PMSZ_HIP(hipMemcpyAsync(c.h_ring, c.ring, sizeof(Counters) * kCtrRing, hipMemcpyDeviceToHost,
                           c.stream));
-  if (dc_out)
-    PMSZ_HIP(hipMemcpyAsync(dc_out, c.dc, (size_t)d.n, hipMemcpyDeviceToDevice, c.stream));
   PMSZ_HIP(hipStreamSynchronize(c.stream));
-  float ms = 0, dms = 0;
-  PMSZ_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
-  PMSZ_HIP(hipEventElapsedTime(&dms, c.ev2, c.ev3));
-  st->t_total_s = ms * 1e-3;
-  st->t_dense_s = dms * 1e-3;
+  PMSZ_HIP(hipEventElapsedTime(&out.ms_total, c.ev0, c.ev1));
+  PMSZ_HIP(hipEventElapsedTime(&out.ms_first, c.ev2, c.ev3));
+  return PMSZ_OK;
+}
 
+// Bitmap words and list capacity of a run, allocating what is missing.
+int prepare(Ctx &c, Job &job) {
+  uint64_t cells = 0, ids = 0;
+  for (const Dom &D : job.doms) cells += (uint64_t)D.d.n;
+  if (job.doms.size() == 1) ids = (uint64_t)job.doms[0].d.n;
+  else ids = (uint64_t)job.doms.size() << job.shift;
+  job.words = (ids + 31) / 32 + 1;
+  job.list_cap = cells;
+  int rc;
+  if ((rc = c.tmark.ensure(job.words)) || (rc = c.wmark.ensure(job.words)) ||
+      (rc = c.tlist.ensure(job.list_cap)) || (rc = c.wlist.ensure(job.list_cap)))
+    return rc;
+  if (job.doms.size() > 1 || job.subdomains) {
+    if ((rc = c.doms.ensure(job.doms.size())) || (rc = c.ds.ensure(job.doms.size())) ||
+        (rc = ensure_host_ds(c, job.doms.size())))
+      return rc;
+    PMSZ_HIP(hipMemcpyAsync(c.doms.p, job.doms.data(), sizeof(Dom) * job.doms.size(),
+                            hipMemcpyHostToDevice, c.stream));
+    DomState init;
+    std::memset(&init, 0, sizeof(init));
+    init.flags = kDomActive | kDomEmit;
+    for (size_t k = 0; k < job.doms.size(); k++) c.h_ds[k] = init;
+    PMSZ_HIP(hipMemcpyAsync(c.ds.p, c.h_ds, sizeof(DomState) * job.doms.size(),
+                            hipMemcpyHostToDevice, c.stream));
+  }
+  return PMSZ_OK;
+}
+
+uint32_t ceil_log2(uint64_t v) {
+  uint32_t s = 0;
+  while (((uint64_t)1 << s) < v) s++;
+  return s;
+}
+
+Sched make_sched(const pmsz_params *p, uint64_t work_voxels, bool tail_ok) {
+  Sched s;
+  s.mode = p->mode;
+  s.max_sweeps = (uint32_t)p->max_sweeps;
+  s.dense_min_work = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(work_voxels / 16, 1),
+                                                  0xffffffffull);
+  s.tail_ok = tail_ok ? 1u : 0u;
+  return s;
+}
+
+void fill_single_stats(pmsz_stats *st, int mode, uint32_t m, bool conv) {
+  if (mode == PMSZ_LAZY) {
+    st->edit_sweeps = m;
+    st->check_sweeps = conv ? 1 : 0;
+    st->outer_iters = conv ? 1 : 0;
+    st->ite = m + 1;
+  } else {
+    int64_t applies = conv ? std::max<int64_t>((int64_t)m - 1, 1) : (int64_t)m - 1;
+    st->edit_sweeps = applies;
+    st->check_sweeps = applies;
+    st->outer_iters = applies;
+    st->ite = 2 * applies;
+  }
+}
+
+template <typename T>
+int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_out,
+                   pmsz_stats *st) {
+  const int64_t n = p->nx * p->ny * p->nz;
+  int rc;
+  if ((rc = c.code.ensure(n)) || (rc = c.dc.ensure(n))) return rc;
+  Job job;
+  job.doms.push_back(field_dom(p, f, g, c.code.p, c.dc.p));
+  job.s = make_sched(p, (uint64_t)n, true);
+  if ((rc = prepare(c, job))) return rc;
+  std::memset(st, 0, sizeof(*st));
+  Outcome o;
+  rc = drive<T, false>(c, job, o);
+  if (rc) return rc;
+  if (dc_out)
+    PMSZ_HIP(hipMemcpyAsync(dc_out, c.dc.p, (size_t)n, hipMemcpyDeviceToDevice, c.stream));
+  PMSZ_HIP(hipStreamSynchronize(c.stream));
+  st->t_total_s = o.ms_total * 1e-3;
+  st->t_dense_s = o.ms_first * 1e-3;
   const RunState &r = *c.h_run;
-  const uint32_t m = r.last_j;
+  const uint32_t m = o.last_j;
   const Counters &cm = c.h_ring[m % kCtrRing];
-  const bool conv = r.done == 1;
+  const bool conv = o.done == 1;
   const int status = conv ? PMSZ_OK : PMSZ_NOT_CONVERGED;
   st->status = status;
-  st->final_cnt[0] = cm.cnt[0];
-  st->final_cnt[1] = cm.cnt[1];
-  st->final_cnt[2] = cm.cnt[2];
+  for (int q = 0; q < 3; q++) st->final_cnt[q] = cm.cnt[q];
   if (m <= (uint32_t)kCtrRing) {
     st->trace_n = std::min<int64_t>(m, 64);
     for (int64_t k = 0; k < st->trace_n; k++) {
@@ -845,25 +1243,239 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
   st->sparse_voxels = (int64_t)r.sparse_voxels;
   const int64_t extra_dense = (int64_t)r.dense_passes;
   st->dense_passes = 1 + extra_dense;
-  if (p->mode == PMSZ_LAZY) {
-    st->edit_sweeps = m;
-    st->check_sweeps = conv ? 1 : 0;
-    st->outer_iters = conv ? 1 : 0;
-    st->ite = m + 1;
-  } else {
-    int64_t applies = conv ? std::max<int64_t>((int64_t)m - 1, 1) : (int64_t)m - 1;
-    st->edit_sweeps = applies;
-    st->check_sweeps = applies;
-    st->outer_iters = applies;
-    st->ite = 2 * applies;
-  }
+  fill_single_stats(st, p->mode, m, conv);
   // first pass: f, g read, code, dc written; later dense sweeps: g, code read;
   // edits: f, g, dc read and g, dc written; listed voxels: g and code read
-  st->bytes_algorithmic = (double)d.n * (2.0 * sizeof(T) + 2.0) +
-                          (double)d.n * (sizeof(T) + 1.0) * (double)extra_dense +
+  st->bytes_algorithmic = (double)n * (2.0 * sizeof(T) + 2.0) +
+                          (double)n * (sizeof(T) + 1.0) * (double)extra_dense +
                           (double)st->total_targets * (3.0 * sizeof(T) + 2.0) +
                           (double)st->sparse_voxels * (sizeof(T) + 1.0);
   return status;
+}
+
+// Subdomains: the global field split into sub[0] x sub[1] x sub[2] padded blocks
+// (pMSz.cu:2690-2711), corrected in lockstep with the ghost syncs of the lazy
+// (pMSz.cu:1872-2074) or per-iteration (sync_pMSz.cu:2102-2200) loop.
+template <typename T>
+int correct_subdomains(Ctx &c, const pmsz_params *p, const T *F, T *G, uint8_t *dc_out,
+                       pmsz_stats *st) {
+  const int sx = p->sub[0], sy = p->sub[1], sz = p->sub[2];
+  const uint32_t nsub = (uint32_t)(sx * sy * sz);
+  if (nsub > kMaxDomains) return PMSZ_EINVAL;
+  Job job;
+  job.subdomains = true;
+  job.sub[0] = sx;
+  job.sub[1] = sy;
+  job.sub[2] = sz;
+  job.ndir = p->ghost_policy == PMSZ_GHOST_FULL ? 14 : 12;
+  job.gn[0] = p->nx;
+  job.gn[1] = p->ny;
+  job.gn[2] = p->nz;
+  // block split of each axis (pMSz.cu:2690-2706): the first n % parts blocks
+  // hold one more cell
+  auto split = [](int64_t n, int parts, int coord, int64_t &len, int64_t &off) {
+    const int64_t base = n / parts, rem = n % parts;
+    len = base + (coord < rem ? 1 : 0);
+    off = (int64_t)coord * base + (coord < rem ? coord : rem);
+  };
+  uint64_t cells = 0, maxcells = 0, core = 0;
+  std::vector<int64_t> offs;
+  for (int cz = 0; cz < sz; cz++)
+    for (int cy = 0; cy < sy; cy++)
+      for (int cx = 0; cx < sx; cx++) {
+        int64_t l[3], o[3];
+        split(p->nx, sx, cx, l[0], o[0]);
+        split(p->ny, sy, cy, l[1], o[1]);
+        split(p->nz, sz, cz, l[2], o[2]);
+        Dom D;
+        std::memset(&D, 0, sizeof(D));
+        D.d = grid_of(l[0] + 2, l[1] + 2, l[2] + 2);
+        for (int a = 0; a < 3; a++) {
+          D.lo[a] = 1;
+          D.hi[a] = (int32_t)l[a] + 1;
+          D.off[a] = (int32_t)o[a];
+        }
+        D.cc[0] = cx;
+        D.cc[1] = cy;
+        D.cc[2] = cz;
+        col_units(l[0], l[1], l[2], D.nux, D.nuy, D.nuz);
+        D.shell_n = shell_count((uint32_t)D.d.nx, (uint32_t)D.d.ny, (uint32_t)D.d.nz);
+        D.padded = 1;
+        D.ep = edit_params(p);
+        offs.push_back((int64_t)cells);
+        cells += (uint64_t)D.d.n;
+        core += (uint64_t)(l[0] * l[1] * l[2]);
+        maxcells = std::max<uint64_t>(maxcells, (uint64_t)D.d.n);
+        job.doms.push_back(D);
+      }
+  job.shift = std::max<uint32_t>(5, ceil_log2(maxcells));
+  if (((uint64_t)nsub << job.shift) > 0xffffffffull || cells >= 0xffffffffull)
+    return PMSZ_ETOOBIG;
+  int rc;
+  if ((rc = c.code.ensure(cells)) || (rc = c.dc.ensure(cells)) || (rc = c.snap.ensure(cells)) ||
+      (rc = c.fpad.ensure(cells * sizeof(T))) || (rc = c.gpad.ensure(cells * sizeof(T))))
+    return rc;
+  uint32_t ubase = 0;
+  for (uint32_t k = 0; k < nsub; k++) {
+    Dom &D = job.doms[k];
+    D.f = (const void *)((T *)c.fpad.p + offs[k]);
+    D.g = (void *)((T *)c.gpad.p + offs[k]);
+    D.code = c.code.p + offs[k];
+    D.dc = c.dc.p + offs[k];
+    D.snap = c.snap.p + offs[k];
+    D.gbase = k << job.shift;
+    D.unit_base = ubase;
+    ubase += dom_units(D);
+  }
+  job.s = make_sched(p, core, p->mode == PMSZ_LAZY);
+  if ((rc = prepare(c, job))) return rc;
+  std::memset(st, 0, sizeof(*st));
+  // padded blocks with their initial ghost values; run->err flags bad inputs
+  PMSZ_HIP(hipMemsetAsync(c.run, 0, sizeof(RunState), c.stream));
+  uint32_t maxn = 0;
+  for (const Dom &D : job.doms) maxn = std::max<uint32_t>(maxn, (uint32_t)D.d.n);
+  const dim3 g_all((unsigned)std::min<uint32_t>((maxn + kSyncThreads - 1) / kSyncThreads,
+                                                (uint32_t)c.cus * 8),
+                   nsub);
+  k_scatter<T><<<g_all, kSyncThreads, 0, c.stream>>>(c.doms.p, F, G, p->nx, p->ny, sx, sy, sz,
+                                                     p->ghost_policy, c.run);
+  PMSZ_HIP(hipGetLastError());
+  PMSZ_HIP(hipMemcpyAsync(c.h_run, c.run, sizeof(RunState), hipMemcpyDeviceToHost, c.stream));
+  PMSZ_HIP(hipStreamSynchronize(c.stream));
+  if (c.h_run->err) return PMSZ_EINVAL;
+  Outcome o;
+  rc = drive<T, true>(c, job, o);
+  if (rc) return rc;
+  k_finish<T><<<g_all, kSyncThreads, 0, c.stream>>>(c.doms.p, c.ds.p, G, dc_out, p->nx, p->ny);
+  PMSZ_HIP(hipGetLastError());
+  PMSZ_HIP(hipMemcpyAsync(c.h_ds, c.ds.p, sizeof(DomState) * nsub, hipMemcpyDeviceToHost,
+                          c.stream));
+  PMSZ_HIP(hipStreamSynchronize(c.stream));
+  const RunState &r = *c.h_run;
+  const bool conv = o.done == 1;
+  st->status = conv ? PMSZ_OK : PMSZ_NOT_CONVERGED;
+  st->t_total_s = o.ms_total * 1e-3;
+  st->t_dense_s = o.ms_first * 1e-3;
+  st->sparse_voxels = (int64_t)r.sparse_voxels;
+  st->dense_passes = 1 + (int64_t)r.dense_passes;
+  int64_t sweeps = 0, ite = 0;
+  for (uint32_t k = 0; k < nsub; k++) {
+    const DomState &s = c.h_ds[k];
+    sweeps += s.sweeps;
+    st->total_targets += (int64_t)s.targets;
+    st->touched_voxels += (int64_t)s.touched;
+    st->clamped_voxels += (int64_t)s.clamped;
+    for (int q = 0; q < 3; q++)
+      if (!conv) st->final_cnt[q] = std::max(st->final_cnt[q], s.last_cnt[q]);
+    ite = std::max<int64_t>(ite, o.outer_started + (int64_t)s.sweeps);
+  }
+  if (p->mode == PMSZ_LAZY) {
+    st->edit_sweeps = sweeps;
+    st->outer_iters = o.outer_done;
+    st->check_sweeps = (int64_t)nsub * o.outer_done;
+    st->ite = ite;
+  } else {
+    // one edit sweep and one check per subdomain and outer iteration
+    const uint32_t m = o.last_j;
+    const int64_t applies = conv ? std::max<int64_t>((int64_t)m - 1, 1) : (int64_t)m - 1;
+    st->edit_sweeps = (int64_t)nsub * applies;
+    st->check_sweeps = (int64_t)nsub * applies;
+    st->outer_iters = applies;
+    st->ite = 2 * applies;
+  }
+  st->bytes_algorithmic = (double)cells * (2.0 * sizeof(T) + 2.0) +
+                          (double)cells * (sizeof(T) + 1.0) * (double)r.dense_passes +
+                          (double)st->total_targets * (3.0 * sizeof(T) + 2.0) +
+                          (double)st->sparse_voxels * (sizeof(T) + 1.0);
+  return st->status;
+}
+
+// Batch: independent fields of one value type corrected in lockstep.  Each field
+// behaves exactly as its own pmsz_correct.
+template <typename T>
+int correct_batch(Ctx &c, int n, const pmsz_params *p, const void *const *d_f,
+                  void *const *d_g, uint8_t *const *d_dc, pmsz_stats *stats) {
+  // groups of fields whose voxel ids fit 32 bits
+  int first = 0;
+  while (first < n) {
+    uint64_t maxn = 0, cells = 0;
+    int last = first;
+    while (last < n && last - first < (int)kMaxDomains) {
+      const uint64_t nk = (uint64_t)(p[last].nx * p[last].ny * p[last].nz);
+      const uint64_t mx = std::max(maxn, nk);
+      const uint32_t sh = std::max<uint32_t>(5, ceil_log2(mx));
+      if (((uint64_t)(last - first + 1) << sh) > 0xffffffffull) break;
+      maxn = mx;
+      cells += nk;
+      last++;
+    }
+    if (last == first) return PMSZ_ETOOBIG;
+    const int cnt = last - first;
+    int rc;
+    if (cnt == 1) {
+      rc = correct_single<T>(c, &p[first], (const T *)d_f[first], (T *)d_g[first],
+                             d_dc ? d_dc[first] : nullptr, &stats[first]);
+      if (rc != PMSZ_OK && rc != PMSZ_NOT_CONVERGED) return rc;
+      first = last;
+      continue;
+    }
+    if ((rc = c.code.ensure(cells)) || (rc = c.dc.ensure(cells))) return rc;
+    Job job;
+    job.shift = std::max<uint32_t>(5, ceil_log2(maxn));
+    uint64_t off = 0;
+    uint32_t ubase = 0;
+    for (int k = 0; k < cnt; k++) {
+      const pmsz_params *pk = &p[first + k];
+      Dom D = field_dom(pk, d_f[first + k], d_g[first + k], c.code.p + off, c.dc.p + off);
+      D.gbase = (uint32_t)k << job.shift;
+      D.unit_base = ubase;
+      ubase += dom_units(D);
+      off += (uint64_t)D.d.n;
+      job.doms.push_back(D);
+    }
+    job.s = make_sched(&p[first], cells, true);
+    if ((rc = prepare(c, job))) return rc;
+    Outcome o;
+    rc = drive<T, true>(c, job, o);
+    if (rc) return rc;
+    uint32_t maxcore = 0;
+    for (const Dom &D : job.doms) maxcore = std::max<uint32_t>(maxcore, (uint32_t)D.d.n);
+    const dim3 g_all((unsigned)std::min<uint32_t>((maxcore + kSyncThreads - 1) / kSyncThreads,
+                                                  (uint32_t)c.cus * 8),
+                     (unsigned)cnt);
+    k_finish<T><<<g_all, kSyncThreads, 0, c.stream>>>(c.doms.p, c.ds.p, nullptr, nullptr, 0, 0);
+    PMSZ_HIP(hipGetLastError());
+    off = 0;
+    for (int k = 0; k < cnt; k++) {
+      const uint64_t nk = (uint64_t)job.doms[k].d.n;
+      if (d_dc && d_dc[first + k])
+        PMSZ_HIP(hipMemcpyAsync(d_dc[first + k], c.dc.p + off, nk, hipMemcpyDeviceToDevice,
+                                c.stream));
+      off += nk;
+    }
+    PMSZ_HIP(hipMemcpyAsync(c.h_ds, c.ds.p, sizeof(DomState) * cnt, hipMemcpyDeviceToHost,
+                            c.stream));
+    PMSZ_HIP(hipStreamSynchronize(c.stream));
+    for (int k = 0; k < cnt; k++) {
+      const DomState &s = c.h_ds[k];
+      pmsz_stats *st = &stats[first + k];
+      std::memset(st, 0, sizeof(*st));
+      const bool conv = s.status == kDomConverged;
+      st->status = conv ? PMSZ_OK : PMSZ_NOT_CONVERGED;
+      for (int q = 0; q < 3; q++) st->final_cnt[q] = s.last_cnt[q];
+      st->total_targets = (int64_t)s.targets;
+      st->touched_voxels = (int64_t)s.touched;
+      st->clamped_voxels = (int64_t)s.clamped;
+      fill_single_stats(st, p[first + k].mode, s.last_j, conv);
+      st->t_total_s = o.ms_total * 1e-3;
+      st->t_dense_s = o.ms_first * 1e-3;
+      const uint64_t nk = (uint64_t)job.doms[k].d.n;
+      st->bytes_algorithmic = (double)nk * (2.0 * sizeof(T) + 2.0) +
+                              (double)st->total_targets * (3.0 * sizeof(T) + 2.0);
+    }
+    first = last;
+  }
+  return PMSZ_OK;
 }
 
 // Timing of the fused first pass alone (events around each launch), for the
@@ -871,20 +1483,23 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
 template <typename T>
 int time_first_pass(Ctx &c, const pmsz_params *p, const T *f, const T *g, int iters,
                     double *avg_s) {
-  const Grid3 d = grid_of(p->nx, p->ny, p->nz);
-  int rc = ensure(c, d.n);
-  if (rc) return rc;
-  const int64_t words = (d.n + 31) / 32 + 1;
+  const int64_t n = p->nx * p->ny * p->nz;
+  int rc;
+  if ((rc = c.code.ensure(n)) || (rc = c.dc.ensure(n))) return rc;
+  Job job;
+  job.doms.push_back(field_dom(p, f, (void *)g, c.code.p, c.dc.p));
+  job.s = make_sched(p, (uint64_t)n, true);
+  if ((rc = prepare(c, job))) return rc;
+  static const int occ = occupancy(k_columns<T, true, true, false>, kColThreads);
+  const unsigned grid = col_blocks(c, dom_units(job.doms[0]), occ);
   double tot = 0;
   for (int it = 0; it < iters; it++) {
-    PMSZ_HIP(hipMemsetAsync(c.tmark, 0, sizeof(uint32_t) * words, c.stream));
+    PMSZ_HIP(hipMemsetAsync(c.tmark.p, 0, sizeof(uint32_t) * job.words, c.stream));
     PMSZ_HIP(hipMemsetAsync(c.ring, 0, sizeof(Counters) * 2, c.stream));
     PMSZ_HIP(hipEventRecord(c.ev0, c.stream));
-    k_columns<T, true, true><<<col_blocks(c, d), kColThreads, 0, c.stream>>>(
-        f, g, c.code, c.dc, c.tmark, c.tlist, c.ring, 0, 1, c.run, d);
+    k_columns<T, true, true, false><<<grid, kColThreads, 0, c.stream>>>(
+        job.doms[0], nullptr, 1, nullptr, c.tmark.p, c.ring, 0, 1, 1u, c.run);
     PMSZ_HIP(hipGetLastError());
-    k_compact<<<compact_blocks(c, words), kCompactThreads, 0, c.stream>>>(
-        c.tmark, c.tlist, c.ring, 0, 1, true, (uint32_t)words);
     PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
     PMSZ_HIP(hipStreamSynchronize(c.stream));
     float ms = 0;
@@ -917,13 +1532,38 @@ int pmsz_correct(const pmsz_params *p, const void *d_f, void *d_g, uint8_t *d_dc
   int rc = check_params(p);
   if (rc) return rc;
   if (!d_f || !d_g) return PMSZ_EINVAL;
-  if (p->sub[0] * p->sub[1] * p->sub[2] != 1) return PMSZ_EINVAL;  // subdomains: separate path
   pmsz_stats tmp;
   if (!stats) stats = &tmp;
   std::lock_guard<std::mutex> lk(g_mu);
-  if (p->dtype == PMSZ_F32)
+  if ((rc = ctx_device(g_ctx))) return rc;
+  const bool split = p->sub[0] * p->sub[1] * p->sub[2] != 1;
+  if (p->dtype == PMSZ_F32) {
+    if (split)
+      return correct_subdomains<float>(g_ctx, p, (const float *)d_f, (float *)d_g, d_dc, stats);
     return correct_single<float>(g_ctx, p, (const float *)d_f, (float *)d_g, d_dc, stats);
+  }
+  if (split)
+    return correct_subdomains<double>(g_ctx, p, (const double *)d_f, (double *)d_g, d_dc, stats);
   return correct_single<double>(g_ctx, p, (const double *)d_f, (double *)d_g, d_dc, stats);
+}
+
+int pmsz_correct_batch(int n, const pmsz_params *params, const void *const *d_f,
+                       void *const *d_g, uint8_t *const *d_dc, pmsz_stats *stats) {
+  if (n < 1 || !params || !d_f || !d_g || !stats) return PMSZ_EINVAL;
+  for (int k = 0; k < n; k++) {
+    int rc = check_params(&params[k]);
+    if (rc) return rc;
+    if (!d_f[k] || !d_g[k]) return PMSZ_EINVAL;
+    if (params[k].sub[0] * params[k].sub[1] * params[k].sub[2] != 1) return PMSZ_EINVAL;
+    if (params[k].dtype != params[0].dtype || params[k].mode != params[0].mode ||
+        params[k].max_sweeps != params[0].max_sweeps)
+      return PMSZ_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  int rc = ctx_device(g_ctx);
+  if (rc) return rc;
+  if (params[0].dtype == PMSZ_F32) return correct_batch<float>(g_ctx, n, params, d_f, d_g, d_dc, stats);
+  return correct_batch<double>(g_ctx, n, params, d_f, d_g, d_dc, stats);
 }
 
 int pmsz_time_first_pass(const pmsz_params *p, const void *d_f, const void *d_g, int iters,
@@ -932,6 +1572,7 @@ int pmsz_time_first_pass(const pmsz_params *p, const void *d_f, const void *d_g,
   if (rc) return rc;
   if (!d_f || !d_g || !avg_seconds || iters < 1) return PMSZ_EINVAL;
   std::lock_guard<std::mutex> lk(g_mu);
+  if ((rc = ctx_device(g_ctx))) return rc;
   if (p->dtype == PMSZ_F32)
     return time_first_pass<float>(g_ctx, p, (const float *)d_f, (const float *)d_g, iters,
                                   avg_seconds);
@@ -983,18 +1624,20 @@ int pmsz_codes(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_fiel
   if (!d_field || !d_codes) return PMSZ_EINVAL;
   std::lock_guard<std::mutex> lk(g_mu);
   Ctx &c = g_ctx;
-  const Grid3 d = grid_of(nx, ny, nz);
-  rc = ensure(c, 1);
-  if (rc) return rc;
+  if ((rc = ctx_device(c))) return rc;
+  Dom D = field_dom(&p, d_field, nullptr, d_codes, nullptr);
   PMSZ_HIP(hipMemsetAsync(c.run, 0, sizeof(RunState), c.stream));
-  if (dtype == PMSZ_F32)
-    k_columns<float, true, false><<<col_blocks(c, d), kColThreads, 0, c.stream>>>(
-        (const float *)d_field, nullptr, d_codes, nullptr, nullptr, nullptr, c.ring, 0, 0, c.run,
-        d);
-  else
-    k_columns<double, true, false><<<col_blocks(c, d), kColThreads, 0, c.stream>>>(
-        (const double *)d_field, nullptr, d_codes, nullptr, nullptr, nullptr, c.ring, 0, 0,
-        c.run, d);
+  if (dtype == PMSZ_F32) {
+    static const int occ = occupancy(k_columns<float, true, false, false>, kColThreads);
+    k_columns<float, true, false, false><<<col_blocks(c, dom_units(D), occ), kColThreads, 0,
+                                           c.stream>>>(D, nullptr, 1, nullptr, nullptr, c.ring, 0,
+                                                       0, 0u, c.run);
+  } else {
+    static const int occ = occupancy(k_columns<double, true, false, false>, kColThreads);
+    k_columns<double, true, false, false><<<col_blocks(c, dom_units(D), occ), kColThreads, 0,
+                                            c.stream>>>(D, nullptr, 1, nullptr, nullptr, c.ring,
+                                                        0, 0, 0u, c.run);
+  }
   PMSZ_HIP(hipGetLastError());
   PMSZ_HIP(hipMemcpyAsync(c.h_run, c.run, sizeof(RunState), hipMemcpyDeviceToHost, c.stream));
   PMSZ_HIP(hipStreamSynchronize(c.stream));
@@ -1038,8 +1681,7 @@ int pmsz_edit_symbols(const pmsz_params *p, const void *d_f, const void *d_g0, c
   if (!d_f || !d_g0 || !d_g || !d_symbols) return PMSZ_EINVAL;
   std::lock_guard<std::mutex> lk(g_mu);
   Ctx &c = g_ctx;
-  rc = ensure(c, 1);
-  if (rc) return rc;
+  if ((rc = ctx_device(c))) return rc;
   int64_t n = p->nx * p->ny * p->nz;
   const EditParams ep = edit_params(p);
   unsigned long long *dcount = &c.run->total_targets;
@@ -1074,6 +1716,6 @@ const char *pmsz_strerror(int code) {
   }
 }
 
-const char *pmsz_version(void) { return "pmsz-mi355x 0.2"; }
+const char *pmsz_version(void) { return "pmsz-mi355x 0.3"; }
 
 }  // extern "C"

@@ -26,8 +26,9 @@ GHOST_REFERENCE, GHOST_FULL = 0, 1
 OK, EINVAL, ENOMEM, NOT_CONVERGED, EDEVICE, ETOOBIG, EBUSY = 0, -1, -2, -3, -4, -5, -6
 
 EXPORTED = [
-    "pmsz_default_params", "pmsz_correct", "pmsz_correct_host", "pmsz_codes", "pmsz_labels",
-    "pmsz_edit_symbols", "pmsz_time_first_pass", "pmsz_strerror", "pmsz_version",
+    "pmsz_default_params", "pmsz_correct", "pmsz_correct_batch", "pmsz_correct_host",
+    "pmsz_codes", "pmsz_labels", "pmsz_edit_symbols", "pmsz_time_first_pass", "pmsz_strerror",
+    "pmsz_version",
 ]
 
 
@@ -106,6 +107,8 @@ def lib():
         L.pmsz_default_params.restype = None
         L.pmsz_correct.argtypes = [P, vp, vp, vp, S]
         L.pmsz_correct_host.argtypes = [P, vp, vp, vp, S]
+        L.pmsz_correct_batch.argtypes = [i32, P, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                         ctypes.POINTER(vp), S]
         L.pmsz_codes.argtypes = [i32, i64, i64, i64, vp, vp]
         L.pmsz_labels.argtypes = [i32, i64, i64, i64, vp, vp, vp]
         L.pmsz_edit_symbols.argtypes = [P, vp, vp, vp, vp, ctypes.POINTER(i64)]
@@ -194,6 +197,48 @@ def correct(f, g, abs_eb, *, dc=None, raise_not_converged=True, **kw):
                                  out.ctypes.data_as(ctypes.c_void_p), dptr, ctypes.byref(st))
     _check(rc, "pmsz_correct_host", allow)
     return out, st.as_dict()
+
+
+def correct_batch(fs, gs, ebs, *, dcs=None, per_field=None, raise_not_converged=True, **kw):
+    """Correct a batch of CUDA tensors ``gs[k]`` in place against ``fs[k]`` with
+    bound ``ebs[k]`` in one run (pmsz_correct_batch).  ``kw`` applies to every
+    field (mode and max_sweeps must be common); ``per_field[k]`` (a dict of
+    make_params keywords, e.g. q or clamp_mode) to field k.  Returns the list of
+    per-field stats dicts."""
+    import torch
+    n = len(fs)
+    if not (len(gs) == n and len(ebs) == n) or n == 0:
+        raise ValueError("fs, gs and ebs must be non-empty lists of one length")
+    params = (Params * n)()
+    for k in range(n):
+        f, g = fs[k], gs[k]
+        if not (f.is_cuda and g.is_cuda and f.is_contiguous() and g.is_contiguous()):
+            raise ValueError("batch fields must be contiguous CUDA tensors")
+        if f.shape != g.shape or f.dtype != g.dtype:
+            raise ValueError("f and g must match in shape and dtype")
+        extra = dict(kw)
+        if per_field is not None:
+            extra.update(per_field[k])
+        params[k] = make_params(f.shape, f.dtype, ebs[k], **extra)
+    vpa = ctypes.c_void_p * n
+    pf = vpa(*[f.data_ptr() for f in fs])
+    pg = vpa(*[g.data_ptr() for g in gs])
+    pd = None
+    if dcs is not None:
+        for k in range(n):
+            if dcs[k] is not None and (dcs[k].dtype != torch.uint8 or dcs[k].numel() != fs[k].numel()):
+                raise ValueError("dcs[k] must be a uint8 CUDA tensor with one entry per voxel")
+        pd = vpa(*[(d.data_ptr() if d is not None else None) for d in dcs])
+    stats = (Stats * n)()
+    torch.cuda.current_stream().synchronize()
+    rc = lib().pmsz_correct_batch(n, params, pf, pg, pd, stats)
+    _check(rc, "pmsz_correct_batch")
+    out = [stats[k].as_dict() for k in range(n)]
+    if raise_not_converged:
+        for k, st in enumerate(out):
+            if st["status"] != OK:
+                raise PmszError(st["status"], f"pmsz_correct_batch field {k}")
+    return out
 
 
 def codes(field):
