@@ -1127,12 +1127,23 @@ int drive(Ctx &c, Job &job, Outcome &out) {
                             sizeof(Counters), hipMemcpyDeviceToHost, c.stream));
     PMSZ_HIP(hipStreamSynchronize(c.stream));
     const Counters &cc = c.h_ring[(m + 1) % kCtrRing];
-    if (cc.nactive == 0) {
+    const bool violations = (cc.cnt[0] | cc.cnt[1] | cc.cnt[2]) != 0;
+    if (!violations || !emit_ok) {
+      // converged, or the cap on outer iterations stops before the next one
       out.last_j = m + 1;
-      out.done = cc.ncapped ? 2 : 1;
+      out.done = violations ? 2 : 1;
       break;
     }
+    // the next outer iteration starts (every subdomain's ite counts it)
     out.outer_started++;
+    if (cc.nactive == 0 && cc.ntargets == 0) {
+      // every subdomain with violations is capped before it may sweep again
+      out.last_j = m + 1;
+      out.done = 2;
+      break;
+    }
+    // apply the check's targets (the first sweep of the new loops); when every
+    // subdomain is capped after it, that apply ends the run
     j0 = m + 1;
   }
   PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
