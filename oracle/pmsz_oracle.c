@@ -93,6 +93,8 @@ static void orc_compress_pointers(int64_t *lab, int64_t n) {
 
 static float next_up_f32(float c) { return nextafterf(c, INFINITY); }
 static double next_up_f64(double c) { return nextafter(c, INFINITY); }
+static float next_down_f32(float c) { return nextafterf(c, -INFINITY); }
+static double next_down_f64(double c) { return nextafter(c, -INFINITY); }
 static float round_half_away_f32(float r) { return roundf(r); }
 static double round_half_away_f64(double r) { return round(r); }
 

@@ -108,6 +108,12 @@ void orc_split(int64_t n, int parts, int coord, int64_t *local, int64_t *offset)
 /* The exact bound predicate |a - b| <= eb over the reals (a, b, eb doubles). */
 int orc_within(double a, double b, double eb);
 
+/* The clamp value of the edit primitive for original value fv in float32
+ * (orc_clamp_f32; fv and eb rounded to float for the reference value) or
+ * float64 (orc_clamp_f64), in clamp mode ORC_CLAMP_REFERENCE / BOUND_SAFE. */
+double orc_clamp_f32(double fv, double eb, int clamp_mode);
+double orc_clamp_f64(double fv, double eb, int clamp_mode);
+
 #ifdef __cplusplus
 }
 #endif

@@ -187,6 +187,15 @@ def edit_symbols(f, g0, g, eb, **kw):
     return sym, ed.value
 
 
+def clamp(fv, eb, clamp_mode, dtype=np.float32):
+    """Clamp value of the edit primitive for original value fv (a float of dtype)."""
+    L = lib()
+    fn = L.orc_clamp_f32 if np.dtype(dtype) == np.float32 else L.orc_clamp_f64
+    fn.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    fn.restype = ctypes.c_double
+    return np.dtype(dtype).type(fn(float(fv), float(eb), int(clamp_mode)))
+
+
 def within(a, b, eb):
     return bool(lib().orc_within(float(a), float(b), float(eb)))
 

@@ -62,6 +62,9 @@ __host__ __device__ inline double next_up(double c) {
   return __builtin_bit_cast(double, b);
 }
 
+__host__ __device__ inline float next_down(float c) { return -next_up(-c); }
+__host__ __device__ inline double next_down(double c) { return -next_up(-c); }
+
 struct EditParams {
   double eb;      // the absolute bound
   double delta_d; // eb / q in double (delta = bound / q, pMSz.cu:2814)
@@ -69,15 +72,20 @@ struct EditParams {
   int clamp_mode; // 0 reference, 1 bound-safe
 };
 
-// Clamp value of the edit primitive (pMSz.cu:636), optionally moved up to the
-// nearest value that honours the bound exactly.
+// Clamp value of the edit primitive (pMSz.cu:636).  Bound-safe mode keeps that
+// value when |f - c| <= eb holds exactly, and otherwise takes the smallest value
+// of T that honours the bound: the real f - eb rounded up.  That value is found
+// from f - eb formed in double (within one unit in the last place of T of it)
+// and settled with the exact predicate, one step each way at most.
 template <typename T>
 __host__ __device__ inline T clamp_value(T fv, const EditParams &ep) {
   T eps = (T)ep.eb;
   T c = fv - eps;
-  if (ep.clamp_mode == 1) {
-    for (int k = 0; k < 64 && !within_exact((double)fv, (double)c, ep.eb) && c < fv; k++)
-      c = next_up(c);
+  if (ep.clamp_mode == 1 && c < fv && !within_exact((double)fv, (double)c, ep.eb)) {
+    c = (T)((double)fv - ep.eb);
+    for (int k = 0; k < 8 && !within_exact((double)fv, (double)c, ep.eb); k++) c = next_up(c);
+    for (int k = 0; k < 8 && within_exact((double)fv, (double)next_down(c), ep.eb); k++)
+      c = next_down(c);
   }
   return c;
 }

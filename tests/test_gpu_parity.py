@@ -105,6 +105,20 @@ def test_random_shapes_parity(dev, dtype, mode, shape, eb_rel):
     _compare(f, g0, eb, dev, mode=mode)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("q", [1, 2])
+def test_clamp_near_zero_parity(dev, dtype, q):
+    # values just above eb, so the clamp value f - eb lies near zero, where the
+    # bound-safe clamp must find the smallest in-bound value exactly
+    from pmsz_anonymous_amd import fieldio
+    eb = 1e-3
+    f = (eb * (1.0 + 0.05 * fieldio.perlin((28, 30, 32), 8, 6, 3, dtype=np.float64))).astype(dtype)
+    g0, _ = fieldio.szlike(f, eb)
+    go, sto = _compare(f, g0, eb, dev, q=q)
+    assert sto["clamped_voxels"] > 0
+    assert fieldio.bound_violations(f, go, eb)[1] == 0
+
+
 def test_not_converged_parity(dev):
     f, g0, eb = _case((32, 32, 32), 5)
     for cap in (1, 2):

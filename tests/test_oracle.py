@@ -209,6 +209,57 @@ def test_within_predicate_exact():
         assert O.within(a, b, eb) == exact
 
 
+def _clamp_exact(fv, eb, mode, dtype):
+    """Independent clamp: the reference value f - (T)eb in T; in bound-safe mode,
+    when that misses the bound, the smallest value of T whose distance to f is at
+    most eb in rational arithmetic (found by stepping from a nearby value)."""
+    from fractions import Fraction
+    T = np.dtype(dtype).type
+    f = T(fv)
+    c = T(f - T(eb))
+    ok = lambda v: abs(Fraction(float(f)) - Fraction(float(v))) <= Fraction(eb)
+    if mode == O.CLAMP_REFERENCE or ok(c):
+        return c
+    lo = T(float(Fraction(float(f)) - Fraction(eb)))
+    for _ in range(4):
+        lo = np.nextafter(lo, T(-np.inf))
+    while not ok(lo):
+        lo = np.nextafter(lo, T(np.inf))
+    return lo
+
+
+# f32 voxels whose bound-safe clamp value lies near zero (f about eb), from the
+# 512^3 workload (Perlin seed 1, frequency 8, 5 octaves, eb = 1e-3 x range)
+_EB_512 = 0.0013311749696731567
+_F_NEAR_EB = [0.0013306595, 0.0013329731, 0.001332906, 0.0013305482, 0.0013326657,
+              0.0013310843, 0.0013325486, 0.0013300448, 0.0013299118, 0.0013307446,
+              0.0013320686, 0.0013315176, 0.0013320247]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_clamp_value_exact(dtype):
+    rng = np.random.default_rng(11)
+    cases = [(fv, _EB_512) for fv in _F_NEAR_EB]
+    for _ in range(3000):
+        eb = float(10.0 ** rng.uniform(-6, 0))
+        kind = rng.integers(0, 3)
+        if kind == 0:
+            fv = eb * rng.uniform(0.98, 1.02)        # clamp value near zero
+        elif kind == 1:
+            fv = rng.standard_normal() * 10.0 ** rng.integers(-3, 3)
+        else:
+            fv = eb * (1 + rng.choice([-1, 1]) * 2.0 ** -rng.integers(1, 30))
+        cases.append((fv, eb))
+    for fv, eb in cases:
+        f = np.dtype(dtype).type(fv)
+        for mode in (O.CLAMP_REFERENCE, O.CLAMP_BOUND_SAFE):
+            got = O.clamp(f, eb, mode, dtype)
+            want = _clamp_exact(f, eb, mode, dtype)
+            assert got == want or (got == 0 and want == 0), (float(f), eb, mode, got, want)
+            if mode == O.CLAMP_BOUND_SAFE:
+                assert O.within(float(f), float(got), eb), (float(f), eb, got)
+
+
 def test_split_matches_reference_rule():
     # pMSz.cu:2690-2706: local = n // p (+1 for the first n % p coords)
     for n in (7, 8, 64, 513):
