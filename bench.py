@@ -74,6 +74,152 @@ def pmc_traffic(nvox):
         return None
 
 
+def c4_fields(n=64, seed0=100, rng_seed=0):
+    """The batched workload: n independent f32 Perlin fields (seeds seed0..), each
+    edge log-uniform in [32, 256] (non-cubic), eb log-uniform in [1e-4, 1e-2] x
+    range, SZ-like and ZFP alternating by seed parity (ZFP only if libzfp loads),
+    base frequency 4, 4 octaves."""
+    from pmsz_anonymous_amd import fieldio
+    rng = np.random.default_rng(rng_seed)
+    out = []
+    for k in range(n):
+        edges = np.exp(rng.uniform(np.log(32), np.log(256), size=3)).round().astype(int)
+        eb_rel = float(10 ** rng.uniform(-4, -2))
+        seed = seed0 + k
+        comp = "zfp" if seed % 2 and fieldio.zfp_available() else "sz3"
+        shape = (int(edges[2]), int(edges[1]), int(edges[0]))
+        out.append(fieldio.perlin_case(shape, seed, 4, 4, eb_rel, comp, np.float32) + (comp,))
+    return out
+
+
+def batched_vs_plain(reps=3):
+    """C4: the 64-field batch corrected in one run vs the same fields one
+    pmsz_correct after another (fresh copies each time, staged before timing);
+    results must be identical per field."""
+    import torch
+    from pmsz_anonymous_amd import engine
+    t0 = time.perf_counter()
+    cases = c4_fields()
+    log(f"c4: {len(cases)} fields generated in {time.perf_counter() - t0:.1f} s")
+    dev = torch.device("cuda:0")
+    fs = [torch.from_numpy(f).to(dev) for f, _, _, _ in cases]
+    g0s = [torch.from_numpy(g).to(dev) for _, g, _, _ in cases]
+    ebs = [eb for _, _, eb, _ in cases]
+    nvox = sum(f.numel() for f in fs)
+    # warm-up of both paths
+    engine.correct_batch(fs, [g.clone() for g in g0s], ebs)
+    for f, g, eb in zip(fs, g0s, ebs):
+        engine.correct(f, g.clone(), eb)
+    plain_t, batch_t = [], []
+    for _ in range(reps):
+        gs = [g.clone() for g in g0s]
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for f, g, eb in zip(fs, gs, ebs):
+            engine.correct(f, g, eb)
+        torch.cuda.synchronize()
+        plain_t.append(time.perf_counter() - t)
+        gb = [g.clone() for g in g0s]
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        sts = engine.correct_batch(fs, gb, ebs)
+        torch.cuda.synchronize()
+        batch_t.append(time.perf_counter() - t)
+        same = all(torch.equal(a, b) for a, b in zip(gs, gb))
+    tp, tb = float(np.median(plain_t)), float(np.median(batch_t))
+    return {
+        "workload": "64 f32 Perlin fields, edges log-uniform [32,256] per axis (rng 0), seeds "
+                    "100-163, eb log-uniform [1e-4,1e-2] x range, SZ-like / ZFP by seed parity",
+        "voxels": int(nvox),
+        "batched_gvox_s": nvox / tb / 1e9,
+        "plain_gvox_s": nvox / tp / 1e9,
+        "speedup": tp / tb,
+        "batched_ms": tb * 1e3,
+        "plain_ms": tp * 1e3,
+        "identical": bool(same),
+        "sweeps_max": int(max(s["edit_sweeps"] for s in sts)),
+        "codecs": {c: sum(1 for *_, cc in cases if cc == c) for c in ("sz3", "zfp")},
+    }
+
+
+def serving(n_req=400, pool=32, rate=200.0, lanes=4, seed=0):
+    """C5: Poisson arrivals (mean `rate` per second) of correction jobs, edges
+    log-uniform in [64, 512] per axis, f32 80% / f64 20%, eb in {1e-2, 1e-3,
+    1e-4} x range, Perlin seed = job counter (pool entry = counter mod `pool`,
+    base frequency 8, 5 octaves), SZ-like codec.  The decompressed fields are
+    staged in HBM beforehand: latency covers admission + correction.  Also the
+    saturated throughput (every job queued at once) against the same jobs
+    corrected one after another."""
+    import torch
+    from pmsz_anonymous_amd import engine, fieldio, serve
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    entries = []
+    for i in range(pool):
+        edges = np.exp(rng.uniform(np.log(64), np.log(512), size=3)).round().astype(int)
+        dt = np.float64 if rng.uniform() < 0.2 else np.float32
+        eb_rel = float(rng.choice([1e-2, 1e-3, 1e-4]))
+        f, g0, eb = fieldio.perlin_case((int(edges[2]), int(edges[1]), int(edges[0])), i, 8, 5,
+                                        eb_rel, "sz3", dt)
+        entries.append((torch.from_numpy(f).to(dev), torch.from_numpy(g0).to(dev), eb))
+    log(f"c5: pool of {pool} jobs generated in {time.perf_counter() - t0:.1f} s")
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+    picks = [k % pool for k in range(n_req)]
+    copies = [entries[k][1].clone() for k in picks]
+    torch.cuda.synchronize()
+    with serve.Server(lanes=lanes, hbm_budget=96 << 30, max_batch=16) as srv:
+        # warm-up: one job per lane shape class
+        for f, g0, eb in entries[:lanes]:
+            srv.submit(f, g0.clone(), eb).wait(120)
+        start = time.perf_counter()
+        jobs = []
+        for k in range(n_req):
+            now = time.perf_counter() - start
+            if arrivals[k] > now:
+                time.sleep(arrivals[k] - now)
+            f, _, eb = entries[picks[k]]
+            jobs.append(srv.submit(f, copies[k], eb))
+        for j in jobs:
+            j.wait(600)
+        end = max(j.t_done for j in jobs)
+        lat = np.array([j.t_done - (start + a) for j, a in zip(jobs, arrivals)])
+        vox = sum(entries[k][0].numel() for k in picks)
+        batch = float(np.mean([j.batch_size for j in jobs]))
+        # saturation: every job of one pool pass queued at once
+        sat_copies = [g0.clone() for _, g0, _ in entries]
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        sj = [srv.submit(f, g, eb) for (f, _, eb), g in zip(entries, sat_copies)]
+        for j in sj:
+            j.wait(600)
+        t_sat = time.perf_counter() - ts
+    seq_copies = [g0.clone() for _, g0, _ in entries]
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for (f, _, eb), g in zip(entries, seq_copies):
+        engine.correct(f, g, eb)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - ts
+    identical = all(torch.equal(a, b) for a, b in zip(sat_copies, seq_copies))
+    pool_vox = sum(f.numel() for f, _, _ in entries)
+    return {
+        "workload": f"{n_req} jobs, Poisson {rate:g}/s, edges log-uniform [64,512], f32 80% / "
+                    f"f64 20%, eb {{1e-2,1e-3,1e-4}} x range, pool of {pool} Perlin fields",
+        "lanes": lanes,
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                       "p99": float(np.percentile(lat, 99) * 1e3),
+                       "max": float(lat.max() * 1e3)},
+        "offered_gvox_s": vox / arrivals[-1] / 1e9,
+        "served_gvox_s": vox / (end - start) / 1e9,
+        "mean_batch": batch,
+        "saturated_gvox_s": pool_vox / t_sat / 1e9,
+        "plain_gvox_s": pool_vox / t_seq / 1e9,
+        "saturated_vs_plain": t_seq / t_sat,
+        "identical": bool(identical),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +228,8 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--cpu-edge", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-field batched run")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the multi-GPU path is a design "
@@ -166,6 +314,10 @@ def main():
             "bytes_per_launch": dense_bytes,
         },
     }
+    if not args.no_batch:
+        res["batched"] = batched_vs_plain()
+    if not args.no_serve:
+        res["serving"] = serving()
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(f, g0, eb, min(args.cpu_edge, n))
     print(json.dumps(res), flush=True)

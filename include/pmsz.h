@@ -15,8 +15,9 @@
  *     prints and calls MPI_Abort instead (pMSz.cu:1899-1910).
  *   - The library owns its scratch state (codes, delta counters, marks, work
  *     lists), allocated on first use per device and reused; the caller owns f / g.
- *   - Calls are serialised per process by an internal lock and run on the
- *     library's own stream; they return when the result is complete.
+ *   - Calls are serialised per context by its lock and run on the context's
+ *     own stream (the functions without a context argument use the library's
+ *     default context); they return when the result is complete.
  */
 #ifndef PMSZ_H
 #define PMSZ_H
@@ -31,7 +32,8 @@ typedef enum { PMSZ_F32 = 0, PMSZ_F64 = 1 } pmsz_dtype;
 /* lazy ghost sync (pMSz.cu) or per-iteration sync (sync_pMSz.cu) */
 typedef enum { PMSZ_LAZY = 0, PMSZ_SYNC = 1 } pmsz_sync_mode;
 /* clamp of the edit primitive: the reference's g = f - eb (pMSz.cu:636) in the
- * value type, or that value moved up by ulps until |f - g| <= eb holds exactly */
+ * value type, or -- when that value misses |f - g| <= eb exactly -- the smallest
+ * value of the type that meets it */
 typedef enum { PMSZ_CLAMP_REFERENCE = 0, PMSZ_CLAMP_BOUND_SAFE = 1 } pmsz_clamp_mode;
 /* ghost cells exchanged between subdomains: the reference's faces + 4 edges
  * (pMSz.cu:394-488) or all 14 stencil directions */
@@ -106,6 +108,29 @@ int pmsz_correct(const pmsz_params *p, const void *d_f, void *d_g, uint8_t *d_dc
  * d_dc may be NULL, as may any of its entries.  sub must be 1,1,1. */
 int pmsz_correct_batch(int n, const pmsz_params *params, const void *const *d_f,
                        void *const *d_g, uint8_t *const *d_dc, pmsz_stats *stats);
+
+/* Contexts.  The functions above run in the library's default context; a
+ * context of one's own has its own HIP stream, scratch state and lock, so runs
+ * in different contexts overlap on the device (the serving loop keeps one per
+ * lane).  A context belongs to the device current at its creation.  The
+ * reference has one correction per MPI rank and no such object; these entry
+ * points are the concurrent form of pmsz_correct / pmsz_correct_batch. */
+typedef struct pmsz_ctx pmsz_ctx;
+int pmsz_ctx_create(pmsz_ctx **out);
+int pmsz_ctx_destroy(pmsz_ctx *ctx);
+int pmsz_ctx_correct(pmsz_ctx *ctx, const pmsz_params *p, const void *d_f, void *d_g,
+                     uint8_t *d_dc, pmsz_stats *stats);
+int pmsz_ctx_correct_batch(pmsz_ctx *ctx, int n, const pmsz_params *params,
+                           const void *const *d_f, void *const *d_g, uint8_t *const *d_dc,
+                           pmsz_stats *stats);
+/* Library-owned device bytes the context holds now (its buffers only grow). */
+int64_t pmsz_ctx_bytes(pmsz_ctx *ctx);
+/* Frees the context's per-field buffers (they are re-grown on the next call). */
+int pmsz_ctx_trim(pmsz_ctx *ctx);
+/* Library-owned device bytes a batch run of these n fields needs (codes, delta
+ * counters, marks, work lists); the caller's f and g are not included.  Used to
+ * admit work by HBM budget. */
+int64_t pmsz_workspace_bytes(int n, const pmsz_params *params);
 
 /* The same with host arrays: copies in, corrects, copies the result back
  * (the H2D / D2H copies of main, pMSz.cu:2870-2900, 2971). */

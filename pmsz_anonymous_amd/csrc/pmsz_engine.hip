@@ -860,6 +860,7 @@ struct Buf {
 };
 
 struct Ctx {
+  std::mutex mu;  // one call at a time per context
   int device = -1;
   int cus = 0;
   hipStream_t stream = nullptr;
@@ -878,15 +879,54 @@ struct Ctx {
   size_t h_ds_cap = 0;
 };
 
-std::mutex g_mu;
-Ctx g_ctx;
+void ctx_release(Ctx &c) {
+  for (Buf<uint8_t> *b : {&c.code, &c.dc, &c.snap, &c.fpad, &c.gpad}) {
+    (void)hipFree(b->p);
+    *b = Buf<uint8_t>();
+  }
+  for (Buf<uint32_t> *b : {&c.tmark, &c.wmark, &c.tlist, &c.wlist}) {
+    (void)hipFree(b->p);
+    *b = Buf<uint32_t>();
+  }
+  (void)hipFree(c.doms.p);
+  c.doms = Buf<Dom>();
+  (void)hipFree(c.ds.p);
+  c.ds = Buf<DomState>();
+  (void)hipFree(c.ring);
+  (void)hipFree(c.run);
+  if (c.h_ring) (void)hipHostFree(c.h_ring);
+  if (c.h_run) (void)hipHostFree(c.h_run);
+  if (c.h_ds) (void)hipHostFree(c.h_ds);
+  c.ring = nullptr;
+  c.run = nullptr;
+  c.h_ring = nullptr;
+  c.h_run = nullptr;
+  c.h_ds = nullptr;
+  c.h_ds_cap = 0;
+  for (hipEvent_t *e : {&c.ev0, &c.ev1, &c.ev2, &c.ev3}) {
+    if (*e) (void)hipEventDestroy(*e);
+    *e = nullptr;
+  }
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  c.stream = nullptr;
+  c.device = -1;
+}
 
+// Library-owned device bytes a context holds.
+size_t ctx_bytes(const Ctx &c) {
+  return c.code.cap + c.dc.cap + c.snap.cap + c.fpad.cap + c.gpad.cap +
+         4 * (c.tmark.cap + c.wmark.cap + c.tlist.cap + c.wlist.cap) +
+         sizeof(Dom) * c.doms.cap + sizeof(DomState) * c.ds.cap +
+         (c.ring ? sizeof(Counters) * kCtrRing + sizeof(RunState) : 0);
+}
+
+// The context of the calling thread's current device (created on first use; a
+// context created on another device is rebuilt).
 int ctx_device(Ctx &c) {
   int dev = 0;
   PMSZ_HIP(hipGetDevice(&dev));
   if (c.device == dev) return PMSZ_OK;
-  // a new device: the state of the previous one is left to process exit
-  c = Ctx();
+  if (c.device >= 0) ctx_release(c);
   c.device = dev;
   PMSZ_HIP(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, dev));
   PMSZ_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
@@ -1525,6 +1565,15 @@ int time_first_pass(Ctx &c, const pmsz_params *p, const T *f, const T *g, int it
 
 using namespace pmsz;
 
+struct pmsz_ctx {
+  Ctx c;
+};
+
+namespace {
+pmsz_ctx g_default;
+Ctx &default_ctx() { return g_default.c; }
+}  // namespace
+
 extern "C" {
 
 void pmsz_default_params(pmsz_params *p) {
@@ -1540,27 +1589,102 @@ void pmsz_default_params(pmsz_params *p) {
 
 int pmsz_correct(const pmsz_params *p, const void *d_f, void *d_g, uint8_t *d_dc,
                  pmsz_stats *stats) {
+  return pmsz_ctx_correct(&g_default, p, d_f, d_g, d_dc, stats);
+}
+
+int pmsz_ctx_correct(pmsz_ctx *ctx, const pmsz_params *p, const void *d_f, void *d_g,
+                     uint8_t *d_dc, pmsz_stats *stats) {
+  if (!ctx) return PMSZ_EINVAL;
   int rc = check_params(p);
   if (rc) return rc;
   if (!d_f || !d_g) return PMSZ_EINVAL;
   pmsz_stats tmp;
   if (!stats) stats = &tmp;
-  std::lock_guard<std::mutex> lk(g_mu);
-  if ((rc = ctx_device(g_ctx))) return rc;
+  Ctx &c = ctx->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if ((rc = ctx_device(c))) return rc;
   const bool split = p->sub[0] * p->sub[1] * p->sub[2] != 1;
   if (p->dtype == PMSZ_F32) {
     if (split)
-      return correct_subdomains<float>(g_ctx, p, (const float *)d_f, (float *)d_g, d_dc, stats);
-    return correct_single<float>(g_ctx, p, (const float *)d_f, (float *)d_g, d_dc, stats);
+      return correct_subdomains<float>(c, p, (const float *)d_f, (float *)d_g, d_dc, stats);
+    return correct_single<float>(c, p, (const float *)d_f, (float *)d_g, d_dc, stats);
   }
   if (split)
-    return correct_subdomains<double>(g_ctx, p, (const double *)d_f, (double *)d_g, d_dc, stats);
-  return correct_single<double>(g_ctx, p, (const double *)d_f, (double *)d_g, d_dc, stats);
+    return correct_subdomains<double>(c, p, (const double *)d_f, (double *)d_g, d_dc, stats);
+  return correct_single<double>(c, p, (const double *)d_f, (double *)d_g, d_dc, stats);
+}
+
+int pmsz_ctx_create(pmsz_ctx **out) {
+  if (!out) return PMSZ_EINVAL;
+  pmsz_ctx *ctx = new (std::nothrow) pmsz_ctx();
+  if (!ctx) return PMSZ_ENOMEM;
+  int rc = ctx_device(ctx->c);
+  if (rc) {
+    ctx_release(ctx->c);
+    delete ctx;
+    return rc;
+  }
+  *out = ctx;
+  return PMSZ_OK;
+}
+
+int pmsz_ctx_destroy(pmsz_ctx *ctx) {
+  if (!ctx || ctx == &g_default) return PMSZ_EINVAL;
+  {
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+    ctx_release(ctx->c);
+  }
+  delete ctx;
+  return PMSZ_OK;
+}
+
+int pmsz_ctx_trim(pmsz_ctx *ctx) {
+  if (!ctx) return PMSZ_EINVAL;
+  Ctx &c = ctx->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (c.stream) PMSZ_HIP(hipStreamSynchronize(c.stream));
+  for (Buf<uint8_t> *b : {&c.code, &c.dc, &c.snap, &c.fpad, &c.gpad}) {
+    (void)hipFree(b->p);
+    *b = Buf<uint8_t>();
+  }
+  for (Buf<uint32_t> *b : {&c.tmark, &c.wmark, &c.tlist, &c.wlist}) {
+    (void)hipFree(b->p);
+    *b = Buf<uint32_t>();
+  }
+  return PMSZ_OK;
+}
+
+int64_t pmsz_ctx_bytes(pmsz_ctx *ctx) {
+  if (!ctx) return PMSZ_EINVAL;
+  std::lock_guard<std::mutex> lk(ctx->c.mu);
+  return (int64_t)ctx_bytes(ctx->c);
+}
+
+int64_t pmsz_workspace_bytes(int n, const pmsz_params *params) {
+  if (n < 1 || !params) return PMSZ_EINVAL;
+  uint64_t cells = 0, maxn = 0;
+  for (int k = 0; k < n; k++) {
+    if (check_params(&params[k])) return PMSZ_EINVAL;
+    const uint64_t nk = (uint64_t)(params[k].nx * params[k].ny * params[k].nz);
+    cells += nk;
+    maxn = std::max(maxn, nk);
+  }
+  const uint64_t ids = n == 1 ? maxn : ((uint64_t)n << std::max<uint32_t>(5, ceil_log2(maxn)));
+  // code + dc per cell, two bitmaps over the ids, two lists of cell ids
+  return (int64_t)(2 * cells + 2 * 4 * ((ids + 31) / 32 + 1) + 2 * 4 * cells +
+                   (sizeof(Dom) + sizeof(DomState)) * (uint64_t)n);
 }
 
 int pmsz_correct_batch(int n, const pmsz_params *params, const void *const *d_f,
                        void *const *d_g, uint8_t *const *d_dc, pmsz_stats *stats) {
-  if (n < 1 || !params || !d_f || !d_g || !stats) return PMSZ_EINVAL;
+  return pmsz_ctx_correct_batch(&g_default, n, params, d_f, d_g, d_dc, stats);
+}
+
+int pmsz_ctx_correct_batch(pmsz_ctx *ctx, int n, const pmsz_params *params,
+                           const void *const *d_f, void *const *d_g, uint8_t *const *d_dc,
+                           pmsz_stats *stats) {
+  if (!ctx || n < 1 || !params || !d_f || !d_g || !stats) return PMSZ_EINVAL;
   for (int k = 0; k < n; k++) {
     int rc = check_params(&params[k]);
     if (rc) return rc;
@@ -1570,11 +1694,12 @@ int pmsz_correct_batch(int n, const pmsz_params *params, const void *const *d_f,
         params[k].max_sweeps != params[0].max_sweeps)
       return PMSZ_EINVAL;
   }
-  std::lock_guard<std::mutex> lk(g_mu);
-  int rc = ctx_device(g_ctx);
+  Ctx &c = ctx->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  int rc = ctx_device(c);
   if (rc) return rc;
-  if (params[0].dtype == PMSZ_F32) return correct_batch<float>(g_ctx, n, params, d_f, d_g, d_dc, stats);
-  return correct_batch<double>(g_ctx, n, params, d_f, d_g, d_dc, stats);
+  if (params[0].dtype == PMSZ_F32) return correct_batch<float>(c, n, params, d_f, d_g, d_dc, stats);
+  return correct_batch<double>(c, n, params, d_f, d_g, d_dc, stats);
 }
 
 int pmsz_time_first_pass(const pmsz_params *p, const void *d_f, const void *d_g, int iters,
@@ -1582,12 +1707,13 @@ int pmsz_time_first_pass(const pmsz_params *p, const void *d_f, const void *d_g,
   int rc = check_params(p);
   if (rc) return rc;
   if (!d_f || !d_g || !avg_seconds || iters < 1) return PMSZ_EINVAL;
-  std::lock_guard<std::mutex> lk(g_mu);
-  if ((rc = ctx_device(g_ctx))) return rc;
+  Ctx &c = default_ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  if ((rc = ctx_device(c))) return rc;
   if (p->dtype == PMSZ_F32)
-    return time_first_pass<float>(g_ctx, p, (const float *)d_f, (const float *)d_g, iters,
+    return time_first_pass<float>(c, p, (const float *)d_f, (const float *)d_g, iters,
                                   avg_seconds);
-  return time_first_pass<double>(g_ctx, p, (const double *)d_f, (const double *)d_g, iters,
+  return time_first_pass<double>(c, p, (const double *)d_f, (const double *)d_g, iters,
                                  avg_seconds);
 }
 
@@ -1633,8 +1759,8 @@ int pmsz_codes(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_fiel
   int rc = check_params(&p);
   if (rc) return rc;
   if (!d_field || !d_codes) return PMSZ_EINVAL;
-  std::lock_guard<std::mutex> lk(g_mu);
-  Ctx &c = g_ctx;
+  Ctx &c = default_ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
   if ((rc = ctx_device(c))) return rc;
   Dom D = field_dom(&p, d_field, nullptr, d_codes, nullptr);
   PMSZ_HIP(hipMemsetAsync(c.run, 0, sizeof(RunState), c.stream));
@@ -1668,8 +1794,8 @@ int pmsz_labels(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_fie
     (void)hipFree(codes);
     return rc;
   }
-  std::lock_guard<std::mutex> lk(g_mu);
-  Ctx &c = g_ctx;
+  Ctx &c = default_ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
   const Grid3 d = grid_of(nx, ny, nz);
   unsigned blocks = (unsigned)((n + 255) / 256);
   k_init_pointers<<<blocks, 256, 0, c.stream>>>(codes, d_asc, d_desc, d);
@@ -1690,8 +1816,8 @@ int pmsz_edit_symbols(const pmsz_params *p, const void *d_f, const void *d_g0, c
   int rc = check_params(p);
   if (rc) return rc;
   if (!d_f || !d_g0 || !d_g || !d_symbols) return PMSZ_EINVAL;
-  std::lock_guard<std::mutex> lk(g_mu);
-  Ctx &c = g_ctx;
+  Ctx &c = default_ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
   if ((rc = ctx_device(c))) return rc;
   int64_t n = p->nx * p->ny * p->nz;
   const EditParams ep = edit_params(p);

@@ -27,6 +27,8 @@ OK, EINVAL, ENOMEM, NOT_CONVERGED, EDEVICE, ETOOBIG, EBUSY = 0, -1, -2, -3, -4, 
 
 EXPORTED = [
     "pmsz_default_params", "pmsz_correct", "pmsz_correct_batch", "pmsz_correct_host",
+    "pmsz_ctx_create", "pmsz_ctx_destroy", "pmsz_ctx_correct", "pmsz_ctx_correct_batch",
+    "pmsz_ctx_bytes", "pmsz_ctx_trim", "pmsz_workspace_bytes",
     "pmsz_codes", "pmsz_labels", "pmsz_edit_symbols", "pmsz_time_first_pass", "pmsz_strerror",
     "pmsz_version",
 ]
@@ -109,6 +111,16 @@ def lib():
         L.pmsz_correct_host.argtypes = [P, vp, vp, vp, S]
         L.pmsz_correct_batch.argtypes = [i32, P, ctypes.POINTER(vp), ctypes.POINTER(vp),
                                          ctypes.POINTER(vp), S]
+        L.pmsz_ctx_create.argtypes = [ctypes.POINTER(vp)]
+        L.pmsz_ctx_destroy.argtypes = [vp]
+        L.pmsz_ctx_correct.argtypes = [vp, P, vp, vp, vp, S]
+        L.pmsz_ctx_correct_batch.argtypes = [vp, i32, P, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                             ctypes.POINTER(vp), S]
+        L.pmsz_ctx_trim.argtypes = [vp]
+        L.pmsz_ctx_bytes.argtypes = [vp]
+        L.pmsz_ctx_bytes.restype = i64
+        L.pmsz_workspace_bytes.argtypes = [i32, P]
+        L.pmsz_workspace_bytes.restype = i64
         L.pmsz_codes.argtypes = [i32, i64, i64, i64, vp, vp]
         L.pmsz_labels.argtypes = [i32, i64, i64, i64, vp, vp, vp]
         L.pmsz_edit_symbols.argtypes = [P, vp, vp, vp, vp, ctypes.POINTER(i64)]
@@ -199,12 +211,7 @@ def correct(f, g, abs_eb, *, dc=None, raise_not_converged=True, **kw):
     return out, st.as_dict()
 
 
-def correct_batch(fs, gs, ebs, *, dcs=None, per_field=None, raise_not_converged=True, **kw):
-    """Correct a batch of CUDA tensors ``gs[k]`` in place against ``fs[k]`` with
-    bound ``ebs[k]`` in one run (pmsz_correct_batch).  ``kw`` applies to every
-    field (mode and max_sweeps must be common); ``per_field[k]`` (a dict of
-    make_params keywords, e.g. q or clamp_mode) to field k.  Returns the list of
-    per-field stats dicts."""
+def _batch_args(fs, gs, ebs, dcs, per_field, kw):
     import torch
     n = len(fs)
     if not (len(gs) == n and len(ebs) == n) or n == 0:
@@ -229,6 +236,88 @@ def correct_batch(fs, gs, ebs, *, dcs=None, per_field=None, raise_not_converged=
             if dcs[k] is not None and (dcs[k].dtype != torch.uint8 or dcs[k].numel() != fs[k].numel()):
                 raise ValueError("dcs[k] must be a uint8 CUDA tensor with one entry per voxel")
         pd = vpa(*[(d.data_ptr() if d is not None else None) for d in dcs])
+    return n, params, pf, pg, pd
+
+
+def workspace_bytes(shapes_dtypes, **kw):
+    """Library-owned device bytes a batch run of fields of these (shape, dtype)
+    pairs needs (pmsz_workspace_bytes)."""
+    n = len(shapes_dtypes)
+    params = (Params * n)()
+    for k, (shape, dt) in enumerate(shapes_dtypes):
+        params[k] = make_params(shape, dt, 1.0, **kw)
+    v = lib().pmsz_workspace_bytes(n, params)
+    if v < 0:
+        raise PmszError(int(v), "pmsz_workspace_bytes")
+    return int(v)
+
+
+class Context:
+    """A correction context of its own (pmsz_ctx): its own HIP stream, scratch
+    state and lock; runs in different contexts overlap on the device."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(lib().pmsz_ctx_create(ctypes.byref(h)), "pmsz_ctx_create")
+        self._h = h
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().pmsz_ctx_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bytes(self):
+        return int(lib().pmsz_ctx_bytes(self._h))
+
+    def trim(self):
+        _check(lib().pmsz_ctx_trim(self._h), "pmsz_ctx_trim")
+
+    def correct(self, f, g, abs_eb, *, dc=None, raise_not_converged=True, **kw):
+        """As ``correct`` for CUDA tensors, in this context."""
+        import torch
+        p = make_params(f.shape, f.dtype, abs_eb, **kw)
+        st = Stats()
+        if not (f.is_cuda and g.is_cuda and f.is_contiguous() and g.is_contiguous()):
+            raise ValueError("context inputs must be contiguous CUDA tensors")
+        dptr = ctypes.c_void_p(dc.data_ptr()) if dc is not None else None
+        torch.cuda.current_stream().synchronize()
+        rc = lib().pmsz_ctx_correct(self._h, ctypes.byref(p), ctypes.c_void_p(f.data_ptr()),
+                                    ctypes.c_void_p(g.data_ptr()), dptr, ctypes.byref(st))
+        _check(rc, "pmsz_ctx_correct", () if raise_not_converged else (NOT_CONVERGED,))
+        return g, st.as_dict()
+
+    def correct_batch(self, fs, gs, ebs, *, dcs=None, per_field=None, raise_not_converged=True,
+                      sync_torch=True, **kw):
+        """As ``correct_batch``, in this context."""
+        import torch
+        n, params, pf, pg, pd = _batch_args(fs, gs, ebs, dcs, per_field, kw)
+        stats = (Stats * n)()
+        if sync_torch:
+            torch.cuda.current_stream().synchronize()
+        rc = lib().pmsz_ctx_correct_batch(self._h, n, params, pf, pg, pd, stats)
+        _check(rc, "pmsz_ctx_correct_batch")
+        out = [stats[k].as_dict() for k in range(n)]
+        if raise_not_converged:
+            for k, st in enumerate(out):
+                if st["status"] != OK:
+                    raise PmszError(st["status"], f"pmsz_ctx_correct_batch field {k}")
+        return out
+
+
+def correct_batch(fs, gs, ebs, *, dcs=None, per_field=None, raise_not_converged=True, **kw):
+    """Correct a batch of CUDA tensors ``gs[k]`` in place against ``fs[k]`` with
+    bound ``ebs[k]`` in one run (pmsz_correct_batch).  ``kw`` applies to every
+    field (mode and max_sweeps must be common); ``per_field[k]`` (a dict of
+    make_params keywords, e.g. q or clamp_mode) to field k.  Returns the list of
+    per-field stats dicts."""
+    import torch
+    n, params, pf, pg, pd = _batch_args(fs, gs, ebs, dcs, per_field, kw)
     stats = (Stats * n)()
     torch.cuda.current_stream().synchronize()
     rc = lib().pmsz_correct_batch(n, params, pf, pg, pd, stats)
