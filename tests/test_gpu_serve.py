@@ -49,7 +49,8 @@ def test_served_equals_alone(dev):
         jobs = []
         for k, (f, g, (_, _, eb)) in enumerate(zip(fs, gs, cases)):
             jobs.append(srv.submit(f, g, eb))
-            time.sleep(float(rng.exponential(0.002)))  # arrivals while others run
+            if k >= 12:  # a burst of 12, then arrivals while others run
+                time.sleep(float(rng.exponential(0.002)))
         for j in jobs:
             j.wait(60)
         assert srv.runs < len(jobs), "no job was batched with another"
