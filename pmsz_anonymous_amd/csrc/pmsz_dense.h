@@ -57,6 +57,12 @@ __device__ __forceinline__ double from_right(double v) {
                           (int)from_right_u((uint32_t)__double2loint(v)));
 }
 
+// Target bit of rank r in the 54-bit plane accumulator of a lane, less the voxel
+// row h: field (plane e-1, e, e+1) * 18 + (dx + 1) * kColH + dy.
+__host__ __device__ constexpr uint32_t target_base(int r) {
+  return (uint32_t)((rank_dz(r) + 1) * 3 * kColH + (rank_dx(r) + 1) * kColH + rank_dy(r));
+}
+
 // Offsets of rank r packed for runtime lookup: (dx+1) | (dy+1) << 2, 4 bits per rank.
 __host__ __device__ constexpr uint64_t rank_xy_table() {
   uint64_t t = 0;
@@ -160,6 +166,42 @@ __device__ __forceinline__ void partial_hi(const ColPlane &c, int h, Ext<T> &e) 
   e.add(c.k[h + 1], 14);    // (x,   y+1, z+1)
 }
 
+// float32 keys with a cell-intrinsic tie-break.  Within one unit a cell at lane
+// L, loaded row h and plane counter pl carries the tag t = L + 3h + 5pl; its key
+// is the exact double 2^52 + 512 * ord(v) + t.  Over the 15 candidates of any
+// voxel the tags increase strictly in linear-index order (the rows of a
+// neighbourhood are 3 apart in tag, the planes 5 apart, and the lanes differ by
+// at most 2), so the numeric order of the keys is the (value, index) order of
+// islarger_shared / isless_shared (pMSz.cu:610-620) with no per-candidate rank.
+// The winner's tag minus the voxel's own is one of -8, -7, -5, -4, ..., 5, 7, 8
+// in rank order, which decodes to the rank with two compares.
+__device__ __forceinline__ double key32(float v, uint32_t t) {
+  const float c = v + 0.0f;  // -0 -> +0: the reference compares -0 == +0
+  const uint32_t b = __float_as_uint(c);
+  const uint32_t ord = b ^ ((uint32_t)((int32_t)b >> 31) | 0x80000000u);
+  return __hiloint2double((int)(0x43300000u | (ord >> 23)), (int)((ord << 9) | t));
+}
+__device__ __forceinline__ int key_rank(double w, uint32_t tself) {
+  const int d = (int)((uint32_t)__double2loint(w) & 511u) - (int)tself + 8;
+  return d - (d > 2 ? 1 : 0) - (d > 14 ? 1 : 0);
+}
+
+// max / min of two keys as single v_max_f64 / v_min_f64.  Keys are exact
+// positive numbers, the quiet NaN of a missing cell, or the 0.0 a DPP shift
+// brings into an edge lane (whose results are never used); none is a signalling
+// NaN, so the instructions need no canonicalised operands, and a quiet NaN
+// operand yields the other one.
+__device__ __forceinline__ double kmax(double a, double b) {
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ double kmin(double a, double b) {
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
 // Flush the completed target plane `pz` of this wave's column block: `m` holds,
 // per lane, bits (dx+1)*kColH + row for targets at (x+dx, y0-1+row, pz).  After
 // the exchange with the neighbouring lanes each lane holds the target bits of its
@@ -238,8 +280,12 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
   if (!kLabel) {
     if (ring[jslot_prev].mode != kModeDense) return;  // uniform
   }
-  constexpr uint64_t kXY = rank_xy_table();
   const double kNaN = __builtin_nan("");
+  __shared__ uint8_t s_tbase[16];
+  if (kDetect) {
+    if (threadIdx.x < kCand) s_tbase[threadIdx.x] = (uint8_t)target_base((int)threadIdx.x);
+    __syncthreads();
+  }
   const uint32_t lane = lane_id();
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
   uint32_t nunits;
@@ -346,100 +392,217 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
       }
     };
 
-    Ext<T> lo_f[kColRows], part_f[kColRows], lo_g[kColRows], part_g[kColRows];
-#pragma unroll
-    for (int h = 0; h < kColRows; h++) {
-      lo_f[h].first(kNaN, 0);
-      lo_g[h].first(kNaN, 0);
-    }
-    uint32_t acc0 = 0, acc1 = 0, acc2 = 0;  // target masks of planes e-1, e, e+1
+    // target bits of planes e-1, e, e+1 in three 18-bit fields: (dx+1)*6 + row
+    uint64_t acc = 0;
     uint32_t uc[3] = {0, 0, 0};              // counters of this unit
     const int64_t gbase = (int64_t)D.gbase;
-    fetch(zs - 1);
-    for (int p = zs - 1; p <= ze; p++) {
-      ColPlane cf, cg;
-      if (kLabel) stage(p, nf, cf, true);
-      if (kDetect) stage(p, ng, cg, kLabel);
-      if (!kLabel) {
-#pragma unroll
-        for (int h = 0; h < kColRows; h++) ccode[h] = ncode[h];
+    // detection at owned voxel row h of plane e: codes, counters, target masks
+    auto visit = [&](int h, int e, int oa, int od, int da, int dd) {
+      const int y = y0m1 + h;
+      if (!(owned && y < yhi)) return;
+      const uint32_t i = (uint32_t)x + nx * (uint32_t)y + sxy * (uint32_t)e;
+      if (kLabel) {
+        code[i] = (uint8_t)(oa | (od << 4));
+        if (kDetect) dc[i] = 0;
       }
-      if (p < ze) {
-        fetch(p + 1);
-        if (!kLabel) fetch_codes(p);
+      if (kDetect) {
+        int t1, t2;
+        bool v1, v2;
+        detect_pair(oa, od, da, dd, uc, t1, v1, t2, v2);
+        if (emit) {
+          const uint64_t b1 = (uint64_t)(v1 ? 1u : 0u) << (s_tbase[t1] + (uint32_t)h);
+          const uint64_t b2 = (uint64_t)(v2 ? 1u : 0u) << (s_tbase[t2] + (uint32_t)h);
+          acc |= b1 | b2;
+        }
       }
-      const int e = p - 1;  // plane completed at this step
-      const bool emitp = e >= zs;
+    };
+    auto flush_plane = [&](int e) {
+      // plane e-1 can receive no more targets
+      flush_targets((uint32_t)acc & ((1u << (3 * kColH)) - 1u), gbase + x, y0m1, (int64_t)e - 1,
+                    D.d, tmark);
+      acc >>= 3 * kColH;
+    };
+    if constexpr (sizeof(T) == 4) {
+      // per voxel row: lo (plane p's share for plane p+1) and part (planes p-1, p
+      // for plane p), as max / min keys
+      double lf_mx[kColRows], lf_mn[kColRows], pf_mx[kColRows], pf_mn[kColRows];
+      double lg_mx[kColRows], lg_mn[kColRows], pg_mx[kColRows], pg_mn[kColRows];
 #pragma unroll
-      for (int h = 1; h <= kColRows; h++) {
-        Ext<T> hf, hg, mf, mg, lf, lg;
-        if (emitp) {
+      for (int h = 0; h < kColRows; h++) lf_mx[h] = lf_mn[h] = lg_mx[h] = lg_mn[h] = kNaN;
+      auto key_of = [&](bool pin, T v, int h, uint32_t tplane, bool check) -> double {
+        bool ok = pin && rok[h];
+        if (padded) {
+          ok = ok && v != sentinel<T>();  // a ghost cell that carries no value
+        } else if (check && ok && (!(v == v) || v == sentinel<T>())) {
+          lerr |= 1u;
+        }
+        return ok ? key32(v, tplane + 3u * (uint32_t)h) : kNaN;
+      };
+      // row extremes: A over (x, x+1), B over (x-1..x+1), C over (x-1, x)
+      struct Win {
+        double a2x, a2n, a1x, a1n, b1x, b1n, c1x, c1n;  // rows k-2 (A) and k-1
+      };
+      fetch(zs - 1);
+      for (int p = zs - 1; p <= ze; p++) {
+        const bool pin = p >= 0 && p < (int)nz;  // uniform
+        const uint32_t tplane = lane + 5u * (uint32_t)(p - zs + 1);
+        T rf[kColH], rg[kColH];
+#pragma unroll
+        for (int h = 0; h < kColH; h++) {
+          if (kLabel) rf[h] = nf[h];
+          if (kDetect) rg[h] = ng[h];
+        }
+        if (!kLabel) {
+#pragma unroll
+          for (int h = 0; h < kColRows; h++) ccode[h] = ncode[h];
+        }
+        if (p < ze) {
+          fetch(p + 1);
+          if (!kLabel) fetch_codes(p);
+        }
+        const int e = p - 1;  // plane completed at this step
+        const bool emitp = e >= zs;
+        const uint32_t tself0 = lane + 5u * (uint32_t)(p - zs);
+        Win wf, wg;
+#pragma unroll
+        for (int k = 0; k < kColH; k++) {
+          double ax = 0, an = 0, bx = 0, bn = 0, cx = 0, cn = 0;
+          double gax = 0, gan = 0, gbx = 0, gbn = 0, gcx = 0, gcn = 0;
           if (kLabel) {
-            partial_hi<T>(cf, h, hf);
-            part_f[h - 1].merge(hf);
+            const double kk = key_of(pin, rf[k], k, tplane, true);
+            const double l = from_left(kk), r = from_right(kk);
+            ax = kmax(kk, r);
+            an = kmin(kk, r);
+            cx = kmax(l, kk);
+            cn = kmin(l, kk);
+            bx = kmax(ax, l);
+            bn = kmin(an, l);
           }
           if (kDetect) {
-            partial_hi<T>(cg, h, hg);
-            part_g[h - 1].merge(hg);
+            const double kk = key_of(pin, rg[k], k, tplane, kLabel);
+            const double l = from_left(kk), r = from_right(kk);
+            gax = kmax(kk, r);
+            gan = kmin(kk, r);
+            gcx = kmax(l, kk);
+            gcn = kmin(l, kk);
+            gbx = kmax(gax, l);
+            gbn = kmin(gan, l);
           }
-          const int y = y0m1 + h;
-          if (owned && y < yhi) {
-            const uint32_t i = (uint32_t)x + nx * (uint32_t)y + sxy * (uint32_t)e;
-            int oa, od;
+          if (k >= 2) {
+            const int h = k - 1;  // voxel row completed by row k
+            if (emitp) {
+              const uint32_t ts = tself0 + 3u * (uint32_t)h;
+              int oa, od, da = 0, dd = 0;
+              if (kLabel) {
+                oa = key_rank(kmax(pf_mx[h - 1], kmax(wf.c1x, cx)), ts);
+                od = key_rank(kmin(pf_mn[h - 1], kmin(wf.c1n, cn)), ts);
+              } else {
+                oa = ccode[h - 1] & 15;
+                od = ccode[h - 1] >> 4;
+              }
+              if (kDetect) {
+                da = key_rank(kmax(pg_mx[h - 1], kmax(wg.c1x, gcx)), ts);
+                dd = key_rank(kmin(pg_mn[h - 1], kmin(wg.c1n, gcn)), ts);
+              }
+              visit(h, e, oa, od, da, dd);
+            }
+            // plane p's partials: mid for plane p, lo for plane p+1
             if (kLabel) {
+              pf_mx[h - 1] = kmax(lf_mx[h - 1], kmax(wf.a2x, kmax(wf.b1x, cx)));
+              pf_mn[h - 1] = kmin(lf_mn[h - 1], kmin(wf.a2n, kmin(wf.b1n, cn)));
+              lf_mx[h - 1] = kmax(wf.a2x, wf.a1x);
+              lf_mn[h - 1] = kmin(wf.a2n, wf.a1n);
+            }
+            if (kDetect) {
+              pg_mx[h - 1] = kmax(lg_mx[h - 1], kmax(wg.a2x, kmax(wg.b1x, gcx)));
+              pg_mn[h - 1] = kmin(lg_mn[h - 1], kmin(wg.a2n, kmin(wg.b1n, gcn)));
+              lg_mx[h - 1] = kmax(wg.a2x, wg.a1x);
+              lg_mn[h - 1] = kmin(wg.a2n, wg.a1n);
+            }
+          }
+          if (kLabel) {
+            wf.a2x = wf.a1x; wf.a2n = wf.a1n;
+            wf.a1x = ax; wf.a1n = an;
+            wf.b1x = bx; wf.b1n = bn;
+            wf.c1x = cx; wf.c1n = cn;
+          }
+          if (kDetect) {
+            wg.a2x = wg.a1x; wg.a2n = wg.a1n;
+            wg.a1x = gax; wg.a1n = gan;
+            wg.b1x = gbx; wg.b1n = gbn;
+            wg.c1x = gcx; wg.c1n = gcn;
+          }
+        }
+        if (kDetect && emitp) flush_plane(e);
+      }
+    } else {
+      Ext<T> lo_f[kColRows], part_f[kColRows], lo_g[kColRows], part_g[kColRows];
+#pragma unroll
+      for (int h = 0; h < kColRows; h++) {
+        lo_f[h].first(kNaN, 0);
+        lo_g[h].first(kNaN, 0);
+      }
+      fetch(zs - 1);
+      for (int p = zs - 1; p <= ze; p++) {
+        ColPlane cf, cg;
+        if (kLabel) stage(p, nf, cf, true);
+        if (kDetect) stage(p, ng, cg, kLabel);
+        if (!kLabel) {
+#pragma unroll
+          for (int h = 0; h < kColRows; h++) ccode[h] = ncode[h];
+        }
+        if (p < ze) {
+          fetch(p + 1);
+          if (!kLabel) fetch_codes(p);
+        }
+        const int e = p - 1;  // plane completed at this step
+        const bool emitp = e >= zs;
+#pragma unroll
+        for (int h = 1; h <= kColRows; h++) {
+          Ext<T> hf, hg, mf, mg, lf, lg;
+          if (emitp) {
+            int oa = 0, od = 0, da = 0, dd = 0;
+            if (kLabel) {
+              partial_hi<T>(cf, h, hf);
+              part_f[h - 1].merge(hf);
               oa = part_f[h - 1].asc();
               od = part_f[h - 1].desc();
-              code[i] = (uint8_t)(oa | (od << 4));
-              if (kDetect) dc[i] = 0;
             } else {
               oa = ccode[h - 1] & 15;
               od = ccode[h - 1] >> 4;
             }
             if (kDetect) {
-              int tr[4];
-              const int nt =
-                  detect_targets(oa, od, part_g[h - 1].asc(), part_g[h - 1].desc(), tr, uc);
-              if (emit) {
-                for (int q = 0; q < nt; q++) {
-                  const int r = tr[q];
-                  const uint32_t xy = (uint32_t)(kXY >> (4 * r)) & 15u;
-                  const uint32_t bit = (xy & 3u) * kColH + (uint32_t)h + (xy >> 2) - 1u;
-                  const uint32_t mb = 1u << bit;
-                  if (r < 4) acc0 |= mb;
-                  else if (r < 11) acc1 |= mb;
-                  else acc2 |= mb;
-                }
-              }
+              partial_hi<T>(cg, h, hg);
+              part_g[h - 1].merge(hg);
+              da = part_g[h - 1].asc();
+              dd = part_g[h - 1].desc();
             }
+            visit(h, e, oa, od, da, dd);
+          }
+          // partials of plane p for the voxels of planes p and p+1
+          if (kLabel) {
+            partial_mid<T>(cf, h, mf);
+            partial_lo<T>(cf, h, lf);
+            part_f[h - 1] = lo_f[h - 1];
+            part_f[h - 1].merge(mf);
+            lo_f[h - 1] = lf;
+          }
+          if (kDetect) {
+            partial_mid<T>(cg, h, mg);
+            partial_lo<T>(cg, h, lg);
+            part_g[h - 1] = lo_g[h - 1];
+            part_g[h - 1].merge(mg);
+            lo_g[h - 1] = lg;
           }
         }
-        // partials of plane p for the voxels of planes p and p+1
-        if (kLabel) {
-          partial_mid<T>(cf, h, mf);
-          partial_lo<T>(cf, h, lf);
-          part_f[h - 1] = lo_f[h - 1];
-          part_f[h - 1].merge(mf);
-          lo_f[h - 1] = lf;
-        }
-        if (kDetect) {
-          partial_mid<T>(cg, h, mg);
-          partial_lo<T>(cg, h, lg);
-          part_g[h - 1] = lo_g[h - 1];
-          part_g[h - 1].merge(mg);
-          lo_g[h - 1] = lg;
-        }
-      }
-      if (kDetect && emitp) {
-        // plane e-1 can receive no more targets
-        flush_targets(acc0, gbase + x, y0m1, (int64_t)e - 1, D.d, tmark);
-        acc0 = acc1;
-        acc1 = acc2;
-        acc2 = 0;
+        if (kDetect && emitp) flush_plane(e);
       }
     }
     if (kDetect) {
-      flush_targets(acc0, gbase + x, y0m1, (int64_t)ze - 1, D.d, tmark);
-      flush_targets(acc1, gbase + x, y0m1, (int64_t)ze, D.d, tmark);
+      flush_targets((uint32_t)acc & ((1u << (3 * kColH)) - 1u), gbase + x, y0m1, (int64_t)ze - 1,
+                    D.d, tmark);
+      flush_targets((uint32_t)(acc >> (3 * kColH)) & ((1u << (3 * kColH)) - 1u), gbase + x, y0m1,
+                    (int64_t)ze, D.d, tmark);
       if (kMulti) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {

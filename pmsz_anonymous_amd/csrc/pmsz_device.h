@@ -270,6 +270,27 @@ __device__ __forceinline__ int detect_targets(int oa, int od, int da, int dd, in
   return nt;
 }
 
+// The same rules without branches: at most one target on the ascent side (t1:
+// the new top neighbour, the false maximum, or the wrong steepest-ascent
+// neighbour) and one on the descent side (t2: the lost minimum, the true descent
+// neighbour, or the true steepest-descent neighbour).  v1 / v2 tell whether each
+// exists.  Counters as detect_targets.
+__device__ __forceinline__ void detect_pair(int oa, int od, int da, int dd, uint32_t (&c)[3],
+                                            int &t1, bool &v1, int &t2, bool &v2) {
+  const bool omax = oa == kSelf;
+  const bool omin = od == kSelf && !omax;
+  const bool fm = (da == kSelf) != omax, fn = (dd == kSelf) != omin;
+  const bool crit = fm || fn;
+  const bool dira = !crit && da != oa, dird = !crit && dd != od;
+  c[0] += fm ? 1u : 0u;
+  c[1] += fn ? 1u : 0u;
+  c[2] += crit ? 1u : (dira ? 1u : 0u) + (dird ? 1u : 0u);
+  t1 = (fm && !omax) ? kSelf : da;
+  v1 = fm || dira;
+  t2 = (fn && omin) ? kSelf : od;
+  v2 = fn || dird;
+}
+
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 // Workgroup barrier ordering LDS only: global stores issued before it need not
