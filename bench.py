@@ -21,14 +21,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
-# Algorithmic HBM bytes per voxel of the fused first pass (labeling + first
-# detection sweep): read f and g (4 B each for f32), write the code byte and
-# the cleared delta counter (1 B each).
-PASS1_BYTES_PER_VOXEL = 2 * 4 + 1 + 1
-# Committed rocprofv3 --pmc summary of the same kernel on the same workload;
-# its "traffic_bytes_per_launch" line is FETCH_SIZE + WRITE_SIZE, each scaled
-# by what the counter reads for a device copy of known size.
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_columns_pass1_512cube_f32_pmc.md")
+# Algorithmic HBM bytes per voxel of a full-volume detection sweep: read g
+# (4 B for f32) and the original code byte.
+DETECT_BYTES_PER_VOXEL = 4 + 1
+# ... and of the original labeling pass: read f, write the code byte and the
+# cleared delta counter.
+LABEL_BYTES_PER_VOXEL = 4 + 1 + 1
+# Committed rocprofv3 --pmc summary of the detection kernel on the same
+# workload; its "traffic_bytes_per_launch" line is FETCH_SIZE + WRITE_SIZE of
+# detection 1, each scaled by what the counter reads for a device copy of known
+# size.
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_columns_detect_512cube_f32_pmc.md")
 
 
 def log(*a):
@@ -270,8 +273,10 @@ def main():
     ms = elapsed / args.steps * 1e3
     s0 = stats[-1]
     dense_s = float(np.mean([s["t_dense_s"] for s in stats]))
-    dense_bytes = nvox * PASS1_BYTES_PER_VOXEL
+    label_s = float(np.mean([s["t_label_s"] for s in stats]))
+    dense_bytes = nvox * DETECT_BYTES_PER_VOXEL
     achieved = dense_bytes / dense_s / 1e9
+    label_gbs = nvox * LABEL_BYTES_PER_VOXEL / label_s / 1e9
     traffic = pmc_traffic(nvox)
     # full check of one result against the original's codes (topology preserved)
     ok = torch.equal(engine.codes(tf), engine.codes(copies[-1]))
@@ -302,7 +307,7 @@ def main():
                        "sparse_voxels": s0["sparse_voxels"], "topology_preserved": bool(ok),
                        "device_ms": s0["t_total_s"] * 1e3},
         "roofline": {
-            "kernel": "k_columns<float,true,true> (labeling + first detection sweep)",
+            "kernel": "k_columns<float,false,true> (full-volume detection sweep, detection 1)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBPS,
@@ -312,6 +317,9 @@ def main():
             "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if traffic else None,
             "kernel_ms": dense_s * 1e3,
             "bytes_per_launch": dense_bytes,
+            "labeling": {"kernel": "k_columns<float,true,false>", "kernel_ms": label_s * 1e3,
+                         "achieved": label_gbs, "frac": label_gbs / HBM_PEAK_GBPS,
+                         "bytes_per_launch": nvox * LABEL_BYTES_PER_VOXEL},
         },
     }
     if not args.no_batch:

@@ -74,7 +74,8 @@ typedef struct {
   int64_t dense_passes;    /* full-volume kernel passes actually run */
   int64_t sparse_voxels;   /* voxels re-examined by work-list sweeps */
   double t_total_s;        /* device time of the correction (events on the library stream) */
-  double t_dense_s;        /* device time of the full first pass (labeling + first sweep) */
+  double t_dense_s;        /* device time of detection 1, a full-volume detection sweep */
+  double t_label_s;        /* device time of the original labeling pass */
   double bytes_algorithmic;/* algorithmic HBM bytes of the passes actually run */
   int64_t trace_n;         /* sweeps recorded below (the first min(64, sweeps)) */
   uint32_t trace_cnt[64][3];   /* per detection: false max, false min, direction counters */
@@ -158,9 +159,10 @@ int pmsz_labels(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_fie
 int pmsz_edit_symbols(const pmsz_params *p, const void *d_f, const void *d_g0,
                       const void *d_g, uint8_t *d_symbols, int64_t *edited);
 
-/* Average device time of the fused first pass (original labeling + first
- * detection sweep) over `iters` launches, timed with HIP events on the library
- * stream.  Measurement aid for the roofline report; it leaves no result. */
+/* Average device times of the original labeling pass (avg_seconds[0]) and of
+ * detection 1, a full-volume detection sweep on g (avg_seconds[1]), over
+ * `iters` launches each, timed with HIP events on the library stream.
+ * Measurement aid for the roofline report; it leaves no result. */
 int pmsz_time_first_pass(const pmsz_params *p, const void *d_f, const void *d_g, int iters,
                          double *avg_seconds);
 

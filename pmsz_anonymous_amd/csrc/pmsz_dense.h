@@ -403,7 +403,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
       const uint32_t i = (uint32_t)x + nx * (uint32_t)y + sxy * (uint32_t)e;
       if (kLabel) {
         code[i] = (uint8_t)(oa | (od << 4));
-        if (kDetect) dc[i] = 0;
+        if (dc) dc[i] = 0;  // the run's delta counters start at zero
       }
       if (kDetect) {
         int t1, t2;

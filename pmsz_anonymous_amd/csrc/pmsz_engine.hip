@@ -864,7 +864,7 @@ struct Ctx {
   int device = -1;
   int cus = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr;
   Buf<uint8_t> code, dc, snap;   // per stored cell
   Buf<uint8_t> fpad, gpad;       // padded subdomain blocks (bytes)
   Buf<uint32_t> tmark, wmark;    // one bit per voxel id
@@ -903,7 +903,7 @@ void ctx_release(Ctx &c) {
   c.h_run = nullptr;
   c.h_ds = nullptr;
   c.h_ds_cap = 0;
-  for (hipEvent_t *e : {&c.ev0, &c.ev1, &c.ev2, &c.ev3}) {
+  for (hipEvent_t *e : {&c.ev0, &c.ev1, &c.ev2, &c.ev3, &c.ev4}) {
     if (*e) (void)hipEventDestroy(*e);
     *e = nullptr;
   }
@@ -934,6 +934,7 @@ int ctx_device(Ctx &c) {
   PMSZ_HIP(hipEventCreate(&c.ev1));
   PMSZ_HIP(hipEventCreate(&c.ev2));
   PMSZ_HIP(hipEventCreate(&c.ev3));
+  PMSZ_HIP(hipEventCreate(&c.ev4));
   PMSZ_HIP(hipMalloc(&c.ring, sizeof(Counters) * kCtrRing));
   PMSZ_HIP(hipMalloc(&c.run, sizeof(RunState)));
   PMSZ_HIP(hipHostMalloc(&c.h_ring, sizeof(Counters) * kCtrRing, hipHostMallocDefault));
@@ -1054,7 +1055,7 @@ struct Outcome {
   uint32_t done = 0;       // 1 converged, 2 capped
   int64_t outer_done = 0;  // ghost syncs + checks completed (subdomains)
   int64_t outer_started = 0;
-  float ms_total = 0, ms_first = 0;
+  float ms_total = 0, ms_label = 0, ms_first = 0;
 };
 
 // The chain of one run: detection 1 (fused with the labeling), then apply /
@@ -1069,9 +1070,9 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   DomState *ds = kMulti ? c.ds.p : nullptr;
   uint32_t nunits = 0;
   for (const Dom &D : job.doms) nunits += dom_units(D);
-  static const int occ_first = occupancy(k_columns<T, true, true, kMulti>, kColThreads);
+  static const int occ_label = occupancy(k_columns<T, true, false, kMulti>, kColThreads);
   static const int occ_dense = occupancy(k_columns<T, false, true, kMulti>, kColThreads);
-  const unsigned g_first = col_blocks(c, nunits, occ_first);
+  const unsigned g_label = col_blocks(c, nunits, occ_label);
   const unsigned g_dense = col_blocks(c, nunits, occ_dense);
   const unsigned g_list = (unsigned)std::max(1, c.cus * 4);
   const unsigned g_compact = compact_blocks(c, (int64_t)job.words);
@@ -1094,11 +1095,16 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   static const uint32_t kRunning = kModeDense;
   PMSZ_HIP(hipMemcpyAsync(&c.ring[0].mode, &kRunning, sizeof(uint32_t), hipMemcpyHostToDevice,
                           c.stream));
+  // the original labeling (codes, delta counters cleared), then detection 1 as
+  // a dense detection sweep (slot 0 holds mode Dense)
   PMSZ_HIP(hipEventRecord(c.ev2, c.stream));
-  k_columns<T, true, true, kMulti><<<g_first, kColThreads, 0, c.stream>>>(
+  k_columns<T, true, false, kMulti><<<g_label, kColThreads, 0, c.stream>>>(
+      d0, doms, ndom, ds, c.tmark.p, c.ring, 0, 1, 1u, c.run);
+  PMSZ_HIP(hipEventRecord(c.ev3, c.stream));
+  k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
       d0, doms, ndom, ds, c.tmark.p, c.ring, 0, 1, 1u, c.run);
   PMSZ_HIP(hipGetLastError());
-  PMSZ_HIP(hipEventRecord(c.ev3, c.stream));
+  PMSZ_HIP(hipEventRecord(c.ev4, c.stream));
   k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(c.tmark.p, c.tlist.p, c.ring, 0, 1,
                                                          true, (uint32_t)job.words);
   if (kMulti)
@@ -1191,7 +1197,8 @@ int drive(Ctx &c, Job &job, Outcome &out) {
                           c.stream));
   PMSZ_HIP(hipStreamSynchronize(c.stream));
   PMSZ_HIP(hipEventElapsedTime(&out.ms_total, c.ev0, c.ev1));
-  PMSZ_HIP(hipEventElapsedTime(&out.ms_first, c.ev2, c.ev3));
+  PMSZ_HIP(hipEventElapsedTime(&out.ms_label, c.ev2, c.ev3));
+  PMSZ_HIP(hipEventElapsedTime(&out.ms_first, c.ev3, c.ev4));
   return PMSZ_OK;
 }
 
@@ -1273,6 +1280,7 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
   PMSZ_HIP(hipStreamSynchronize(c.stream));
   st->t_total_s = o.ms_total * 1e-3;
   st->t_dense_s = o.ms_first * 1e-3;
+  st->t_label_s = o.ms_label * 1e-3;
   const RunState &r = *c.h_run;
   const uint32_t m = o.last_j;
   const Counters &cm = c.h_ring[m % kCtrRing];
@@ -1292,13 +1300,12 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
   st->touched_voxels = (int64_t)r.touched;
   st->clamped_voxels = (int64_t)r.clamped;
   st->sparse_voxels = (int64_t)r.sparse_voxels;
-  const int64_t extra_dense = (int64_t)r.dense_passes;
-  st->dense_passes = 1 + extra_dense;
+  st->dense_passes = 1 + (int64_t)r.dense_passes;  // the labeling + detection sweeps
   fill_single_stats(st, p->mode, m, conv);
-  // first pass: f, g read, code, dc written; later dense sweeps: g, code read;
-  // edits: f, g, dc read and g, dc written; listed voxels: g and code read
-  st->bytes_algorithmic = (double)n * (2.0 * sizeof(T) + 2.0) +
-                          (double)n * (sizeof(T) + 1.0) * (double)extra_dense +
+  // labeling: f read, code and dc written; full-volume detections: g and code
+  // read; edits: f, g, dc read and g, dc written; listed voxels: g and code read
+  st->bytes_algorithmic = (double)n * (sizeof(T) + 2.0) +
+                          (double)n * (sizeof(T) + 1.0) * (double)r.dense_passes +
                           (double)st->total_targets * (3.0 * sizeof(T) + 2.0) +
                           (double)st->sparse_voxels * (sizeof(T) + 1.0);
   return status;
@@ -1407,6 +1414,7 @@ int correct_subdomains(Ctx &c, const pmsz_params *p, const T *F, T *G, uint8_t *
   st->status = conv ? PMSZ_OK : PMSZ_NOT_CONVERGED;
   st->t_total_s = o.ms_total * 1e-3;
   st->t_dense_s = o.ms_first * 1e-3;
+  st->t_label_s = o.ms_label * 1e-3;
   st->sparse_voxels = (int64_t)r.sparse_voxels;
   st->dense_passes = 1 + (int64_t)r.dense_passes;
   int64_t sweeps = 0, ite = 0;
@@ -1434,7 +1442,7 @@ int correct_subdomains(Ctx &c, const pmsz_params *p, const T *F, T *G, uint8_t *
     st->outer_iters = applies;
     st->ite = 2 * applies;
   }
-  st->bytes_algorithmic = (double)cells * (2.0 * sizeof(T) + 2.0) +
+  st->bytes_algorithmic = (double)cells * (sizeof(T) + 2.0) +
                           (double)cells * (sizeof(T) + 1.0) * (double)r.dense_passes +
                           (double)st->total_targets * (3.0 * sizeof(T) + 2.0) +
                           (double)st->sparse_voxels * (sizeof(T) + 1.0);
@@ -1520,8 +1528,9 @@ int correct_batch(Ctx &c, int n, const pmsz_params *p, const void *const *d_f,
       fill_single_stats(st, p[first + k].mode, s.last_j, conv);
       st->t_total_s = o.ms_total * 1e-3;
       st->t_dense_s = o.ms_first * 1e-3;
+      st->t_label_s = o.ms_label * 1e-3;
       const uint64_t nk = (uint64_t)job.doms[k].d.n;
-      st->bytes_algorithmic = (double)nk * (2.0 * sizeof(T) + 2.0) +
+      st->bytes_algorithmic = (double)nk * (2.0 * sizeof(T) + 3.0) +
                               (double)st->total_targets * (3.0 * sizeof(T) + 2.0);
     }
     first = last;
@@ -1541,23 +1550,34 @@ int time_first_pass(Ctx &c, const pmsz_params *p, const T *f, const T *g, int it
   job.doms.push_back(field_dom(p, f, (void *)g, c.code.p, c.dc.p));
   job.s = make_sched(p, (uint64_t)n, true);
   if ((rc = prepare(c, job))) return rc;
-  static const int occ = occupancy(k_columns<T, true, true, false>, kColThreads);
-  const unsigned grid = col_blocks(c, dom_units(job.doms[0]), occ);
-  double tot = 0;
+  static const int occ_l = occupancy(k_columns<T, true, false, false>, kColThreads);
+  static const int occ_d = occupancy(k_columns<T, false, true, false>, kColThreads);
+  const unsigned gl = col_blocks(c, dom_units(job.doms[0]), occ_l);
+  const unsigned gd = col_blocks(c, dom_units(job.doms[0]), occ_d);
+  double tl = 0, td = 0;
+  static const uint32_t kRunning = kModeDense;
   for (int it = 0; it < iters; it++) {
     PMSZ_HIP(hipMemsetAsync(c.tmark.p, 0, sizeof(uint32_t) * job.words, c.stream));
     PMSZ_HIP(hipMemsetAsync(c.ring, 0, sizeof(Counters) * 2, c.stream));
+    PMSZ_HIP(hipMemcpyAsync(&c.ring[0].mode, &kRunning, sizeof(uint32_t), hipMemcpyHostToDevice,
+                            c.stream));
     PMSZ_HIP(hipEventRecord(c.ev0, c.stream));
-    k_columns<T, true, true, false><<<grid, kColThreads, 0, c.stream>>>(
+    k_columns<T, true, false, false><<<gl, kColThreads, 0, c.stream>>>(
+        job.doms[0], nullptr, 1, nullptr, c.tmark.p, c.ring, 0, 1, 1u, c.run);
+    PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
+    k_columns<T, false, true, false><<<gd, kColThreads, 0, c.stream>>>(
         job.doms[0], nullptr, 1, nullptr, c.tmark.p, c.ring, 0, 1, 1u, c.run);
     PMSZ_HIP(hipGetLastError());
-    PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
+    PMSZ_HIP(hipEventRecord(c.ev2, c.stream));
     PMSZ_HIP(hipStreamSynchronize(c.stream));
-    float ms = 0;
-    PMSZ_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
-    tot += ms * 1e-3;
+    float ms1 = 0, ms2 = 0;
+    PMSZ_HIP(hipEventElapsedTime(&ms1, c.ev0, c.ev1));
+    PMSZ_HIP(hipEventElapsedTime(&ms2, c.ev1, c.ev2));
+    tl += ms1 * 1e-3;
+    td += ms2 * 1e-3;
   }
-  *avg_s = tot / std::max(1, iters);
+  avg_s[0] = tl / std::max(1, iters);
+  avg_s[1] = td / std::max(1, iters);
   return PMSZ_OK;
 }
 

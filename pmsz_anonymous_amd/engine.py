@@ -72,6 +72,7 @@ class Stats(ctypes.Structure):
         ("sparse_voxels", ctypes.c_int64),
         ("t_total_s", ctypes.c_double),
         ("t_dense_s", ctypes.c_double),
+        ("t_label_s", ctypes.c_double),
         ("bytes_algorithmic", ctypes.c_double),
         ("trace_n", ctypes.c_int64),
         ("trace_cnt", (ctypes.c_uint32 * 3) * 64),
@@ -374,15 +375,16 @@ def edit_symbols(f, g0, g, abs_eb, **kw):
 
 
 def time_first_pass(f, g, abs_eb, iters=5, **kw):
-    """Average seconds of the fused first pass on CUDA tensors f, g (g is not modified)."""
+    """(labeling seconds, detection-1 seconds), averaged, on CUDA tensors f, g
+    (g is not modified)."""
     import torch
     p = make_params(f.shape, f.dtype, abs_eb, **kw)
-    out = ctypes.c_double(0)
+    out = (ctypes.c_double * 2)()
     torch.cuda.current_stream().synchronize()
     rc = lib().pmsz_time_first_pass(ctypes.byref(p), ctypes.c_void_p(f.data_ptr()),
-                                    ctypes.c_void_p(g.data_ptr()), int(iters), ctypes.byref(out))
+                                    ctypes.c_void_p(g.data_ptr()), int(iters), out)
     _check(rc, "pmsz_time_first_pass")
-    return out.value
+    return out[0], out[1]
 
 
 def version():
