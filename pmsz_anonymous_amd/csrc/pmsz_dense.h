@@ -276,7 +276,7 @@ template <typename T, bool kLabel, bool kDetect, bool kMulti>
 __global__ __launch_bounds__(kColThreads) void k_columns(
     Dom d0, const Dom *__restrict__ doms, uint32_t ndom, DomState *__restrict__ ds,
     uint32_t *__restrict__ tmark, Counters *__restrict__ ring, uint32_t jslot_prev,
-    uint32_t jslot, uint32_t par, RunState *run) {
+    uint32_t jslot, uint32_t par, RunState *run, uint32_t check_g) {
   if (!kLabel) {
     if (ring[jslot_prev].mode != kModeDense) return;  // uniform
   }
@@ -479,7 +479,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
             bn = kmin(an, l);
           }
           if (kDetect) {
-            const double kk = key_of(pin, rg[k], k, tplane, kLabel);
+            const double kk = key_of(pin, rg[k], k, tplane, kLabel || check_g != 0);
             const double l = from_left(kk), r = from_right(kk);
             gax = kmax(kk, r);
             gan = kmin(kk, r);
@@ -546,7 +546,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
       for (int p = zs - 1; p <= ze; p++) {
         ColPlane cf, cg;
         if (kLabel) stage(p, nf, cf, true);
-        if (kDetect) stage(p, ng, cg, kLabel);
+        if (kDetect) stage(p, ng, cg, kLabel || check_g != 0);
         if (!kLabel) {
 #pragma unroll
           for (int h = 0; h < kColRows; h++) ccode[h] = ncode[h];
@@ -615,7 +615,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
     }
   }
   const uint64_t em = __ballot(lerr != 0);
-  if (kLabel && em && lane == 0) atomicOr(&run->err, 1u);
+  if (em && lane == 0) atomicOr(&run->err, 1u);
   if (!kDetect) return;
 #pragma unroll
   for (int c = 0; c < 3; c++) {

@@ -1099,10 +1099,11 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   // a dense detection sweep (slot 0 holds mode Dense)
   PMSZ_HIP(hipEventRecord(c.ev2, c.stream));
   k_columns<T, true, false, kMulti><<<g_label, kColThreads, 0, c.stream>>>(
-      d0, doms, ndom, ds, c.tmark.p, c.ring, 0, 1, 1u, c.run);
+      d0, doms, ndom, ds, c.tmark.p, c.ring, 0, 1, 1u, c.run, 0u);
   PMSZ_HIP(hipEventRecord(c.ev3, c.stream));
+  // detection 1 also rejects NaN / sentinel values of g (padded blocks: the scatter did)
   k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
-      d0, doms, ndom, ds, c.tmark.p, c.ring, 0, 1, 1u, c.run);
+      d0, doms, ndom, ds, c.tmark.p, c.ring, 0, 1, 1u, c.run, job.subdomains ? 0u : 1u);
   PMSZ_HIP(hipGetLastError());
   PMSZ_HIP(hipEventRecord(c.ev4, c.stream));
   k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(c.tmark.p, c.tlist.p, c.ring, 0, 1,
@@ -1125,7 +1126,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
       }
       k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
           d0, doms, ndom, ds, c.tmark.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, (j + 1) & 1u,
-          c.run);
+          c.run, 0u);
       k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
           c.tmark.p, c.tlist.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false,
           (uint32_t)job.words);
@@ -1563,10 +1564,10 @@ int time_first_pass(Ctx &c, const pmsz_params *p, const T *f, const T *g, int it
                             c.stream));
     PMSZ_HIP(hipEventRecord(c.ev0, c.stream));
     k_columns<T, true, false, false><<<gl, kColThreads, 0, c.stream>>>(
-        job.doms[0], nullptr, 1, nullptr, c.tmark.p, c.ring, 0, 1, 1u, c.run);
+        job.doms[0], nullptr, 1, nullptr, c.tmark.p, c.ring, 0, 1, 1u, c.run, 0u);
     PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
     k_columns<T, false, true, false><<<gd, kColThreads, 0, c.stream>>>(
-        job.doms[0], nullptr, 1, nullptr, c.tmark.p, c.ring, 0, 1, 1u, c.run);
+        job.doms[0], nullptr, 1, nullptr, c.tmark.p, c.ring, 0, 1, 1u, c.run, 0u);
     PMSZ_HIP(hipGetLastError());
     PMSZ_HIP(hipEventRecord(c.ev2, c.stream));
     PMSZ_HIP(hipStreamSynchronize(c.stream));
@@ -1788,12 +1789,12 @@ int pmsz_codes(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_fiel
     static const int occ = occupancy(k_columns<float, true, false, false>, kColThreads);
     k_columns<float, true, false, false><<<col_blocks(c, dom_units(D), occ), kColThreads, 0,
                                            c.stream>>>(D, nullptr, 1, nullptr, nullptr, c.ring, 0,
-                                                       0, 0u, c.run);
+                                                       0, 0u, c.run, 0u);
   } else {
     static const int occ = occupancy(k_columns<double, true, false, false>, kColThreads);
     k_columns<double, true, false, false><<<col_blocks(c, dom_units(D), occ), kColThreads, 0,
                                             c.stream>>>(D, nullptr, 1, nullptr, nullptr, c.ring,
-                                                        0, 0, 0u, c.run);
+                                                        0, 0, 0u, c.run, 0u);
   }
   PMSZ_HIP(hipGetLastError());
   PMSZ_HIP(hipMemcpyAsync(c.h_run, c.run, sizeof(RunState), hipMemcpyDeviceToHost, c.stream));
