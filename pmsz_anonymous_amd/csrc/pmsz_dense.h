@@ -205,56 +205,77 @@ __device__ __forceinline__ double kmin(double a, double b) {
 // Flush the completed target plane `pz` of this wave's column block: `m` holds,
 // per lane, bits (dx+1)*kColH + row for targets at (x+dx, y0-1+row, pz).  After
 // the exchange with the neighbouring lanes each lane holds the target bits of its
-// own x and ORs each one into the bitmap without waiting for the old word (the
-// target list is compacted from the bitmap afterwards).
+// own x and marks each target's byte in the run's target byte map with a plain
+// store: several waves may mark one cell, all with the same value, so no atomic
+// is needed.  The target list is compacted from the byte map afterwards.
 __device__ __forceinline__ void flush_targets(uint32_t m, int64_t xl, int64_t y0m1, int64_t pz,
-                                              const Grid3 &d, uint32_t *tmark) {
+                                              const Grid3 &d, uint8_t *tbyte) {
   const uint32_t dxm = m & ((1u << kColH) - 1);                   // dx = -1
   const uint32_t dx0 = (m >> kColH) & ((1u << kColH) - 1);        // dx =  0
   const uint32_t dxp = (m >> (2 * kColH)) & ((1u << kColH) - 1);  // dx = +1
   uint32_t col = dx0 | from_left_u(dxp) | from_right_u(dxm);
-  // a set bit is a target, so its cell lies in the domain
+  // a set bit is a target, so its cell lies in the block
   const int64_t base = xl + d.nx * y0m1 + d.sxy * pz;
   while (col) {
     const int h = __ffs(col) - 1;
     col &= col - 1;
-    const uint64_t i = (uint64_t)(base + d.nx * h);
-    __hip_atomic_fetch_or(&tmark[i >> 5], 1u << (i & 31u), __ATOMIC_RELAXED,
-                          __HIP_MEMORY_SCOPE_AGENT);
+    tbyte[base + d.nx * h] = 1;
   }
 }
 
-// Target list from the target bitmap after a full-volume detection: every set
-// bit is one distinct target.  Each block takes kCompactWords words per step and
-// reserves its list range with one atomic.
+// Target list from the target byte map after a full-volume detection: every
+// nonzero byte is one distinct target, listed as its voxel id, and cleared.
+// Each thread takes 16 bytes per step; a block reserves its list range with one
+// atomic.
 constexpr int kCompactThreads = 256;
-constexpr int kCompactPT = 8;  // words per thread per step
-__global__ __launch_bounds__(kCompactThreads) void k_compact(const uint32_t *__restrict__ tmark,
-                                                             uint32_t *__restrict__ tlist,
-                                                             Counters *__restrict__ ring,
-                                                             uint32_t jslot_prev, uint32_t jslot,
-                                                             bool always, uint32_t nwords) {
+constexpr int kCompactPT = 4;  // 16-byte groups per thread per step
+__global__ __launch_bounds__(kCompactThreads) void k_compact(
+    uint8_t *__restrict__ tbyte, uint32_t *__restrict__ tlist, Counters *__restrict__ ring,
+    uint32_t jslot_prev, uint32_t jslot, bool always, uint32_t ngroups,
+    const Dom *__restrict__ doms, uint32_t ndom) {
   if (!always && ring[jslot_prev].mode != kModeDense) return;  // uniform
   uint32_t *counter = &ring[jslot].ntargets;
+  uint4 *g16 = reinterpret_cast<uint4 *>(tbyte);
   const uint32_t step = gridDim.x * kCompactThreads * kCompactPT;
-  for (uint32_t base = blockIdx.x * kCompactThreads * kCompactPT; base < nwords; base += step) {
-    uint32_t w[kCompactPT];
+  for (uint32_t base = blockIdx.x * kCompactThreads * kCompactPT; base < ngroups; base += step) {
+    uint32_t w[kCompactPT][4];
     uint32_t n = 0;
 #pragma unroll
     for (int q = 0; q < kCompactPT; q++) {
       const uint32_t i = base + (uint32_t)q * kCompactThreads + threadIdx.x;
-      w[q] = i < nwords ? tmark[i] : 0u;
-      n += __popc(w[q]);
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (i < ngroups) v = g16[i];
+      w[q][0] = v.x;
+      w[q][1] = v.y;
+      w[q][2] = v.z;
+      w[q][3] = v.w;
+#pragma unroll
+      for (int c = 0; c < 4; c++) n += __popc(w[q][c] & 0x01010101u);
+      if (v.x | v.y | v.z | v.w) g16[i] = make_uint4(0, 0, 0, 0);
     }
     uint32_t pos = block_reserve<kCompactThreads>(n, counter);
 #pragma unroll
     for (int q = 0; q < kCompactPT; q++) {
       const uint32_t i = base + (uint32_t)q * kCompactThreads + threadIdx.x;
-      uint32_t m = w[q];
-      while (m) {
-        const int b = __ffs(m) - 1;
-        m &= m - 1;
-        tlist[pos++] = i * 32u + (uint32_t)b;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        uint32_t m = w[q][c] & 0x01010101u;
+        while (m) {
+          const int b = (__ffs(m) - 1) >> 3;
+          m &= m - 1;
+          const uint32_t cell = i * 16u + (uint32_t)(c * 4 + b);
+          uint32_t id = cell;
+          if (ndom > 1) {  // the domain whose cells start at or before `cell`
+            uint32_t lo = 0, hi = ndom;
+            while (hi - lo > 1) {
+              const uint32_t mid = (lo + hi) / 2;
+              if (doms[mid].cbase <= cell) lo = mid;
+              else hi = mid;
+            }
+            id = doms[lo].gbase + (cell - doms[lo].cbase);
+          }
+          tlist[pos++] = id;
+        }
       }
     }
   }
@@ -268,14 +289,16 @@ __host__ inline void col_units(int64_t bx, int64_t by, int64_t bz, uint32_t &nux
   nuz = (uint32_t)((bz + kColZ - 1) / kColZ);
 }
 
-// The full-volume kernel.  With kLabel the original labeling of f (codes), with
-// kDetect a detection sweep on g (detection 1 when both are set; a later dense
-// detection reading the codes when only kDetect is set).  kMulti: the units of
-// all domains of the run, domain state in ds (par = parity of the detection).
+// The full-volume kernel.  With kLabel the original labeling of f (codes; the
+// delta counters are cleared when the domain has them), with kDetect a
+// detection sweep on g reading the codes (check_g: reject NaN / sentinel values
+// of g, in the first sweep).  kMulti: the units of all domains of the run,
+// domain state in ds (par = parity of the detection).  Targets go to the byte
+// map tbyte.
 template <typename T, bool kLabel, bool kDetect, bool kMulti>
 __global__ __launch_bounds__(kColThreads) void k_columns(
     Dom d0, const Dom *__restrict__ doms, uint32_t ndom, DomState *__restrict__ ds,
-    uint32_t *__restrict__ tmark, Counters *__restrict__ ring, uint32_t jslot_prev,
+    uint8_t *__restrict__ tbyte, Counters *__restrict__ ring, uint32_t jslot_prev,
     uint32_t jslot, uint32_t par, RunState *run, uint32_t check_g) {
   if (!kLabel) {
     if (ring[jslot_prev].mode != kModeDense) return;  // uniform
@@ -395,7 +418,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
     // target bits of planes e-1, e, e+1 in three 18-bit fields: (dx+1)*6 + row
     uint64_t acc = 0;
     uint32_t uc[3] = {0, 0, 0};              // counters of this unit
-    const int64_t gbase = (int64_t)D.gbase;
+    const int64_t cbase = (int64_t)D.cbase;
     // detection at owned voxel row h of plane e: codes, counters, target masks
     auto visit = [&](int h, int e, int oa, int od, int da, int dd) {
       const int y = y0m1 + h;
@@ -418,8 +441,8 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
     };
     auto flush_plane = [&](int e) {
       // plane e-1 can receive no more targets
-      flush_targets((uint32_t)acc & ((1u << (3 * kColH)) - 1u), gbase + x, y0m1, (int64_t)e - 1,
-                    D.d, tmark);
+      flush_targets((uint32_t)acc & ((1u << (3 * kColH)) - 1u), cbase + x, y0m1, (int64_t)e - 1,
+                    D.d, tbyte);
       acc >>= 3 * kColH;
     };
     if constexpr (sizeof(T) == 4) {
@@ -599,10 +622,10 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
       }
     }
     if (kDetect) {
-      flush_targets((uint32_t)acc & ((1u << (3 * kColH)) - 1u), gbase + x, y0m1, (int64_t)ze - 1,
-                    D.d, tmark);
-      flush_targets((uint32_t)(acc >> (3 * kColH)) & ((1u << (3 * kColH)) - 1u), gbase + x, y0m1,
-                    (int64_t)ze, D.d, tmark);
+      flush_targets((uint32_t)acc & ((1u << (3 * kColH)) - 1u), cbase + x, y0m1, (int64_t)ze - 1,
+                    D.d, tbyte);
+      flush_targets((uint32_t)(acc >> (3 * kColH)) & ((1u << (3 * kColH)) - 1u), cbase + x, y0m1,
+                    (int64_t)ze, D.d, tbyte);
       if (kMulti) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {

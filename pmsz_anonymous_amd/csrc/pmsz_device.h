@@ -65,7 +65,7 @@ struct Dom {
   uint32_t shell_n;   // cells of the 2-layer shell (padded blocks)
   uint32_t padded;    // 1: cells holding the sentinel are missing neighbours
   uint32_t gbase;     // domain << shift
-  uint32_t pad_;
+  uint32_t cbase;     // offset of the block's first cell in the run's target byte map
   EditParams ep;      // bound, step, q and clamp mode of this domain's edits
 };
 

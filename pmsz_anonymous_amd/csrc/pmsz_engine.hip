@@ -866,6 +866,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr;
   Buf<uint8_t> code, dc, snap;   // per stored cell
+  Buf<uint8_t> tbyte;            // target byte map of full-volume detections (16-padded)
   Buf<uint8_t> fpad, gpad;       // padded subdomain blocks (bytes)
   Buf<uint32_t> tmark, wmark;    // one bit per voxel id
   Buf<uint32_t> tlist, wlist;    // voxel ids
@@ -880,7 +881,7 @@ struct Ctx {
 };
 
 void ctx_release(Ctx &c) {
-  for (Buf<uint8_t> *b : {&c.code, &c.dc, &c.snap, &c.fpad, &c.gpad}) {
+  for (Buf<uint8_t> *b : {&c.code, &c.dc, &c.snap, &c.fpad, &c.gpad, &c.tbyte}) {
     (void)hipFree(b->p);
     *b = Buf<uint8_t>();
   }
@@ -914,7 +915,7 @@ void ctx_release(Ctx &c) {
 
 // Library-owned device bytes a context holds.
 size_t ctx_bytes(const Ctx &c) {
-  return c.code.cap + c.dc.cap + c.snap.cap + c.fpad.cap + c.gpad.cap +
+  return c.code.cap + c.dc.cap + c.snap.cap + c.fpad.cap + c.gpad.cap + c.tbyte.cap +
          4 * (c.tmark.cap + c.wmark.cap + c.tlist.cap + c.wlist.cap) +
          sizeof(Dom) * c.doms.cap + sizeof(DomState) * c.ds.cap +
          (c.ring ? sizeof(Counters) * kCtrRing + sizeof(RunState) : 0);
@@ -1041,6 +1042,7 @@ struct Job {
   std::vector<Dom> doms;
   uint32_t shift = 32;     // id = (domain << shift) | local (multi-domain runs)
   uint64_t words = 0;      // bitmap words
+  uint64_t groups = 0;     // 16-byte groups of the target byte map
   uint64_t list_cap = 0;   // list entries (distinct cells)
   Sched s;
   bool subdomains = false; // padded blocks with ghost syncs
@@ -1075,7 +1077,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   const unsigned g_label = col_blocks(c, nunits, occ_label);
   const unsigned g_dense = col_blocks(c, nunits, occ_dense);
   const unsigned g_list = (unsigned)std::max(1, c.cus * 4);
-  const unsigned g_compact = compact_blocks(c, (int64_t)job.words);
+  const unsigned g_compact = compact_blocks(c, (int64_t)job.groups);
   const bool sync_every = job.subdomains && job.s.mode == PMSZ_SYNC;
   const bool sync_lazy = job.subdomains && job.s.mode == PMSZ_LAZY;
   const int sx = job.sub[0], sy = job.sub[1], sz = job.sub[2];
@@ -1089,6 +1091,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   PMSZ_HIP(hipEventRecord(c.ev0, c.stream));
   PMSZ_HIP(hipMemsetAsync(c.tmark.p, 0, sizeof(uint32_t) * job.words, c.stream));
   PMSZ_HIP(hipMemsetAsync(c.wmark.p, 0, sizeof(uint32_t) * job.words, c.stream));
+  PMSZ_HIP(hipMemsetAsync(c.tbyte.p, 0, 16 * job.groups, c.stream));
   PMSZ_HIP(hipMemsetAsync(c.ring, 0, sizeof(Counters) * kCtrRing, c.stream));
   PMSZ_HIP(hipMemsetAsync(c.run, 0, sizeof(RunState), c.stream));
   // apply(1) reads slot 0's mode: mark detection 1 as a running sweep
@@ -1099,15 +1102,15 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   // a dense detection sweep (slot 0 holds mode Dense)
   PMSZ_HIP(hipEventRecord(c.ev2, c.stream));
   k_columns<T, true, false, kMulti><<<g_label, kColThreads, 0, c.stream>>>(
-      d0, doms, ndom, ds, c.tmark.p, c.ring, 0, 1, 1u, c.run, 0u);
+      d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, 0u);
   PMSZ_HIP(hipEventRecord(c.ev3, c.stream));
   // detection 1 also rejects NaN / sentinel values of g (padded blocks: the scatter did)
   k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
-      d0, doms, ndom, ds, c.tmark.p, c.ring, 0, 1, 1u, c.run, job.subdomains ? 0u : 1u);
+      d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, job.subdomains ? 0u : 1u);
   PMSZ_HIP(hipGetLastError());
   PMSZ_HIP(hipEventRecord(c.ev4, c.stream));
-  k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(c.tmark.p, c.tlist.p, c.ring, 0, 1,
-                                                         true, (uint32_t)job.words);
+  k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(c.tbyte.p, c.tlist.p, c.ring, 0, 1,
+                                                         true, (uint32_t)job.groups, doms, ndom);
   if (kMulti)
     k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, 0, 0, 1, kOpDetect, job.s, 1);
 
@@ -1125,11 +1128,11 @@ int drive(Ctx &c, Job &job, Outcome &out) {
                                                             c.wmark.p, c.wlist.p, c.ring, j, 1);
       }
       k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
-          d0, doms, ndom, ds, c.tmark.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, (j + 1) & 1u,
+          d0, doms, ndom, ds, c.tbyte.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, (j + 1) & 1u,
           c.run, 0u);
       k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
-          c.tmark.p, c.tlist.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false,
-          (uint32_t)job.words);
+          c.tbyte.p, c.tlist.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false,
+          (uint32_t)job.groups, doms, ndom);
       k_detect_list<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
           d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j);
       if (kMulti)
@@ -1210,9 +1213,11 @@ int prepare(Ctx &c, Job &job) {
   if (job.doms.size() == 1) ids = (uint64_t)job.doms[0].d.n;
   else ids = (uint64_t)job.doms.size() << job.shift;
   job.words = (ids + 31) / 32 + 1;
+  job.groups = (cells + 15) / 16;
   job.list_cap = cells;
   int rc;
-  if ((rc = c.tmark.ensure(job.words)) || (rc = c.wmark.ensure(job.words)) ||
+  if ((rc = c.tbyte.ensure(16 * job.groups)) || (rc = c.tmark.ensure(job.words)) ||
+      (rc = c.wmark.ensure(job.words)) ||
       (rc = c.tlist.ensure(job.list_cap)) || (rc = c.wlist.ensure(job.list_cap)))
     return rc;
   if (job.doms.size() > 1 || job.subdomains) {
@@ -1383,6 +1388,7 @@ int correct_subdomains(Ctx &c, const pmsz_params *p, const T *F, T *G, uint8_t *
     D.dc = c.dc.p + offs[k];
     D.snap = c.snap.p + offs[k];
     D.gbase = k << job.shift;
+    D.cbase = (uint32_t)offs[k];
     D.unit_base = ubase;
     ubase += dom_units(D);
   }
@@ -1488,6 +1494,7 @@ int correct_batch(Ctx &c, int n, const pmsz_params *p, const void *const *d_f,
       const pmsz_params *pk = &p[first + k];
       Dom D = field_dom(pk, d_f[first + k], d_g[first + k], c.code.p + off, c.dc.p + off);
       D.gbase = (uint32_t)k << job.shift;
+      D.cbase = (uint32_t)off;
       D.unit_base = ubase;
       ubase += dom_units(D);
       off += (uint64_t)D.d.n;
@@ -1558,16 +1565,16 @@ int time_first_pass(Ctx &c, const pmsz_params *p, const T *f, const T *g, int it
   double tl = 0, td = 0;
   static const uint32_t kRunning = kModeDense;
   for (int it = 0; it < iters; it++) {
-    PMSZ_HIP(hipMemsetAsync(c.tmark.p, 0, sizeof(uint32_t) * job.words, c.stream));
+    PMSZ_HIP(hipMemsetAsync(c.tbyte.p, 0, 16 * job.groups, c.stream));
     PMSZ_HIP(hipMemsetAsync(c.ring, 0, sizeof(Counters) * 2, c.stream));
     PMSZ_HIP(hipMemcpyAsync(&c.ring[0].mode, &kRunning, sizeof(uint32_t), hipMemcpyHostToDevice,
                             c.stream));
     PMSZ_HIP(hipEventRecord(c.ev0, c.stream));
     k_columns<T, true, false, false><<<gl, kColThreads, 0, c.stream>>>(
-        job.doms[0], nullptr, 1, nullptr, c.tmark.p, c.ring, 0, 1, 1u, c.run, 0u);
+        job.doms[0], nullptr, 1, nullptr, c.tbyte.p, c.ring, 0, 1, 1u, c.run, 0u);
     PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
     k_columns<T, false, true, false><<<gd, kColThreads, 0, c.stream>>>(
-        job.doms[0], nullptr, 1, nullptr, c.tmark.p, c.ring, 0, 1, 1u, c.run, 0u);
+        job.doms[0], nullptr, 1, nullptr, c.tbyte.p, c.ring, 0, 1, 1u, c.run, 0u);
     PMSZ_HIP(hipGetLastError());
     PMSZ_HIP(hipEventRecord(c.ev2, c.stream));
     PMSZ_HIP(hipStreamSynchronize(c.stream));
@@ -1665,7 +1672,7 @@ int pmsz_ctx_trim(pmsz_ctx *ctx) {
   Ctx &c = ctx->c;
   std::lock_guard<std::mutex> lk(c.mu);
   if (c.stream) PMSZ_HIP(hipStreamSynchronize(c.stream));
-  for (Buf<uint8_t> *b : {&c.code, &c.dc, &c.snap, &c.fpad, &c.gpad}) {
+  for (Buf<uint8_t> *b : {&c.code, &c.dc, &c.snap, &c.fpad, &c.gpad, &c.tbyte}) {
     (void)hipFree(b->p);
     *b = Buf<uint8_t>();
   }
@@ -1692,8 +1699,8 @@ int64_t pmsz_workspace_bytes(int n, const pmsz_params *params) {
     maxn = std::max(maxn, nk);
   }
   const uint64_t ids = n == 1 ? maxn : ((uint64_t)n << std::max<uint32_t>(5, ceil_log2(maxn)));
-  // code + dc per cell, two bitmaps over the ids, two lists of cell ids
-  return (int64_t)(2 * cells + 2 * 4 * ((ids + 31) / 32 + 1) + 2 * 4 * cells +
+  // code, dc and target byte per cell, two bitmaps over the ids, two lists of ids
+  return (int64_t)(3 * cells + 16 + 2 * 4 * ((ids + 31) / 32 + 1) + 2 * 4 * cells +
                    (sizeof(Dom) + sizeof(DomState)) * (uint64_t)n);
 }
 
@@ -1788,12 +1795,12 @@ int pmsz_codes(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_fiel
   if (dtype == PMSZ_F32) {
     static const int occ = occupancy(k_columns<float, true, false, false>, kColThreads);
     k_columns<float, true, false, false><<<col_blocks(c, dom_units(D), occ), kColThreads, 0,
-                                           c.stream>>>(D, nullptr, 1, nullptr, nullptr, c.ring, 0,
+                                           c.stream>>>(D, nullptr, 1, nullptr, (uint8_t *)nullptr, c.ring, 0,
                                                        0, 0u, c.run, 0u);
   } else {
     static const int occ = occupancy(k_columns<double, true, false, false>, kColThreads);
     k_columns<double, true, false, false><<<col_blocks(c, dom_units(D), occ), kColThreads, 0,
-                                            c.stream>>>(D, nullptr, 1, nullptr, nullptr, c.ring,
+                                            c.stream>>>(D, nullptr, 1, nullptr, (uint8_t *)nullptr, c.ring,
                                                         0, 0, 0u, c.run, 0u);
   }
   PMSZ_HIP(hipGetLastError());
