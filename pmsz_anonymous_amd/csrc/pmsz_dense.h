@@ -223,64 +223,6 @@ __device__ __forceinline__ void flush_targets(uint32_t m, int64_t xl, int64_t y0
   }
 }
 
-// Target list from the target byte map after a full-volume detection: every
-// nonzero byte is one distinct target, listed as its voxel id, and cleared.
-// Each thread takes 16 bytes per step; a block reserves its list range with one
-// atomic.
-constexpr int kCompactThreads = 256;
-constexpr int kCompactPT = 4;  // 16-byte groups per thread per step
-__global__ __launch_bounds__(kCompactThreads) void k_compact(
-    uint8_t *__restrict__ tbyte, uint32_t *__restrict__ tlist, Counters *__restrict__ ring,
-    uint32_t jslot_prev, uint32_t jslot, bool always, uint32_t ngroups,
-    const Dom *__restrict__ doms, uint32_t ndom) {
-  if (!always && ring[jslot_prev].mode != kModeDense) return;  // uniform
-  uint32_t *counter = &ring[jslot].ntargets;
-  uint4 *g16 = reinterpret_cast<uint4 *>(tbyte);
-  const uint32_t step = gridDim.x * kCompactThreads * kCompactPT;
-  for (uint32_t base = blockIdx.x * kCompactThreads * kCompactPT; base < ngroups; base += step) {
-    uint32_t w[kCompactPT][4];
-    uint32_t n = 0;
-#pragma unroll
-    for (int q = 0; q < kCompactPT; q++) {
-      const uint32_t i = base + (uint32_t)q * kCompactThreads + threadIdx.x;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (i < ngroups) v = g16[i];
-      w[q][0] = v.x;
-      w[q][1] = v.y;
-      w[q][2] = v.z;
-      w[q][3] = v.w;
-#pragma unroll
-      for (int c = 0; c < 4; c++) n += __popc(w[q][c] & 0x01010101u);
-      if (v.x | v.y | v.z | v.w) g16[i] = make_uint4(0, 0, 0, 0);
-    }
-    uint32_t pos = block_reserve<kCompactThreads>(n, counter);
-#pragma unroll
-    for (int q = 0; q < kCompactPT; q++) {
-      const uint32_t i = base + (uint32_t)q * kCompactThreads + threadIdx.x;
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        uint32_t m = w[q][c] & 0x01010101u;
-        while (m) {
-          const int b = (__ffs(m) - 1) >> 3;
-          m &= m - 1;
-          const uint32_t cell = i * 16u + (uint32_t)(c * 4 + b);
-          uint32_t id = cell;
-          if (ndom > 1) {  // the domain whose cells start at or before `cell`
-            uint32_t lo = 0, hi = ndom;
-            while (hi - lo > 1) {
-              const uint32_t mid = (lo + hi) / 2;
-              if (doms[mid].cbase <= cell) lo = mid;
-              else hi = mid;
-            }
-            id = doms[lo].gbase + (cell - doms[lo].cbase);
-          }
-          tlist[pos++] = id;
-        }
-      }
-    }
-  }
-}
-
 // Units (62 x 4 x 64 column blocks) of one domain's box.
 __host__ inline void col_units(int64_t bx, int64_t by, int64_t bz, uint32_t &nux, uint32_t &nuy,
                                uint32_t &nuz) {
@@ -327,6 +269,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
   Counters *out = kDetect ? &ring[jslot] : nullptr;
   uint32_t lc[3] = {0, 0, 0};
   uint32_t lerr = 0;
+  uint32_t nem = 0;  // target marks emitted (an upper bound of the distinct targets)
 
   for (uint32_t unit = u_begin + (blockIdx.x / 8) * kColWaves + wave; unit < u_end;
        unit += waves_per_xcd) {
@@ -436,6 +379,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
           const uint64_t b1 = (uint64_t)(v1 ? 1u : 0u) << (s_tbase[t1] + (uint32_t)h);
           const uint64_t b2 = (uint64_t)(v2 ? 1u : 0u) << (s_tbase[t2] + (uint32_t)h);
           acc |= b1 | b2;
+          nem += (v1 ? 1u : 0u) + (v2 ? 1u : 0u);
         }
       }
     };
@@ -645,6 +589,8 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
     const uint32_t s = wave_sum(lc[c]);
     if (lane == 0 && s) atomicAdd(&out->cnt[c], s);
   }
+  const uint32_t se = wave_sum(nem);
+  if (lane == 0 && se) atomicAdd(&out->nemit, se);
   if (threadIdx.x == 0 && blockIdx.x == 0 && !kLabel) atomicAdd(&run->dense_passes, 1ull);
 }
 

@@ -85,6 +85,8 @@ __device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) {
 // ---------------------------------------------------------------------------
 // Decisions of the apply of sweep j (identical in every thread): the mode of
 // detection j+1, or kModeTail, or kModeDone with *done = 1 (converged) / 2 (cap).
+// They read only what detection j wrote (counters, the target estimate nemit),
+// never what the apply itself updates, so every block decides alike.
 
 // One domain: from the counters of detection j.
 __device__ __forceinline__ uint32_t decide(const Counters &cj, uint32_t j, const Sched &s,
@@ -99,12 +101,12 @@ __device__ __forceinline__ uint32_t decide(const Counters &cj, uint32_t j, const
     *done = 2;
     return kModeDone;
   }
-  if (tail_allowed && cj.ntargets <= kTailTargets) return kModeTail;
+  if (tail_allowed && cj.nemit <= kTailTargets) return kModeTail;
   if (s.mode == PMSZ_LAZY && j >= s.max_sweeps) {
     *done = 2;  // the edits of sweep j still land (they belong to sweep j)
     return kModeDone;
   }
-  return (uint64_t)cj.ntargets * kCand >= s.dense_min_work ? kModeDense : kModeSparse;
+  return (uint64_t)cj.nemit * kCand >= s.dense_min_work ? kModeDense : kModeSparse;
 }
 
 // Several domains: from the domain bookkeeping after detection j (the caps are
@@ -117,8 +119,8 @@ __device__ __forceinline__ uint32_t decide_multi(const Counters &cj, const Sched
     *done = cj.ncapped ? 2 : 1;
     return kModeDone;
   }
-  if (tail_allowed && cj.ntargets <= kTailTargets) return kModeTail;
-  return (uint64_t)cj.ntargets * kCand >= s.dense_min_work ? kModeDense : kModeSparse;
+  if (tail_allowed && cj.nemit <= kTailTargets) return kModeTail;
+  return (uint64_t)cj.nemit * kCand >= s.dense_min_work ? kModeDense : kModeSparse;
 }
 
 // ---------------------------------------------------------------------------
@@ -295,8 +297,9 @@ __device__ __forceinline__ int apply_target(uint32_t t, const Dom &d0, const Dom
   const Dom &D = dom_of<kMulti>(t, d0, doms, shift, dk, local);
   apply_one<T>((T *)D.g, (const T *)D.f, D.dc, local, D.ep, touched, clamped);
   // every set bit of tmark is a target of this sweep and is applied in this phase,
-  // so the word can be cleared with a plain store
-  tmark[t >> 5] = 0u;
+  // so the word can be cleared with a plain store (a full-volume detection marks
+  // targets in the byte map instead: tmark is null then)
+  if (tmark) tmark[t >> 5] = 0u;
   int64_t x, y, z;
   decompose(local, D.d, x, y, z);
   if (kMulti && D.padded) {
@@ -320,6 +323,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
   Counters &cj = ring[j % kCtrRing];
   const uint32_t pm = prev.mode;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (pm == kModeDense) return;  // detection j marked the byte map: k_apply_dense
   if (pm == kModeTail || pm == kModeDone || pm == kModeUnset) {
     if (lead) cj.mode = pm == kModeUnset ? kModeDone : pm;
     return;
@@ -368,6 +372,98 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
   if (lead) atomicAdd(&run->total_targets, (unsigned long long)nt);
 }
 
+// Apply of sweep j after a full-volume detection: every marked byte of the
+// target byte map is one distinct target.  Each thread takes 16 cells, edits the
+// marked ones, clears their bytes, queues their neighbourhoods when detection
+// j+1 is sparse, and counts them (the exact target count of sweep j).
+template <typename T, bool kMulti>
+__global__ __launch_bounds__(kApplyThreads) void k_apply_dense(
+    Dom d0, const Dom *__restrict__ doms, uint32_t ndom, uint32_t shift,
+    DomState *__restrict__ ds, uint8_t *__restrict__ tbyte, uint32_t ngroups,
+    uint32_t *__restrict__ wmark, uint32_t *__restrict__ wlist, Counters *__restrict__ ring,
+    RunState *run, uint32_t j, Sched s) {
+  Counters &prev = ring[(j - 1) % kCtrRing];
+  Counters &cj = ring[j % kCtrRing];
+  if (prev.mode != kModeDense) return;  // the list apply handles this sweep
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  uint32_t done;
+  const uint32_t mode =
+      kMulti ? decide_multi(cj, s, false, &done) : decide(cj, j, s, false, &done);
+  if (lead) {
+    cj.mode = mode;
+    if (done) {
+      run->done = done;
+      run->last_j = j;
+    }
+  }
+  if (mode == kModeDone && (done == 1 || (!kMulti && s.mode == PMSZ_SYNC))) return;
+  if (lead && mode != kModeDone) zero_counters(&ring[(j + 1) % kCtrRing]);
+  const bool queue = mode == kModeSparse;
+  uint4 *g16 = reinterpret_cast<uint4 *>(tbyte);
+  const uint32_t lane = lane_id();
+  uint32_t touched = 0, clamped = 0, cnt = 0;
+  const uint32_t stride = gridDim.x * kApplyThreads;
+  for (uint32_t base = blockIdx.x * kApplyThreads; base < ngroups; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    uint32_t m16 = 0;  // bit b: cell i*16 + b is a target
+    if (i < ngroups) {
+      const uint4 v = g16[i];
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) m16 |= ((w[c] >> (8 * b)) & 1u) << (4 * c + b);
+      if (m16) g16[i] = make_uint4(0, 0, 0, 0);
+    }
+    while (__ballot(m16 != 0)) {
+      uint32_t u[kCand];
+      int n = 0;
+      uint32_t dk = 0;
+      const bool has = m16 != 0;
+      if (has) {
+        const uint32_t cell = i * 16u + (uint32_t)(__ffs(m16) - 1);
+        m16 &= m16 - 1;
+        uint32_t id = cell;
+        if (kMulti) {  // the domain whose cells start at or before `cell`
+          uint32_t lo = 0, hi = ndom;
+          while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (doms[mid].cbase <= cell) lo = mid;
+            else hi = mid;
+          }
+          id = doms[lo].gbase + (cell - doms[lo].cbase);
+        }
+        n = apply_target<T, kMulti>(id, d0, doms, shift, ds, nullptr, wmark, queue, dk, touched,
+                                    clamped, u);
+        cnt++;
+      }
+      if (kMulti) dom_add_targets(has, dk, ds);
+      if (queue) {
+        // one list reservation per wave
+        const uint32_t incl = wave_incl_scan((uint32_t)n);
+        const uint32_t tot = (uint32_t)__shfl((int)incl, 63);
+        uint32_t wbase = 0;
+        if (lane == 0 && tot) wbase = atomicAdd(&cj.nwork, tot);
+        wbase = (uint32_t)__shfl((int)wbase, 0);
+        const uint32_t pos = wbase + incl - (uint32_t)n;
+        for (int q = 0; q < n; q++) wlist[pos + q] = u[q];
+      }
+    }
+  }
+  const uint32_t sc = wave_sum(cnt);
+  if (lane == 0 && sc) {
+    atomicAdd(&cj.ntargets, sc);
+    atomicAdd(&run->total_targets, (unsigned long long)sc);
+  }
+  if (!kMulti) {
+    uint32_t ts = wave_sum(touched), cs = wave_sum(clamped);
+    if (lane == 0) {
+      if (ts) atomicAdd(&run->touched, (unsigned long long)ts);
+      if (cs) atomicAdd(&run->clamped, (unsigned long long)cs);
+    }
+  }
+}
+
 // Work-list detection of one listed voxel id (shared by k_detect_list / k_tail).
 template <typename T, bool kMulti>
 __device__ __forceinline__ int detect_listed(uint32_t v, const Dom &d0, const Dom *doms,
@@ -395,6 +491,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
   const uint32_t par = (j + 1) & 1u;
   const uint32_t nw = cj.nwork;
   uint32_t lc[3] = {0, 0, 0};
+  uint32_t ln = 0;
   const uint32_t stride = gridDim.x * kApplyThreads;
   for (uint32_t base = blockIdx.x * kApplyThreads; base < nw; base += stride) {
     const uint32_t i = base + threadIdx.x;
@@ -413,8 +510,11 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
     for (int q = 0; q < 3; q++) lc[q] += vc[q];
     uint32_t pos = block_reserve<kApplyThreads>((uint32_t)n, &cn.ntargets);
     for (int q = 0; q < n; q++) tlist[pos + q] = out[q];
+    ln += (uint32_t)n;
   }
   block_add3<kApplyThreads>(lc, cn.cnt);
+  const uint32_t sn = wave_sum(ln);
+  if (lane_id() == 0 && sn) atomicAdd(&cn.nemit, sn);  // exact for a work-list detection
   if (blockIdx.x == 0 && threadIdx.x == 0)
     atomicAdd(&run->sparse_voxels, (unsigned long long)nw);
 }
@@ -498,6 +598,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
       for (int q = 0; q < n; q++) tlist[pos + q] = out[q];
     }
     block_add3<kTailThreads>(lc, cn.cnt);
+    if (lead) cn.nemit = ld_agent(&cn.ntargets);
     sv += nw;
     if (kMulti) {
       if (lead) {
@@ -1030,10 +1131,6 @@ unsigned col_blocks(const Ctx &c, uint32_t units, int occ) {
   return (unsigned)(8 * std::max<int64_t>(1, std::min(per_xcd, cap)));
 }
 
-unsigned compact_blocks(const Ctx &c, int64_t words) {
-  const int64_t per = (int64_t)kCompactThreads * kCompactPT;
-  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((words + per - 1) / per, c.cus * 4));
-}
 
 constexpr uint32_t kChain = 8;  // apply/detect steps enqueued before each host check
 
@@ -1077,7 +1174,8 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   const unsigned g_label = col_blocks(c, nunits, occ_label);
   const unsigned g_dense = col_blocks(c, nunits, occ_dense);
   const unsigned g_list = (unsigned)std::max(1, c.cus * 4);
-  const unsigned g_compact = compact_blocks(c, (int64_t)job.groups);
+  const unsigned g_adense = (unsigned)std::max<uint64_t>(
+      1, std::min<uint64_t>((job.groups + kApplyThreads - 1) / kApplyThreads, (uint64_t)c.cus * 8));
   const bool sync_every = job.subdomains && job.s.mode == PMSZ_SYNC;
   const bool sync_lazy = job.subdomains && job.s.mode == PMSZ_LAZY;
   const int sx = job.sub[0], sy = job.sub[1], sz = job.sub[2];
@@ -1109,8 +1207,6 @@ int drive(Ctx &c, Job &job, Outcome &out) {
       d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, job.subdomains ? 0u : 1u);
   PMSZ_HIP(hipGetLastError());
   PMSZ_HIP(hipEventRecord(c.ev4, c.stream));
-  k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(c.tbyte.p, c.tlist.p, c.ring, 0, 1,
-                                                         true, (uint32_t)job.groups, doms, ndom);
   if (kMulti)
     k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, 0, 0, 1, kOpDetect, job.s, 1);
 
@@ -1118,6 +1214,9 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   out.outer_started = 1;
   while (true) {
     for (uint32_t j = j0; j < j0 + kChain; j++) {
+      k_apply_dense<T, kMulti><<<g_adense, kApplyThreads, 0, c.stream>>>(
+          d0, doms, ndom, job.shift, ds, c.tbyte.p, (uint32_t)job.groups, c.wmark.p, c.wlist.p,
+          c.ring, c.run, j, job.s);
       k_apply<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
           d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j,
           job.s);
@@ -1130,9 +1229,6 @@ int drive(Ctx &c, Job &job, Outcome &out) {
       k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
           d0, doms, ndom, ds, c.tbyte.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, (j + 1) & 1u,
           c.run, 0u);
-      k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
-          c.tbyte.p, c.tlist.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false,
-          (uint32_t)job.groups, doms, ndom);
       k_detect_list<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
           d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j);
       if (kMulti)
