@@ -373,13 +373,15 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
 }
 
 // Apply of sweep j after a full-volume detection: every marked byte of the
-// target byte map is one distinct target.  Each thread takes 16 cells, edits the
-// marked ones, clears their bytes, queues their neighbourhoods when detection
-// j+1 is sparse, and counts them (the exact target count of sweep j).
+// target byte map is one distinct target.  Each thread takes a 4-cell word of the
+// map (neighbouring threads, neighbouring cells), edits the marked cells, clears
+// the word, queues the targets' neighbourhoods when detection j+1 is sparse, and
+// counts the targets (the exact target count of sweep j).  Domains start on
+// 16-cell boundaries of the map, so a word lies in one domain.
 template <typename T, bool kMulti>
 __global__ __launch_bounds__(kApplyThreads) void k_apply_dense(
     Dom d0, const Dom *__restrict__ doms, uint32_t ndom, uint32_t shift,
-    DomState *__restrict__ ds, uint8_t *__restrict__ tbyte, uint32_t ngroups,
+    DomState *__restrict__ ds, uint8_t *__restrict__ tbyte, uint32_t nwords,
     uint32_t *__restrict__ wmark, uint32_t *__restrict__ wlist, Counters *__restrict__ ring,
     RunState *run, uint32_t j, Sched s) {
   Counters &prev = ring[(j - 1) % kCtrRing];
@@ -399,42 +401,37 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_dense(
   if (mode == kModeDone && (done == 1 || (!kMulti && s.mode == PMSZ_SYNC))) return;
   if (lead && mode != kModeDone) zero_counters(&ring[(j + 1) % kCtrRing]);
   const bool queue = mode == kModeSparse;
-  uint4 *g16 = reinterpret_cast<uint4 *>(tbyte);
+  uint32_t *t32 = reinterpret_cast<uint32_t *>(tbyte);
   const uint32_t lane = lane_id();
   uint32_t touched = 0, clamped = 0, cnt = 0;
   const uint32_t stride = gridDim.x * kApplyThreads;
-  for (uint32_t base = blockIdx.x * kApplyThreads; base < ngroups; base += stride) {
-    const uint32_t i = base + threadIdx.x;
-    uint32_t m16 = 0;  // bit b: cell i*16 + b is a target
-    if (i < ngroups) {
-      const uint4 v = g16[i];
-      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-      for (int c = 0; c < 4; c++)
-#pragma unroll
-        for (int b = 0; b < 4; b++) m16 |= ((w[c] >> (8 * b)) & 1u) << (4 * c + b);
-      if (m16) g16[i] = make_uint4(0, 0, 0, 0);
+  for (uint32_t base = blockIdx.x * kApplyThreads; base < nwords; base += stride) {
+    const uint32_t w = base + threadIdx.x;
+    uint32_t b4 = 0;
+    if (w < nwords) {
+      b4 = t32[w] & 0x01010101u;
+      if (b4) t32[w] = 0u;
     }
-    while (__ballot(m16 != 0)) {
+    uint32_t id0 = w * 4u;  // voxel id of the word's first cell
+    if (kMulti && b4) {
+      const uint32_t cell = w * 4u;
+      uint32_t lo = 0, hi = ndom;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (doms[mid].cbase <= cell) lo = mid;
+        else hi = mid;
+      }
+      id0 = doms[lo].gbase + (cell - doms[lo].cbase);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
       uint32_t u[kCand];
       int n = 0;
       uint32_t dk = 0;
-      const bool has = m16 != 0;
+      const bool has = ((b4 >> (8 * k)) & 1u) != 0;
       if (has) {
-        const uint32_t cell = i * 16u + (uint32_t)(__ffs(m16) - 1);
-        m16 &= m16 - 1;
-        uint32_t id = cell;
-        if (kMulti) {  // the domain whose cells start at or before `cell`
-          uint32_t lo = 0, hi = ndom;
-          while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) / 2;
-            if (doms[mid].cbase <= cell) lo = mid;
-            else hi = mid;
-          }
-          id = doms[lo].gbase + (cell - doms[lo].cbase);
-        }
-        n = apply_target<T, kMulti>(id, d0, doms, shift, ds, nullptr, wmark, queue, dk, touched,
-                                    clamped, u);
+        n = apply_target<T, kMulti>(id0 + (uint32_t)k, d0, doms, shift, ds, nullptr, wmark,
+                                    queue, dk, touched, clamped, u);
         cnt++;
       }
       if (kMulti) dom_add_targets(has, dk, ds);
@@ -1175,7 +1172,8 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   const unsigned g_dense = col_blocks(c, nunits, occ_dense);
   const unsigned g_list = (unsigned)std::max(1, c.cus * 4);
   const unsigned g_adense = (unsigned)std::max<uint64_t>(
-      1, std::min<uint64_t>((job.groups + kApplyThreads - 1) / kApplyThreads, (uint64_t)c.cus * 8));
+      1, std::min<uint64_t>((4 * job.groups + kApplyThreads - 1) / kApplyThreads,
+                            (uint64_t)c.cus * 16));
   const bool sync_every = job.subdomains && job.s.mode == PMSZ_SYNC;
   const bool sync_lazy = job.subdomains && job.s.mode == PMSZ_LAZY;
   const int sx = job.sub[0], sy = job.sub[1], sz = job.sub[2];
@@ -1215,8 +1213,8 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   while (true) {
     for (uint32_t j = j0; j < j0 + kChain; j++) {
       k_apply_dense<T, kMulti><<<g_adense, kApplyThreads, 0, c.stream>>>(
-          d0, doms, ndom, job.shift, ds, c.tbyte.p, (uint32_t)job.groups, c.wmark.p, c.wlist.p,
-          c.ring, c.run, j, job.s);
+          d0, doms, ndom, job.shift, ds, c.tbyte.p, (uint32_t)(4 * job.groups), c.wmark.p,
+          c.wlist.p, c.ring, c.run, j, job.s);
       k_apply<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
           d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j,
           job.s);
@@ -1304,12 +1302,15 @@ int drive(Ctx &c, Job &job, Outcome &out) {
 
 // Bitmap words and list capacity of a run, allocating what is missing.
 int prepare(Ctx &c, Job &job) {
-  uint64_t cells = 0, ids = 0;
-  for (const Dom &D : job.doms) cells += (uint64_t)D.d.n;
+  uint64_t cells = 0, ids = 0, mapcells = 0;
+  for (const Dom &D : job.doms) {
+    cells += (uint64_t)D.d.n;
+    mapcells = std::max<uint64_t>(mapcells, (uint64_t)D.cbase + (uint64_t)D.d.n);
+  }
   if (job.doms.size() == 1) ids = (uint64_t)job.doms[0].d.n;
   else ids = (uint64_t)job.doms.size() << job.shift;
   job.words = (ids + 31) / 32 + 1;
-  job.groups = (cells + 15) / 16;
+  job.groups = (mapcells + 15) / 16;
   job.list_cap = cells;
   int rc;
   if ((rc = c.tbyte.ensure(16 * job.groups)) || (rc = c.tmark.ensure(job.words)) ||
@@ -1463,7 +1464,7 @@ int correct_subdomains(Ctx &c, const pmsz_params *p, const T *F, T *G, uint8_t *
         D.padded = 1;
         D.ep = edit_params(p);
         offs.push_back((int64_t)cells);
-        cells += (uint64_t)D.d.n;
+        cells += ((uint64_t)D.d.n + 15) & ~(uint64_t)15;  // blocks start on 16-cell boundaries
         core += (uint64_t)(l[0] * l[1] * l[2]);
         maxcells = std::max<uint64_t>(maxcells, (uint64_t)D.d.n);
         job.doms.push_back(D);
@@ -1568,7 +1569,7 @@ int correct_batch(Ctx &c, int n, const pmsz_params *p, const void *const *d_f,
       const uint32_t sh = std::max<uint32_t>(5, ceil_log2(mx));
       if (((uint64_t)(last - first + 1) << sh) > 0xffffffffull) break;
       maxn = mx;
-      cells += nk;
+      cells += (nk + 15) & ~(uint64_t)15;  // fields start on 16-cell boundaries
       last++;
     }
     if (last == first) return PMSZ_ETOOBIG;
@@ -1593,7 +1594,7 @@ int correct_batch(Ctx &c, int n, const pmsz_params *p, const void *const *d_f,
       D.cbase = (uint32_t)off;
       D.unit_base = ubase;
       ubase += dom_units(D);
-      off += (uint64_t)D.d.n;
+      off += ((uint64_t)D.d.n + 15) & ~(uint64_t)15;  // fields start on 16-cell boundaries
       job.doms.push_back(D);
     }
     job.s = make_sched(&p[first], cells, true);
@@ -1608,13 +1609,11 @@ int correct_batch(Ctx &c, int n, const pmsz_params *p, const void *const *d_f,
                      (unsigned)cnt);
     k_finish<T><<<g_all, kSyncThreads, 0, c.stream>>>(c.doms.p, c.ds.p, nullptr, nullptr, 0, 0);
     PMSZ_HIP(hipGetLastError());
-    off = 0;
     for (int k = 0; k < cnt; k++) {
       const uint64_t nk = (uint64_t)job.doms[k].d.n;
       if (d_dc && d_dc[first + k])
-        PMSZ_HIP(hipMemcpyAsync(d_dc[first + k], c.dc.p + off, nk, hipMemcpyDeviceToDevice,
+        PMSZ_HIP(hipMemcpyAsync(d_dc[first + k], job.doms[k].dc, nk, hipMemcpyDeviceToDevice,
                                 c.stream));
-      off += nk;
     }
     PMSZ_HIP(hipMemcpyAsync(c.h_ds, c.ds.p, sizeof(DomState) * cnt, hipMemcpyDeviceToHost,
                             c.stream));
@@ -1791,7 +1790,7 @@ int64_t pmsz_workspace_bytes(int n, const pmsz_params *params) {
   for (int k = 0; k < n; k++) {
     if (check_params(&params[k])) return PMSZ_EINVAL;
     const uint64_t nk = (uint64_t)(params[k].nx * params[k].ny * params[k].nz);
-    cells += nk;
+    cells += (nk + 15) & ~(uint64_t)15;
     maxn = std::max(maxn, nk);
   }
   const uint64_t ids = n == 1 ? maxn : ((uint64_t)n << std::max<uint32_t>(5, ceil_log2(maxn)));
