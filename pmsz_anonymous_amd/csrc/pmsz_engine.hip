@@ -423,7 +423,8 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_dense(
       }
       id0 = doms[lo].gbase + (cell - doms[lo].cbase);
     }
-#pragma unroll
+    // one copy of the edit / queue code (kept small for the instruction cache)
+#pragma unroll 1
     for (int k = 0; k < 4; k++) {
       uint32_t u[kCand];
       int n = 0;
