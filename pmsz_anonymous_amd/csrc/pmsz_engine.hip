@@ -372,16 +372,25 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
   if (lead) atomicAdd(&run->total_targets, (unsigned long long)nt);
 }
 
+// Ordering of a wave's own LDS writes before its other lanes read them (LDS
+// operations of one wave complete in order; this stops the compiler moving them).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Apply of sweep j after a full-volume detection: every marked byte of the
-// target byte map is one distinct target.  Each thread takes a 4-cell word of the
-// map (neighbouring threads, neighbouring cells), edits the marked cells, clears
-// the word, queues the targets' neighbourhoods when detection j+1 is sparse, and
-// counts the targets (the exact target count of sweep j).  Domains start on
-// 16-cell boundaries of the map, so a word lies in one domain.
+// target byte map is one distinct target.  Each wave takes 1024 cells of the map
+// at a time (16 per lane, neighbouring lanes on neighbouring cells), clears
+// them, lists the marked cells in its LDS slice, and applies that list with
+// every lane busy: the edits, the neighbourhood queue when detection j+1 is
+// sparse, and the exact target count of sweep j.
+constexpr int kAdWaves = kApplyThreads / 64;
 template <typename T, bool kMulti>
 __global__ __launch_bounds__(kApplyThreads) void k_apply_dense(
     Dom d0, const Dom *__restrict__ doms, uint32_t ndom, uint32_t shift,
-    DomState *__restrict__ ds, uint8_t *__restrict__ tbyte, uint32_t nwords,
+    DomState *__restrict__ ds, uint8_t *__restrict__ tbyte, uint32_t ngroups,
     uint32_t *__restrict__ wmark, uint32_t *__restrict__ wlist, Counters *__restrict__ ring,
     RunState *run, uint32_t j, Sched s) {
   Counters &prev = ring[(j - 1) % kCtrRing];
@@ -401,52 +410,70 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply_dense(
   if (mode == kModeDone && (done == 1 || (!kMulti && s.mode == PMSZ_SYNC))) return;
   if (lead && mode != kModeDone) zero_counters(&ring[(j + 1) % kCtrRing]);
   const bool queue = mode == kModeSparse;
-  uint32_t *t32 = reinterpret_cast<uint32_t *>(tbyte);
+  __shared__ uint32_t s_ids[kAdWaves][64 * 16];
+  uint4 *g16 = reinterpret_cast<uint4 *>(tbyte);
   const uint32_t lane = lane_id();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  uint32_t *ids = s_ids[wave];
   uint32_t touched = 0, clamped = 0, cnt = 0;
-  const uint32_t stride = gridDim.x * kApplyThreads;
-  for (uint32_t base = blockIdx.x * kApplyThreads; base < nwords; base += stride) {
-    const uint32_t w = base + threadIdx.x;
-    uint32_t b4 = 0;
-    if (w < nwords) {
-      b4 = t32[w] & 0x01010101u;
-      if (b4) t32[w] = 0u;
+  const uint32_t nwaves = gridDim.x * kAdWaves;
+  for (uint32_t chunk = blockIdx.x * kAdWaves + wave; chunk * 64 < ngroups; chunk += nwaves) {
+    const uint32_t i = chunk * 64 + lane;
+    uint32_t m16 = 0;  // bit b: cell i*16 + b is a target
+    if (i < ngroups) {
+      const uint4 v = g16[i];
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) m16 |= ((w[c] >> (8 * b)) & 1u) << (4 * c + b);
+      if (m16) g16[i] = make_uint4(0, 0, 0, 0);
     }
-    uint32_t id0 = w * 4u;  // voxel id of the word's first cell
-    if (kMulti && b4) {
-      const uint32_t cell = w * 4u;
-      uint32_t lo = 0, hi = ndom;
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (doms[mid].cbase <= cell) lo = mid;
-        else hi = mid;
-      }
-      id0 = doms[lo].gbase + (cell - doms[lo].cbase);
+    const uint32_t n = __popc(m16);
+    const uint32_t incl = wave_incl_scan(n);
+    const uint32_t tot = (uint32_t)__shfl((int)incl, 63);
+    if (tot == 0) continue;  // uniform
+    uint32_t pos = incl - n;
+    while (m16) {
+      ids[pos++] = i * 16u + (uint32_t)(__ffs(m16) - 1);
+      m16 &= m16 - 1;
     }
-    // one copy of the edit / queue code (kept small for the instruction cache)
-#pragma unroll 1
-    for (int k = 0; k < 4; k++) {
+    wave_lds_sync();
+    for (uint32_t r = 0; r < tot; r += 64) {
+      const uint32_t q = r + lane;
+      const bool has = q < tot;
       uint32_t u[kCand];
-      int n = 0;
+      int nq = 0;
       uint32_t dk = 0;
-      const bool has = ((b4 >> (8 * k)) & 1u) != 0;
       if (has) {
-        n = apply_target<T, kMulti>(id0 + (uint32_t)k, d0, doms, shift, ds, nullptr, wmark,
-                                    queue, dk, touched, clamped, u);
+        const uint32_t cell = ids[q];
+        uint32_t id = cell;
+        if (kMulti) {  // the domain whose cells start at or before `cell`
+          uint32_t lo = 0, hi = ndom;
+          while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (doms[mid].cbase <= cell) lo = mid;
+            else hi = mid;
+          }
+          id = doms[lo].gbase + (cell - doms[lo].cbase);
+        }
+        nq = apply_target<T, kMulti>(id, d0, doms, shift, ds, nullptr, wmark, queue, dk,
+                                     touched, clamped, u);
         cnt++;
       }
       if (kMulti) dom_add_targets(has, dk, ds);
       if (queue) {
         // one list reservation per wave
-        const uint32_t incl = wave_incl_scan((uint32_t)n);
-        const uint32_t tot = (uint32_t)__shfl((int)incl, 63);
+        const uint32_t qi = wave_incl_scan((uint32_t)nq);
+        const uint32_t qt = (uint32_t)__shfl((int)qi, 63);
         uint32_t wbase = 0;
-        if (lane == 0 && tot) wbase = atomicAdd(&cj.nwork, tot);
+        if (lane == 0 && qt) wbase = atomicAdd(&cj.nwork, qt);
         wbase = (uint32_t)__shfl((int)wbase, 0);
-        const uint32_t pos = wbase + incl - (uint32_t)n;
-        for (int q = 0; q < n; q++) wlist[pos + q] = u[q];
+        const uint32_t p0 = wbase + qi - (uint32_t)nq;
+        for (int k = 0; k < nq; k++) wlist[p0 + k] = u[k];
       }
     }
+    wave_lds_sync();  // the slice is rewritten by the next chunk
   }
   const uint32_t sc = wave_sum(cnt);
   if (lane == 0 && sc) {
@@ -1172,9 +1199,10 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   const unsigned g_label = col_blocks(c, nunits, occ_label);
   const unsigned g_dense = col_blocks(c, nunits, occ_dense);
   const unsigned g_list = (unsigned)std::max(1, c.cus * 4);
+  // k_apply_dense: a wave per 64 groups of 16 cells, persistent
   const unsigned g_adense = (unsigned)std::max<uint64_t>(
-      1, std::min<uint64_t>((4 * job.groups + kApplyThreads - 1) / kApplyThreads,
-                            (uint64_t)c.cus * 16));
+      1, std::min<uint64_t>((job.groups + 64 * kAdWaves - 1) / (64 * kAdWaves),
+                            (uint64_t)c.cus * 8));
   const bool sync_every = job.subdomains && job.s.mode == PMSZ_SYNC;
   const bool sync_lazy = job.subdomains && job.s.mode == PMSZ_LAZY;
   const int sx = job.sub[0], sy = job.sub[1], sz = job.sub[2];
@@ -1214,7 +1242,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   while (true) {
     for (uint32_t j = j0; j < j0 + kChain; j++) {
       k_apply_dense<T, kMulti><<<g_adense, kApplyThreads, 0, c.stream>>>(
-          d0, doms, ndom, job.shift, ds, c.tbyte.p, (uint32_t)(4 * job.groups), c.wmark.p,
+          d0, doms, ndom, job.shift, ds, c.tbyte.p, (uint32_t)job.groups, c.wmark.p,
           c.wlist.p, c.ring, c.run, j, job.s);
       k_apply<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
           d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j,
