@@ -205,11 +205,12 @@ __device__ __forceinline__ double kmin(double a, double b) {
 // Flush the completed target plane `pz` of this wave's column block: `m` holds,
 // per lane, bits (dx+1)*kColH + row for targets at (x+dx, y0-1+row, pz).  After
 // the exchange with the neighbouring lanes each lane holds the target bits of its
-// own x and marks each target's byte in the run's target byte map with a plain
-// store: several waves may mark one cell, all with the same value, so no atomic
-// is needed.  The target list is compacted from the byte map afterwards.
+// own x and marks each target's byte in the run's target byte map with the
+// sweep's tag, by a plain store: several waves may mark one cell, all with the
+// same value, so no atomic is needed.  The target list is compacted from the
+// byte map afterwards; bytes holding other tags are marks of earlier sweeps.
 __device__ __forceinline__ void flush_targets(uint32_t m, int64_t xl, int64_t y0m1, int64_t pz,
-                                              const Grid3 &d, uint8_t *tbyte) {
+                                              const Grid3 &d, uint8_t *tbyte, uint8_t tag) {
   const uint32_t dxm = m & ((1u << kColH) - 1);                   // dx = -1
   const uint32_t dx0 = (m >> kColH) & ((1u << kColH) - 1);        // dx =  0
   const uint32_t dxp = (m >> (2 * kColH)) & ((1u << kColH) - 1);  // dx = +1
@@ -219,9 +220,77 @@ __device__ __forceinline__ void flush_targets(uint32_t m, int64_t xl, int64_t y0
   while (col) {
     const int h = __ffs(col) - 1;
     col &= col - 1;
-    tbyte[base + d.nx * h] = 1;
+    tbyte[base + d.nx * h] = tag;
   }
 }
+
+// Target list from the target byte map after a full-volume detection: every
+// byte equal to the sweep's tag is one distinct target, listed as its voxel id.
+// Each thread takes 16-byte groups; a block reserves its list range with one
+// atomic per step.  Nothing is cleared: the next sweep uses another tag.
+constexpr int kCompactThreads = 256;
+constexpr int kCompactPT = 8;  // 16-byte groups per thread per step
+__device__ __forceinline__ uint32_t tag_bytes(uint32_t w, uint32_t rep) {
+  // 0x01 in each byte of w equal to the tag (rep = tag * 0x01010101), else 0
+  const uint32_t x = w ^ rep;
+  const uint32_t nz = (((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) & 0x80808080u;
+  return (~nz >> 7) & 0x01010101u;
+}
+__global__ __launch_bounds__(kCompactThreads) void k_compact(
+    const uint8_t *__restrict__ tbyte, uint32_t *__restrict__ tlist, Counters *__restrict__ ring,
+    uint32_t jslot_prev, uint32_t jslot, bool always, uint32_t ngroups,
+    const Dom *__restrict__ doms, uint32_t ndom, uint32_t tag) {
+  if (!always && ring[jslot_prev].mode != kModeDense) return;  // uniform
+  uint32_t *counter = &ring[jslot].ntargets;
+  const uint4 *g16 = reinterpret_cast<const uint4 *>(tbyte);
+  const uint32_t rep = tag * 0x01010101u;
+  const uint32_t step = gridDim.x * kCompactThreads * kCompactPT;
+  for (uint32_t base = blockIdx.x * kCompactThreads * kCompactPT; base < ngroups; base += step) {
+    uint32_t w[kCompactPT][4];
+    uint32_t n = 0;
+#pragma unroll
+    for (int q = 0; q < kCompactPT; q++) {
+      const uint32_t i = base + (uint32_t)q * kCompactThreads + threadIdx.x;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (i < ngroups) v = g16[i];
+      w[q][0] = tag_bytes(v.x, rep);
+      w[q][1] = tag_bytes(v.y, rep);
+      w[q][2] = tag_bytes(v.z, rep);
+      w[q][3] = tag_bytes(v.w, rep);
+#pragma unroll
+      for (int c = 0; c < 4; c++) n += __popc(w[q][c]);
+    }
+    uint32_t pos = block_reserve<kCompactThreads>(n, counter);
+#pragma unroll
+    for (int q = 0; q < kCompactPT; q++) {
+      const uint32_t i = base + (uint32_t)q * kCompactThreads + threadIdx.x;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        uint32_t m = w[q][c];
+        while (m) {
+          const int b = (__ffs(m) - 1) >> 3;
+          m &= m - 1;
+          const uint32_t cell = i * 16u + (uint32_t)(c * 4 + b);
+          uint32_t id = cell;
+          if (ndom > 1) {  // the domain whose cells start at or before `cell`
+            uint32_t lo = 0, hi = ndom;
+            while (hi - lo > 1) {
+              const uint32_t mid = (lo + hi) / 2;
+              if (doms[mid].cbase <= cell) lo = mid;
+              else hi = mid;
+            }
+            id = doms[lo].gbase + (cell - doms[lo].cbase);
+          }
+          tlist[pos++] = id;
+        }
+      }
+    }
+  }
+}
+
+// Tag of the marks of detection jd in the target byte map: 1..255 in turn (the
+// map is cleared whenever the tags start over).
+__host__ __device__ inline uint32_t detect_tag(uint32_t jd) { return 1u + (jd - 1u) % 255u; }
 
 // Units (62 x 4 x 64 column blocks) of one domain's box.
 __host__ inline void col_units(int64_t bx, int64_t by, int64_t bz, uint32_t &nux, uint32_t &nuy,
@@ -241,7 +310,7 @@ template <typename T, bool kLabel, bool kDetect, bool kMulti>
 __global__ __launch_bounds__(kColThreads) void k_columns(
     Dom d0, const Dom *__restrict__ doms, uint32_t ndom, DomState *__restrict__ ds,
     uint8_t *__restrict__ tbyte, Counters *__restrict__ ring, uint32_t jslot_prev,
-    uint32_t jslot, uint32_t par, RunState *run, uint32_t check_g) {
+    uint32_t jslot, uint32_t par, RunState *run, uint32_t check_g, uint32_t tag) {
   if (!kLabel) {
     if (ring[jslot_prev].mode != kModeDense) return;  // uniform
   }
@@ -269,7 +338,6 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
   Counters *out = kDetect ? &ring[jslot] : nullptr;
   uint32_t lc[3] = {0, 0, 0};
   uint32_t lerr = 0;
-  uint32_t nem = 0;  // target marks emitted (an upper bound of the distinct targets)
 
   for (uint32_t unit = u_begin + (blockIdx.x / 8) * kColWaves + wave; unit < u_end;
        unit += waves_per_xcd) {
@@ -379,14 +447,13 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
           const uint64_t b1 = (uint64_t)(v1 ? 1u : 0u) << (s_tbase[t1] + (uint32_t)h);
           const uint64_t b2 = (uint64_t)(v2 ? 1u : 0u) << (s_tbase[t2] + (uint32_t)h);
           acc |= b1 | b2;
-          nem += (v1 ? 1u : 0u) + (v2 ? 1u : 0u);
         }
       }
     };
     auto flush_plane = [&](int e) {
       // plane e-1 can receive no more targets
       flush_targets((uint32_t)acc & ((1u << (3 * kColH)) - 1u), cbase + x, y0m1, (int64_t)e - 1,
-                    D.d, tbyte);
+                    D.d, tbyte, (uint8_t)tag);
       acc >>= 3 * kColH;
     };
     if constexpr (sizeof(T) == 4) {
@@ -567,9 +634,9 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
     }
     if (kDetect) {
       flush_targets((uint32_t)acc & ((1u << (3 * kColH)) - 1u), cbase + x, y0m1, (int64_t)ze - 1,
-                    D.d, tbyte);
+                    D.d, tbyte, (uint8_t)tag);
       flush_targets((uint32_t)(acc >> (3 * kColH)) & ((1u << (3 * kColH)) - 1u), cbase + x, y0m1,
-                    (int64_t)ze, D.d, tbyte);
+                    (int64_t)ze, D.d, tbyte, (uint8_t)tag);
       if (kMulti) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -589,8 +656,6 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
     const uint32_t s = wave_sum(lc[c]);
     if (lane == 0 && s) atomicAdd(&out->cnt[c], s);
   }
-  const uint32_t se = wave_sum(nem);
-  if (lane == 0 && se) atomicAdd(&out->nemit, se);
   if (threadIdx.x == 0 && blockIdx.x == 0 && !kLabel) atomicAdd(&run->dense_passes, 1ull);
 }
 

@@ -111,9 +111,7 @@ struct Counters {
   uint32_t err;
   uint32_t nactive;  // multi-domain: domains taking part in detection j+1
   uint32_t ncapped;  // multi-domain: domains stopped by the sweep cap
-  uint32_t nemit;    // the schedule's target estimate: distinct targets of a work-list
-                     // detection, target marks emitted by a full-volume one
-  uint32_t pad[6];
+  uint32_t pad[7];
 };
 static_assert(sizeof(Counters) == 64, "Counters layout");
 

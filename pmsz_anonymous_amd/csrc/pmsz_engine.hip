@@ -85,8 +85,6 @@ __device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) {
 // ---------------------------------------------------------------------------
 // Decisions of the apply of sweep j (identical in every thread): the mode of
 // detection j+1, or kModeTail, or kModeDone with *done = 1 (converged) / 2 (cap).
-// They read only what detection j wrote (counters, the target estimate nemit),
-// never what the apply itself updates, so every block decides alike.
 
 // One domain: from the counters of detection j.
 __device__ __forceinline__ uint32_t decide(const Counters &cj, uint32_t j, const Sched &s,
@@ -101,12 +99,12 @@ __device__ __forceinline__ uint32_t decide(const Counters &cj, uint32_t j, const
     *done = 2;
     return kModeDone;
   }
-  if (tail_allowed && cj.nemit <= kTailTargets) return kModeTail;
+  if (tail_allowed && cj.ntargets <= kTailTargets) return kModeTail;
   if (s.mode == PMSZ_LAZY && j >= s.max_sweeps) {
     *done = 2;  // the edits of sweep j still land (they belong to sweep j)
     return kModeDone;
   }
-  return (uint64_t)cj.nemit * kCand >= s.dense_min_work ? kModeDense : kModeSparse;
+  return (uint64_t)cj.ntargets * kCand >= s.dense_min_work ? kModeDense : kModeSparse;
 }
 
 // Several domains: from the domain bookkeeping after detection j (the caps are
@@ -119,8 +117,8 @@ __device__ __forceinline__ uint32_t decide_multi(const Counters &cj, const Sched
     *done = cj.ncapped ? 2 : 1;
     return kModeDone;
   }
-  if (tail_allowed && cj.nemit <= kTailTargets) return kModeTail;
-  return (uint64_t)cj.nemit * kCand >= s.dense_min_work ? kModeDense : kModeSparse;
+  if (tail_allowed && cj.ntargets <= kTailTargets) return kModeTail;
+  return (uint64_t)cj.ntargets * kCand >= s.dense_min_work ? kModeDense : kModeSparse;
 }
 
 // ---------------------------------------------------------------------------
@@ -297,9 +295,8 @@ __device__ __forceinline__ int apply_target(uint32_t t, const Dom &d0, const Dom
   const Dom &D = dom_of<kMulti>(t, d0, doms, shift, dk, local);
   apply_one<T>((T *)D.g, (const T *)D.f, D.dc, local, D.ep, touched, clamped);
   // every set bit of tmark is a target of this sweep and is applied in this phase,
-  // so the word can be cleared with a plain store (a full-volume detection marks
-  // targets in the byte map instead: tmark is null then)
-  if (tmark) tmark[t >> 5] = 0u;
+  // so the word can be cleared with a plain store
+  tmark[t >> 5] = 0u;
   int64_t x, y, z;
   decompose(local, D.d, x, y, z);
   if (kMulti && D.padded) {
@@ -323,7 +320,6 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
   Counters &cj = ring[j % kCtrRing];
   const uint32_t pm = prev.mode;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-  if (pm == kModeDense) return;  // detection j marked the byte map: k_apply_dense
   if (pm == kModeTail || pm == kModeDone || pm == kModeUnset) {
     if (lead) cj.mode = pm == kModeUnset ? kModeDone : pm;
     return;
@@ -372,123 +368,6 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
   if (lead) atomicAdd(&run->total_targets, (unsigned long long)nt);
 }
 
-// Ordering of a wave's own LDS writes before its other lanes read them (LDS
-// operations of one wave complete in order; this stops the compiler moving them).
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Apply of sweep j after a full-volume detection: every marked byte of the
-// target byte map is one distinct target.  Each wave takes 1024 cells of the map
-// at a time (16 per lane, neighbouring lanes on neighbouring cells), clears
-// them, lists the marked cells in its LDS slice, and applies that list with
-// every lane busy: the edits, the neighbourhood queue when detection j+1 is
-// sparse, and the exact target count of sweep j.
-constexpr int kAdWaves = kApplyThreads / 64;
-template <typename T, bool kMulti>
-__global__ __launch_bounds__(kApplyThreads) void k_apply_dense(
-    Dom d0, const Dom *__restrict__ doms, uint32_t ndom, uint32_t shift,
-    DomState *__restrict__ ds, uint8_t *__restrict__ tbyte, uint32_t ngroups,
-    uint32_t *__restrict__ wmark, uint32_t *__restrict__ wlist, Counters *__restrict__ ring,
-    RunState *run, uint32_t j, Sched s) {
-  Counters &prev = ring[(j - 1) % kCtrRing];
-  Counters &cj = ring[j % kCtrRing];
-  if (prev.mode != kModeDense) return;  // the list apply handles this sweep
-  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-  uint32_t done;
-  const uint32_t mode =
-      kMulti ? decide_multi(cj, s, false, &done) : decide(cj, j, s, false, &done);
-  if (lead) {
-    cj.mode = mode;
-    if (done) {
-      run->done = done;
-      run->last_j = j;
-    }
-  }
-  if (mode == kModeDone && (done == 1 || (!kMulti && s.mode == PMSZ_SYNC))) return;
-  if (lead && mode != kModeDone) zero_counters(&ring[(j + 1) % kCtrRing]);
-  const bool queue = mode == kModeSparse;
-  __shared__ uint32_t s_ids[kAdWaves][64 * 16];
-  uint4 *g16 = reinterpret_cast<uint4 *>(tbyte);
-  const uint32_t lane = lane_id();
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
-  uint32_t *ids = s_ids[wave];
-  uint32_t touched = 0, clamped = 0, cnt = 0;
-  const uint32_t nwaves = gridDim.x * kAdWaves;
-  for (uint32_t chunk = blockIdx.x * kAdWaves + wave; chunk * 64 < ngroups; chunk += nwaves) {
-    const uint32_t i = chunk * 64 + lane;
-    uint32_t m16 = 0;  // bit b: cell i*16 + b is a target
-    if (i < ngroups) {
-      const uint4 v = g16[i];
-      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-      for (int c = 0; c < 4; c++)
-#pragma unroll
-        for (int b = 0; b < 4; b++) m16 |= ((w[c] >> (8 * b)) & 1u) << (4 * c + b);
-      if (m16) g16[i] = make_uint4(0, 0, 0, 0);
-    }
-    const uint32_t n = __popc(m16);
-    const uint32_t incl = wave_incl_scan(n);
-    const uint32_t tot = (uint32_t)__shfl((int)incl, 63);
-    if (tot == 0) continue;  // uniform
-    uint32_t pos = incl - n;
-    while (m16) {
-      ids[pos++] = i * 16u + (uint32_t)(__ffs(m16) - 1);
-      m16 &= m16 - 1;
-    }
-    wave_lds_sync();
-    for (uint32_t r = 0; r < tot; r += 64) {
-      const uint32_t q = r + lane;
-      const bool has = q < tot;
-      uint32_t u[kCand];
-      int nq = 0;
-      uint32_t dk = 0;
-      if (has) {
-        const uint32_t cell = ids[q];
-        uint32_t id = cell;
-        if (kMulti) {  // the domain whose cells start at or before `cell`
-          uint32_t lo = 0, hi = ndom;
-          while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) / 2;
-            if (doms[mid].cbase <= cell) lo = mid;
-            else hi = mid;
-          }
-          id = doms[lo].gbase + (cell - doms[lo].cbase);
-        }
-        nq = apply_target<T, kMulti>(id, d0, doms, shift, ds, nullptr, wmark, queue, dk,
-                                     touched, clamped, u);
-        cnt++;
-      }
-      if (kMulti) dom_add_targets(has, dk, ds);
-      if (queue) {
-        // one list reservation per wave
-        const uint32_t qi = wave_incl_scan((uint32_t)nq);
-        const uint32_t qt = (uint32_t)__shfl((int)qi, 63);
-        uint32_t wbase = 0;
-        if (lane == 0 && qt) wbase = atomicAdd(&cj.nwork, qt);
-        wbase = (uint32_t)__shfl((int)wbase, 0);
-        const uint32_t p0 = wbase + qi - (uint32_t)nq;
-        for (int k = 0; k < nq; k++) wlist[p0 + k] = u[k];
-      }
-    }
-    wave_lds_sync();  // the slice is rewritten by the next chunk
-  }
-  const uint32_t sc = wave_sum(cnt);
-  if (lane == 0 && sc) {
-    atomicAdd(&cj.ntargets, sc);
-    atomicAdd(&run->total_targets, (unsigned long long)sc);
-  }
-  if (!kMulti) {
-    uint32_t ts = wave_sum(touched), cs = wave_sum(clamped);
-    if (lane == 0) {
-      if (ts) atomicAdd(&run->touched, (unsigned long long)ts);
-      if (cs) atomicAdd(&run->clamped, (unsigned long long)cs);
-    }
-  }
-}
-
 // Work-list detection of one listed voxel id (shared by k_detect_list / k_tail).
 template <typename T, bool kMulti>
 __device__ __forceinline__ int detect_listed(uint32_t v, const Dom &d0, const Dom *doms,
@@ -516,7 +395,6 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
   const uint32_t par = (j + 1) & 1u;
   const uint32_t nw = cj.nwork;
   uint32_t lc[3] = {0, 0, 0};
-  uint32_t ln = 0;
   const uint32_t stride = gridDim.x * kApplyThreads;
   for (uint32_t base = blockIdx.x * kApplyThreads; base < nw; base += stride) {
     const uint32_t i = base + threadIdx.x;
@@ -535,11 +413,8 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
     for (int q = 0; q < 3; q++) lc[q] += vc[q];
     uint32_t pos = block_reserve<kApplyThreads>((uint32_t)n, &cn.ntargets);
     for (int q = 0; q < n; q++) tlist[pos + q] = out[q];
-    ln += (uint32_t)n;
   }
   block_add3<kApplyThreads>(lc, cn.cnt);
-  const uint32_t sn = wave_sum(ln);
-  if (lane_id() == 0 && sn) atomicAdd(&cn.nemit, sn);  // exact for a work-list detection
   if (blockIdx.x == 0 && threadIdx.x == 0)
     atomicAdd(&run->sparse_voxels, (unsigned long long)nw);
 }
@@ -623,7 +498,6 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
       for (int q = 0; q < n; q++) tlist[pos + q] = out[q];
     }
     block_add3<kTailThreads>(lc, cn.cnt);
-    if (lead) cn.nemit = ld_agent(&cn.ntargets);
     sv += nw;
     if (kMulti) {
       if (lead) {
@@ -1156,6 +1030,10 @@ unsigned col_blocks(const Ctx &c, uint32_t units, int occ) {
   return (unsigned)(8 * std::max<int64_t>(1, std::min(per_xcd, cap)));
 }
 
+unsigned compact_blocks(const Ctx &c, int64_t groups) {
+  const int64_t per = (int64_t)kCompactThreads * kCompactPT;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((groups + per - 1) / per, c.cus * 8));
+}
 
 constexpr uint32_t kChain = 8;  // apply/detect steps enqueued before each host check
 
@@ -1199,10 +1077,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   const unsigned g_label = col_blocks(c, nunits, occ_label);
   const unsigned g_dense = col_blocks(c, nunits, occ_dense);
   const unsigned g_list = (unsigned)std::max(1, c.cus * 4);
-  // k_apply_dense: a wave per 64 groups of 16 cells, persistent
-  const unsigned g_adense = (unsigned)std::max<uint64_t>(
-      1, std::min<uint64_t>((job.groups + 64 * kAdWaves - 1) / (64 * kAdWaves),
-                            (uint64_t)c.cus * 8));
+  const unsigned g_compact = compact_blocks(c, (int64_t)job.groups);
   const bool sync_every = job.subdomains && job.s.mode == PMSZ_SYNC;
   const bool sync_lazy = job.subdomains && job.s.mode == PMSZ_LAZY;
   const int sx = job.sub[0], sy = job.sub[1], sz = job.sub[2];
@@ -1227,13 +1102,17 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   // a dense detection sweep (slot 0 holds mode Dense)
   PMSZ_HIP(hipEventRecord(c.ev2, c.stream));
   k_columns<T, true, false, kMulti><<<g_label, kColThreads, 0, c.stream>>>(
-      d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, 0u);
+      d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, 0u, 0u);
   PMSZ_HIP(hipEventRecord(c.ev3, c.stream));
   // detection 1 also rejects NaN / sentinel values of g (padded blocks: the scatter did)
   k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
-      d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, job.subdomains ? 0u : 1u);
+      d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, job.subdomains ? 0u : 1u,
+      detect_tag(1));
   PMSZ_HIP(hipGetLastError());
   PMSZ_HIP(hipEventRecord(c.ev4, c.stream));
+  k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(c.tbyte.p, c.tlist.p, c.ring, 0, 1,
+                                                         true, (uint32_t)job.groups, doms, ndom,
+                                                         detect_tag(1));
   if (kMulti)
     k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, 0, 0, 1, kOpDetect, job.s, 1);
 
@@ -1241,9 +1120,6 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   out.outer_started = 1;
   while (true) {
     for (uint32_t j = j0; j < j0 + kChain; j++) {
-      k_apply_dense<T, kMulti><<<g_adense, kApplyThreads, 0, c.stream>>>(
-          d0, doms, ndom, job.shift, ds, c.tbyte.p, (uint32_t)job.groups, c.wmark.p,
-          c.wlist.p, c.ring, c.run, j, job.s);
       k_apply<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
           d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j,
           job.s);
@@ -1253,9 +1129,15 @@ int drive(Ctx &c, Job &job, Outcome &out) {
         k_unpack<T><<<g_shell, kSyncThreads, 0, c.stream>>>(doms, ds, sx, sy, sz, job.ndir,
                                                             c.wmark.p, c.wlist.p, c.ring, j, 1);
       }
+      // the tags start over: no mark of an earlier sweep may carry detection j+1's
+      if (detect_tag(j + 1) == 1u)
+        PMSZ_HIP(hipMemsetAsync(c.tbyte.p, 0, 16 * job.groups, c.stream));
       k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
           d0, doms, ndom, ds, c.tbyte.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, (j + 1) & 1u,
-          c.run, 0u);
+          c.run, 0u, detect_tag(j + 1));
+      k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
+          c.tbyte.p, c.tlist.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false,
+          (uint32_t)job.groups, doms, ndom, detect_tag(j + 1));
       k_detect_list<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
           d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j);
       if (kMulti)
@@ -1331,15 +1213,12 @@ int drive(Ctx &c, Job &job, Outcome &out) {
 
 // Bitmap words and list capacity of a run, allocating what is missing.
 int prepare(Ctx &c, Job &job) {
-  uint64_t cells = 0, ids = 0, mapcells = 0;
-  for (const Dom &D : job.doms) {
-    cells += (uint64_t)D.d.n;
-    mapcells = std::max<uint64_t>(mapcells, (uint64_t)D.cbase + (uint64_t)D.d.n);
-  }
+  uint64_t cells = 0, ids = 0;
+  for (const Dom &D : job.doms) cells += (uint64_t)D.d.n;
   if (job.doms.size() == 1) ids = (uint64_t)job.doms[0].d.n;
   else ids = (uint64_t)job.doms.size() << job.shift;
   job.words = (ids + 31) / 32 + 1;
-  job.groups = (mapcells + 15) / 16;
+  job.groups = (cells + 15) / 16;
   job.list_cap = cells;
   int rc;
   if ((rc = c.tbyte.ensure(16 * job.groups)) || (rc = c.tmark.ensure(job.words)) ||
@@ -1493,7 +1372,7 @@ int correct_subdomains(Ctx &c, const pmsz_params *p, const T *F, T *G, uint8_t *
         D.padded = 1;
         D.ep = edit_params(p);
         offs.push_back((int64_t)cells);
-        cells += ((uint64_t)D.d.n + 15) & ~(uint64_t)15;  // blocks start on 16-cell boundaries
+        cells += (uint64_t)D.d.n;
         core += (uint64_t)(l[0] * l[1] * l[2]);
         maxcells = std::max<uint64_t>(maxcells, (uint64_t)D.d.n);
         job.doms.push_back(D);
@@ -1598,7 +1477,7 @@ int correct_batch(Ctx &c, int n, const pmsz_params *p, const void *const *d_f,
       const uint32_t sh = std::max<uint32_t>(5, ceil_log2(mx));
       if (((uint64_t)(last - first + 1) << sh) > 0xffffffffull) break;
       maxn = mx;
-      cells += (nk + 15) & ~(uint64_t)15;  // fields start on 16-cell boundaries
+      cells += nk;
       last++;
     }
     if (last == first) return PMSZ_ETOOBIG;
@@ -1623,7 +1502,7 @@ int correct_batch(Ctx &c, int n, const pmsz_params *p, const void *const *d_f,
       D.cbase = (uint32_t)off;
       D.unit_base = ubase;
       ubase += dom_units(D);
-      off += ((uint64_t)D.d.n + 15) & ~(uint64_t)15;  // fields start on 16-cell boundaries
+      off += (uint64_t)D.d.n;
       job.doms.push_back(D);
     }
     job.s = make_sched(&p[first], cells, true);
@@ -1638,11 +1517,13 @@ int correct_batch(Ctx &c, int n, const pmsz_params *p, const void *const *d_f,
                      (unsigned)cnt);
     k_finish<T><<<g_all, kSyncThreads, 0, c.stream>>>(c.doms.p, c.ds.p, nullptr, nullptr, 0, 0);
     PMSZ_HIP(hipGetLastError());
+    off = 0;
     for (int k = 0; k < cnt; k++) {
       const uint64_t nk = (uint64_t)job.doms[k].d.n;
       if (d_dc && d_dc[first + k])
-        PMSZ_HIP(hipMemcpyAsync(d_dc[first + k], job.doms[k].dc, nk, hipMemcpyDeviceToDevice,
+        PMSZ_HIP(hipMemcpyAsync(d_dc[first + k], c.dc.p + off, nk, hipMemcpyDeviceToDevice,
                                 c.stream));
+      off += nk;
     }
     PMSZ_HIP(hipMemcpyAsync(c.h_ds, c.ds.p, sizeof(DomState) * cnt, hipMemcpyDeviceToHost,
                             c.stream));
@@ -1695,10 +1576,10 @@ int time_first_pass(Ctx &c, const pmsz_params *p, const T *f, const T *g, int it
                             c.stream));
     PMSZ_HIP(hipEventRecord(c.ev0, c.stream));
     k_columns<T, true, false, false><<<gl, kColThreads, 0, c.stream>>>(
-        job.doms[0], nullptr, 1, nullptr, c.tbyte.p, c.ring, 0, 1, 1u, c.run, 0u);
+        job.doms[0], nullptr, 1, nullptr, c.tbyte.p, c.ring, 0, 1, 1u, c.run, 0u, 0u);
     PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
     k_columns<T, false, true, false><<<gd, kColThreads, 0, c.stream>>>(
-        job.doms[0], nullptr, 1, nullptr, c.tbyte.p, c.ring, 0, 1, 1u, c.run, 0u);
+        job.doms[0], nullptr, 1, nullptr, c.tbyte.p, c.ring, 0, 1, 1u, c.run, 0u, 1u);
     PMSZ_HIP(hipGetLastError());
     PMSZ_HIP(hipEventRecord(c.ev2, c.stream));
     PMSZ_HIP(hipStreamSynchronize(c.stream));
@@ -1819,7 +1700,7 @@ int64_t pmsz_workspace_bytes(int n, const pmsz_params *params) {
   for (int k = 0; k < n; k++) {
     if (check_params(&params[k])) return PMSZ_EINVAL;
     const uint64_t nk = (uint64_t)(params[k].nx * params[k].ny * params[k].nz);
-    cells += (nk + 15) & ~(uint64_t)15;
+    cells += nk;
     maxn = std::max(maxn, nk);
   }
   const uint64_t ids = n == 1 ? maxn : ((uint64_t)n << std::max<uint32_t>(5, ceil_log2(maxn)));
@@ -1920,12 +1801,12 @@ int pmsz_codes(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_fiel
     static const int occ = occupancy(k_columns<float, true, false, false>, kColThreads);
     k_columns<float, true, false, false><<<col_blocks(c, dom_units(D), occ), kColThreads, 0,
                                            c.stream>>>(D, nullptr, 1, nullptr, (uint8_t *)nullptr, c.ring, 0,
-                                                       0, 0u, c.run, 0u);
+                                                       0, 0u, c.run, 0u, 0u);
   } else {
     static const int occ = occupancy(k_columns<double, true, false, false>, kColThreads);
     k_columns<double, true, false, false><<<col_blocks(c, dom_units(D), occ), kColThreads, 0,
                                             c.stream>>>(D, nullptr, 1, nullptr, (uint8_t *)nullptr, c.ring,
-                                                        0, 0, 0u, c.run, 0u);
+                                                        0, 0, 0u, c.run, 0u, 0u);
   }
   PMSZ_HIP(hipGetLastError());
   PMSZ_HIP(hipMemcpyAsync(c.h_run, c.run, sizeof(RunState), hipMemcpyDeviceToHost, c.stream));
