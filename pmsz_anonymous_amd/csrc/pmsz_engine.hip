@@ -48,17 +48,20 @@
 #include <mutex>
 #include <vector>
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "../../include/pmsz.h"
 #include "pmsz_dense.h"
 #include "pmsz_device.h"
 
 namespace pmsz {
 
+namespace cg = cooperative_groups;
+
 constexpr int kApplyThreads = 256;
-constexpr int kTailThreads = 1024;
-constexpr uint32_t kTailTargets = 2048;  // the tail kernel takes over at or below this
+constexpr int kTailThreads = 256;
 constexpr int kSyncThreads = 256;
-constexpr uint32_t kMaxDomains = kTailThreads;  // domain bookkeeping: one thread each
+constexpr uint32_t kMaxDomains = 1024;  // domain bookkeeping: one thread each
 
 struct Sched {
   int mode;  // PMSZ_LAZY / PMSZ_SYNC
@@ -99,12 +102,13 @@ __device__ __forceinline__ uint32_t decide(const Counters &cj, uint32_t j, const
     *done = 2;
     return kModeDone;
   }
-  if (tail_allowed && cj.ntargets <= kTailTargets) return kModeTail;
+  const bool dense = (uint64_t)cj.ntargets * kCand >= s.dense_min_work;
+  if (tail_allowed && !dense) return kModeTail;  // the tail runs every work-list sweep
   if (s.mode == PMSZ_LAZY && j >= s.max_sweeps) {
     *done = 2;  // the edits of sweep j still land (they belong to sweep j)
     return kModeDone;
   }
-  return (uint64_t)cj.ntargets * kCand >= s.dense_min_work ? kModeDense : kModeSparse;
+  return dense ? kModeDense : kModeSparse;
 }
 
 // Several domains: from the domain bookkeeping after detection j (the caps are
@@ -117,8 +121,9 @@ __device__ __forceinline__ uint32_t decide_multi(const Counters &cj, const Sched
     *done = cj.ncapped ? 2 : 1;
     return kModeDone;
   }
-  if (tail_allowed && cj.ntargets <= kTailTargets) return kModeTail;
-  return (uint64_t)cj.ntargets * kCand >= s.dense_min_work ? kModeDense : kModeSparse;
+  const bool dense = (uint64_t)cj.ntargets * kCand >= s.dense_min_work;
+  if (tail_allowed && !dense) return kModeTail;  // the tail runs every work-list sweep
+  return dense ? kModeDense : kModeSparse;
 }
 
 // ---------------------------------------------------------------------------
@@ -419,44 +424,50 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
     atomicAdd(&run->sparse_voxels, (unsigned long long)nw);
 }
 
-// Single-workgroup tail: runs apply / detect sweeps to the end once the target
-// sets are small, with workgroup barriers between the phases.
+// The tail: once detection j+1 would run over a work list, one cooperative
+// kernel runs the remaining sweeps -- apply, grid barrier, work-list detection,
+// grid barrier -- instead of several launches per sweep.  Every block decides
+// alike from counters read after a barrier.  Launched only through
+// hipLaunchCooperativeKernel, which guarantees that all blocks are resident.
 template <typename T, bool kMulti>
 __global__ __launch_bounds__(kTailThreads) void k_tail(
     Dom d0, const Dom *__restrict__ doms, uint32_t shift, uint32_t ndom,
     DomState *__restrict__ ds, uint32_t *__restrict__ tmark, uint32_t *__restrict__ tlist,
     uint32_t *__restrict__ wmark, uint32_t *__restrict__ wlist, Counters *__restrict__ ring,
     RunState *run, Sched s) {
-  __shared__ uint32_t s_ctl[2];
+  cg::grid_group grid = cg::this_grid();
   __shared__ uint32_t s_act, s_cap;
-  uint32_t j = run->tail_j;
-  if (j == 0 || run->done) return;
-  const bool lead = threadIdx.x == 0;
+  uint32_t j = ld_agent(&run->tail_j);
+  if (j == 0 || ld_agent(&run->done)) return;  // uniform: written by earlier kernels
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const uint32_t nthreads = gridDim.x * kTailThreads;
   unsigned long long tt = 0, sv = 0;
   uint32_t touched = 0, clamped = 0;
   while (true) {
     Counters &cj = ring[j % kCtrRing];
+    Counters cv;
+#pragma unroll
+    for (int q = 0; q < 3; q++) cv.cnt[q] = ld_agent(&cj.cnt[q]);
+    cv.ntargets = ld_agent(&cj.ntargets);
+    cv.nactive = ld_agent(&cj.nactive);
+    cv.ncapped = ld_agent(&cj.ncapped);
+    uint32_t done;
+    const uint32_t mode =
+        kMulti ? decide_multi(cv, s, false, &done) : decide(cv, j, s, false, &done);
     if (lead) {
-      uint32_t done;
-      const uint32_t mode =
-          kMulti ? decide_multi(cj, s, false, &done) : decide(cj, j, s, false, &done);
-      s_ctl[0] = mode;
-      s_ctl[1] = done;
       cj.mode = mode == kModeDone ? kModeDone : kModeTail;
       if (done) {
         run->done = done;
         run->last_j = j;
       }
     }
-    __syncthreads();
-    const uint32_t mode = s_ctl[0], done = s_ctl[1];
     if (!kMulti && (done == 1 || (done == 2 && s.mode == PMSZ_SYNC))) break;
-    const uint32_t nt = ld_agent(&cj.ntargets);
+    const uint32_t nt = cv.ntargets;
     if (kMulti && mode == kModeDone && nt == 0) break;
     if (lead && mode != kModeDone) zero_counters(&ring[(j + 1) % kCtrRing]);
     // apply sweep j, queueing the next work list unless this was the last sweep
     const bool queue = mode != kModeDone;
-    for (uint32_t base = 0; base < nt; base += kTailThreads) {
+    for (uint32_t base = blockIdx.x * kTailThreads; base < nt; base += nthreads) {
       const uint32_t i = base + threadIdx.x;
       uint32_t u[kCand];
       int n = 0;
@@ -471,15 +482,15 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
         for (int q = 0; q < n; q++) wlist[pos + q] = u[q];
       }
     }
-    tt += nt;
-    __syncthreads();
+    if (lead) tt += nt;
+    grid.sync();
     if (!queue) break;  // sweep cap: sweep j's edits landed, no detection j+1
     // detection j+1 over the work list
     Counters &cn = ring[(j + 1) % kCtrRing];
     const uint32_t par = (j + 1) & 1u;
     const uint32_t nw = ld_agent(&cj.nwork);
     uint32_t lc[3] = {0, 0, 0};
-    for (uint32_t base = 0; base < nw; base += kTailThreads) {
+    for (uint32_t base = blockIdx.x * kTailThreads; base < nw; base += nthreads) {
       const uint32_t i = base + threadIdx.x;
       uint32_t out[4];
       uint32_t vc[3] = {0, 0, 0};
@@ -488,7 +499,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
       int n = 0;
       if (i < nw) {
         const uint32_t v = wlist[i];
-        atomicAnd(&wmark[v >> 5], ~(1u << (v & 31u)));
+        wmark[v >> 5] = 0u;  // every set bit is a listed voxel of this sweep
         n = detect_listed<T, kMulti>(v, d0, doms, shift, ds, tmark, dk, has, vc, out);
       }
       if (kMulti) dom_add3(has, dk, vc, ds, par);
@@ -498,24 +509,28 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
       for (int q = 0; q < n; q++) tlist[pos + q] = out[q];
     }
     block_add3<kTailThreads>(lc, cn.cnt);
-    sv += nw;
+    if (lead) sv += nw;
+    grid.sync();
     if (kMulti) {
-      if (lead) {
-        s_act = 0;
-        s_cap = 0;
+      if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) {
+          s_act = 0;
+          s_cap = 0;
+        }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < ndom; k += kTailThreads) {
+          uint32_t a, c;
+          dom_update(ds[k], j + 1, kOpDetect, s, 1u, a, c);
+          if (a) atomicAdd(&s_act, 1u);
+          if (c) atomicAdd(&s_cap, 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          cn.nactive = s_act;
+          cn.ncapped = s_cap;
+        }
       }
-      __syncthreads();
-      for (uint32_t k = threadIdx.x; k < ndom; k += kTailThreads) {
-        uint32_t a, c;
-        dom_update(ds[k], j + 1, kOpDetect, s, 1u, a, c);
-        if (a) atomicAdd(&s_act, 1u);
-        if (c) atomicAdd(&s_cap, 1u);
-      }
-      __syncthreads();
-      if (lead) {
-        cn.nactive = s_act;
-        cn.ncapped = s_cap;
-      }
+      grid.sync();
     }
     j++;
   }
@@ -1077,6 +1092,9 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   const unsigned g_label = col_blocks(c, nunits, occ_label);
   const unsigned g_dense = col_blocks(c, nunits, occ_dense);
   const unsigned g_list = (unsigned)std::max(1, c.cus * 4);
+  // the tail: one block per CU, within what can be resident at once
+  static const int occ_tail = occupancy(k_tail<T, kMulti>, kTailThreads);
+  const unsigned g_tail = (unsigned)std::max(1, std::min(c.cus, c.cus * occ_tail));
   const unsigned g_compact = compact_blocks(c, (int64_t)job.groups);
   const bool sync_every = job.subdomains && job.s.mode == PMSZ_SYNC;
   const bool sync_lazy = job.subdomains && job.s.mode == PMSZ_LAZY;
@@ -1144,10 +1162,21 @@ int drive(Ctx &c, Job &job, Outcome &out) {
         k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, j, 1, j + 1, kOpDetect,
                                                    job.s, 1);
     }
-    if (job.s.tail_ok)
-      k_tail<T, kMulti><<<1, kTailThreads, 0, c.stream>>>(d0, doms, job.shift, ndom, ds,
-                                                          c.tmark.p, c.tlist.p, c.wmark.p,
-                                                          c.wlist.p, c.ring, c.run, job.s);
+    if (job.s.tail_ok) {
+      Dom a_d0 = d0;
+      const Dom *a_doms = doms;
+      uint32_t a_shift = job.shift, a_ndom = ndom;
+      DomState *a_ds = ds;
+      uint32_t *a_tmark = c.tmark.p, *a_tlist = c.tlist.p, *a_wmark = c.wmark.p,
+               *a_wlist = c.wlist.p;
+      Counters *a_ring = c.ring;
+      RunState *a_run = c.run;
+      Sched a_s = job.s;
+      void *args[] = {&a_d0,    &a_doms,  &a_shift, &a_ndom, &a_ds,  &a_tmark,
+                      &a_tlist, &a_wmark, &a_wlist, &a_ring, &a_run, &a_s};
+      PMSZ_HIP(hipLaunchCooperativeKernel((const void *)k_tail<T, kMulti>, dim3(g_tail),
+                                          dim3(kTailThreads), args, 0, c.stream));
+    }
     PMSZ_HIP(hipGetLastError());
     PMSZ_HIP(hipMemcpyAsync(c.h_run, c.run, sizeof(RunState), hipMemcpyDeviceToHost, c.stream));
     PMSZ_HIP(hipStreamSynchronize(c.stream));
