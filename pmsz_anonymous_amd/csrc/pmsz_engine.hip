@@ -60,6 +60,9 @@ namespace cg = cooperative_groups;
 
 constexpr int kApplyThreads = 256;
 constexpr int kTailThreads = 256;
+// the tail takes over once a sweep has at most this many targets: heavier work-list
+// sweeps run as separate launches, which bring more threads to their atomics
+constexpr uint32_t kTailTargets = 65536;
 constexpr int kSyncThreads = 256;
 constexpr uint32_t kMaxDomains = 1024;  // domain bookkeeping: one thread each
 
@@ -103,7 +106,7 @@ __device__ __forceinline__ uint32_t decide(const Counters &cj, uint32_t j, const
     return kModeDone;
   }
   const bool dense = (uint64_t)cj.ntargets * kCand >= s.dense_min_work;
-  if (tail_allowed && !dense) return kModeTail;  // the tail runs every work-list sweep
+  if (tail_allowed && !dense && cj.ntargets <= kTailTargets) return kModeTail;
   if (s.mode == PMSZ_LAZY && j >= s.max_sweeps) {
     *done = 2;  // the edits of sweep j still land (they belong to sweep j)
     return kModeDone;
@@ -122,7 +125,7 @@ __device__ __forceinline__ uint32_t decide_multi(const Counters &cj, const Sched
     return kModeDone;
   }
   const bool dense = (uint64_t)cj.ntargets * kCand >= s.dense_min_work;
-  if (tail_allowed && !dense) return kModeTail;  // the tail runs every work-list sweep
+  if (tail_allowed && !dense && cj.ntargets <= kTailTargets) return kModeTail;
   return dense ? kModeDense : kModeSparse;
 }
 
@@ -424,8 +427,8 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
     atomicAdd(&run->sparse_voxels, (unsigned long long)nw);
 }
 
-// The tail: once detection j+1 would run over a work list, one cooperative
-// kernel runs the remaining sweeps -- apply, grid barrier, work-list detection,
+// The tail: once a sweep's targets are few and detection j+1 would run over a
+// work list, one cooperative kernel runs the remaining sweeps -- apply, grid barrier, work-list detection,
 // grid barrier -- instead of several launches per sweep.  Every block decides
 // alike from counters read after a barrier.  Launched only through
 // hipLaunchCooperativeKernel, which guarantees that all blocks are resident.
@@ -1092,9 +1095,9 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   const unsigned g_label = col_blocks(c, nunits, occ_label);
   const unsigned g_dense = col_blocks(c, nunits, occ_dense);
   const unsigned g_list = (unsigned)std::max(1, c.cus * 4);
-  // the tail: one block per CU, within what can be resident at once
+  // the tail: up to four blocks per CU, within what can be resident at once
   static const int occ_tail = occupancy(k_tail<T, kMulti>, kTailThreads);
-  const unsigned g_tail = (unsigned)std::max(1, std::min(c.cus, c.cus * occ_tail));
+  const unsigned g_tail = (unsigned)std::max(1, c.cus * std::min(occ_tail, 4));
   const unsigned g_compact = compact_blocks(c, (int64_t)job.groups);
   const bool sync_every = job.subdomains && job.s.mode == PMSZ_SYNC;
   const bool sync_lazy = job.subdomains && job.s.mode == PMSZ_LAZY;
