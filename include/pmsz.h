@@ -115,7 +115,9 @@ int pmsz_correct_batch(int n, const pmsz_params *params, const void *const *d_f,
  * in different contexts overlap on the device (the serving loop keeps one per
  * lane).  A context belongs to the device current at its creation.  The
  * reference has one correction per MPI rank and no such object; these entry
- * points are the concurrent form of pmsz_correct / pmsz_correct_batch. */
+ * points are the concurrent form of pmsz_correct / pmsz_correct_batch.  At most
+ * 8 contexts exist at once, the default one included (PMSZ_EBUSY beyond):
+ * their runs' device-wide phases are sized to fit on the device together. */
 typedef struct pmsz_ctx pmsz_ctx;
 int pmsz_ctx_create(pmsz_ctx **out);
 int pmsz_ctx_destroy(pmsz_ctx *ctx);

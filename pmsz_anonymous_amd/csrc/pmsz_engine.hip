@@ -42,6 +42,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -65,6 +66,7 @@ constexpr int kTailThreads = 256;
 constexpr uint32_t kTailTargets = 65536;
 constexpr int kSyncThreads = 256;
 constexpr uint32_t kMaxDomains = 1024;  // domain bookkeeping: one thread each
+constexpr int kMaxContexts = 8;         // live contexts, the default one included
 
 struct Sched {
   int mode;  // PMSZ_LAZY / PMSZ_SYNC
@@ -1095,9 +1097,12 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   const unsigned g_label = col_blocks(c, nunits, occ_label);
   const unsigned g_dense = col_blocks(c, nunits, occ_dense);
   const unsigned g_list = (unsigned)std::max(1, c.cus * 4);
-  // the tail: up to four blocks per CU, within what can be resident at once
+  // the tail: its grid barriers need every block resident, also while the tails of
+  // other contexts run -- each tail takes at most 1/(kMaxContexts+1) of what the
+  // device can hold of it, and at most kMaxContexts contexts exist
   static const int occ_tail = occupancy(k_tail<T, kMulti>, kTailThreads);
-  const unsigned g_tail = (unsigned)std::max(1, c.cus * std::min(occ_tail, 4));
+  const unsigned g_tail =
+      (unsigned)std::max(1, std::min(c.cus * 4, c.cus * occ_tail / (kMaxContexts + 1)));
   const unsigned g_compact = compact_blocks(c, (int64_t)job.groups);
   const bool sync_every = job.subdomains && job.s.mode == PMSZ_SYNC;
   const bool sync_lazy = job.subdomains && job.s.mode == PMSZ_LAZY;
@@ -1637,6 +1642,7 @@ struct pmsz_ctx {
 namespace {
 pmsz_ctx g_default;
 Ctx &default_ctx() { return g_default.c; }
+std::atomic<int> g_contexts{1};  // the default context counts
 }  // namespace
 
 extern "C" {
@@ -1681,12 +1687,20 @@ int pmsz_ctx_correct(pmsz_ctx *ctx, const pmsz_params *p, const void *d_f, void 
 
 int pmsz_ctx_create(pmsz_ctx **out) {
   if (!out) return PMSZ_EINVAL;
+  if (g_contexts.fetch_add(1) >= kMaxContexts) {
+    g_contexts.fetch_sub(1);
+    return PMSZ_EBUSY;
+  }
   pmsz_ctx *ctx = new (std::nothrow) pmsz_ctx();
-  if (!ctx) return PMSZ_ENOMEM;
+  if (!ctx) {
+    g_contexts.fetch_sub(1);
+    return PMSZ_ENOMEM;
+  }
   int rc = ctx_device(ctx->c);
   if (rc) {
     ctx_release(ctx->c);
     delete ctx;
+    g_contexts.fetch_sub(1);
     return rc;
   }
   *out = ctx;
@@ -1701,6 +1715,7 @@ int pmsz_ctx_destroy(pmsz_ctx *ctx) {
     ctx_release(ctx->c);
   }
   delete ctx;
+  g_contexts.fetch_sub(1);
   return PMSZ_OK;
 }
 
