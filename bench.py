@@ -3,9 +3,13 @@
 Workload: 512^3 float32 Perlin field (seed 1, base frequency 8, 5 octaves,
 persistence 0.5), SZ-like codec at abs eb = 1e-3 x value range.  A step is one
 full correction (pmsz_correct) of a fresh decompressed copy; every step's input
-is staged in HBM before the timed region.  Prints one JSON line.
+is staged in HBM before the timed region.  Prints one JSON line, which also
+carries the 64-field batched run against plain runs of the same fields, the
+serving run (Poisson arrivals, p50 / p99), the 768^3 float64 2x2x2 subdomain
+run (lazy vs per-iteration sync) and the oracle on the host cores.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 512] [--no-cpu]
+                  [--no-batch] [--no-serve] [--no-subdomains]
 """
 from __future__ import annotations
 
@@ -223,6 +227,47 @@ def serving(n_req=400, pool=32, rate=200.0, lanes=4, seed=0):
     }
 
 
+def subdomains_lazy_vs_sync(edge=768, reps=2):
+    """The subdomain ghost path on one GPU: a 768^3 float64 Perlin field (seed 2,
+    base frequency 16, 6 octaves) at abs eb = 1e-4 x range, split 2x2x2 into
+    in-HBM padded subdomains, corrected with the lazy ghost sync and with the
+    per-iteration sync (full ghost policy).  Each result is checked for the
+    original's codes over the whole field."""
+    import torch
+    from pmsz_anonymous_amd import engine, fieldio
+    t0 = time.perf_counter()
+    f, g0, eb = fieldio.perlin_case((edge,) * 3, 2, 16.0, 6, 1e-4, "sz3", np.float64)
+    log(f"subdomain workload generated in {time.perf_counter() - t0:.1f} s")
+    dev = torch.device("cuda:0")
+    tf = torch.from_numpy(f).to(dev)
+    tg0 = torch.from_numpy(g0).to(dev)
+    del f, g0
+    want = engine.codes(tf)
+    out = {"workload": f"{edge}^3 f64 Perlin (seed 2, freq 16, 6 octaves), SZ-like codec, "
+                       "abs eb = 1e-4 x range, 2x2x2 subdomains, full ghost policy",
+           "voxels": edge ** 3}
+    for name, mode in (("lazy", engine.LAZY), ("sync", engine.SYNC)):
+        g = tg0.clone()
+        engine.correct(tf, g, eb, sub=(2, 2, 2), mode=mode)  # warm-up, sizes the buffers
+        ts = []
+        for _ in range(reps):
+            g.copy_(tg0)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            _, st = engine.correct(tf, g, eb, sub=(2, 2, 2), mode=mode)
+            ts.append(time.perf_counter() - t1)
+        ok = torch.equal(engine.codes(g), want)
+        t = float(np.median(ts))
+        out[name] = {"ms": t * 1e3, "gvoxels_per_s": edge ** 3 / t / 1e9,
+                     "edit_sweeps": st["edit_sweeps"], "outer_iters": st["outer_iters"],
+                     "total_targets": st["total_targets"], "touched": st["touched_voxels"],
+                     "topology_preserved": bool(ok)}
+        log(f"subdomains {name}: {t * 1e3:.1f} ms, {st['edit_sweeps']} sweeps, "
+            f"{st['outer_iters']} syncs, codes equal: {ok}")
+        del g
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,6 +278,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-field batched run")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
+    ap.add_argument("--no-subdomains", action="store_true",
+                    help="skip the 768^3 f64 2x2x2 lazy-vs-sync run")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the multi-GPU path is a design "
@@ -326,6 +373,9 @@ def main():
         res["batched"] = batched_vs_plain()
     if not args.no_serve:
         res["serving"] = serving()
+    if not args.no_subdomains:
+        del copies
+        res["subdomains"] = subdomains_lazy_vs_sync()
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(f, g0, eb, min(args.cpu_edge, n))
     print(json.dumps(res), flush=True)
