@@ -1,11 +1,12 @@
-// pmsz_dense.h -- the full-volume kernel of the correction: original labeling
-// of f and/or one detection sweep over g, for every voxel.
+// pmsz_dense.h -- the full-volume kernel of the correction: the original
+// labeling of f, or one detection sweep over g, for every voxel.
 //
 // Layout of the work: a wave owns 62 consecutive x (lanes 1..62; lanes 0 and 63
-// carry the x-1 / x+1 halo), kColRows consecutive y, and walks z.  No LDS and no
-// barriers: a lane loads its x column of kColRows+2 rows per plane from HBM
-// (coalesced across the wave), the x-1 / x+1 neighbours arrive through DPP wave
-// shifts, and z is a register pipeline.
+// carry the x-1 / x+1 halo), kColRows consecutive y, and walks z.  A lane loads
+// its x column of kColRows+2 rows per plane from HBM (coalesced across the
+// wave), the x-1 / x+1 neighbours arrive through DPP wave shifts, and z is a
+// register pipeline.  No barriers; LDS holds only the detection's fast-test
+// table and each wave's violation queue.
 //
 // The 15 candidates of voxel (x, y, z) -- itself and its 14 stencil neighbours,
 // ranked by linear index (pmsz_common.h) -- split by plane: ranks 0..3 lie in
@@ -15,20 +16,17 @@
 // p-1).  Voxel p-1 is then complete: extreme(lo(p-2), mid(p-1), hi(p)).
 // Every cell is loaded once per pass.
 //
-// Candidate order.  float32: the staged key 2^52 + 16 * ord(v) plus the rank is
-// an exact integer in a double whose numeric order is the (value, index) order
-// of islarger_shared / isless_shared (pMSz.cu:610-620); max / min are single
-// v_max_f64 / v_min_f64 and the winner's rank is the key's low 4 bits.
+// Candidate order.  float32: each cell becomes an exact double key (key32
+// below) whose numeric order is the (value, index) order of islarger_shared /
+// isless_shared (pMSz.cu:610-620); max / min are single v_max_f64 / v_min_f64.
 // float64: (value, rank) pairs combined in increasing rank order, the later one
 // winning ties for the maximum and the earlier one for the minimum.
 // Out-of-domain cells (the reference's INVALID halo, pMSz.cu:709) are quiet NaN:
 // maxnum / minnum and the pair combines skip them.
 //
-// Targets (with kDetect) go to per-lane bit masks over (dx, row) for the three
-// planes a voxel's targets can lie in; a completed plane is exchanged with the
-// neighbouring lanes, turned into one 64-lane mask per row by ballots and OR-ed
-// into the target bitmap with one atomic per 32 cells; cells whose bit was new
-// are appended to the target list with one reservation per wave.
+// Detection compares each voxel's winners in g with its code; the voxels that
+// differ go through a per-wave LDS queue to the exact rules, which count them
+// and mark their targets in the target byte map.
 #pragma once
 #include "pmsz_device.h"
 
@@ -41,12 +39,12 @@ constexpr int kColZ = 64;      // planes per work unit
 constexpr int kColWaves = 4;   // waves per block
 constexpr int kColThreads = 64 * kColWaves;
 
-// Value of lane i-1 / lane i+1 (lane 0 / lane 63 receive 0).
+// Value of lane i-1 / lane i+1 (lane 0 / lane 63 receive 0: bound_ctrl).
 __device__ __forceinline__ uint32_t from_left_u(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x138, 0xf, 0xf, true);
 }
 __device__ __forceinline__ uint32_t from_right_u(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xf, 0xf, true);
 }
 __device__ __forceinline__ double from_left(double v) {
   return __hiloint2double((int)from_left_u((uint32_t)__double2hiint(v)),
@@ -55,20 +53,6 @@ __device__ __forceinline__ double from_left(double v) {
 __device__ __forceinline__ double from_right(double v) {
   return __hiloint2double((int)from_right_u((uint32_t)__double2hiint(v)),
                           (int)from_right_u((uint32_t)__double2loint(v)));
-}
-
-// Target bit of rank r in the 54-bit plane accumulator of a lane, less the voxel
-// row h: field (plane e-1, e, e+1) * 18 + (dx + 1) * kColH + dy.
-__host__ __device__ constexpr uint32_t target_base(int r) {
-  return (uint32_t)((rank_dz(r) + 1) * 3 * kColH + (rank_dx(r) + 1) * kColH + rank_dy(r));
-}
-
-// Offsets of rank r packed for runtime lookup: (dx+1) | (dy+1) << 2, 4 bits per rank.
-__host__ __device__ constexpr uint64_t rank_xy_table() {
-  uint64_t t = 0;
-  for (int r = 0; r < kCand; r++)
-    t |= (uint64_t)((rank_dx(r) + 1) | ((rank_dy(r) + 1) << 2)) << (4 * r);
-  return t;
 }
 
 // Running extremes over candidates visited in increasing rank order.
@@ -202,28 +186,6 @@ __device__ __forceinline__ double kmin(double a, double b) {
   return r;
 }
 
-// Flush the completed target plane `pz` of this wave's column block: `m` holds,
-// per lane, bits (dx+1)*kColH + row for targets at (x+dx, y0-1+row, pz).  After
-// the exchange with the neighbouring lanes each lane holds the target bits of its
-// own x and marks each target's byte in the run's target byte map with the
-// sweep's tag, by a plain store: several waves may mark one cell, all with the
-// same value, so no atomic is needed.  The target list is compacted from the
-// byte map afterwards; bytes holding other tags are marks of earlier sweeps.
-__device__ __forceinline__ void flush_targets(uint32_t m, int64_t xl, int64_t y0m1, int64_t pz,
-                                              const Grid3 &d, uint8_t *tbyte, uint8_t tag) {
-  const uint32_t dxm = m & ((1u << kColH) - 1);                   // dx = -1
-  const uint32_t dx0 = (m >> kColH) & ((1u << kColH) - 1);        // dx =  0
-  const uint32_t dxp = (m >> (2 * kColH)) & ((1u << kColH) - 1);  // dx = +1
-  uint32_t col = dx0 | from_left_u(dxp) | from_right_u(dxm);
-  // a set bit is a target, so its cell lies in the block
-  const int64_t base = xl + d.nx * y0m1 + d.sxy * pz;
-  while (col) {
-    const int h = __ffs(col) - 1;
-    col &= col - 1;
-    tbyte[base + d.nx * h] = tag;
-  }
-}
-
 // Target list from the target byte map after a full-volume detection: every
 // byte equal to the sweep's tag is one distinct target, listed as its voxel id.
 // Each thread takes 16-byte groups; a block reserves its list range with one
@@ -300,24 +262,94 @@ __host__ inline void col_units(int64_t bx, int64_t by, int64_t bz, uint32_t &nux
   nuz = (uint32_t)((bz + kColZ - 1) / kColZ);
 }
 
-// The full-volume kernel.  With kLabel the original labeling of f (codes; the
+
+// Violations of a full-volume detection are resolved off the per-voxel path.
+// Every voxel runs a fast test -- are both winners in g the original's? -- and
+// the lanes that fail append (cell, info) to their wave's queue in LDS; the queue
+// is resolved 64 entries at a time, one per lane, by the exact rules
+// (detect_pair): counters and target bytes.  info = the code byte | the
+// detection's winners above it (float32: winner tag - voxel tag + 8, 5 bits
+// each; float64: ranks, 4 bits each).
+constexpr int kQCap = 512;  // queue entries per wave (a power of two)
+
+// Orders one wave's LDS writes before its later LDS reads by other lanes (and
+// reads before later overwrites): a wavefront-scope fence around a wave barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Winner tag - voxel tag + 8 of the candidate of rank r (float32 keys).
+__host__ __device__ constexpr uint32_t tag_diff(int r) {
+  return (uint32_t)(r + (r >= 2 ? 1 : 0) + (r >= 13 ? 1 : 0));
+}
+__device__ __forceinline__ int diff_rank(int d) { return d - (d > 2 ? 1 : 0) - (d > 14 ? 1 : 0); }
+
+// Fast-test table of the float32 detection, indexed by the code byte: the tag
+// differences (mod 512) that the original's ascent and descent winners have, in
+// the low and high 16 bits.  A voxel that is both maximum and minimum of its
+// neighbourhood (it has no valid neighbour) gets an ascent entry no winner can
+// match, so the exact rules always see it.
+__host__ __device__ inline uint32_t expect_entry(uint32_t c) {
+  const int oa = (int)(c & 15u), od = (int)(c >> 4);
+  uint32_t a = (tag_diff(oa) + 504u) & 511u;
+  const uint32_t d = (tag_diff(od) + 504u) & 511u;
+  if (oa == kSelf && od == kSelf) a = 256u;
+  return a | (d << 16);
+}
+
+// Loads through a buffer resource whose base is a plane's first byte: the
+// in-plane byte offset is the instruction's 32-bit VGPR offset, so a load
+// costs no address arithmetic.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)0xffffffff,
+                                           0x00020000);
+}
+__device__ __forceinline__ void plane_load(__amdgpu_buffer_rsrc_t rs, uint32_t off, float &v) {
+  v = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+}
+__device__ __forceinline__ void plane_load(__amdgpu_buffer_rsrc_t rs, uint32_t off, double &v) {
+  const auto w = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
+  v = __hiloint2double((int)w[1], (int)w[0]);
+}
+
+// Candidate offset of rank r in a block of extent d.
+__device__ __forceinline__ int64_t cand_off(int r, const Grid3 &d) {
+  constexpr uint64_t kXY = [] {
+    uint64_t t = 0;
+    for (int q = 0; q < kCand; q++)
+      t |= (uint64_t)((rank_dx(q) + 1) | ((rank_dy(q) + 1) << 2)) << (4 * q);
+    return t;
+  }();
+  const uint32_t f = (uint32_t)(kXY >> (4 * r)) & 15u;
+  const int dz = (r >= 4 ? 1 : 0) + (r >= 11 ? 1 : 0) - 1;
+  return (int64_t)((int)(f & 3u) - 1) + d.nx * (int64_t)((int)(f >> 2) - 1) + d.sxy * dz;
+}
+
+// The full-volume kernel: with kLabel the original labeling of f (codes; the
 // delta counters are cleared when the domain has them), with kDetect a
-// detection sweep on g reading the codes (check_g: reject NaN / sentinel values
-// of g, in the first sweep).  kMulti: the units of all domains of the run,
-// domain state in ds (par = parity of the detection).  Targets go to the byte
-// map tbyte.
-template <typename T, bool kLabel, bool kDetect, bool kMulti>
+// detection sweep on g against the codes (kCheck: reject NaN / sentinel values
+// of g, in the first detection).  kMulti: the units of all domains of the run,
+// domain state in ds (par = parity of the detection).  Targets are marked in
+// the byte map tbyte with the detection's tag.
+template <typename T, bool kLabel, bool kDetect, bool kMulti, bool kCheck = false>
 __global__ __launch_bounds__(kColThreads) void k_columns(
     Dom d0, const Dom *__restrict__ doms, uint32_t ndom, DomState *__restrict__ ds,
     uint8_t *__restrict__ tbyte, Counters *__restrict__ ring, uint32_t jslot_prev,
-    uint32_t jslot, uint32_t par, RunState *run, uint32_t check_g, uint32_t tag) {
-  if (!kLabel) {
+    uint32_t jslot, uint32_t par, RunState *run, uint32_t tag) {
+  static_assert(kLabel != kDetect, "a pass labels f or detects on g");
+  static_assert(kColThreads == 256, "one fast-test table entry per thread");
+  constexpr bool kF32 = sizeof(T) == 4;
+  constexpr bool kCheckV = kLabel || kCheck;  // validate the loaded values
+  if (kDetect) {
     if (ring[jslot_prev].mode != kModeDense) return;  // uniform
   }
   const double kNaN = __builtin_nan("");
-  __shared__ uint8_t s_tbase[16];
-  if (kDetect) {
-    if (threadIdx.x < kCand) s_tbase[threadIdx.x] = (uint8_t)target_base((int)threadIdx.x);
+  __shared__ uint32_t s_exp[kDetect && kF32 ? 256 : 1];
+  __shared__ uint2 s_q[kDetect ? kColWaves * kQCap : 1];
+  if (kDetect && kF32) {
+    s_exp[threadIdx.x] = expect_entry(threadIdx.x);
     __syncthreads();
   }
   const uint32_t lane = lane_id();
@@ -338,6 +370,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
   Counters *out = kDetect ? &ring[jslot] : nullptr;
   uint32_t lc[3] = {0, 0, 0};
   uint32_t lerr = 0;
+  const uint32_t qbase = wave * (uint32_t)kQCap;
 
   for (uint32_t unit = u_begin + (blockIdx.x / 8) * kColWaves + wave; unit < u_end;
        unit += waves_per_xcd) {
@@ -359,12 +392,13 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
       emit = (fl & kDomEmit) != 0;
     }
     const bool padded = kMulti && D.padded;
-    const T *__restrict__ f = (const T *)D.f;
-    const T *__restrict__ g = (const T *)D.g;
+    // the field this pass reads, and per plane its first byte: loads and stores
+    // take a uniform plane base plus a 32-bit in-plane byte offset
+    const char *__restrict__ src = (const char *)(kLabel ? D.f : (const void *)D.g);
     uint8_t *__restrict__ code = D.code;
     uint8_t *__restrict__ dc = D.dc;
     const uint32_t nx = (uint32_t)D.d.nx, ny = (uint32_t)D.d.ny, nz = (uint32_t)D.d.nz;
-    const uint32_t sxy = (uint32_t)D.d.sxy;
+    const uint64_t sxy = (uint64_t)D.d.sxy;
     const uint32_t u = unit - D.unit_base;
     const uint32_t uz = u / (D.nux * D.nuy), rem = u - uz * D.nux * D.nuy;
     const uint32_t uy = rem / D.nux, ux = rem - uy * D.nux;
@@ -374,269 +408,237 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
     const int ze = zs + kColZ < D.hi[2] ? zs + kColZ : D.hi[2];
     const bool xok = x >= 0 && x < (int)nx;
     const bool owned = lane >= 1 && lane <= (uint32_t)kColOwned && x < D.hi[0];
-    const int yhi = D.hi[1];
-    // in-plane offsets of the loaded rows, fixed for the unit
-    uint32_t roff[kColH];
-    bool rok[kColH];
+    // in-plane offsets of the loaded rows (bytes of T) and of the voxel rows
+    // (cells; 0 where the lane has no such cell), fixed for the unit
+    uint32_t roff[kColH], coff[kColRows];
+    bool rok[kColH], vis[kColRows];
 #pragma unroll
     for (int h = 0; h < kColH; h++) {
       const int y = y0m1 + h;
       rok[h] = xok && y >= 0 && y < (int)ny;
-      roff[h] = rok[h] ? (uint32_t)x + nx * (uint32_t)y : 0u;
+      roff[h] = rok[h] ? ((uint32_t)x + nx * (uint32_t)y) * (uint32_t)sizeof(T) : 0u;
     }
-    // raw values of the next plane, in flight; the plane's validity is uniform
-    // over the wave, so an out-of-domain plane is a branch, not a select
-    T nf[kColH], ng[kColH];
-    auto fetch = [&](int pz) {
+#pragma unroll
+    for (int h = 0; h < kColRows; h++) {
+      const int y = y0m1 + 1 + h;
+      vis[h] = owned && y < D.hi[1];
+      coff[h] = vis[h] ? (uint32_t)x + nx * (uint32_t)y : 0u;
+    }
+    // the voxel rows this lane resolves, as wave masks
+    uint64_t vmask[kColRows];
+#pragma unroll
+    for (int h = 0; h < kColRows; h++) vmask[h] = __ballot(vis[h]);
+    // raw values of plane pz into nv; the plane's validity is uniform over the
+    // wave, so an out-of-domain plane is a branch, not a select
+    auto fetch = [&](int pz, T (&nv)[kColH]) {
       if (pz < 0 || pz >= (int)nz) return;
-      const uint32_t zoff = sxy * (uint32_t)pz;
+      const auto rs = plane_rsrc(src + (uint64_t)pz * sxy * sizeof(T));
 #pragma unroll
-      for (int h = 0; h < kColH; h++) {
-        const uint32_t gi = zoff + roff[h];
-        if (kLabel) nf[h] = f[gi];
-        if (kDetect) ng[h] = g[gi];
-      }
+      for (int h = 0; h < kColH; h++) plane_load(rs, roff[h], nv[h]);
     };
-    auto stage = [&](int pz, const T (&raw)[kColH], ColPlane &c, bool check) {
-      if (pz < 0 || pz >= (int)nz) {
+    auto fetch_codes = [&](int pz, uint8_t (&nc)[kColRows]) {  // pz lies in the domain
+      const auto rs = plane_rsrc(code + (uint64_t)pz * sxy);
 #pragma unroll
-        for (int h = 0; h < kColH; h++) c.k[h] = c.l[h] = c.r[h] = kNaN;
-        return;
-      }
-#pragma unroll
-      for (int h = 0; h < kColH; h++) {
-        const T v = raw[h];
-        bool ok = rok[h];
-        if (padded) {
-          ok = ok && v != sentinel<T>();  // a ghost cell that carries no value
-        } else if (check && ok && (!(v == v) || v == sentinel<T>())) {
-          lerr |= 1u;
-        }
-        c.k[h] = ok ? Ext<T>::stage(v) : kNaN;
-      }
-      shift_plane(c);
+      for (int h = 0; h < kColRows; h++)
+        nc[h] = (uint8_t)__builtin_amdgcn_raw_buffer_load_b8(rs, coff[h], 0, 0);
     };
-    uint8_t ccode[kColRows], ncode[kColRows];  // codes of the plane to emit / of the next
-    auto fetch_codes = [&](int pz) {
-#pragma unroll
-      for (int h = 0; h < kColRows; h++) {
-        const int y = y0m1 + 1 + h;
-        const bool ok = owned && y < yhi && pz >= 0 && pz < (int)nz;
-        ncode[h] = code[ok ? (uint32_t)x + nx * (uint32_t)y + sxy * (uint32_t)pz : 0u];
-      }
+    // the value's validity: a padded block's sentinel cells carry no value; in
+    // a field, NaN or the sentinel is an input error
+    auto valid = [&](bool ok, T v) -> bool {
+      if (padded) return ok && v != sentinel<T>();
+      if (kCheckV && ok && (!(v == v) || v == sentinel<T>())) lerr |= 1u;
+      return ok;
     };
 
-    // target bits of planes e-1, e, e+1 in three 18-bit fields: (dx+1)*6 + row
-    uint64_t acc = 0;
-    uint32_t uc[3] = {0, 0, 0};              // counters of this unit
+    // the violation queue of this wave: entries [qh, qt), resolved per unit
+    uint32_t uc[3] = {0, 0, 0};  // counters of this unit
+    uint32_t qh = 0, qt = 0;
     const int64_t cbase = (int64_t)D.cbase;
-    // detection at owned voxel row h of plane e: codes, counters, target masks
-    auto visit = [&](int h, int e, int oa, int od, int da, int dd) {
-      const int y = y0m1 + h;
-      if (!(owned && y < yhi)) return;
-      const uint32_t i = (uint32_t)x + nx * (uint32_t)y + sxy * (uint32_t)e;
-      if (kLabel) {
-        code[i] = (uint8_t)(oa | (od << 4));
-        if (dc) dc[i] = 0;  // the run's delta counters start at zero
-      }
-      if (kDetect) {
+    auto resolve = [&](uint32_t count) {  // the `count` oldest entries (count <= 64)
+      if (lane < count) {
+        const uint2 q = s_q[qbase + ((qh + lane) & (uint32_t)(kQCap - 1))];
+        const int oa = (int)(q.y & 15u), od = (int)((q.y >> 4) & 15u);
+        int da, dd;
+        if constexpr (kF32) {
+          da = diff_rank((int)((q.y >> 8) & 31u));
+          dd = diff_rank((int)((q.y >> 13) & 31u));
+        } else {
+          da = (int)((q.y >> 8) & 15u);
+          dd = (int)((q.y >> 12) & 15u);
+        }
         int t1, t2;
         bool v1, v2;
         detect_pair(oa, od, da, dd, uc, t1, v1, t2, v2);
-        if (emit) {
-          const uint64_t b1 = (uint64_t)(v1 ? 1u : 0u) << (s_tbase[t1] + (uint32_t)h);
-          const uint64_t b2 = (uint64_t)(v2 ? 1u : 0u) << (s_tbase[t2] + (uint32_t)h);
-          acc |= b1 | b2;
+        if (emit) {  // a target is a cell of the block
+          const int64_t c0 = cbase + (int64_t)q.x;
+          if (v1) tbyte[c0 + cand_off(t1, D.d)] = (uint8_t)tag;
+          if (v2) tbyte[c0 + cand_off(t2, D.d)] = (uint8_t)tag;
         }
       }
+      qh += count;
     };
-    auto flush_plane = [&](int e) {
-      // plane e-1 can receive no more targets
-      flush_targets((uint32_t)acc & ((1u << (3 * kColH)) - 1u), cbase + x, y0m1, (int64_t)e - 1,
-                    D.d, tbyte, (uint8_t)tag);
-      acc >>= 3 * kColH;
+    // append the lanes of wave mask bm; `info` is evaluated only if bm is not 0
+    auto enqueue = [&](uint64_t bm, int h, int e, auto info) {
+      if (bm == 0) return;  // uniform
+      if ((bm >> lane) & 1u) {
+        const uint32_t pos = qt + __builtin_amdgcn_mbcnt_hi(
+                                      (uint32_t)(bm >> 32),
+                                      __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+        const uint32_t cell = coff[h] + (uint32_t)((uint64_t)e * sxy);
+        s_q[qbase + (pos & (uint32_t)(kQCap - 1))] = make_uint2(cell, info());
+      }
+      qt += (uint32_t)__popcll(bm);
+      if (qt - qh > (uint32_t)(kQCap - 64)) {  // room for the next row's appends
+        wave_lds_sync();
+        while (qt - qh >= 64u) resolve(64u);
+        wave_lds_sync();
+      }
     };
-    if constexpr (sizeof(T) == 4) {
+    auto label = [&](int h, int e, int oa, int od) {
+      if (!vis[h]) return;
+      const uint64_t i = (uint64_t)coff[h] + (uint64_t)e * sxy;
+      code[i] = (uint8_t)(oa | (od << 4));
+      if (dc) dc[i] = 0;  // the run's delta counters start at zero
+    };
+
+    if constexpr (kF32) {
       // per voxel row: lo (plane p's share for plane p+1) and part (planes p-1, p
       // for plane p), as max / min keys
-      double lf_mx[kColRows], lf_mn[kColRows], pf_mx[kColRows], pf_mn[kColRows];
-      double lg_mx[kColRows], lg_mn[kColRows], pg_mx[kColRows], pg_mn[kColRows];
+      double lo_mx[kColRows], lo_mn[kColRows], pt_mx[kColRows], pt_mn[kColRows];
 #pragma unroll
-      for (int h = 0; h < kColRows; h++) lf_mx[h] = lf_mn[h] = lg_mx[h] = lg_mn[h] = kNaN;
-      auto key_of = [&](bool pin, T v, int h, uint32_t tplane, bool check) -> double {
-        bool ok = pin && rok[h];
-        if (padded) {
-          ok = ok && v != sentinel<T>();  // a ghost cell that carries no value
-        } else if (check && ok && (!(v == v) || v == sentinel<T>())) {
-          lerr |= 1u;
-        }
-        return ok ? key32(v, tplane + 3u * (uint32_t)h) : kNaN;
-      };
-      // row extremes: A over (x, x+1), B over (x-1..x+1), C over (x-1, x)
-      struct Win {
-        double a2x, a2n, a1x, a1n, b1x, b1n, c1x, c1n;  // rows k-2 (A) and k-1
-      };
-      fetch(zs - 1);
-      for (int p = zs - 1; p <= ze; p++) {
+      for (int h = 0; h < kColRows; h++) lo_mx[h] = lo_mn[h] = kNaN;
+      // plane step p: rows rv of plane p are in registers, cc holds the codes of
+      // plane p-1; plane p+1 is loaded into nv and the codes of plane p into nc
+      auto step = [&](int p, const T (&rv)[kColH], const uint8_t (&cc)[kColRows],
+                      T (&nv)[kColH], uint8_t (&nc)[kColRows]) {
         const bool pin = p >= 0 && p < (int)nz;  // uniform
-        const uint32_t tplane = lane + 5u * (uint32_t)(p - zs + 1);
-        T rf[kColH], rg[kColH];
-#pragma unroll
-        for (int h = 0; h < kColH; h++) {
-          if (kLabel) rf[h] = nf[h];
-          if (kDetect) rg[h] = ng[h];
-        }
-        if (!kLabel) {
-#pragma unroll
-          for (int h = 0; h < kColRows; h++) ccode[h] = ncode[h];
-        }
         if (p < ze) {
-          fetch(p + 1);
-          if (!kLabel) fetch_codes(p);
+          fetch(p + 1, nv);
+          if (kDetect && p >= zs) fetch_codes(p, nc);
         }
+        const uint32_t tplane = lane + 5u * (uint32_t)(p - zs + 1);
         const int e = p - 1;  // plane completed at this step
         const bool emitp = e >= zs;
         const uint32_t tself0 = lane + 5u * (uint32_t)(p - zs);
-        Win wf, wg;
+        // row extremes: A over (x, x+1), B over (x-1..x+1), C over (x-1, x) of
+        // rows k-2 (A only) and k-1
+        double a2x = 0, a2n = 0, a1x = 0, a1n = 0, b1x = 0, b1n = 0, c1x = 0, c1n = 0;
 #pragma unroll
         for (int k = 0; k < kColH; k++) {
-          double ax = 0, an = 0, bx = 0, bn = 0, cx = 0, cn = 0;
-          double gax = 0, gan = 0, gbx = 0, gbn = 0, gcx = 0, gcn = 0;
-          if (kLabel) {
-            const double kk = key_of(pin, rf[k], k, tplane, true);
-            const double l = from_left(kk), r = from_right(kk);
-            ax = kmax(kk, r);
-            an = kmin(kk, r);
-            cx = kmax(l, kk);
-            cn = kmin(l, kk);
-            bx = kmax(ax, l);
-            bn = kmin(an, l);
-          }
-          if (kDetect) {
-            const double kk = key_of(pin, rg[k], k, tplane, kLabel || check_g != 0);
-            const double l = from_left(kk), r = from_right(kk);
-            gax = kmax(kk, r);
-            gan = kmin(kk, r);
-            gcx = kmax(l, kk);
-            gcn = kmin(l, kk);
-            gbx = kmax(gax, l);
-            gbn = kmin(gan, l);
-          }
+          const bool ok = valid(pin && rok[k], rv[k]);
+          const float cv = rv[k] + 0.0f;  // -0 -> +0: the reference compares -0 == +0
+          const uint32_t b = __float_as_uint(cv);
+          const uint32_t ord = b ^ ((uint32_t)((int32_t)b >> 31) | 0x80000000u);
+          // key32(v, tag): high word 0x43300000 | ord >> 23 by one funnel shift; a
+          // missing cell is a quiet NaN whatever its low word
+          const uint32_t khi = ok ? __builtin_amdgcn_alignbit(0x219800u, ord, 23) : 0x7ff80000u;
+          const double kk =
+              __hiloint2double((int)khi, (int)((ord << 9) | (tplane + 3u * (uint32_t)k)));
+          const double l = from_left(kk), r = from_right(kk);
+          const double ax = kmax(kk, r), an = kmin(kk, r);
+          const double cx = kmax(l, kk), cn = kmin(l, kk);
+          const double bx = kmax(ax, l), bn = kmin(an, l);
           if (k >= 2) {
             const int h = k - 1;  // voxel row completed by row k
             if (emitp) {
+              const double wmx = kmax(pt_mx[h - 1], kmax(c1x, cx));
+              const double wmn = kmin(pt_mn[h - 1], kmin(c1n, cn));
               const uint32_t ts = tself0 + 3u * (uint32_t)h;
-              int oa, od, da = 0, dd = 0;
               if (kLabel) {
-                oa = key_rank(kmax(pf_mx[h - 1], kmax(wf.c1x, cx)), ts);
-                od = key_rank(kmin(pf_mn[h - 1], kmin(wf.c1n, cn)), ts);
+                label(h - 1, e, key_rank(wmx, ts), key_rank(wmn, ts));
               } else {
-                oa = ccode[h - 1] & 15;
-                od = ccode[h - 1] >> 4;
+                const uint32_t c = cc[h - 1];
+                const uint32_t ex = s_exp[c] + ts * 0x10001u;
+                const uint32_t wa = (uint32_t)__double2loint(wmx);
+                const uint32_t wd = (uint32_t)__double2loint(wmn);
+                const uint64_t bm =
+                    __ballot((((wa - (ex & 0xffffu)) | (wd - (ex >> 16))) & 511u) != 0u) &
+                    vmask[h - 1];
+                enqueue(bm, h - 1, e, [&] {
+                  return c | (((wa - ts + 8u) & 31u) << 8) | (((wd - ts + 8u) & 31u) << 13);
+                });
               }
-              if (kDetect) {
-                da = key_rank(kmax(pg_mx[h - 1], kmax(wg.c1x, gcx)), ts);
-                dd = key_rank(kmin(pg_mn[h - 1], kmin(wg.c1n, gcn)), ts);
-              }
-              visit(h, e, oa, od, da, dd);
             }
             // plane p's partials: mid for plane p, lo for plane p+1
-            if (kLabel) {
-              pf_mx[h - 1] = kmax(lf_mx[h - 1], kmax(wf.a2x, kmax(wf.b1x, cx)));
-              pf_mn[h - 1] = kmin(lf_mn[h - 1], kmin(wf.a2n, kmin(wf.b1n, cn)));
-              lf_mx[h - 1] = kmax(wf.a2x, wf.a1x);
-              lf_mn[h - 1] = kmin(wf.a2n, wf.a1n);
-            }
-            if (kDetect) {
-              pg_mx[h - 1] = kmax(lg_mx[h - 1], kmax(wg.a2x, kmax(wg.b1x, gcx)));
-              pg_mn[h - 1] = kmin(lg_mn[h - 1], kmin(wg.a2n, kmin(wg.b1n, gcn)));
-              lg_mx[h - 1] = kmax(wg.a2x, wg.a1x);
-              lg_mn[h - 1] = kmin(wg.a2n, wg.a1n);
-            }
+            pt_mx[h - 1] = kmax(lo_mx[h - 1], kmax(a2x, kmax(b1x, cx)));
+            pt_mn[h - 1] = kmin(lo_mn[h - 1], kmin(a2n, kmin(b1n, cn)));
+            lo_mx[h - 1] = kmax(a2x, a1x);
+            lo_mn[h - 1] = kmin(a2n, a1n);
           }
-          if (kLabel) {
-            wf.a2x = wf.a1x; wf.a2n = wf.a1n;
-            wf.a1x = ax; wf.a1n = an;
-            wf.b1x = bx; wf.b1n = bn;
-            wf.c1x = cx; wf.c1n = cn;
-          }
-          if (kDetect) {
-            wg.a2x = wg.a1x; wg.a2n = wg.a1n;
-            wg.a1x = gax; wg.a1n = gan;
-            wg.b1x = gbx; wg.b1n = gbn;
-            wg.c1x = gcx; wg.c1n = gcn;
-          }
+          a2x = a1x; a2n = a1n;
+          a1x = ax; a1n = an;
+          b1x = bx; b1n = bn;
+          c1x = cx; c1n = cn;
         }
-        if (kDetect && emitp) flush_plane(e);
+      };
+      // two register sets for the planes, alternating: no copies between steps
+      T va[kColH], vb[kColH];
+      uint8_t ca[kColRows] = {}, cb[kColRows] = {};
+      fetch(zs - 1, va);
+      int p = zs - 1;
+      for (; p + 1 <= ze; p += 2) {
+        step(p, va, ca, vb, cb);
+        step(p + 1, vb, cb, va, ca);
       }
+      if (p <= ze) step(p, va, ca, vb, cb);
     } else {
-      Ext<T> lo_f[kColRows], part_f[kColRows], lo_g[kColRows], part_g[kColRows];
+      Ext<T> lo_e[kColRows], part[kColRows];
 #pragma unroll
-      for (int h = 0; h < kColRows; h++) {
-        lo_f[h].first(kNaN, 0);
-        lo_g[h].first(kNaN, 0);
-      }
-      fetch(zs - 1);
+      for (int h = 0; h < kColRows; h++) lo_e[h].first(kNaN, 0);
+      T nv[kColH];
+      uint8_t ccode[kColRows], ncode[kColRows] = {};
+      fetch(zs - 1, nv);
       for (int p = zs - 1; p <= ze; p++) {
-        ColPlane cf, cg;
-        if (kLabel) stage(p, nf, cf, true);
-        if (kDetect) stage(p, ng, cg, kLabel || check_g != 0);
-        if (!kLabel) {
+        ColPlane cp;
+        if (p < 0 || p >= (int)nz) {
+#pragma unroll
+          for (int h = 0; h < kColH; h++) cp.k[h] = cp.l[h] = cp.r[h] = kNaN;
+        } else {
+#pragma unroll
+          for (int h = 0; h < kColH; h++) cp.k[h] = valid(rok[h], nv[h]) ? (double)nv[h] : kNaN;
+          shift_plane(cp);
+        }
+        if (kDetect) {
 #pragma unroll
           for (int h = 0; h < kColRows; h++) ccode[h] = ncode[h];
         }
         if (p < ze) {
-          fetch(p + 1);
-          if (!kLabel) fetch_codes(p);
+          fetch(p + 1, nv);
+          if (kDetect && p >= zs) fetch_codes(p, ncode);
         }
         const int e = p - 1;  // plane completed at this step
         const bool emitp = e >= zs;
 #pragma unroll
         for (int h = 1; h <= kColRows; h++) {
-          Ext<T> hf, hg, mf, mg, lf, lg;
           if (emitp) {
-            int oa = 0, od = 0, da = 0, dd = 0;
+            Ext<T> hi;
+            partial_hi<T>(cp, h, hi);
+            part[h - 1].merge(hi);
+            const int da = part[h - 1].asc(), dd = part[h - 1].desc();
             if (kLabel) {
-              partial_hi<T>(cf, h, hf);
-              part_f[h - 1].merge(hf);
-              oa = part_f[h - 1].asc();
-              od = part_f[h - 1].desc();
+              label(h - 1, e, da, dd);
             } else {
-              oa = ccode[h - 1] & 15;
-              od = ccode[h - 1] >> 4;
+              const uint32_t c = ccode[h - 1];
+              const uint32_t w = (uint32_t)da | ((uint32_t)dd << 4);
+              const uint64_t bm =
+                  __ballot(w != c || c == (uint32_t)(kSelf | (kSelf << 4))) & vmask[h - 1];
+              enqueue(bm, h - 1, e, [&] { return c | (w << 8); });
             }
-            if (kDetect) {
-              partial_hi<T>(cg, h, hg);
-              part_g[h - 1].merge(hg);
-              da = part_g[h - 1].asc();
-              dd = part_g[h - 1].desc();
-            }
-            visit(h, e, oa, od, da, dd);
           }
           // partials of plane p for the voxels of planes p and p+1
-          if (kLabel) {
-            partial_mid<T>(cf, h, mf);
-            partial_lo<T>(cf, h, lf);
-            part_f[h - 1] = lo_f[h - 1];
-            part_f[h - 1].merge(mf);
-            lo_f[h - 1] = lf;
-          }
-          if (kDetect) {
-            partial_mid<T>(cg, h, mg);
-            partial_lo<T>(cg, h, lg);
-            part_g[h - 1] = lo_g[h - 1];
-            part_g[h - 1].merge(mg);
-            lo_g[h - 1] = lg;
-          }
+          Ext<T> mid, lo;
+          partial_mid<T>(cp, h, mid);
+          partial_lo<T>(cp, h, lo);
+          part[h - 1] = lo_e[h - 1];
+          part[h - 1].merge(mid);
+          lo_e[h - 1] = lo;
         }
-        if (kDetect && emitp) flush_plane(e);
       }
     }
     if (kDetect) {
-      flush_targets((uint32_t)acc & ((1u << (3 * kColH)) - 1u), cbase + x, y0m1, (int64_t)ze - 1,
-                    D.d, tbyte, (uint8_t)tag);
-      flush_targets((uint32_t)(acc >> (3 * kColH)) & ((1u << (3 * kColH)) - 1u), cbase + x, y0m1,
-                    (int64_t)ze, D.d, tbyte, (uint8_t)tag);
+      wave_lds_sync();
+      while (qt != qh) resolve(qt - qh < 64u ? qt - qh : 64u);
+      wave_lds_sync();
       if (kMulti) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -656,7 +658,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
     const uint32_t s = wave_sum(lc[c]);
     if (lane == 0 && s) atomicAdd(&out->cnt[c], s);
   }
-  if (threadIdx.x == 0 && blockIdx.x == 0 && !kLabel) atomicAdd(&run->dense_passes, 1ull);
+  if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(&run->dense_passes, 1ull);
 }
 
 }  // namespace pmsz

@@ -973,6 +973,13 @@ int ensure_host_ds(Ctx &c, size_t n) {
   return PMSZ_OK;
 }
 
+// The full-volume kernel addresses a plane with 32-bit byte offsets: a plane of
+// the field, and of a padded block of it, stays below 4 GiB.
+bool plane_fits(int dtype, int64_t nx, int64_t ny) {
+  const int64_t s = dtype == PMSZ_F64 ? 8 : 4;
+  return (nx + 2) * (ny + 2) * s < ((int64_t)1 << 32);
+}
+
 int check_params(const pmsz_params *p) {
   if (!p) return PMSZ_EINVAL;
   if (p->dtype != PMSZ_F32 && p->dtype != PMSZ_F64) return PMSZ_EINVAL;
@@ -989,6 +996,7 @@ int check_params(const pmsz_params *p) {
     if (p->sub[a] < 1) return PMSZ_EINVAL;
   if (p->sub[0] > p->nx || p->sub[1] > p->ny || p->sub[2] > p->nz) return PMSZ_EINVAL;
   if (n >= (int64_t)0xffffffffLL) return PMSZ_ETOOBIG;
+  if (!plane_fits(p->dtype, p->nx, p->ny)) return PMSZ_ETOOBIG;
   return PMSZ_OK;
 }
 
@@ -1094,8 +1102,10 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   for (const Dom &D : job.doms) nunits += dom_units(D);
   static const int occ_label = occupancy(k_columns<T, true, false, kMulti>, kColThreads);
   static const int occ_dense = occupancy(k_columns<T, false, true, kMulti>, kColThreads);
+  static const int occ_dense1 = occupancy(k_columns<T, false, true, kMulti, true>, kColThreads);
   const unsigned g_label = col_blocks(c, nunits, occ_label);
   const unsigned g_dense = col_blocks(c, nunits, occ_dense);
+  const unsigned g_dense1 = col_blocks(c, nunits, occ_dense1);
   const unsigned g_list = (unsigned)std::max(1, c.cus * 4);
   // the tail: its grid barriers need every block resident, also while the tails of
   // other contexts run -- each tail takes at most 1/(kMaxContexts+1) of what the
@@ -1128,12 +1138,15 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   // a dense detection sweep (slot 0 holds mode Dense)
   PMSZ_HIP(hipEventRecord(c.ev2, c.stream));
   k_columns<T, true, false, kMulti><<<g_label, kColThreads, 0, c.stream>>>(
-      d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, 0u, 0u);
+      d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, 0u);
   PMSZ_HIP(hipEventRecord(c.ev3, c.stream));
   // detection 1 also rejects NaN / sentinel values of g (padded blocks: the scatter did)
-  k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
-      d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, job.subdomains ? 0u : 1u,
-      detect_tag(1));
+  if (job.subdomains)
+    k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
+        d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, detect_tag(1));
+  else
+    k_columns<T, false, true, kMulti, true><<<g_dense1, kColThreads, 0, c.stream>>>(
+        d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, detect_tag(1));
   PMSZ_HIP(hipGetLastError());
   PMSZ_HIP(hipEventRecord(c.ev4, c.stream));
   k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(c.tbyte.p, c.tlist.p, c.ring, 0, 1,
@@ -1160,7 +1173,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
         PMSZ_HIP(hipMemsetAsync(c.tbyte.p, 0, 16 * job.groups, c.stream));
       k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
           d0, doms, ndom, ds, c.tbyte.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, (j + 1) & 1u,
-          c.run, 0u, detect_tag(j + 1));
+          c.run, detect_tag(j + 1));
       k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
           c.tbyte.p, c.tlist.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false,
           (uint32_t)job.groups, doms, ndom, detect_tag(j + 1));
@@ -1601,7 +1614,7 @@ int time_first_pass(Ctx &c, const pmsz_params *p, const T *f, const T *g, int it
   job.s = make_sched(p, (uint64_t)n, true);
   if ((rc = prepare(c, job))) return rc;
   static const int occ_l = occupancy(k_columns<T, true, false, false>, kColThreads);
-  static const int occ_d = occupancy(k_columns<T, false, true, false>, kColThreads);
+  static const int occ_d = occupancy(k_columns<T, false, true, false, true>, kColThreads);
   const unsigned gl = col_blocks(c, dom_units(job.doms[0]), occ_l);
   const unsigned gd = col_blocks(c, dom_units(job.doms[0]), occ_d);
   double tl = 0, td = 0;
@@ -1613,10 +1626,10 @@ int time_first_pass(Ctx &c, const pmsz_params *p, const T *f, const T *g, int it
                             c.stream));
     PMSZ_HIP(hipEventRecord(c.ev0, c.stream));
     k_columns<T, true, false, false><<<gl, kColThreads, 0, c.stream>>>(
-        job.doms[0], nullptr, 1, nullptr, c.tbyte.p, c.ring, 0, 1, 1u, c.run, 0u, 0u);
+        job.doms[0], nullptr, 1, nullptr, c.tbyte.p, c.ring, 0, 1, 1u, c.run, 0u);
     PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
-    k_columns<T, false, true, false><<<gd, kColThreads, 0, c.stream>>>(
-        job.doms[0], nullptr, 1, nullptr, c.tbyte.p, c.ring, 0, 1, 1u, c.run, 0u, 1u);
+    k_columns<T, false, true, false, true><<<gd, kColThreads, 0, c.stream>>>(
+        job.doms[0], nullptr, 1, nullptr, c.tbyte.p, c.ring, 0, 1, 1u, c.run, 1u);
     PMSZ_HIP(hipGetLastError());
     PMSZ_HIP(hipEventRecord(c.ev2, c.stream));
     PMSZ_HIP(hipStreamSynchronize(c.stream));
@@ -1848,12 +1861,12 @@ int pmsz_codes(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_fiel
     static const int occ = occupancy(k_columns<float, true, false, false>, kColThreads);
     k_columns<float, true, false, false><<<col_blocks(c, dom_units(D), occ), kColThreads, 0,
                                            c.stream>>>(D, nullptr, 1, nullptr, (uint8_t *)nullptr, c.ring, 0,
-                                                       0, 0u, c.run, 0u, 0u);
+                                                       0, 0u, c.run, 0u);
   } else {
     static const int occ = occupancy(k_columns<double, true, false, false>, kColThreads);
     k_columns<double, true, false, false><<<col_blocks(c, dom_units(D), occ), kColThreads, 0,
                                             c.stream>>>(D, nullptr, 1, nullptr, (uint8_t *)nullptr, c.ring,
-                                                        0, 0, 0u, c.run, 0u, 0u);
+                                                        0, 0, 0u, c.run, 0u);
   }
   PMSZ_HIP(hipGetLastError());
   PMSZ_HIP(hipMemcpyAsync(c.h_run, c.run, sizeof(RunState), hipMemcpyDeviceToHost, c.stream));
