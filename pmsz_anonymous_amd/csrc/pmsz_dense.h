@@ -188,15 +188,22 @@ __device__ __forceinline__ double kmin(double a, double b) {
 
 // Target list from the target byte map after a full-volume detection: every
 // byte equal to the sweep's tag is one distinct target, listed as its voxel id.
-// Each thread takes 16-byte groups; a block reserves its list range with one
-// atomic per step.  Nothing is cleared: the next sweep uses another tag.
+// A thread takes kCompactPT 16-byte groups per step (interleaved across the
+// block, so loads coalesce) and keeps one 16-bit mask per group; a block makes
+// one list reservation per step (same-address atomics serialise, so steps are
+// large).  Nothing is cleared: the next sweep uses another tag.
 constexpr int kCompactThreads = 256;
-constexpr int kCompactPT = 8;  // 16-byte groups per thread per step
+constexpr int kCompactPT = 64;     // groups per thread per step
+constexpr int kCompactBatch = 16;  // loads in flight per thread
 __device__ __forceinline__ uint32_t tag_bytes(uint32_t w, uint32_t rep) {
   // 0x01 in each byte of w equal to the tag (rep = tag * 0x01010101), else 0
   const uint32_t x = w ^ rep;
   const uint32_t nz = (((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) & 0x80808080u;
   return (~nz >> 7) & 0x01010101u;
+}
+// The four 0x01 / 0x00 bytes of tag_bytes() as 4 bits.
+__device__ __forceinline__ uint32_t byte_bits(uint32_t t) {
+  return (t | (t >> 7) | (t >> 14) | (t >> 21)) & 15u;
 }
 __global__ __launch_bounds__(kCompactThreads) void k_compact(
     const uint8_t *__restrict__ tbyte, uint32_t *__restrict__ tlist, Counters *__restrict__ ring,
@@ -206,45 +213,45 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(
   uint32_t *counter = &ring[jslot].ntargets;
   const uint4 *g16 = reinterpret_cast<const uint4 *>(tbyte);
   const uint32_t rep = tag * 0x01010101u;
-  const uint32_t step = gridDim.x * kCompactThreads * kCompactPT;
-  for (uint32_t base = blockIdx.x * kCompactThreads * kCompactPT; base < ngroups; base += step) {
-    uint32_t w[kCompactPT][4];
+  const uint32_t per_step = kCompactThreads * kCompactPT;
+  for (uint32_t base = blockIdx.x * per_step; base < ngroups; base += gridDim.x * per_step) {
+    uint32_t m[kCompactPT];
     uint32_t n = 0;
 #pragma unroll
-    for (int q = 0; q < kCompactPT; q++) {
-      const uint32_t i = base + (uint32_t)q * kCompactThreads + threadIdx.x;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (i < ngroups) v = g16[i];
-      w[q][0] = tag_bytes(v.x, rep);
-      w[q][1] = tag_bytes(v.y, rep);
-      w[q][2] = tag_bytes(v.z, rep);
-      w[q][3] = tag_bytes(v.w, rep);
+    for (int q0 = 0; q0 < kCompactPT; q0 += kCompactBatch) {
+      uint4 v[kCompactBatch];
 #pragma unroll
-      for (int c = 0; c < 4; c++) n += __popc(w[q][c]);
+      for (int q = 0; q < kCompactBatch; q++) {
+        const uint32_t i = base + (uint32_t)(q0 + q) * kCompactThreads + threadIdx.x;
+        v[q] = i < ngroups ? g16[i] : make_uint4(0, 0, 0, 0);
+      }
+#pragma unroll
+      for (int q = 0; q < kCompactBatch; q++) {
+        m[q0 + q] = byte_bits(tag_bytes(v[q].x, rep)) | (byte_bits(tag_bytes(v[q].y, rep)) << 4) |
+                    (byte_bits(tag_bytes(v[q].z, rep)) << 8) |
+                    (byte_bits(tag_bytes(v[q].w, rep)) << 12);
+        n += __popc(m[q0 + q]);
+      }
     }
     uint32_t pos = block_reserve<kCompactThreads>(n, counter);
 #pragma unroll
     for (int q = 0; q < kCompactPT; q++) {
+      uint32_t mq = m[q];
       const uint32_t i = base + (uint32_t)q * kCompactThreads + threadIdx.x;
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        uint32_t m = w[q][c];
-        while (m) {
-          const int b = (__ffs(m) - 1) >> 3;
-          m &= m - 1;
-          const uint32_t cell = i * 16u + (uint32_t)(c * 4 + b);
-          uint32_t id = cell;
-          if (ndom > 1) {  // the domain whose cells start at or before `cell`
-            uint32_t lo = 0, hi = ndom;
-            while (hi - lo > 1) {
-              const uint32_t mid = (lo + hi) / 2;
-              if (doms[mid].cbase <= cell) lo = mid;
-              else hi = mid;
-            }
-            id = doms[lo].gbase + (cell - doms[lo].cbase);
+      while (mq) {
+        const uint32_t cell = i * 16u + (uint32_t)(__ffs(mq) - 1);
+        mq &= mq - 1;
+        uint32_t id = cell;
+        if (ndom > 1) {  // the domain whose cells start at or before `cell`
+          uint32_t lo = 0, hi = ndom;
+          while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (doms[mid].cbase <= cell) lo = mid;
+            else hi = mid;
           }
-          tlist[pos++] = id;
+          id = doms[lo].gbase + (cell - doms[lo].cbase);
         }
+        tlist[pos++] = id;
       }
     }
   }
@@ -656,7 +663,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const uint32_t s = wave_sum(lc[c]);
-    if (lane == 0 && s) atomicAdd(&out->cnt[c], s);
+    if (lane == 0) cnt_add(*out, c, s);
   }
   if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(&run->dense_passes, 1ull);
 }

@@ -99,35 +99,86 @@ enum : uint32_t {
   kModeDone = 4,    // finished (converged or capped)
 };
 
-// Counters of detection j and of the apply that follows it (one 64-byte block per
-// sweep, in a ring).  cnt: false max / false min / direction (count_f_max,
-// count_f_min, count_f_dir, pMSz.cu:174); ntargets: entries of tlist; nwork: entries
-// of wlist queued by apply j; mode: the decision of apply j.
+// Device-scope atomics on one address serialise (about 11 ns each on MI355X), so
+// a counter that every block or wave of a launch adds to is kept in kStripes
+// copies 16 bytes apart: a block adds to copy blockIdx.x % kStripes, readers sum.
+constexpr int kStripes = 16;
+
+// Counters of detection j and of the apply that follows it (one block per sweep,
+// in a ring).  cs: false max / false min / direction (count_f_max, count_f_min,
+// count_f_dir, pMSz.cu:174), by stripe; ntargets: entries of tlist; nwork:
+// entries of wlist queued by apply j; mode: the decision of apply j.
 struct Counters {
-  uint32_t cnt[3];
+  uint32_t cs[kStripes][4];
   uint32_t ntargets;
   uint32_t nwork;
   uint32_t mode;
   uint32_t err;
   uint32_t nactive;  // multi-domain: domains taking part in detection j+1
   uint32_t ncapped;  // multi-domain: domains stopped by the sweep cap
-  uint32_t pad[7];
+  uint32_t pad[10];
 };
-static_assert(sizeof(Counters) == 64, "Counters layout");
+static_assert(sizeof(Counters) == 320, "Counters layout");
+
+__device__ __forceinline__ void cnt_add(Counters &c, int q, uint32_t v) {
+  if (v) atomicAdd(&c.cs[blockIdx.x % kStripes][q], v);
+}
+// Counter q of a detection (relaxed device-scope loads: valid after a grid
+// barrier as well as after a kernel boundary).
+__device__ __forceinline__ uint32_t cnt_get(const Counters &c, int q) {
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kStripes; k++)
+    s += __hip_atomic_load(&c.cs[k][q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return s;
+}
+// Nonzero iff any counter of the detection is (plain loads: after a kernel
+// boundary; uniform addresses, so scalar loads).
+__device__ __forceinline__ uint32_t cnt_any(const Counters &c) {
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kStripes; k++) s |= c.cs[k][0] | c.cs[k][1] | c.cs[k][2];
+  return s;
+}
+__device__ __forceinline__ uint32_t cnt_any_agent(const Counters &c) {
+  return cnt_get(c, 0) | cnt_get(c, 1) | cnt_get(c, 2);
+}
+__host__ inline uint32_t cnt_host(const Counters &c, int q) {
+  uint32_t s = 0;
+  for (int k = 0; k < kStripes; k++) s += c.cs[k][q];
+  return s;
+}
+
+// Schedule parameters of a run.
+struct Sched {
+  int mode;  // PMSZ_LAZY / PMSZ_SYNC
+  uint32_t max_sweeps;
+  uint32_t dense_min_work;  // 15 * targets at or above this -> next detection dense
+  uint32_t tail_ok;         // the cooperative tail may take over
+};
 
 // Run-wide state, written only by kernels that end a phase.
+enum : int { kAccTargets = 0, kAccTouched = 1, kAccClamped = 2, kAccSparse = 3 };
 struct RunState {
   uint32_t tail_j;     // sweep whose apply the tail kernel starts with (0: none)
   uint32_t done;       // 0 running, 1 converged, 2 sweep cap reached
   uint32_t last_j;     // detections performed when done
   uint32_t err;        // bit 0: NaN or sentinel value in an input
-  unsigned long long total_targets;
-  unsigned long long touched;
-  unsigned long long clamped;
-  unsigned long long sparse_voxels;
   unsigned long long dense_passes;
-  unsigned long long pad[3];
+  unsigned long long pad[5];
+  // by stripe: edits applied, voxels edited at least once, voxels clamped,
+  // voxels re-examined by work-list sweeps
+  unsigned long long acc[kStripes][4];
 };
+
+__device__ __forceinline__ void acc_add(RunState *r, int k, unsigned long long v) {
+  if (v) atomicAdd(&r->acc[blockIdx.x % kStripes][k], v);
+}
+__host__ inline unsigned long long acc_host(const RunState &r, int k) {
+  unsigned long long s = 0;
+  for (int q = 0; q < kStripes; q++) s += r.acc[q][k];
+  return s;
+}
 
 constexpr int kCtrRing = 256;  // counter blocks kept; consecutive sweeps use distinct slots
 
@@ -351,9 +402,9 @@ __device__ __forceinline__ uint32_t block_reserve(uint32_t n, uint32_t *counter)
   return pos;
 }
 
-// Block-wide sum of three counters added to dst[0..2] with one atomic each.
+// Block-wide sums of three detection counters added to dst with one atomic each.
 template <int kBlock>
-__device__ __forceinline__ void block_add3(const uint32_t (&c)[3], uint32_t *dst) {
+__device__ __forceinline__ void block_add3(const uint32_t (&c)[3], Counters &dst) {
   constexpr int kWaves = kBlock / 64;
   __shared__ uint32_t s_c[3][kWaves];
   const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
@@ -366,7 +417,7 @@ __device__ __forceinline__ void block_add3(const uint32_t (&c)[3], uint32_t *dst
   if (threadIdx.x < 3) {
     uint32_t s = 0;
     for (int w = 0; w < kWaves; w++) s += s_c[threadIdx.x][w];
-    if (s) atomicAdd(&dst[threadIdx.x], s);
+    cnt_add(dst, (int)threadIdx.x, s);
   }
   __syncthreads();
 }

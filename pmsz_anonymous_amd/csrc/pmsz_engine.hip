@@ -68,13 +68,6 @@ constexpr int kSyncThreads = 256;
 constexpr uint32_t kMaxDomains = 1024;  // domain bookkeeping: one thread each
 constexpr int kMaxContexts = 8;         // live contexts, the default one included
 
-struct Sched {
-  int mode;  // PMSZ_LAZY / PMSZ_SYNC
-  uint32_t max_sweeps;
-  uint32_t dense_min_work;  // 15 * targets at or above this -> next detection dense
-  uint32_t tail_ok;         // the single-workgroup tail may take over
-};
-
 // Stencil directions of the reference in its order d = 0..13 (directions_host,
 // pMSz.cu:186-193): component a of direction d.  The opposite of d is d ^ 1.
 __host__ __device__ constexpr int dir_comp(int d, int a) {
@@ -94,10 +87,14 @@ __device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) {
 // Decisions of the apply of sweep j (identical in every thread): the mode of
 // detection j+1, or kModeTail, or kModeDone with *done = 1 (converged) / 2 (cap).
 
-// One domain: from the counters of detection j.
+// One domain: from the counters of detection j.  kAtomic: relaxed device-scope
+// loads, for counters written by other blocks of the same launch (the tail);
+// plain loads suffice after a kernel boundary.
+template <bool kAtomic>
 __device__ __forceinline__ uint32_t decide(const Counters &cj, uint32_t j, const Sched &s,
                                            bool tail_allowed, uint32_t *done) {
-  bool zero = cj.cnt[0] == 0 && cj.cnt[1] == 0 && cj.cnt[2] == 0;
+  const bool zero = (kAtomic ? cnt_any_agent(cj) : cnt_any(cj)) == 0;
+  const uint32_t nt = kAtomic ? ld_agent(&cj.ntargets) : cj.ntargets;
   *done = 0;
   if (zero) {
     *done = 1;
@@ -107,8 +104,8 @@ __device__ __forceinline__ uint32_t decide(const Counters &cj, uint32_t j, const
     *done = 2;
     return kModeDone;
   }
-  const bool dense = (uint64_t)cj.ntargets * kCand >= s.dense_min_work;
-  if (tail_allowed && !dense && cj.ntargets <= kTailTargets) return kModeTail;
+  const bool dense = (uint64_t)nt * kCand >= s.dense_min_work;
+  if (tail_allowed && !dense && nt <= kTailTargets) return kModeTail;
   if (s.mode == PMSZ_LAZY && j >= s.max_sweeps) {
     *done = 2;  // the edits of sweep j still land (they belong to sweep j)
     return kModeDone;
@@ -122,12 +119,13 @@ __device__ __forceinline__ uint32_t decide(const Counters &cj, uint32_t j, const
 __device__ __forceinline__ uint32_t decide_multi(const Counters &cj, const Sched &s,
                                                  bool tail_allowed, uint32_t *done) {
   *done = 0;
-  if (cj.nactive == 0) {
-    *done = cj.ncapped ? 2 : 1;
+  if (ld_agent(&cj.nactive) == 0) {
+    *done = ld_agent(&cj.ncapped) ? 2 : 1;
     return kModeDone;
   }
-  const bool dense = (uint64_t)cj.ntargets * kCand >= s.dense_min_work;
-  if (tail_allowed && !dense && cj.ntargets <= kTailTargets) return kModeTail;
+  const uint32_t nt = ld_agent(&cj.ntargets);
+  const bool dense = (uint64_t)nt * kCand >= s.dense_min_work;
+  if (tail_allowed && !dense && nt <= kTailTargets) return kModeTail;
   return dense ? kModeDense : kModeSparse;
 }
 
@@ -251,6 +249,103 @@ __device__ __forceinline__ int queue_neighbourhood(uint32_t t, int64_t x, int64_
   return n;
 }
 
+// Marks the classified voxels of the closed stencil neighbourhood of cell t of a
+// single domain in wmark, without looking at the old bits: the stencil's 7 rows
+// of consecutive x take one atomic OR each (two where a row crosses a word).
+__device__ __forceinline__ void mark_neighbourhood(uint32_t t, const Dom &D, uint32_t *wmark) {
+  int64_t x, y, z;
+  decompose(t, D.d, x, y, z);
+  const uint32_t vm = box_mask(x, y, z, D.lo, D.hi);
+  constexpr int kRowFirst[7] = {0, 2, 4, 6, 9, 11, 13};  // first rank of each row
+  constexpr int kRowLen[7] = {2, 2, 2, 3, 2, 2, 2};
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    const uint32_t rb = (vm >> kRowFirst[k]) & ((1u << kRowLen[k]) - 1u);
+    if (!rb) continue;
+    const uint32_t v0 = (uint32_t)((int64_t)t + rank_off(kRowFirst[k], D.d));
+    const uint64_t m = (uint64_t)rb << (v0 & 31u);
+    atomicOr(&wmark[v0 >> 5], (uint32_t)m);
+    if (m >> 32) atomicOr(&wmark[(v0 >> 5) + 1], (uint32_t)(m >> 32));
+  }
+}
+
+// Detection at one listed voxel of a single domain; its targets are marked in
+// tmark without looking at the old bits (the list is compacted afterwards).
+template <typename T>
+__device__ __forceinline__ void detect_mark(const Dom &D, uint32_t *tmark, uint32_t local,
+                                            uint32_t (&lc)[3]) {
+  const T *g = (const T *)D.g;
+  int64_t x, y, z;
+  decompose(local, D.d, x, y, z);
+  const uint32_t vm = valid_mask(x, y, z, D.d);
+  T cg[kCand];
+#pragma unroll
+  for (int r = 0; r < kCand; r++)
+    cg[r] = ((vm >> r) & 1u) ? g[(int64_t)local + rank_off(r, D.d)] : (T)0;
+  int da, dd;
+  if (vm == 0x7fffu) scan_full(cg, da, dd);
+  else scan_masked(cg, vm, da, dd);
+  const uint8_t oc = D.code[local];
+  int tr[4];
+  const int nt = detect_targets(oc & 15, oc >> 4, da, dd, tr, lc);
+  for (int q = 0; q < nt; q++) {
+    const uint32_t v = (uint32_t)((int64_t)local + rank_off(tr[q], D.d));
+    atomicOr(&tmark[v >> 5], 1u << (v & 31u));
+  }
+}
+
+// List of the set bits of a bitmap (ascending within each thread's words),
+// each nonzero word cleared as it is read; runs when gate->mode == need.  A
+// block takes 16K words per step with one list reservation.
+constexpr int kBitsThreads = 256;
+constexpr int kBitsVec = 16;  // 16-byte vectors per thread per step
+__global__ __launch_bounds__(kBitsThreads) void k_compact_bits(
+    uint32_t *__restrict__ bits, uint32_t nwords, uint32_t *__restrict__ list,
+    uint32_t *__restrict__ count, const Counters *__restrict__ gate, uint32_t need) {
+  if (gate->mode != need) return;  // uniform
+  uint4 *b4 = reinterpret_cast<uint4 *>(bits);
+  const uint32_t per_step = kBitsThreads * kBitsVec * 4;
+  for (uint32_t w0 = blockIdx.x * per_step; w0 < nwords; w0 += gridDim.x * per_step) {
+    uint4 q[kBitsVec];
+    uint32_t n = 0;
+#pragma unroll
+    for (int v = 0; v < kBitsVec; v++) {
+      const uint32_t wi = w0 + (uint32_t)(v * kBitsThreads + threadIdx.x) * 4u;
+      if (wi + 3 < nwords) {
+        q[v] = b4[wi / 4];
+      } else {
+        q[v].x = wi < nwords ? bits[wi] : 0u;
+        q[v].y = wi + 1 < nwords ? bits[wi + 1] : 0u;
+        q[v].z = wi + 2 < nwords ? bits[wi + 2] : 0u;
+        q[v].w = 0u;
+      }
+      n += __popc(q[v].x) + __popc(q[v].y) + __popc(q[v].z) + __popc(q[v].w);
+    }
+    uint32_t pos = block_reserve<kBitsThreads>(n, count);
+#pragma unroll
+    for (int v = 0; v < kBitsVec; v++) {
+      const uint32_t wi = w0 + (uint32_t)(v * kBitsThreads + threadIdx.x) * 4u;
+      const uint32_t ws[4] = {q[v].x, q[v].y, q[v].z, q[v].w};
+      if ((ws[0] | ws[1] | ws[2] | ws[3]) == 0) continue;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        uint32_t m = ws[c];
+        while (m) {
+          const uint32_t b = (uint32_t)(__ffs(m) - 1);
+          m &= m - 1;
+          list[pos++] = (wi + (uint32_t)c) * 32u + b;
+        }
+      }
+      if (wi + 3 < nwords) {
+        b4[wi / 4] = make_uint4(0u, 0u, 0u, 0u);
+      } else {
+        for (uint32_t c = 0; c < 3; c++)
+          if (wi + c < nwords) bits[wi + c] = 0u;
+      }
+    }
+  }
+}
+
 // Detection at one listed voxel (global loads); returns the claimed targets.  In a
 // padded block a neighbour holding the sentinel is missing (pMSz.cu:709).
 template <typename T>
@@ -305,8 +400,9 @@ __device__ __forceinline__ int apply_target(uint32_t t, const Dom &d0, const Dom
   const Dom &D = dom_of<kMulti>(t, d0, doms, shift, dk, local);
   apply_one<T>((T *)D.g, (const T *)D.f, D.dc, local, D.ep, touched, clamped);
   // every set bit of tmark is a target of this sweep and is applied in this phase,
-  // so the word can be cleared with a plain store
-  tmark[t >> 5] = 0u;
+  // so the word can be cleared with a plain store (tmark is null when the targets
+  // come from a full-volume detection, which marks none)
+  if (tmark) tmark[t >> 5] = 0u;
   int64_t x, y, z;
   decompose(local, D.d, x, y, z);
   if (kMulti && D.padded) {
@@ -336,7 +432,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
   }
   uint32_t done;
   const uint32_t mode =
-      kMulti ? decide_multi(cj, s, s.tail_ok != 0, &done) : decide(cj, j, s, true, &done);
+      kMulti ? decide_multi(cj, s, s.tail_ok != 0, &done) : decide<false>(cj, j, s, true, &done);
   if (lead) {
     cj.mode = mode;
     if (mode == kModeTail) run->tail_j = j;
@@ -353,6 +449,51 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
   const bool queue = mode == kModeSparse;
   uint32_t touched = 0, clamped = 0;
   const uint32_t stride = gridDim.x * kApplyThreads;
+  if (!kMulti && queue) {
+    // one domain, work-list detection next: the edit, then the neighbourhood's
+    // bits in wmark (the list is compacted from the bitmap afterwards)
+    for (uint32_t i = blockIdx.x * kApplyThreads + threadIdx.x; i < nt; i += stride) {
+      const uint32_t t = tlist[i];
+      apply_one<T>((T *)d0.g, (const T *)d0.f, d0.dc, t, d0.ep, touched, clamped);
+      mark_neighbourhood(t, d0, wmark);
+    }
+  } else if (!kMulti) {
+    // edits only (the targets of a full-volume detection, no work list): four
+    // targets per thread, all their loads issued before any edit is stored
+    constexpr int kU = 4;
+    T *g = (T *)d0.g;
+    const T *f = (const T *)d0.f;
+    uint8_t *dc = d0.dc;
+    for (uint32_t base = blockIdx.x * kApplyThreads * kU; base < nt; base += stride * kU) {
+      uint32_t t[kU];
+      T fv[kU], gv[kU];
+      uint8_t cv[kU];
+#pragma unroll
+      for (int q = 0; q < kU; q++) {
+        const uint32_t i = base + (uint32_t)q * kApplyThreads + threadIdx.x;
+        t[q] = i < nt ? tlist[i] : 0xffffffffu;
+      }
+#pragma unroll
+      for (int q = 0; q < kU; q++) {
+        if (t[q] != 0xffffffffu) {
+          fv[q] = f[t[q]];
+          gv[q] = g[t[q]];
+          cv[q] = dc[t[q]];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kU; q++) {
+        if (t[q] != 0xffffffffu) {
+          const uint8_t c0 = cv[q];
+          edit_value<T>(fv[q], gv[q], cv[q], d0.ep);
+          g[t[q]] = gv[q];
+          dc[t[q]] = cv[q];
+          touched += c0 == 0;
+          clamped += (cv[q] == d0.ep.q && c0 != d0.ep.q);
+        }
+      }
+    }
+  } else
   for (uint32_t base = blockIdx.x * kApplyThreads; base < nt; base += stride) {
     const uint32_t i = base + threadIdx.x;
     uint32_t u[kCand];
@@ -371,11 +512,11 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
   if (!kMulti) {
     uint32_t ts = wave_sum(touched), cs = wave_sum(clamped);
     if (lane_id() == 0) {
-      if (ts) atomicAdd(&run->touched, (unsigned long long)ts);
-      if (cs) atomicAdd(&run->clamped, (unsigned long long)cs);
+      acc_add(run, kAccTouched, ts);
+      acc_add(run, kAccClamped, cs);
     }
   }
-  if (lead) atomicAdd(&run->total_targets, (unsigned long long)nt);
+  if (lead) acc_add(run, kAccTargets, nt);
 }
 
 // Work-list detection of one listed voxel id (shared by k_detect_list / k_tail).
@@ -406,6 +547,11 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
   const uint32_t nw = cj.nwork;
   uint32_t lc[3] = {0, 0, 0};
   const uint32_t stride = gridDim.x * kApplyThreads;
+  if (!kMulti) {
+    // one domain: targets go to tmark bits, compacted into tlist afterwards
+    for (uint32_t i = blockIdx.x * kApplyThreads + threadIdx.x; i < nw; i += stride)
+      detect_mark<T>(d0, tmark, wlist[i], lc);
+  } else
   for (uint32_t base = blockIdx.x * kApplyThreads; base < nw; base += stride) {
     const uint32_t i = base + threadIdx.x;
     uint32_t out[4];
@@ -424,9 +570,9 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
     uint32_t pos = block_reserve<kApplyThreads>((uint32_t)n, &cn.ntargets);
     for (int q = 0; q < n; q++) tlist[pos + q] = out[q];
   }
-  block_add3<kApplyThreads>(lc, cn.cnt);
+  block_add3<kApplyThreads>(lc, cn);
   if (blockIdx.x == 0 && threadIdx.x == 0)
-    atomicAdd(&run->sparse_voxels, (unsigned long long)nw);
+    acc_add(run, kAccSparse, nw);
 }
 
 // The tail: once a sweep's targets are few and detection j+1 would run over a
@@ -448,17 +594,20 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
   const uint32_t nthreads = gridDim.x * kTailThreads;
   unsigned long long tt = 0, sv = 0;
   uint32_t touched = 0, clamped = 0;
+  __shared__ uint32_t s_mode, s_done, s_nt;
   while (true) {
     Counters &cj = ring[j % kCtrRing];
-    Counters cv;
-#pragma unroll
-    for (int q = 0; q < 3; q++) cv.cnt[q] = ld_agent(&cj.cnt[q]);
-    cv.ntargets = ld_agent(&cj.ntargets);
-    cv.nactive = ld_agent(&cj.nactive);
-    cv.ncapped = ld_agent(&cj.ncapped);
-    uint32_t done;
-    const uint32_t mode =
-        kMulti ? decide_multi(cv, s, false, &done) : decide(cv, j, s, false, &done);
+    // one thread per block reads the counters (after the grid barrier) and
+    // decides; every block decides alike
+    if (threadIdx.x == 0) {
+      uint32_t dn;
+      s_mode = kMulti ? decide_multi(cj, s, false, &dn) : decide<true>(cj, j, s, false, &dn);
+      s_done = dn;
+      s_nt = ld_agent(&cj.ntargets);
+    }
+    __syncthreads();
+    const uint32_t mode = s_mode, done = s_done, nt = s_nt;
+    __syncthreads();  // s_* are rewritten in the next sweep
     if (lead) {
       cj.mode = mode == kModeDone ? kModeDone : kModeTail;
       if (done) {
@@ -467,7 +616,6 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
       }
     }
     if (!kMulti && (done == 1 || (done == 2 && s.mode == PMSZ_SYNC))) break;
-    const uint32_t nt = cv.ntargets;
     if (kMulti && mode == kModeDone && nt == 0) break;
     if (lead && mode != kModeDone) zero_counters(&ring[(j + 1) % kCtrRing]);
     // apply sweep j, queueing the next work list unless this was the last sweep
@@ -513,7 +661,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
       uint32_t pos = block_reserve<kTailThreads>((uint32_t)n, &cn.ntargets);
       for (int q = 0; q < n; q++) tlist[pos + q] = out[q];
     }
-    block_add3<kTailThreads>(lc, cn.cnt);
+    block_add3<kTailThreads>(lc, cn);
     if (lead) sv += nw;
     grid.sync();
     if (kMulti) {
@@ -542,13 +690,13 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
   if (!kMulti) {
     uint32_t ts = wave_sum(touched), cs = wave_sum(clamped);
     if (lane_id() == 0) {
-      if (ts) atomicAdd(&run->touched, (unsigned long long)ts);
-      if (cs) atomicAdd(&run->clamped, (unsigned long long)cs);
+      acc_add(run, kAccTouched, ts);
+      acc_add(run, kAccClamped, cs);
     }
   }
   if (lead) {
-    atomicAdd(&run->total_targets, tt);
-    atomicAdd(&run->sparse_voxels, sv);
+    acc_add(run, kAccTargets, tt);
+    acc_add(run, kAccSparse, sv);
   }
 }
 
@@ -1106,7 +1254,11 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   const unsigned g_label = col_blocks(c, nunits, occ_label);
   const unsigned g_dense = col_blocks(c, nunits, occ_dense);
   const unsigned g_dense1 = col_blocks(c, nunits, occ_dense1);
-  const unsigned g_list = (unsigned)std::max(1, c.cus * 4);
+  // list kernels are latency-bound: as many resident waves as the device holds
+  static const int occ_apply = occupancy(k_apply<T, kMulti>, kApplyThreads);
+  static const int occ_dlist = occupancy(k_detect_list<T, kMulti>, kApplyThreads);
+  const unsigned g_apply = (unsigned)std::max(1, c.cus * std::min(occ_apply, 8));
+  const unsigned g_list = (unsigned)std::max(1, c.cus * std::min(occ_dlist, 8));
   // the tail: its grid barriers need every block resident, also while the tails of
   // other contexts run -- each tail takes at most 1/(kMaxContexts+1) of what the
   // device can hold of it, and at most kMaxContexts contexts exist
@@ -1114,6 +1266,10 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   const unsigned g_tail =
       (unsigned)std::max(1, std::min(c.cus * 4, c.cus * occ_tail / (kMaxContexts + 1)));
   const unsigned g_compact = compact_blocks(c, (int64_t)job.groups);
+  const unsigned g_bits = (unsigned)std::max<int64_t>(
+      1, std::min<int64_t>(((int64_t)job.words + kBitsThreads * kBitsVec * 4 - 1) /
+                               (kBitsThreads * kBitsVec * 4),
+                           (int64_t)c.cus * 4));
   const bool sync_every = job.subdomains && job.s.mode == PMSZ_SYNC;
   const bool sync_lazy = job.subdomains && job.s.mode == PMSZ_LAZY;
   const int sx = job.sub[0], sy = job.sub[1], sz = job.sub[2];
@@ -1149,17 +1305,24 @@ int drive(Ctx &c, Job &job, Outcome &out) {
         d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, detect_tag(1));
   PMSZ_HIP(hipGetLastError());
   PMSZ_HIP(hipEventRecord(c.ev4, c.stream));
-  k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(c.tbyte.p, c.tlist.p, c.ring, 0, 1,
-                                                         true, (uint32_t)job.groups, doms, ndom,
-                                                         detect_tag(1));
-  if (kMulti)
+  if (kMulti) {
+    k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(c.tbyte.p, c.tlist.p, c.ring, 0, 1,
+                                                           true, (uint32_t)job.groups, doms,
+                                                           ndom, detect_tag(1));
     k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, 0, 0, 1, kOpDetect, job.s, 1);
+  }
 
   uint32_t j0 = 1;
   out.outer_started = 1;
   while (true) {
     for (uint32_t j = j0; j < j0 + kChain; j++) {
-      k_apply<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
+      if (!kMulti) {
+        // after a full-volume detection j: its target list
+        k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
+            c.tbyte.p, c.tlist.p, c.ring, (j - 1) % kCtrRing, j % kCtrRing, false,
+            (uint32_t)job.groups, doms, ndom, detect_tag(j));
+      }
+      k_apply<T, kMulti><<<g_apply, kApplyThreads, 0, c.stream>>>(
           d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j,
           job.s);
       if (sync_every) {
@@ -1168,17 +1331,26 @@ int drive(Ctx &c, Job &job, Outcome &out) {
         k_unpack<T><<<g_shell, kSyncThreads, 0, c.stream>>>(doms, ds, sx, sy, sz, job.ndir,
                                                             c.wmark.p, c.wlist.p, c.ring, j, 1);
       }
+      if (!kMulti)  // the work list of detection j+1 from the bitmap apply j marked
+        k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
+            c.wmark.p, (uint32_t)job.words, c.wlist.p, &c.ring[j % kCtrRing].nwork,
+            &c.ring[j % kCtrRing], kModeSparse);
       // the tags start over: no mark of an earlier sweep may carry detection j+1's
       if (detect_tag(j + 1) == 1u)
         PMSZ_HIP(hipMemsetAsync(c.tbyte.p, 0, 16 * job.groups, c.stream));
       k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
           d0, doms, ndom, ds, c.tbyte.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, (j + 1) & 1u,
           c.run, detect_tag(j + 1));
-      k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
-          c.tbyte.p, c.tlist.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false,
-          (uint32_t)job.groups, doms, ndom, detect_tag(j + 1));
+      if (kMulti)
+        k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
+            c.tbyte.p, c.tlist.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false,
+            (uint32_t)job.groups, doms, ndom, detect_tag(j + 1));
       k_detect_list<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
           d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j);
+      if (!kMulti)  // the targets of a work-list detection j+1, from the bitmap
+        k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
+            c.tmark.p, (uint32_t)job.words, c.tlist.p, &c.ring[(j + 1) % kCtrRing].ntargets,
+            &c.ring[j % kCtrRing], kModeSparse);
       if (kMulti)
         k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, j, 1, j + 1, kOpDetect,
                                                    job.s, 1);
@@ -1232,7 +1404,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
                             sizeof(Counters), hipMemcpyDeviceToHost, c.stream));
     PMSZ_HIP(hipStreamSynchronize(c.stream));
     const Counters &cc = c.h_ring[(m + 1) % kCtrRing];
-    const bool violations = (cc.cnt[0] | cc.cnt[1] | cc.cnt[2]) != 0;
+    const bool violations = (cnt_host(cc, 0) | cnt_host(cc, 1) | cnt_host(cc, 2)) != 0;
     if (!violations || !emit_ok) {
       // converged, or the cap on outer iterations stops before the next one
       out.last_j = m + 1;
@@ -1348,19 +1520,19 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
   const bool conv = o.done == 1;
   const int status = conv ? PMSZ_OK : PMSZ_NOT_CONVERGED;
   st->status = status;
-  for (int q = 0; q < 3; q++) st->final_cnt[q] = cm.cnt[q];
+  for (int q = 0; q < 3; q++) st->final_cnt[q] = cnt_host(cm, q);
   if (m <= (uint32_t)kCtrRing) {
     st->trace_n = std::min<int64_t>(m, 64);
     for (int64_t k = 0; k < st->trace_n; k++) {
       const Counters &ck = c.h_ring[(k + 1) % kCtrRing];
-      for (int q = 0; q < 3; q++) st->trace_cnt[k][q] = ck.cnt[q];
+      for (int q = 0; q < 3; q++) st->trace_cnt[k][q] = cnt_host(ck, q);
       st->trace_targets[k] = ck.ntargets;
     }
   }
-  st->total_targets = (int64_t)r.total_targets;
-  st->touched_voxels = (int64_t)r.touched;
-  st->clamped_voxels = (int64_t)r.clamped;
-  st->sparse_voxels = (int64_t)r.sparse_voxels;
+  st->total_targets = (int64_t)acc_host(r, kAccTargets);
+  st->touched_voxels = (int64_t)acc_host(r, kAccTouched);
+  st->clamped_voxels = (int64_t)acc_host(r, kAccClamped);
+  st->sparse_voxels = (int64_t)acc_host(r, kAccSparse);
   st->dense_passes = 1 + (int64_t)r.dense_passes;  // the labeling + detection sweeps
   fill_single_stats(st, p->mode, m, conv);
   // labeling: f read, code and dc written; full-volume detections: g and code
@@ -1477,7 +1649,7 @@ int correct_subdomains(Ctx &c, const pmsz_params *p, const T *F, T *G, uint8_t *
   st->t_total_s = o.ms_total * 1e-3;
   st->t_dense_s = o.ms_first * 1e-3;
   st->t_label_s = o.ms_label * 1e-3;
-  st->sparse_voxels = (int64_t)r.sparse_voxels;
+  st->sparse_voxels = (int64_t)acc_host(r, kAccSparse);
   st->dense_passes = 1 + (int64_t)r.dense_passes;
   int64_t sweeps = 0, ite = 0;
   for (uint32_t k = 0; k < nsub; k++) {
@@ -1914,7 +2086,7 @@ int pmsz_edit_symbols(const pmsz_params *p, const void *d_f, const void *d_g0, c
   if ((rc = ctx_device(c))) return rc;
   int64_t n = p->nx * p->ny * p->nz;
   const EditParams ep = edit_params(p);
-  unsigned long long *dcount = &c.run->total_targets;
+  unsigned long long *dcount = &c.run->acc[0][kAccTargets];
   PMSZ_HIP(hipMemsetAsync(dcount, 0, sizeof(unsigned long long), c.stream));
   unsigned blocks = (unsigned)((n + 255) / 256);
   if (p->dtype == PMSZ_F32)
