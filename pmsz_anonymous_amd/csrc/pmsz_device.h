@@ -165,7 +165,9 @@ struct RunState {
   uint32_t last_j;     // detections performed when done
   uint32_t err;        // bit 0: NaN or sentinel value in an input
   unsigned long long dense_passes;
-  unsigned long long pad[5];
+  uint32_t bar_epoch;  // last epoch of the tail's grid barrier in this run
+  uint32_t pad0;
+  unsigned long long pad[4];
   // by stripe: edits applied, voxels edited at least once, voxels clamped,
   // voxels re-examined by work-list sweeps
   unsigned long long acc[kStripes][4];

@@ -63,7 +63,10 @@ constexpr int kApplyThreads = 256;
 constexpr int kTailThreads = 256;
 // the tail takes over once a sweep has at most this many targets: heavier work-list
 // sweeps run as separate launches, which bring more threads to their atomics
-constexpr uint32_t kTailTargets = 65536;
+constexpr uint32_t kTailTargets = 16384;
+// blocks of the tail: its grid barriers cost about 45 ns per block (MI355X)
+constexpr int kTailBlocks = 64;
+constexpr size_t kBarBytes = sizeof(uint32_t) * 16 * (kTailBlocks + 1);
 constexpr int kSyncThreads = 256;
 constexpr uint32_t kMaxDomains = 1024;  // domain bookkeeping: one thread each
 constexpr int kMaxContexts = 8;         // live contexts, the default one included
@@ -575,6 +578,32 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
     acc_add(run, kAccSparse, nw);
 }
 
+// Grid barrier of the tail (all its blocks are resident: cooperative launch).
+// Block b publishes epoch in its own flag (release); block 0 gathers every flag
+// (acquire) and publishes the epoch in the generation word; the others wait for
+// it.  Epochs only grow within a run (the state is cleared before each run), so
+// no flag needs resetting.  bar[16 * b]: flag of block b; bar[16 * kTailBlocks]:
+// the generation.
+__device__ __forceinline__ void tail_barrier(uint32_t *bar, uint32_t epoch) {
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&bar[16 * blockIdx.x], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (blockIdx.x == 0) {
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x)
+      while (__hip_atomic_load(&bar[16 * b], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch)
+        __builtin_amdgcn_s_sleep(1);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_store(&bar[16 * kTailBlocks], epoch, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  } else if (threadIdx.x == 0) {
+    while (__hip_atomic_load(&bar[16 * kTailBlocks], __ATOMIC_ACQUIRE,
+                             __HIP_MEMORY_SCOPE_AGENT) != epoch)
+      __builtin_amdgcn_s_sleep(1);
+  }
+  __syncthreads();
+}
+
 // The tail: once a sweep's targets are few and detection j+1 would run over a
 // work list, one cooperative kernel runs the remaining sweeps -- apply, grid barrier, work-list detection,
 // grid barrier -- instead of several launches per sweep.  Every block decides
@@ -585,8 +614,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
     Dom d0, const Dom *__restrict__ doms, uint32_t shift, uint32_t ndom,
     DomState *__restrict__ ds, uint32_t *__restrict__ tmark, uint32_t *__restrict__ tlist,
     uint32_t *__restrict__ wmark, uint32_t *__restrict__ wlist, Counters *__restrict__ ring,
-    RunState *run, Sched s) {
-  cg::grid_group grid = cg::this_grid();
+    RunState *run, Sched s, uint32_t *bar) {
+  uint32_t epoch = ld_agent(&run->bar_epoch);
   __shared__ uint32_t s_act, s_cap;
   uint32_t j = ld_agent(&run->tail_j);
   if (j == 0 || ld_agent(&run->done)) return;  // uniform: written by earlier kernels
@@ -636,7 +665,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
       }
     }
     if (lead) tt += nt;
-    grid.sync();
+    tail_barrier(bar, ++epoch);
     if (!queue) break;  // sweep cap: sweep j's edits landed, no detection j+1
     // detection j+1 over the work list
     Counters &cn = ring[(j + 1) % kCtrRing];
@@ -663,7 +692,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
     }
     block_add3<kTailThreads>(lc, cn);
     if (lead) sv += nw;
-    grid.sync();
+    tail_barrier(bar, ++epoch);
     if (kMulti) {
       if (blockIdx.x == 0) {
         if (threadIdx.x == 0) {
@@ -683,7 +712,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
           cn.ncapped = s_cap;
         }
       }
-      grid.sync();
+      tail_barrier(bar, ++epoch);
     }
     j++;
   }
@@ -697,6 +726,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
   if (lead) {
     acc_add(run, kAccTargets, tt);
     acc_add(run, kAccSparse, sv);
+    run->bar_epoch = epoch;  // the next tail launch of this run continues from here
   }
 }
 
@@ -1042,6 +1072,7 @@ struct Ctx {
   Buf<DomState> ds;
   Counters *ring = nullptr;
   RunState *run = nullptr;
+  uint32_t *bar = nullptr;       // grid barrier state of the tail (kBarBytes)
   Counters *h_ring = nullptr;  // pinned mirror of the ring
   RunState *h_run = nullptr;   // pinned mirror of the run state
   DomState *h_ds = nullptr;    // pinned mirror of the domain states
@@ -1063,11 +1094,13 @@ void ctx_release(Ctx &c) {
   c.ds = Buf<DomState>();
   (void)hipFree(c.ring);
   (void)hipFree(c.run);
+  (void)hipFree(c.bar);
   if (c.h_ring) (void)hipHostFree(c.h_ring);
   if (c.h_run) (void)hipHostFree(c.h_run);
   if (c.h_ds) (void)hipHostFree(c.h_ds);
   c.ring = nullptr;
   c.run = nullptr;
+  c.bar = nullptr;
   c.h_ring = nullptr;
   c.h_run = nullptr;
   c.h_ds = nullptr;
@@ -1086,7 +1119,7 @@ size_t ctx_bytes(const Ctx &c) {
   return c.code.cap + c.dc.cap + c.snap.cap + c.fpad.cap + c.gpad.cap + c.tbyte.cap +
          4 * (c.tmark.cap + c.wmark.cap + c.tlist.cap + c.wlist.cap) +
          sizeof(Dom) * c.doms.cap + sizeof(DomState) * c.ds.cap +
-         (c.ring ? sizeof(Counters) * kCtrRing + sizeof(RunState) : 0);
+         (c.ring ? sizeof(Counters) * kCtrRing + sizeof(RunState) + kBarBytes : 0);
 }
 
 // The context of the calling thread's current device (created on first use; a
@@ -1106,6 +1139,7 @@ int ctx_device(Ctx &c) {
   PMSZ_HIP(hipEventCreate(&c.ev4));
   PMSZ_HIP(hipMalloc(&c.ring, sizeof(Counters) * kCtrRing));
   PMSZ_HIP(hipMalloc(&c.run, sizeof(RunState)));
+  PMSZ_HIP(hipMalloc(&c.bar, kBarBytes));
   PMSZ_HIP(hipHostMalloc(&c.h_ring, sizeof(Counters) * kCtrRing, hipHostMallocDefault));
   PMSZ_HIP(hipHostMalloc(&c.h_run, sizeof(RunState), hipHostMallocDefault));
   return PMSZ_OK;
@@ -1263,8 +1297,8 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   // other contexts run -- each tail takes at most 1/(kMaxContexts+1) of what the
   // device can hold of it, and at most kMaxContexts contexts exist
   static const int occ_tail = occupancy(k_tail<T, kMulti>, kTailThreads);
-  const unsigned g_tail =
-      (unsigned)std::max(1, std::min(c.cus * 4, c.cus * occ_tail / (kMaxContexts + 1)));
+  const unsigned g_tail = (unsigned)std::max(
+      1, std::min(kTailBlocks, c.cus * occ_tail / (kMaxContexts + 1)));
   const unsigned g_compact = compact_blocks(c, (int64_t)job.groups);
   const unsigned g_bits = (unsigned)std::max<int64_t>(
       1, std::min<int64_t>(((int64_t)job.words + kBitsThreads * kBitsVec * 4 - 1) /
@@ -1286,6 +1320,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   PMSZ_HIP(hipMemsetAsync(c.tbyte.p, 0, 16 * job.groups, c.stream));
   PMSZ_HIP(hipMemsetAsync(c.ring, 0, sizeof(Counters) * kCtrRing, c.stream));
   PMSZ_HIP(hipMemsetAsync(c.run, 0, sizeof(RunState), c.stream));
+  PMSZ_HIP(hipMemsetAsync(c.bar, 0, kBarBytes, c.stream));
   // apply(1) reads slot 0's mode: mark detection 1 as a running sweep
   static const uint32_t kRunning = kModeDense;
   PMSZ_HIP(hipMemcpyAsync(&c.ring[0].mode, &kRunning, sizeof(uint32_t), hipMemcpyHostToDevice,
@@ -1365,8 +1400,9 @@ int drive(Ctx &c, Job &job, Outcome &out) {
       Counters *a_ring = c.ring;
       RunState *a_run = c.run;
       Sched a_s = job.s;
+      uint32_t *a_bar = c.bar;
       void *args[] = {&a_d0,    &a_doms,  &a_shift, &a_ndom, &a_ds,  &a_tmark,
-                      &a_tlist, &a_wmark, &a_wlist, &a_ring, &a_run, &a_s};
+                      &a_tlist, &a_wmark, &a_wlist, &a_ring, &a_run, &a_s, &a_bar};
       PMSZ_HIP(hipLaunchCooperativeKernel((const void *)k_tail<T, kMulti>, dim3(g_tail),
                                           dim3(kTailThreads), args, 0, c.stream));
     }
