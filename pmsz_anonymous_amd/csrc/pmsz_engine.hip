@@ -10,12 +10,14 @@
 //   dc          u8[n]      delta counters (delta_counter, pMSz.cu:173)
 //   snap        u8[n]      subdomains: the dc of the 2-layer shell at a ghost sync
 // Per run:
-//   tmark       u32[]      targets of the running sweep, one bit per voxel id: the
+//   tbyte       u8[]       target byte map of full-volume detections: detection
+//                          jd marks each target cell with its tag (1..255)
+//   tmark       u32[]      targets of a work-list sweep, one bit per voxel id: the
 //                          per-sweep edit dedup (edits[] + atomicCAS, pMSz.cu:795)
-//   wmark       u32[]      voxels queued for the next sweep
-//   tlist/wlist u32[]      the same two sets as compact lists of voxel ids
-//   ring        Counters[] per-sweep counters and the schedule decisions
-//   run         RunState   phase hand-over between kernels, run totals
+//   wmark       u32[]      voxels queued for the next work-list sweep
+//   tlist/wlist u32[]      the same sets as lists of voxel ids
+//   ring        Counters[] per-sweep counters (striped) and schedule decisions
+//   run         RunState   phase hand-over between kernels, run totals (striped)
 //   doms, ds    Dom[], DomState[]  per-domain description and state
 // A voxel id is (domain << shift) | local index; a single field is domain 0 with
 // id = index.
@@ -24,16 +26,19 @@
 // marks, lists and counters; a separate phase then applies one edit per target
 // (applyDeltaBuffer_local, pMSz.cu:623-645).
 //
-// Schedule.  Detection 1 is one full pass fused with the original labeling.
-// After that, the apply of sweep j decides on the device how detection j+1 runs:
-//   - over every voxel (column kernel) while the targets are dense,
+// Schedule.  The original labeling and detection 1 are full passes
+// (pmsz_dense.h).  After that, the apply of sweep j decides on the device how
+// detection j+1 runs:
+//   - over every voxel (column kernel, targets to tbyte, then compacted) while
+//     the targets are dense;
 //   - over the closed stencil neighbourhoods of sweep j's targets otherwise: a
 //     voxel's classification depends only on g over its closed neighbourhood and
 //     the stencil is symmetric, so no other voxel can change -- the counters and
 //     targets equal those of a full sweep;
-//   - in a single-workgroup tail kernel once the targets are few, where a sweep
-//     costs workgroup barriers instead of kernel launches.
-// The host enqueues the chain without waiting and synchronises once per chain.
+//   - in the tail kernel once the targets are few: one cooperative launch runs
+//     the remaining sweeps with grid barriers instead of kernel launches.
+// The host enqueues a chain of sweeps without waiting and synchronises once per
+// chain.
 //
 // Several domains run in lockstep in the same launches.  Each keeps its own
 // counters and flags: it takes part in detections while its last detection
@@ -49,15 +54,12 @@
 #include <mutex>
 #include <vector>
 
-#include <hip/hip_cooperative_groups.h>
-
 #include "../../include/pmsz.h"
 #include "pmsz_dense.h"
 #include "pmsz_device.h"
 
 namespace pmsz {
 
-namespace cg = cooperative_groups;
 
 constexpr int kApplyThreads = 256;
 constexpr int kTailThreads = 256;
@@ -981,18 +983,35 @@ __global__ void k_init_pointers(const uint8_t *__restrict__ code, int32_t *__res
   desc[i] = (int32_t)(i + rank_off(c >> 4, d));
 }
 
-__global__ void k_compress(int32_t *lab, Grid3 d) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
-  int32_t cur = lab[i];
-  // chains strictly increase in the (value, index) order, so they end at a root
-  // within N steps; the cap only guards against a corrupted pointer field
-  for (int64_t step = 0; step < d.n; step++) {
-    int32_t nx = lab[cur];
-    if (nx == cur) break;
-    cur = nx;
+// Path compression by pointer jumping (the label of a voxel is the root of its
+// pointer chain, PathCompression, pMSz.cu:255-271): in each round every voxel's
+// pointer becomes its pointer's pointer, so chains of length L settle in about
+// log2(L) rounds.  Two label fields (ascending, descending) share each round:
+// threads [0, n) take lab0, [n, 2n) lab1.  A wave whose 64 voxels all point at
+// roots (ballot) sets its bit in `settled` and later rounds skip it; a wave that
+// still moved a pointer sets *changed.  Updates in place are safe: a read of a
+// pointer being rewritten sees either value, and both are ancestors.
+__global__ void k_jump(int32_t *__restrict__ lab0, int32_t *__restrict__ lab1, int64_t n,
+                       uint32_t *__restrict__ settled, uint32_t *__restrict__ changed) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t w = (uint64_t)t >> 6;  // wave index (blocks are whole waves)
+  if ((settled[w >> 5] >> (w & 31u)) & 1u) return;  // uniform over the wave
+  bool moved = false;
+  if (t < 2 * n) {
+    int32_t *lab = t < n ? lab0 : lab1;
+    const int64_t i = t < n ? t : t - n;
+    const int32_t p = lab[i];
+    const int32_t q = lab[p];
+    if (q != p) {
+      lab[i] = q;
+      moved = true;
+    }
   }
-  lab[i] = cur;
+  const uint64_t mv = __ballot(moved);
+  if (lane_id() == 0) {
+    if (mv == 0) atomicOr(&settled[w >> 5], 1u << (w & 31u));
+    else *changed = 1u;
+  }
 }
 
 // extract_edits_kernel (pMSz.cu:2090-2138).
@@ -2098,13 +2117,39 @@ int pmsz_labels(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_fie
   Ctx &c = default_ctx();
   std::lock_guard<std::mutex> lk(c.mu);
   const Grid3 d = grid_of(nx, ny, nz);
-  unsigned blocks = (unsigned)((n + 255) / 256);
-  k_init_pointers<<<blocks, 256, 0, c.stream>>>(codes, d_asc, d_desc, d);
-  k_compress<<<blocks, 256, 0, c.stream>>>(d_asc, d);
-  k_compress<<<blocks, 256, 0, c.stream>>>(d_desc, d);
-  hipError_t e = hipGetLastError();
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  const unsigned jblocks = (unsigned)((2 * n + 255) / 256);
+  const int64_t waves = (2 * n + 63) / 64;
+  const size_t sbytes = sizeof(uint32_t) * (size_t)((waves + 31) / 32 + 1);
+  uint32_t *settled = nullptr;
+  if (hipMalloc(&settled, sbytes + sizeof(uint32_t)) != hipSuccess) {
+    (void)hipFree(codes);
+    return PMSZ_ENOMEM;
+  }
+  uint32_t *changed = settled + sbytes / sizeof(uint32_t);
+  hipError_t e = hipMemsetAsync(settled, 0, sbytes, c.stream);
+  if (e == hipSuccess) {
+    k_init_pointers<<<blocks, 256, 0, c.stream>>>(codes, d_asc, d_desc, d);
+    // rounds in batches of 4; the batch's last round tells whether anything moved
+    // (a chain is shorter than n < 2^31, so 32 rounds settle any field)
+    bool settled_all = false;
+    for (int round = 0; round < 36 && e == hipSuccess && !settled_all; round += 4) {
+      for (int k = 0; k < 3; k++)
+        k_jump<<<jblocks, 256, 0, c.stream>>>(d_asc, d_desc, n, settled, changed);
+      e = hipMemsetAsync(changed, 0, sizeof(uint32_t), c.stream);
+      k_jump<<<jblocks, 256, 0, c.stream>>>(d_asc, d_desc, n, settled, changed);
+      uint32_t h = 1;
+      if (e == hipSuccess) e = hipGetLastError();
+      if (e == hipSuccess) e = hipMemcpyAsync(&h, changed, sizeof(h), hipMemcpyDeviceToHost, c.stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+      settled_all = e == hipSuccess && h == 0;
+    }
+    if (e == hipSuccess && !settled_all) e = hipErrorUnknown;  // a pointer field with a cycle
+  }
+  if (e == hipSuccess) e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
   (void)hipFree(codes);
+  (void)hipFree(settled);
   if (e != hipSuccess) {
     last_hip_error() = e;
     return PMSZ_EDEVICE;
