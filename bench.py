@@ -369,6 +369,19 @@ def main():
                          "bytes_per_launch": nvox * LABEL_BYTES_PER_VOXEL},
         },
     }
+    # End to end against the reference algorithm's own schedule: the labeling
+    # pass plus one full-volume detection per detection of the loop (read g and
+    # the code: s + 1 bytes per voxel each), plus 3s + 2 bytes per edit (read f,
+    # g, dc; write g, dc).  This build skips most of that work (work-list sweeps),
+    # so the figure is what the reference's dense schedule would have to move.
+    s_bytes = 4
+    detections = s0["edit_sweeps"] + 1
+    e2e_bytes = nvox * (s_bytes + 1) * (1 + detections) + s0["total_targets"] * (3 * s_bytes + 2)
+    e2e = e2e_bytes / (elapsed / args.steps) / 1e9
+    res["end_to_end"] = {
+        "definition": "(N*(s+1)*(1+detections) + edits*(3s+2)) / time per correction",
+        "detections": detections, "algorithmic_bytes": e2e_bytes,
+        "achieved": e2e, "unit": "GB/s", "frac": e2e / HBM_PEAK_GBPS}
     if not args.no_batch:
         res["batched"] = batched_vs_plain()
     if not args.no_serve:

@@ -195,6 +195,7 @@ __device__ __forceinline__ double kmin(double a, double b) {
 constexpr int kCompactThreads = 256;
 constexpr int kCompactPT = 64;     // groups per thread per step
 constexpr int kCompactBatch = 16;  // loads in flight per thread
+constexpr int kCompactDoms = 1024;  // most domains of a run
 __device__ __forceinline__ uint32_t tag_bytes(uint32_t w, uint32_t rep) {
   // 0x01 in each byte of w equal to the tag (rep = tag * 0x01010101), else 0
   const uint32_t x = w ^ rep;
@@ -210,6 +211,15 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(
     uint32_t jslot_prev, uint32_t jslot, bool always, uint32_t ngroups,
     const Dom *__restrict__ doms, uint32_t ndom, uint32_t tag) {
   if (!always && ring[jslot_prev].mode != kModeDense) return;  // uniform
+  // the domains' first cells and ids, for the cell -> id mapping (ndom <= 1024)
+  __shared__ uint32_t s_cb[kCompactDoms], s_gb[kCompactDoms];
+  if (ndom > 1) {
+    for (uint32_t k = threadIdx.x; k < ndom; k += kCompactThreads) {
+      s_cb[k] = doms[k].cbase;
+      s_gb[k] = doms[k].gbase;
+    }
+    __syncthreads();
+  }
   uint32_t *counter = &ring[jslot].ntargets;
   const uint4 *g16 = reinterpret_cast<const uint4 *>(tbyte);
   const uint32_t rep = tag * 0x01010101u;
@@ -246,10 +256,10 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(
           uint32_t lo = 0, hi = ndom;
           while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) / 2;
-            if (doms[mid].cbase <= cell) lo = mid;
+            if (s_cb[mid] <= cell) lo = mid;
             else hi = mid;
           }
-          id = doms[lo].gbase + (cell - doms[lo].cbase);
+          id = s_gb[lo] + (cell - s_cb[lo]);
         }
         tlist[pos++] = id;
       }

@@ -66,6 +66,7 @@ struct Dom {
   uint32_t padded;    // 1: cells holding the sentinel are missing neighbours
   uint32_t gbase;     // domain << shift
   uint32_t cbase;     // offset of the block's first cell in the run's target byte map
+  uint32_t wpre;      // bitmap words of the domains before this one (gbase is 32-aligned)
   EditParams ep;      // bound, step, q and clamp mode of this domain's edits
 };
 

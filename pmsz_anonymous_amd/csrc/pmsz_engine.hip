@@ -71,6 +71,7 @@ constexpr int kTailBlocks = 64;
 constexpr size_t kBarBytes = sizeof(uint32_t) * 16 * (kTailBlocks + 1);
 constexpr int kSyncThreads = 256;
 constexpr uint32_t kMaxDomains = 1024;  // domain bookkeeping: one thread each
+static_assert(kMaxDomains <= (uint32_t)kCompactDoms, "compaction domain table");
 constexpr int kMaxContexts = 8;         // live contexts, the default one included
 
 // Stencil directions of the reference in its order d = 0..13 (directions_host,
@@ -254,12 +255,12 @@ __device__ __forceinline__ int queue_neighbourhood(uint32_t t, int64_t x, int64_
   return n;
 }
 
-// Marks the classified voxels of the closed stencil neighbourhood of cell t of a
-// single domain in wmark, without looking at the old bits: the stencil's 7 rows
-// of consecutive x take one atomic OR each (two where a row crosses a word).
-__device__ __forceinline__ void mark_neighbourhood(uint32_t t, const Dom &D, uint32_t *wmark) {
-  int64_t x, y, z;
-  decompose(t, D.d, x, y, z);
+// Marks the classified voxels of the closed stencil neighbourhood of local cell
+// t = (x, y, z) of domain D in wmark, without looking at the old bits: the
+// stencil's 7 rows of consecutive x take one atomic OR each (two where a row
+// crosses a word; ids of a domain are consecutive from the 32-aligned D.gbase).
+__device__ __forceinline__ void mark_neighbourhood(uint32_t t, int64_t x, int64_t y, int64_t z,
+                                                   const Dom &D, uint32_t *wmark) {
   const uint32_t vm = box_mask(x, y, z, D.lo, D.hi);
   constexpr int kRowFirst[7] = {0, 2, 4, 6, 9, 11, 13};  // first rank of each row
   constexpr int kRowLen[7] = {2, 2, 2, 3, 2, 2, 2};
@@ -267,85 +268,103 @@ __device__ __forceinline__ void mark_neighbourhood(uint32_t t, const Dom &D, uin
   for (int k = 0; k < 7; k++) {
     const uint32_t rb = (vm >> kRowFirst[k]) & ((1u << kRowLen[k]) - 1u);
     if (!rb) continue;
-    const uint32_t v0 = (uint32_t)((int64_t)t + rank_off(kRowFirst[k], D.d));
+    const uint32_t v0 = D.gbase + (uint32_t)((int64_t)t + rank_off(kRowFirst[k], D.d));
     const uint64_t m = (uint64_t)rb << (v0 & 31u);
     atomicOr(&wmark[v0 >> 5], (uint32_t)m);
     if (m >> 32) atomicOr(&wmark[(v0 >> 5) + 1], (uint32_t)(m >> 32));
   }
 }
+__device__ __forceinline__ void mark_neighbourhood(uint32_t t, const Dom &D, uint32_t *wmark) {
+  int64_t x, y, z;
+  decompose(t, D.d, x, y, z);
+  mark_neighbourhood(t, x, y, z, D, wmark);
+}
 
-// Detection at one listed voxel of a single domain; its targets are marked in
-// tmark without looking at the old bits (the list is compacted afterwards).
+// Detection at one listed voxel (global loads); its targets are marked in tmark
+// without looking at the old bits (the list is compacted afterwards).  In a
+// padded block a neighbour holding the sentinel is missing (pMSz.cu:709); a
+// domain that must not edit counts only.
 template <typename T>
 __device__ __forceinline__ void detect_mark(const Dom &D, uint32_t *tmark, uint32_t local,
-                                            uint32_t (&lc)[3]) {
+                                            bool padded, bool emit, uint32_t (&lc)[3]) {
   const T *g = (const T *)D.g;
   int64_t x, y, z;
   decompose(local, D.d, x, y, z);
-  const uint32_t vm = valid_mask(x, y, z, D.d);
+  uint32_t vm = valid_mask(x, y, z, D.d);
   T cg[kCand];
 #pragma unroll
   for (int r = 0; r < kCand; r++)
     cg[r] = ((vm >> r) & 1u) ? g[(int64_t)local + rank_off(r, D.d)] : (T)0;
+  if (padded) {
+#pragma unroll
+    for (int r = 0; r < kCand; r++)
+      if (cg[r] == sentinel<T>()) vm &= ~(1u << r);
+  }
   int da, dd;
   if (vm == 0x7fffu) scan_full(cg, da, dd);
   else scan_masked(cg, vm, da, dd);
   const uint8_t oc = D.code[local];
   int tr[4];
   const int nt = detect_targets(oc & 15, oc >> 4, da, dd, tr, lc);
+  if (!emit) return;
   for (int q = 0; q < nt; q++) {
-    const uint32_t v = (uint32_t)((int64_t)local + rank_off(tr[q], D.d));
+    const uint32_t v = D.gbase + (uint32_t)((int64_t)local + rank_off(tr[q], D.d));
     atomicOr(&tmark[v >> 5], 1u << (v & 31u));
   }
 }
 
-// List of the set bits of a bitmap (ascending within each thread's words),
-// each nonzero word cleared as it is read; runs when gate->mode == need.  A
-// block takes 16K words per step with one list reservation.
+// List of the set bits of a bitmap (ids), each nonzero word cleared as it is
+// read; runs when gate->mode == need.  Only the words that hold ids are read:
+// the domains' word ranges, back to back (dense index u; domain k covers
+// [wpre_k, wpre_k + ceil(n_k / 32)) and starts at word gbase_k / 32).  A thread
+// takes kBitsPT words per step, interleaved across the block so loads coalesce;
+// a block makes one list reservation per step.
 constexpr int kBitsThreads = 256;
-constexpr int kBitsVec = 16;  // 16-byte vectors per thread per step
+constexpr int kBitsPT = 64;
 __global__ __launch_bounds__(kBitsThreads) void k_compact_bits(
-    uint32_t *__restrict__ bits, uint32_t nwords, uint32_t *__restrict__ list,
-    uint32_t *__restrict__ count, const Counters *__restrict__ gate, uint32_t need) {
+    uint32_t *__restrict__ bits, const Dom *__restrict__ doms, uint32_t ndom, uint32_t wdense,
+    uint32_t *__restrict__ list, uint32_t *__restrict__ count, const Counters *__restrict__ gate,
+    uint32_t need) {
   if (gate->mode != need) return;  // uniform
-  uint4 *b4 = reinterpret_cast<uint4 *>(bits);
-  const uint32_t per_step = kBitsThreads * kBitsVec * 4;
-  for (uint32_t w0 = blockIdx.x * per_step; w0 < nwords; w0 += gridDim.x * per_step) {
-    uint4 q[kBitsVec];
+  __shared__ uint32_t s_wp[kCompactDoms], s_wb[kCompactDoms];
+  if (ndom > 1) {
+    for (uint32_t k = threadIdx.x; k < ndom; k += kBitsThreads) {
+      s_wp[k] = doms[k].wpre;
+      s_wb[k] = doms[k].gbase >> 5;
+    }
+    __syncthreads();
+  }
+  auto word_of = [&](uint32_t u) -> uint32_t {  // dense index -> bitmap word
+    if (ndom <= 1) return u;
+    uint32_t lo = 0, hi = ndom;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (s_wp[mid] <= u) lo = mid;
+      else hi = mid;
+    }
+    return s_wb[lo] + (u - s_wp[lo]);
+  };
+  const uint32_t per_step = kBitsThreads * kBitsPT;
+  for (uint32_t u0 = blockIdx.x * per_step; u0 < wdense; u0 += gridDim.x * per_step) {
+    uint32_t w[kBitsPT];
     uint32_t n = 0;
 #pragma unroll
-    for (int v = 0; v < kBitsVec; v++) {
-      const uint32_t wi = w0 + (uint32_t)(v * kBitsThreads + threadIdx.x) * 4u;
-      if (wi + 3 < nwords) {
-        q[v] = b4[wi / 4];
-      } else {
-        q[v].x = wi < nwords ? bits[wi] : 0u;
-        q[v].y = wi + 1 < nwords ? bits[wi + 1] : 0u;
-        q[v].z = wi + 2 < nwords ? bits[wi + 2] : 0u;
-        q[v].w = 0u;
-      }
-      n += __popc(q[v].x) + __popc(q[v].y) + __popc(q[v].z) + __popc(q[v].w);
+    for (int v = 0; v < kBitsPT; v++) {
+      const uint32_t u = u0 + (uint32_t)v * kBitsThreads + threadIdx.x;
+      w[v] = u < wdense ? bits[word_of(u)] : 0u;
+      n += __popc(w[v]);
     }
     uint32_t pos = block_reserve<kBitsThreads>(n, count);
 #pragma unroll
-    for (int v = 0; v < kBitsVec; v++) {
-      const uint32_t wi = w0 + (uint32_t)(v * kBitsThreads + threadIdx.x) * 4u;
-      const uint32_t ws[4] = {q[v].x, q[v].y, q[v].z, q[v].w};
-      if ((ws[0] | ws[1] | ws[2] | ws[3]) == 0) continue;
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        uint32_t m = ws[c];
-        while (m) {
-          const uint32_t b = (uint32_t)(__ffs(m) - 1);
-          m &= m - 1;
-          list[pos++] = (wi + (uint32_t)c) * 32u + b;
-        }
-      }
-      if (wi + 3 < nwords) {
-        b4[wi / 4] = make_uint4(0u, 0u, 0u, 0u);
-      } else {
-        for (uint32_t c = 0; c < 3; c++)
-          if (wi + c < nwords) bits[wi + c] = 0u;
+    for (int v = 0; v < kBitsPT; v++) {
+      uint32_t m = w[v];
+      if (!m) continue;
+      const uint32_t wi = word_of(u0 + (uint32_t)v * kBitsThreads + threadIdx.x);
+      bits[wi] = 0u;
+      while (m) {
+        const uint32_t b = (uint32_t)(__ffs(m) - 1);
+        m &= m - 1;
+        list[pos++] = wi * 32u + b;
       }
     }
   }
@@ -398,7 +417,7 @@ __device__ __forceinline__ void zero_counters(Counters *c) {
 template <typename T, bool kMulti>
 __device__ __forceinline__ int apply_target(uint32_t t, const Dom &d0, const Dom *doms,
                                             uint32_t shift, DomState *ds, uint32_t *tmark,
-                                            uint32_t *wmark, bool queue, uint32_t &dk,
+                                            uint32_t *wmark, bool queue, bool mark, uint32_t &dk,
                                             uint32_t &touched, uint32_t &clamped,
                                             uint32_t (&u)[kCand]) {
   uint32_t local;
@@ -415,6 +434,7 @@ __device__ __forceinline__ int apply_target(uint32_t t, const Dom &d0, const Dom
     if (x <= 1 || x >= D.d.nx - 2 || y <= 1 || y >= D.d.ny - 2 || z <= 1 || z >= D.d.nz - 2)
       if (!(ld_agent(&ds[dk].flags) & kDomSend)) atomicOr(&ds[dk].flags, kDomSend);
   }
+  if (mark) mark_neighbourhood(local, x, y, z, D, wmark);
   return queue ? queue_neighbourhood(local, x, y, z, D, wmark, u) : 0;
 }
 
@@ -462,7 +482,20 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
       apply_one<T>((T *)d0.g, (const T *)d0.f, d0.dc, t, d0.ep, touched, clamped);
       mark_neighbourhood(t, d0, wmark);
     }
-  } else if (!kMulti) {
+  } else if (kMulti) {
+    // several domains: the edit (and the ghost-sync flag), the neighbourhood's
+    // bits in wmark when detection j+1 runs over a work list
+    for (uint32_t base = blockIdx.x * kApplyThreads; base < nt; base += stride) {
+      const uint32_t i = base + threadIdx.x;
+      uint32_t u[kCand];
+      uint32_t dk = 0;
+      const bool has = i < nt;
+      if (has)
+        apply_target<T, kMulti>(tlist[i], d0, doms, shift, ds, nullptr, wmark, false, queue, dk,
+                                touched, clamped, u);
+      dom_add_targets(has, dk, ds);
+    }
+  } else {
     // edits only (the targets of a full-volume detection, no work list): four
     // targets per thread, all their loads issued before any edit is stored
     constexpr int kU = 4;
@@ -497,21 +530,6 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
           clamped += (cv[q] == d0.ep.q && c0 != d0.ep.q);
         }
       }
-    }
-  } else
-  for (uint32_t base = blockIdx.x * kApplyThreads; base < nt; base += stride) {
-    const uint32_t i = base + threadIdx.x;
-    uint32_t u[kCand];
-    int n = 0;
-    uint32_t dk = 0;
-    const bool has = i < nt;
-    if (has)
-      n = apply_target<T, kMulti>(tlist[i], d0, doms, shift, ds, tmark, wmark, queue, dk,
-                                  touched, clamped, u);
-    if (kMulti) dom_add_targets(has, dk, ds);
-    if (queue) {
-      uint32_t pos = block_reserve<kApplyThreads>((uint32_t)n, &cj.nwork);
-      for (int q = 0; q < n; q++) wlist[pos + q] = u[q];
     }
   }
   if (!kMulti) {
@@ -552,28 +570,27 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
   const uint32_t nw = cj.nwork;
   uint32_t lc[3] = {0, 0, 0};
   const uint32_t stride = gridDim.x * kApplyThreads;
+  // targets go to tmark bits, compacted into tlist afterwards
   if (!kMulti) {
-    // one domain: targets go to tmark bits, compacted into tlist afterwards
     for (uint32_t i = blockIdx.x * kApplyThreads + threadIdx.x; i < nw; i += stride)
-      detect_mark<T>(d0, tmark, wlist[i], lc);
-  } else
-  for (uint32_t base = blockIdx.x * kApplyThreads; base < nw; base += stride) {
-    const uint32_t i = base + threadIdx.x;
-    uint32_t out[4];
-    uint32_t vc[3] = {0, 0, 0};
-    uint32_t dk = 0;
-    bool has = false;
-    int n = 0;
-    if (i < nw) {
-      const uint32_t v = wlist[i];
-      wmark[v >> 5] = 0u;  // every set bit is a listed voxel of this sweep
-      n = detect_listed<T, kMulti>(v, d0, doms, shift, ds, tmark, dk, has, vc, out);
-    }
-    if (kMulti) dom_add3(has, dk, vc, ds, par);
+      detect_mark<T>(d0, tmark, wlist[i], false, true, lc);
+  } else {
+    for (uint32_t base = blockIdx.x * kApplyThreads; base < nw; base += stride) {
+      const uint32_t i = base + threadIdx.x;
+      uint32_t vc[3] = {0, 0, 0};
+      uint32_t dk = 0;
+      bool has = false;
+      if (i < nw) {
+        uint32_t local;
+        const Dom &D = dom_of<kMulti>(wlist[i], d0, doms, shift, dk, local);
+        const uint32_t fl = ds[dk].flags;
+        has = (fl & kDomActive) != 0;
+        if (has) detect_mark<T>(D, tmark, local, D.padded != 0, (fl & kDomEmit) != 0, vc);
+      }
+      dom_add3(has, dk, vc, ds, par);
 #pragma unroll
-    for (int q = 0; q < 3; q++) lc[q] += vc[q];
-    uint32_t pos = block_reserve<kApplyThreads>((uint32_t)n, &cn.ntargets);
-    for (int q = 0; q < n; q++) tlist[pos + q] = out[q];
+      for (int q = 0; q < 3; q++) lc[q] += vc[q];
+    }
   }
   block_add3<kApplyThreads>(lc, cn);
   if (blockIdx.x == 0 && threadIdx.x == 0)
@@ -658,7 +675,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
       uint32_t dk = 0;
       const bool has = i < nt;
       if (has)
-        n = apply_target<T, kMulti>(tlist[i], d0, doms, shift, ds, tmark, wmark, queue, dk,
+        n = apply_target<T, kMulti>(tlist[i], d0, doms, shift, ds, tmark, wmark, queue, false, dk,
                                     touched, clamped, u);
       if (kMulti) dom_add_targets(has, dk, ds);
       if (queue) {
@@ -874,8 +891,6 @@ __global__ __launch_bounds__(kSyncThreads) void k_unpack(const Dom *__restrict__
   for (uint32_t base = blockIdx.x * kSyncThreads; base < D.shell_n;
        base += gridDim.x * kSyncThreads) {
     const uint32_t i = base + threadIdx.x;
-    uint32_t u[kCand];
-    int n = 0;
     if (i < D.shell_n) {
       uint32_t x, y, z;
       shell_cell(i, P0, P1, P2, x, y, z);
@@ -921,13 +936,10 @@ __global__ __launch_bounds__(kSyncThreads) void k_unpack(const Dom *__restrict__
         if (changed) {
           g[li] = gv;
           D.dc[li] = cur;
-          if (queue) n = queue_neighbourhood(li, x, y, z, D, wmark, u);
+          // the next work list is compacted from wmark
+          if (queue) mark_neighbourhood(li, x, y, z, D, wmark);
         }
       }
-    }
-    if (queue) {
-      uint32_t pos = block_reserve<kSyncThreads>((uint32_t)n, &cj.nwork);
-      for (int q = 0; q < n; q++) wlist[pos + q] = u[q];
     }
   }
   // send_flag = 0 after the exchange (pMSz.cu:2000-2007); the pack has read it
@@ -947,27 +959,43 @@ __global__ __launch_bounds__(kSyncThreads) void k_finish(const Dom *__restrict__
   const Dom &D = doms[k];
   const uint32_t bx = (uint32_t)(D.hi[0] - D.lo[0]), by = (uint32_t)(D.hi[1] - D.lo[1]),
                  bz = (uint32_t)(D.hi[2] - D.lo[2]);
-  const uint32_t n = bx * by * bz;
+  const uint32_t rows = by * bz;
+  const uint8_t q = (uint8_t)D.ep.q;
   uint32_t touched = 0, clamped = 0;
-  for (uint32_t i = blockIdx.x * kSyncThreads + threadIdx.x; i < n;
-       i += gridDim.x * kSyncThreads) {
-    const uint32_t x = i % bx, y = (i / bx) % by, z = i / (bx * by);
-    const int64_t li = (int64_t)(x + D.lo[0]) +
+  // one x row per block step (coalesced), rows dealt grid-stride
+  for (uint32_t r = blockIdx.x; r < rows; r += gridDim.x) {
+    const uint32_t y = r % by, z = r / by;  // uniform
+    const int64_t l0 = (int64_t)D.lo[0] +
                        D.d.nx * ((int64_t)(y + D.lo[1]) + D.d.ny * (int64_t)(z + D.lo[2]));
-    const uint8_t c = D.dc[li];
-    touched += c > 0;
-    clamped += c == D.ep.q;
-    if (G) {
-      const int64_t gi = (int64_t)(x + D.off[0]) + gnx * ((int64_t)(y + D.off[1]) +
-                                                          gny * (int64_t)(z + D.off[2]));
-      G[gi] = ((const T *)D.g)[li];
-      if (DC) DC[gi] = c;
+    const int64_t g0 = G ? (int64_t)D.off[0] + gnx * ((int64_t)(y + D.off[1]) +
+                                                       gny * (int64_t)(z + D.off[2]))
+                         : 0;
+    for (uint32_t x = threadIdx.x; x < bx; x += kSyncThreads) {
+      const uint8_t c = D.dc[l0 + x];
+      touched += c > 0;
+      clamped += c == q;
+      if (G) {
+        G[g0 + x] = ((const T *)D.g)[l0 + x];
+        if (DC) DC[g0 + x] = c;
+      }
     }
   }
+  // one pair of atomics per block
+  __shared__ uint32_t s_t[kSyncThreads / 64], s_c[kSyncThreads / 64];
   const uint32_t ts = wave_sum(touched), cs = wave_sum(clamped);
   if (lane_id() == 0) {
-    if (ts) atomicAdd(&ds[k].touched, (unsigned long long)ts);
-    if (cs) atomicAdd(&ds[k].clamped, (unsigned long long)cs);
+    s_t[threadIdx.x / 64] = ts;
+    s_c[threadIdx.x / 64] = cs;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t a = 0, c = 0;
+    for (int w = 0; w < kSyncThreads / 64; w++) {
+      a += s_t[w];
+      c += s_c[w];
+    }
+    if (a) atomicAdd(&ds[k].touched, (unsigned long long)a);
+    if (c) atomicAdd(&ds[k].clamped, (unsigned long long)c);
   }
 }
 
@@ -1270,6 +1298,7 @@ constexpr uint32_t kChain = 8;  // apply/detect steps enqueued before each host 
 struct Job {
   std::vector<Dom> doms;
   uint32_t shift = 32;     // id = (domain << shift) | local (multi-domain runs)
+  uint64_t wdense = 0;     // bitmap words holding ids, over all domains
   uint64_t words = 0;      // bitmap words
   uint64_t groups = 0;     // 16-byte groups of the target byte map
   uint64_t list_cap = 0;   // list entries (distinct cells)
@@ -1320,8 +1349,8 @@ int drive(Ctx &c, Job &job, Outcome &out) {
       1, std::min(kTailBlocks, c.cus * occ_tail / (kMaxContexts + 1)));
   const unsigned g_compact = compact_blocks(c, (int64_t)job.groups);
   const unsigned g_bits = (unsigned)std::max<int64_t>(
-      1, std::min<int64_t>(((int64_t)job.words + kBitsThreads * kBitsVec * 4 - 1) /
-                               (kBitsThreads * kBitsVec * 4),
+      1, std::min<int64_t>(((int64_t)job.wdense + kBitsThreads * kBitsPT - 1) /
+                               (kBitsThreads * kBitsPT),
                            (int64_t)c.cus * 4));
   const bool sync_every = job.subdomains && job.s.mode == PMSZ_SYNC;
   const bool sync_lazy = job.subdomains && job.s.mode == PMSZ_LAZY;
@@ -1385,10 +1414,11 @@ int drive(Ctx &c, Job &job, Outcome &out) {
         k_unpack<T><<<g_shell, kSyncThreads, 0, c.stream>>>(doms, ds, sx, sy, sz, job.ndir,
                                                             c.wmark.p, c.wlist.p, c.ring, j, 1);
       }
-      if (!kMulti)  // the work list of detection j+1 from the bitmap apply j marked
-        k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
-            c.wmark.p, (uint32_t)job.words, c.wlist.p, &c.ring[j % kCtrRing].nwork,
-            &c.ring[j % kCtrRing], kModeSparse);
+      // the work list of detection j+1 from the bitmap that apply j (and a ghost
+      // sync) marked
+      k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
+          c.wmark.p, doms, ndom, (uint32_t)job.wdense, c.wlist.p, &c.ring[j % kCtrRing].nwork,
+          &c.ring[j % kCtrRing], kModeSparse);
       // the tags start over: no mark of an earlier sweep may carry detection j+1's
       if (detect_tag(j + 1) == 1u)
         PMSZ_HIP(hipMemsetAsync(c.tbyte.p, 0, 16 * job.groups, c.stream));
@@ -1401,10 +1431,10 @@ int drive(Ctx &c, Job &job, Outcome &out) {
             (uint32_t)job.groups, doms, ndom, detect_tag(j + 1));
       k_detect_list<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
           d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j);
-      if (!kMulti)  // the targets of a work-list detection j+1, from the bitmap
-        k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
-            c.tmark.p, (uint32_t)job.words, c.tlist.p, &c.ring[(j + 1) % kCtrRing].ntargets,
-            &c.ring[j % kCtrRing], kModeSparse);
+      // the targets of a work-list detection j+1, from the bitmap
+      k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
+          c.tmark.p, doms, ndom, (uint32_t)job.wdense, c.tlist.p,
+          &c.ring[(j + 1) % kCtrRing].ntargets, &c.ring[j % kCtrRing], kModeSparse);
       if (kMulti)
         k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, j, 1, j + 1, kOpDetect,
                                                    job.s, 1);
@@ -1449,10 +1479,16 @@ int drive(Ctx &c, Job &job, Outcome &out) {
                                                       c.ring, m, 0);
     k_unpack<T><<<g_shell, kSyncThreads, 0, c.stream>>>(doms, ds, sx, sy, sz, job.ndir,
                                                         c.wmark.p, c.wlist.p, c.ring, m, 0);
+    k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
+        c.wmark.p, doms, ndom, (uint32_t)job.wdense, c.wlist.p, &c.ring[m % kCtrRing].nwork,
+        &c.ring[m % kCtrRing], kModeSparse);
     k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, 0, 0, m + 1, kOpPreCheck, job.s,
                                                emit_ok);
     k_detect_list<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
         d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, m);
+    k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
+        c.tmark.p, doms, ndom, (uint32_t)job.wdense, c.tlist.p,
+        &c.ring[(m + 1) % kCtrRing].ntargets, &c.ring[m % kCtrRing], kModeSparse);
     k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, 0, 0, m + 1, kOpCheck, job.s, 1);
     PMSZ_HIP(hipGetLastError());
     PMSZ_HIP(hipMemcpyAsync(c.h_ring + ((m + 1) % kCtrRing), c.ring + ((m + 1) % kCtrRing),
@@ -1495,6 +1531,12 @@ int prepare(Ctx &c, Job &job) {
   if (job.doms.size() == 1) ids = (uint64_t)job.doms[0].d.n;
   else ids = (uint64_t)job.doms.size() << job.shift;
   job.words = (ids + 31) / 32 + 1;
+  // the words the domains' ids occupy, back to back (each gbase is a multiple of 32)
+  job.wdense = 0;
+  for (Dom &D : job.doms) {
+    D.wpre = (uint32_t)job.wdense;
+    job.wdense += ((uint64_t)D.d.n + 31) / 32;
+  }
   job.groups = (cells + 15) / 16;
   job.list_cap = cells;
   int rc;
@@ -1693,7 +1735,11 @@ int correct_subdomains(Ctx &c, const pmsz_params *p, const T *F, T *G, uint8_t *
   Outcome o;
   rc = drive<T, true>(c, job, o);
   if (rc) return rc;
-  k_finish<T><<<g_all, kSyncThreads, 0, c.stream>>>(c.doms.p, c.ds.p, G, dc_out, p->nx, p->ny);
+  uint32_t maxrows = 0;
+  for (const Dom &D : job.doms)
+    maxrows = std::max<uint32_t>(maxrows, (uint32_t)((D.hi[1] - D.lo[1]) * (D.hi[2] - D.lo[2])));
+  const dim3 g_fin(std::min<uint32_t>(maxrows, 512u), nsub);
+  k_finish<T><<<g_fin, kSyncThreads, 0, c.stream>>>(c.doms.p, c.ds.p, G, dc_out, p->nx, p->ny);
   PMSZ_HIP(hipGetLastError());
   PMSZ_HIP(hipMemcpyAsync(c.h_ds, c.ds.p, sizeof(DomState) * nsub, hipMemcpyDeviceToHost,
                           c.stream));
@@ -1787,12 +1833,11 @@ int correct_batch(Ctx &c, int n, const pmsz_params *p, const void *const *d_f,
     Outcome o;
     rc = drive<T, true>(c, job, o);
     if (rc) return rc;
-    uint32_t maxcore = 0;
-    for (const Dom &D : job.doms) maxcore = std::max<uint32_t>(maxcore, (uint32_t)D.d.n);
-    const dim3 g_all((unsigned)std::min<uint32_t>((maxcore + kSyncThreads - 1) / kSyncThreads,
-                                                  (uint32_t)c.cus * 8),
-                     (unsigned)cnt);
-    k_finish<T><<<g_all, kSyncThreads, 0, c.stream>>>(c.doms.p, c.ds.p, nullptr, nullptr, 0, 0);
+    uint32_t maxrows = 0;
+    for (const Dom &D : job.doms)
+      maxrows = std::max<uint32_t>(maxrows, (uint32_t)((D.hi[1] - D.lo[1]) * (D.hi[2] - D.lo[2])));
+    const dim3 g_fin(std::min<uint32_t>(maxrows, 256u), (unsigned)cnt);
+    k_finish<T><<<g_fin, kSyncThreads, 0, c.stream>>>(c.doms.p, c.ds.p, nullptr, nullptr, 0, 0);
     PMSZ_HIP(hipGetLastError());
     off = 0;
     for (int k = 0; k < cnt; k++) {
