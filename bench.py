@@ -134,6 +134,20 @@ def batched_vs_plain(reps=3):
         batch_t.append(time.perf_counter() - t)
         same = all(torch.equal(a, b) for a, b in zip(gs, gb))
     tp, tb = float(np.median(plain_t)), float(np.median(batch_t))
+    # the autotuner on this batch (tile depth and schedule thresholds), then the
+    # batch again under the tuning it kept
+    tuning, _ = engine.autotune(fs, g0s, ebs, reps=1)
+    tuned_t = []
+    for _ in range(reps):
+        gt = [g.clone() for g in g0s]
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        engine.correct_batch(fs, gt, ebs)
+        torch.cuda.synchronize()
+        tuned_t.append(time.perf_counter() - t)
+        same = same and all(torch.equal(a, b) for a, b in zip(gt, gb))
+    engine.set_tuning({})
+    tt = float(np.median(tuned_t))
     return {
         "workload": "64 f32 Perlin fields, edges log-uniform [32,256] per axis (rng 0), seeds "
                     "100-163, eb log-uniform [1e-4,1e-2] x range, SZ-like / ZFP by seed parity",
@@ -143,6 +157,8 @@ def batched_vs_plain(reps=3):
         "speedup": tp / tb,
         "batched_ms": tb * 1e3,
         "plain_ms": tp * 1e3,
+        "autotuned": {"tuning": tuning, "batched_ms": tt * 1e3,
+                      "batched_gvox_s": nvox / tt / 1e9, "speedup_vs_plain": tp / tt},
         "identical": bool(same),
         "sweeps_max": int(max(s["edit_sweeps"] for s in sts)),
         "codecs": {c: sum(1 for *_, cc in cases if cc == c) for c in ("sz3", "zfp")},

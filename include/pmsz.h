@@ -135,6 +135,29 @@ int pmsz_ctx_trim(pmsz_ctx *ctx);
  * admit work by HBM budget. */
 int64_t pmsz_workspace_bytes(int n, const pmsz_params *params);
 
+/* Tuning of a context: parameters of the schedule and of the full-volume
+ * kernel's tiles.  They change the speed of a run, never its result.  The
+ * reference has no counterpart (its schedule is fixed: a full sweep per
+ * iteration, c_loops pMSz.cu:1943-2060). */
+typedef struct {
+  int dense_div;     /* detection j+1 over every voxel when 15 x targets >= voxels / dense_div (16) */
+  int tail_targets;  /* the tail kernel takes over at or below this many targets (16384) */
+  int tail_blocks;   /* workgroups of the tail kernel, 1..256 (64) */
+  int zchunk;        /* planes per column unit of the full-volume kernel: 16, 32 or 64 (64) */
+} pmsz_tuning;
+void pmsz_default_tuning(pmsz_tuning *t);
+/* ctx NULL: the default context. */
+int pmsz_ctx_set_tuning(pmsz_ctx *ctx, const pmsz_tuning *t);
+int pmsz_ctx_get_tuning(pmsz_ctx *ctx, pmsz_tuning *t);
+/* Autotuner: corrects copies of the given fields (n == 1: one field, any sub;
+ * n > 1: a batch, as pmsz_ctx_correct_batch) under candidate tunings -- a
+ * coordinate descent over zchunk, dense_div, tail_targets and tail_blocks, each
+ * candidate timed as the best of `reps` runs -- and keeps the fastest in the
+ * context.  d_g0 are not modified; the library allocates the copies for the
+ * duration of the call.  *best / *best_seconds (nullable) receive the result. */
+int pmsz_autotune(pmsz_ctx *ctx, int n, const pmsz_params *params, const void *const *d_f,
+                  const void *const *d_g0, int reps, pmsz_tuning *best, double *best_seconds);
+
 /* The same with host arrays: copies in, corrects, copies the result back
  * (the H2D / D2H copies of main, pMSz.cu:2870-2900, 2971). */
 int pmsz_correct_host(const pmsz_params *p, const void *h_f, void *h_g, uint8_t *h_dc,

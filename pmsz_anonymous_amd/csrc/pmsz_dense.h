@@ -35,7 +35,7 @@ namespace pmsz {
 constexpr int kColOwned = 62;  // x owned by a wave (64 lanes minus the two halo lanes)
 constexpr int kColRows = 4;    // y rows per wave
 constexpr int kColH = kColRows + 2;
-constexpr int kColZ = 64;      // planes per work unit
+constexpr int kColZ = 64;      // planes per work unit by default (a domain's zc: 16, 32 or 64)
 constexpr int kColWaves = 4;   // waves per block
 constexpr int kColThreads = 64 * kColWaves;
 
@@ -193,7 +193,7 @@ __device__ __forceinline__ double kmin(double a, double b) {
 // one list reservation per step (same-address atomics serialise, so steps are
 // large).  Nothing is cleared: the next sweep uses another tag.
 constexpr int kCompactThreads = 256;
-constexpr int kCompactPT = 64;     // groups per thread per step
+constexpr int kCompactPT = 16;     // groups per thread per step (all loads in flight)
 constexpr int kCompactBatch = 16;  // loads in flight per thread
 constexpr int kCompactDoms = 1024;  // most domains of a run
 __device__ __forceinline__ uint32_t tag_bytes(uint32_t w, uint32_t rep) {
@@ -271,12 +271,12 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(
 // map is cleared whenever the tags start over).
 __host__ __device__ inline uint32_t detect_tag(uint32_t jd) { return 1u + (jd - 1u) % 255u; }
 
-// Units (62 x 4 x 64 column blocks) of one domain's box.
+// Units (62 x 4 x zc column blocks) of one domain's box.
 __host__ inline void col_units(int64_t bx, int64_t by, int64_t bz, uint32_t &nux, uint32_t &nuy,
-                               uint32_t &nuz) {
+                               uint32_t &nuz, int64_t zc = kColZ) {
   nux = (uint32_t)((bx + kColOwned - 1) / kColOwned);
   nuy = (uint32_t)((by + kColRows - 1) / kColRows);
-  nuz = (uint32_t)((bz + kColZ - 1) / kColZ);
+  nuz = (uint32_t)((bz + zc - 1) / zc);
 }
 
 
@@ -421,8 +421,8 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
     const uint32_t uy = rem / D.nux, ux = rem - uy * D.nux;
     const int x = D.lo[0] + (int)(ux * kColOwned) - 1 + (int)lane;  // this lane's x
     const int y0m1 = D.lo[1] + (int)(uy * kColRows) - 1;             // y of loaded row 0
-    const int zs = D.lo[2] + (int)(uz * kColZ);
-    const int ze = zs + kColZ < D.hi[2] ? zs + kColZ : D.hi[2];
+    const int zs = D.lo[2] + (int)(uz * D.zc);
+    const int ze = zs + (int)D.zc < D.hi[2] ? zs + (int)D.zc : D.hi[2];
     const bool xok = x >= 0 && x < (int)nx;
     const bool owned = lane >= 1 && lane <= (uint32_t)kColOwned && x < D.hi[0];
     // in-plane offsets of the loaded rows (bytes of T) and of the voxel rows

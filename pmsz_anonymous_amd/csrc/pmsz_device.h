@@ -62,6 +62,7 @@ struct Dom {
   int32_t cc[3];      // coordinates in the subdomain grid
   uint32_t unit_base; // first column unit of this domain
   uint32_t nux, nuy, nuz;  // column units of the box
+  uint32_t zc;             // planes per column unit (16, 32 or 64)
   uint32_t shell_n;   // cells of the 2-layer shell (padded blocks)
   uint32_t padded;    // 1: cells holding the sentinel are missing neighbours
   uint32_t gbase;     // domain << shift
@@ -156,6 +157,7 @@ struct Sched {
   uint32_t max_sweeps;
   uint32_t dense_min_work;  // 15 * targets at or above this -> next detection dense
   uint32_t tail_ok;         // the cooperative tail may take over
+  uint32_t tail_targets;    // the tail takes over at or below this many targets
 };
 
 // Run-wide state, written only by kernels that end a phase.

@@ -48,6 +48,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -65,9 +66,9 @@ constexpr int kApplyThreads = 256;
 constexpr int kTailThreads = 256;
 // the tail takes over once a sweep has at most this many targets: heavier work-list
 // sweeps run as separate launches, which bring more threads to their atomics
-constexpr uint32_t kTailTargets = 16384;
-// blocks of the tail: its grid barriers cost about 45 ns per block (MI355X)
-constexpr int kTailBlocks = 64;
+// most blocks of the tail (its grid barriers cost about 45 ns per block on MI355X;
+// the tuning picks the grid, 64 by default)
+constexpr int kTailBlocks = 256;
 constexpr size_t kBarBytes = sizeof(uint32_t) * 16 * (kTailBlocks + 1);
 constexpr int kSyncThreads = 256;
 constexpr uint32_t kMaxDomains = 1024;  // domain bookkeeping: one thread each
@@ -111,7 +112,7 @@ __device__ __forceinline__ uint32_t decide(const Counters &cj, uint32_t j, const
     return kModeDone;
   }
   const bool dense = (uint64_t)nt * kCand >= s.dense_min_work;
-  if (tail_allowed && !dense && nt <= kTailTargets) return kModeTail;
+  if (tail_allowed && !dense && nt <= s.tail_targets) return kModeTail;
   if (s.mode == PMSZ_LAZY && j >= s.max_sweeps) {
     *done = 2;  // the edits of sweep j still land (they belong to sweep j)
     return kModeDone;
@@ -131,7 +132,7 @@ __device__ __forceinline__ uint32_t decide_multi(const Counters &cj, const Sched
   }
   const uint32_t nt = ld_agent(&cj.ntargets);
   const bool dense = (uint64_t)nt * kCand >= s.dense_min_work;
-  if (tail_allowed && !dense && nt <= kTailTargets) return kModeTail;
+  if (tail_allowed && !dense && nt <= s.tail_targets) return kModeTail;
   return dense ? kModeDense : kModeSparse;
 }
 
@@ -320,7 +321,7 @@ __device__ __forceinline__ void detect_mark(const Dom &D, uint32_t *tmark, uint3
 // takes kBitsPT words per step, interleaved across the block so loads coalesce;
 // a block makes one list reservation per step.
 constexpr int kBitsThreads = 256;
-constexpr int kBitsPT = 64;
+constexpr int kBitsPT = 16;
 __global__ __launch_bounds__(kBitsThreads) void k_compact_bits(
     uint32_t *__restrict__ bits, const Dom *__restrict__ doms, uint32_t ndom, uint32_t wdense,
     uint32_t *__restrict__ list, uint32_t *__restrict__ count, const Counters *__restrict__ gate,
@@ -1104,8 +1105,24 @@ struct Buf {
   }
 };
 
+pmsz_tuning default_tuning() {
+  pmsz_tuning t;
+  t.dense_div = 16;
+  t.tail_targets = 16384;
+  t.tail_blocks = 64;
+  t.zchunk = kColZ;
+  return t;
+}
+
+bool tuning_valid(const pmsz_tuning &t) {
+  return t.dense_div >= 1 && t.dense_div <= 4096 && t.tail_targets >= 0 &&
+         t.tail_blocks >= 1 && t.tail_blocks <= kTailBlocks &&
+         (t.zchunk == 16 || t.zchunk == 32 || t.zchunk == 64);
+}
+
 struct Ctx {
   std::mutex mu;  // one call at a time per context
+  pmsz_tuning tune = default_tuning();
   int device = -1;
   int cus = 0;
   hipStream_t stream = nullptr;
@@ -1262,6 +1279,7 @@ Dom field_dom(const pmsz_params *p, const void *f, void *g, uint8_t *code, uint8
   D.hi[0] = (int32_t)p->nx;
   D.hi[1] = (int32_t)p->ny;
   D.hi[2] = (int32_t)p->nz;
+  D.zc = (uint32_t)kColZ;
   col_units(p->nx, p->ny, p->nz, D.nux, D.nuy, D.nuz);
   D.ep = edit_params(p);
   return D;
@@ -1346,7 +1364,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   // device can hold of it, and at most kMaxContexts contexts exist
   static const int occ_tail = occupancy(k_tail<T, kMulti>, kTailThreads);
   const unsigned g_tail = (unsigned)std::max(
-      1, std::min(kTailBlocks, c.cus * occ_tail / (kMaxContexts + 1)));
+      1, std::min(std::min(kTailBlocks, c.tune.tail_blocks), c.cus * occ_tail / (kMaxContexts + 1)));
   const unsigned g_compact = compact_blocks(c, (int64_t)job.groups);
   const unsigned g_bits = (unsigned)std::max<int64_t>(
       1, std::min<int64_t>(((int64_t)job.wdense + kBitsThreads * kBitsPT - 1) /
@@ -1526,6 +1544,15 @@ int drive(Ctx &c, Job &job, Outcome &out) {
 
 // Bitmap words and list capacity of a run, allocating what is missing.
 int prepare(Ctx &c, Job &job) {
+  // column units with the context's tile depth
+  uint32_t ubase = 0;
+  for (Dom &D : job.doms) {
+    D.zc = (uint32_t)c.tune.zchunk;
+    col_units(D.hi[0] - D.lo[0], D.hi[1] - D.lo[1], D.hi[2] - D.lo[2], D.nux, D.nuy, D.nuz,
+              D.zc);
+    D.unit_base = ubase;
+    ubase += D.nux * D.nuy * D.nuz;
+  }
   uint64_t cells = 0, ids = 0;
   for (const Dom &D : job.doms) cells += (uint64_t)D.d.n;
   if (job.doms.size() == 1) ids = (uint64_t)job.doms[0].d.n;
@@ -1566,13 +1593,14 @@ uint32_t ceil_log2(uint64_t v) {
   return s;
 }
 
-Sched make_sched(const pmsz_params *p, uint64_t work_voxels, bool tail_ok) {
+Sched make_sched(const Ctx &c, const pmsz_params *p, uint64_t work_voxels, bool tail_ok) {
   Sched s;
   s.mode = p->mode;
   s.max_sweeps = (uint32_t)p->max_sweeps;
-  s.dense_min_work = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(work_voxels / 16, 1),
-                                                  0xffffffffull);
+  s.dense_min_work = (uint32_t)std::min<uint64_t>(
+      std::max<uint64_t>(work_voxels / (uint64_t)c.tune.dense_div, 1), 0xffffffffull);
   s.tail_ok = tail_ok ? 1u : 0u;
+  s.tail_targets = (uint32_t)c.tune.tail_targets;
   return s;
 }
 
@@ -1599,7 +1627,7 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
   if ((rc = c.code.ensure(n)) || (rc = c.dc.ensure(n))) return rc;
   Job job;
   job.doms.push_back(field_dom(p, f, g, c.code.p, c.dc.p));
-  job.s = make_sched(p, (uint64_t)n, true);
+  job.s = make_sched(c, p, (uint64_t)n, true);
   if ((rc = prepare(c, job))) return rc;
   std::memset(st, 0, sizeof(*st));
   Outcome o;
@@ -1686,6 +1714,7 @@ int correct_subdomains(Ctx &c, const pmsz_params *p, const T *F, T *G, uint8_t *
         D.cc[0] = cx;
         D.cc[1] = cy;
         D.cc[2] = cz;
+        D.zc = (uint32_t)kColZ;
         col_units(l[0], l[1], l[2], D.nux, D.nuy, D.nuz);
         D.shell_n = shell_count((uint32_t)D.d.nx, (uint32_t)D.d.ny, (uint32_t)D.d.nz);
         D.padded = 1;
@@ -1716,7 +1745,7 @@ int correct_subdomains(Ctx &c, const pmsz_params *p, const T *F, T *G, uint8_t *
     D.unit_base = ubase;
     ubase += dom_units(D);
   }
-  job.s = make_sched(p, core, p->mode == PMSZ_LAZY);
+  job.s = make_sched(c, p, core, p->mode == PMSZ_LAZY);
   if ((rc = prepare(c, job))) return rc;
   std::memset(st, 0, sizeof(*st));
   // padded blocks with their initial ghost values; run->err flags bad inputs
@@ -1828,7 +1857,7 @@ int correct_batch(Ctx &c, int n, const pmsz_params *p, const void *const *d_f,
       off += (uint64_t)D.d.n;
       job.doms.push_back(D);
     }
-    job.s = make_sched(&p[first], cells, true);
+    job.s = make_sched(c, &p[first], cells, true);
     if ((rc = prepare(c, job))) return rc;
     Outcome o;
     rc = drive<T, true>(c, job, o);
@@ -1883,7 +1912,7 @@ int time_first_pass(Ctx &c, const pmsz_params *p, const T *f, const T *g, int it
   if ((rc = c.code.ensure(n)) || (rc = c.dc.ensure(n))) return rc;
   Job job;
   job.doms.push_back(field_dom(p, f, (void *)g, c.code.p, c.dc.p));
-  job.s = make_sched(p, (uint64_t)n, true);
+  job.s = make_sched(c, p, (uint64_t)n, true);
   if ((rc = prepare(c, job))) return rc;
   static const int occ_l = occupancy(k_columns<T, true, false, false>, kColThreads);
   static const int occ_d = occupancy(k_columns<T, false, true, false, true>, kColThreads);
@@ -1924,6 +1953,23 @@ struct pmsz_ctx {
   Ctx c;
 };
 
+namespace pmsz {
+// One field (whole or split into subdomains), by value type; the caller holds
+// the context's lock.
+int correct_dispatch(Ctx &c, const pmsz_params *p, const void *d_f, void *d_g, uint8_t *d_dc,
+                     pmsz_stats *stats) {
+  const bool split = p->sub[0] * p->sub[1] * p->sub[2] != 1;
+  if (p->dtype == PMSZ_F32) {
+    if (split)
+      return correct_subdomains<float>(c, p, (const float *)d_f, (float *)d_g, d_dc, stats);
+    return correct_single<float>(c, p, (const float *)d_f, (float *)d_g, d_dc, stats);
+  }
+  if (split)
+    return correct_subdomains<double>(c, p, (const double *)d_f, (double *)d_g, d_dc, stats);
+  return correct_single<double>(c, p, (const double *)d_f, (double *)d_g, d_dc, stats);
+}
+}  // namespace pmsz
+
 namespace {
 pmsz_ctx g_default;
 Ctx &default_ctx() { return g_default.c; }
@@ -1959,15 +2005,7 @@ int pmsz_ctx_correct(pmsz_ctx *ctx, const pmsz_params *p, const void *d_f, void 
   Ctx &c = ctx->c;
   std::lock_guard<std::mutex> lk(c.mu);
   if ((rc = ctx_device(c))) return rc;
-  const bool split = p->sub[0] * p->sub[1] * p->sub[2] != 1;
-  if (p->dtype == PMSZ_F32) {
-    if (split)
-      return correct_subdomains<float>(c, p, (const float *)d_f, (float *)d_g, d_dc, stats);
-    return correct_single<float>(c, p, (const float *)d_f, (float *)d_g, d_dc, stats);
-  }
-  if (split)
-    return correct_subdomains<double>(c, p, (const double *)d_f, (double *)d_g, d_dc, stats);
-  return correct_single<double>(c, p, (const double *)d_f, (double *)d_g, d_dc, stats);
+  return correct_dispatch(c, p, d_f, d_g, d_dc, stats);
 }
 
 int pmsz_ctx_create(pmsz_ctx **out) {
@@ -2065,6 +2103,114 @@ int pmsz_ctx_correct_batch(pmsz_ctx *ctx, int n, const pmsz_params *params,
   if (rc) return rc;
   if (params[0].dtype == PMSZ_F32) return correct_batch<float>(c, n, params, d_f, d_g, d_dc, stats);
   return correct_batch<double>(c, n, params, d_f, d_g, d_dc, stats);
+}
+
+void pmsz_default_tuning(pmsz_tuning *t) {
+  if (t) *t = default_tuning();
+}
+
+int pmsz_ctx_set_tuning(pmsz_ctx *ctx, const pmsz_tuning *t) {
+  if (!t || !tuning_valid(*t)) return PMSZ_EINVAL;
+  Ctx &c = (ctx ? ctx : &g_default)->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  c.tune = *t;
+  return PMSZ_OK;
+}
+
+int pmsz_ctx_get_tuning(pmsz_ctx *ctx, pmsz_tuning *t) {
+  if (!t) return PMSZ_EINVAL;
+  Ctx &c = (ctx ? ctx : &g_default)->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  *t = c.tune;
+  return PMSZ_OK;
+}
+
+int pmsz_autotune(pmsz_ctx *ctx, int n, const pmsz_params *params, const void *const *d_f,
+                  const void *const *d_g0, int reps, pmsz_tuning *best, double *best_seconds) {
+  if (n < 1 || !params || !d_f || !d_g0 || reps < 1) return PMSZ_EINVAL;
+  for (int k = 0; k < n; k++) {
+    int rc = check_params(&params[k]);
+    if (rc) return rc;
+    if (!d_f[k] || !d_g0[k]) return PMSZ_EINVAL;
+    if (n > 1 && (params[k].sub[0] * params[k].sub[1] * params[k].sub[2] != 1 ||
+                  params[k].dtype != params[0].dtype || params[k].mode != params[0].mode ||
+                  params[k].max_sweeps != params[0].max_sweeps))
+      return PMSZ_EINVAL;
+  }
+  Ctx &c = (ctx ? ctx : &g_default)->c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  int rc = ctx_device(c);
+  if (rc) return rc;
+  // scratch copies of the decompressed fields: the inputs stay untouched
+  const size_t s = params[0].dtype == PMSZ_F64 ? 8 : 4;
+  std::vector<size_t> bytes(n), offs(n);
+  size_t total = 0;
+  for (int k = 0; k < n; k++) {
+    bytes[k] = s * (size_t)(params[k].nx * params[k].ny * params[k].nz);
+    offs[k] = total;
+    total += (bytes[k] + 255) & ~(size_t)255;
+  }
+  char *scratch = nullptr;
+  if (hipMalloc(&scratch, total) != hipSuccess) return PMSZ_ENOMEM;
+  std::vector<void *> gs(n);
+  for (int k = 0; k < n; k++) gs[k] = scratch + offs[k];
+  std::vector<pmsz_stats> st(n);
+  const pmsz_tuning start = c.tune;
+  // one candidate: the best of `reps` timed runs (host clock around the call,
+  // which returns when the result is complete)
+  auto measure = [&](const pmsz_tuning &t, double &secs) -> int {
+    c.tune = t;
+    secs = 1e30;
+    for (int r = 0; r < reps; r++) {
+      for (int k = 0; k < n; k++)
+        PMSZ_HIP(hipMemcpyAsync(gs[k], d_g0[k], bytes[k], hipMemcpyDeviceToDevice, c.stream));
+      PMSZ_HIP(hipStreamSynchronize(c.stream));
+      const auto t0 = std::chrono::steady_clock::now();
+      int e;
+      if (n == 1) {
+        e = correct_dispatch(c, &params[0], d_f[0], gs[0], nullptr, st.data());
+      } else if (params[0].dtype == PMSZ_F32) {
+        e = correct_batch<float>(c, n, params, d_f, gs.data(), nullptr, st.data());
+      } else {
+        e = correct_batch<double>(c, n, params, d_f, gs.data(), nullptr, st.data());
+      }
+      const auto t1 = std::chrono::steady_clock::now();
+      if (e && e != PMSZ_NOT_CONVERGED) return e;
+      secs = std::min(secs, std::chrono::duration<double>(t1 - t0).count());
+    }
+    return PMSZ_OK;
+  };
+  // coordinate descent over the tile depth and the schedule thresholds, one pass
+  pmsz_tuning cur = start;
+  double cur_s = 0;
+  rc = measure(cur, cur_s);
+  const int zc[] = {64, 32, 16};
+  const int dd[] = {16, 8, 32, 4, 64};
+  const int tt[] = {16384, 4096, 65536, 1024};
+  const int tb[] = {64, 32, 128, 16};
+  auto sweep = [&](int pmsz_tuning::*field, const int *vals, int nv) {
+    for (int v = 0; v < nv && rc == PMSZ_OK; v++) {
+      if (cur.*field == vals[v]) continue;
+      pmsz_tuning t = cur;
+      t.*field = vals[v];
+      double secs = 0;
+      rc = measure(t, secs);
+      if (rc == PMSZ_OK && secs < cur_s) {
+        cur = t;
+        cur_s = secs;
+      }
+    }
+  };
+  if (rc == PMSZ_OK) sweep(&pmsz_tuning::zchunk, zc, 3);
+  if (rc == PMSZ_OK) sweep(&pmsz_tuning::dense_div, dd, 5);
+  if (rc == PMSZ_OK) sweep(&pmsz_tuning::tail_targets, tt, 4);
+  if (rc == PMSZ_OK) sweep(&pmsz_tuning::tail_blocks, tb, 4);
+  (void)hipFree(scratch);
+  c.tune = rc == PMSZ_OK ? cur : start;
+  if (rc) return rc;
+  if (best) *best = cur;
+  if (best_seconds) *best_seconds = cur_s;
+  return PMSZ_OK;
 }
 
 int pmsz_time_first_pass(const pmsz_params *p, const void *d_f, const void *d_g, int iters,

@@ -30,7 +30,8 @@ EXPORTED = [
     "pmsz_ctx_create", "pmsz_ctx_destroy", "pmsz_ctx_correct", "pmsz_ctx_correct_batch",
     "pmsz_ctx_bytes", "pmsz_ctx_trim", "pmsz_workspace_bytes",
     "pmsz_codes", "pmsz_labels", "pmsz_edit_symbols", "pmsz_time_first_pass", "pmsz_strerror",
-    "pmsz_version",
+    "pmsz_version", "pmsz_default_tuning", "pmsz_ctx_set_tuning", "pmsz_ctx_get_tuning",
+    "pmsz_autotune",
 ]
 
 
@@ -92,6 +93,51 @@ class Stats(ctypes.Structure):
 _lib = None
 
 
+class Tuning(ctypes.Structure):
+    """pmsz_tuning: schedule and tile parameters (speed only, never results)."""
+    _fields_ = [("dense_div", ctypes.c_int), ("tail_targets", ctypes.c_int),
+                ("tail_blocks", ctypes.c_int), ("zchunk", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _tuning(d):
+    t = Tuning()
+    lib().pmsz_default_tuning(ctypes.byref(t))
+    for k, v in (d or {}).items():
+        setattr(t, k, int(v))
+    return t
+
+
+def set_tuning(tuning, ctx=None):
+    """Set the tuning (a dict of Tuning fields; missing ones take the defaults) of
+    a context (None: the default context)."""
+    _check(lib().pmsz_ctx_set_tuning(ctx._h if ctx else None, ctypes.byref(_tuning(tuning))),
+           "pmsz_ctx_set_tuning")
+
+
+def get_tuning(ctx=None):
+    t = Tuning()
+    _check(lib().pmsz_ctx_get_tuning(ctx._h if ctx else None, ctypes.byref(t)),
+           "pmsz_ctx_get_tuning")
+    return t.as_dict()
+
+
+def autotune(fs, g0s, ebs, *, reps=2, ctx=None, per_field=None, **kw):
+    """Autotune on copies of CUDA tensors g0s[k] against fs[k] (one field, or a
+    batch): the fastest tuning is kept in the context and returned as
+    (dict, seconds).  g0s are not modified."""
+    import torch
+    n, params, pf, pg, _ = _batch_args(fs, g0s, ebs, None, per_field, kw)
+    best = Tuning()
+    secs = ctypes.c_double()
+    torch.cuda.current_stream().synchronize()
+    _check(lib().pmsz_autotune(ctx._h if ctx else None, n, params, pf, pg, reps,
+                               ctypes.byref(best), ctypes.byref(secs)), "pmsz_autotune")
+    return best.as_dict(), secs.value
+
+
 def lib_path():
     return _LIB_PATH
 
@@ -130,6 +176,13 @@ def lib():
         L.pmsz_strerror.restype = ctypes.c_char_p
         L.pmsz_version.argtypes = []
         L.pmsz_version.restype = ctypes.c_char_p
+        T = ctypes.POINTER(Tuning)
+        L.pmsz_default_tuning.argtypes = [T]
+        L.pmsz_default_tuning.restype = None
+        L.pmsz_ctx_set_tuning.argtypes = [vp, T]
+        L.pmsz_ctx_get_tuning.argtypes = [vp, T]
+        L.pmsz_autotune.argtypes = [vp, i32, P, ctypes.POINTER(vp), ctypes.POINTER(vp), i32, T,
+                                    ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
