@@ -52,6 +52,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -762,6 +763,14 @@ __global__ void k_reset_phase(Counters *ring, RunState *run, uint32_t m) {
   run->tail_j = 0;
 }
 
+// Zero fill of n16 16-byte groups (the target byte map when the tags start over;
+// a kernel, not a memset node, so that a captured chain replays it reliably).
+__global__ void k_fill16(uint4 *p, uint64_t n16) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 // ---------------------------------------------------------------------------
 // Subdomains: padded blocks, the delta-counter ghost sync, the gather.
 
@@ -1111,18 +1120,21 @@ pmsz_tuning default_tuning() {
   t.tail_targets = 16384;
   t.tail_blocks = 64;
   t.zchunk = kColZ;
+  t.graphs = 0;
   return t;
 }
 
 bool tuning_valid(const pmsz_tuning &t) {
   return t.dense_div >= 1 && t.dense_div <= 4096 && t.tail_targets >= 0 &&
          t.tail_blocks >= 1 && t.tail_blocks <= kTailBlocks &&
-         (t.zchunk == 16 || t.zchunk == 32 || t.zchunk == 64);
+         (t.zchunk == 16 || t.zchunk == 32 || t.zchunk == 64) && (t.graphs == 0 || t.graphs == 1);
 }
 
 struct Ctx {
   std::mutex mu;  // one call at a time per context
   pmsz_tuning tune = default_tuning();
+  // captured chains of multi-domain runs, by every value their launches take
+  std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
   int device = -1;
   int cus = 0;
   hipStream_t stream = nullptr;
@@ -1143,7 +1155,13 @@ struct Ctx {
   size_t h_ds_cap = 0;
 };
 
+void clear_graphs(Ctx &c) {
+  for (auto &kv : c.graphs) (void)hipGraphExecDestroy(kv.second);
+  c.graphs.clear();
+}
+
 void ctx_release(Ctx &c) {
+  clear_graphs(c);
   for (Buf<uint8_t> *b : {&c.code, &c.dc, &c.snap, &c.fpad, &c.gpad, &c.tbyte}) {
     (void)hipFree(b->p);
     *b = Buf<uint8_t>();
@@ -1340,6 +1358,34 @@ struct Outcome {
 // detect steps decided on the device, the tail, and -- for subdomains in lazy
 // mode -- a ghost sync and a check detection whenever every subdomain's loop has
 // ended.  In sync mode with subdomains the ghost sync follows every apply.
+constexpr size_t kMaxGraphs = 64;  // captured chains a context keeps
+
+// Launches the captured chain with this key, capturing and instantiating it on
+// first use (enqueue(j0) records the chain's launches on the context's stream).
+template <typename F>
+int chain_graph(Ctx &c, const std::vector<uint64_t> &key, uint32_t j0, F &&enqueue) {
+  auto it = c.graphs.find(key);
+  if (it == c.graphs.end()) {
+    if (c.graphs.size() >= kMaxGraphs) clear_graphs(c);
+    hipGraph_t g = nullptr;
+    PMSZ_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue(j0);
+    const hipError_t e = hipStreamEndCapture(c.stream, &g);
+    if (rc) {
+      if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    PMSZ_HIP(e);
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    PMSZ_HIP(ei);
+    it = c.graphs.emplace(key, x).first;
+  }
+  PMSZ_HIP(hipGraphLaunch(it->second, c.stream));
+  return PMSZ_OK;
+}
+
 template <typename T, bool kMulti>
 int drive(Ctx &c, Job &job, Outcome &out) {
   const uint32_t ndom = (uint32_t)job.doms.size();
@@ -1366,6 +1412,8 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   const unsigned g_tail = (unsigned)std::max(
       1, std::min(std::min(kTailBlocks, c.tune.tail_blocks), c.cus * occ_tail / (kMaxContexts + 1)));
   const unsigned g_compact = compact_blocks(c, (int64_t)job.groups);
+  const unsigned g_fill =
+      (unsigned)std::max<int64_t>(1, std::min<int64_t>(((int64_t)job.groups + 255) / 256, 2048));
   const unsigned g_bits = (unsigned)std::max<int64_t>(
       1, std::min<int64_t>(((int64_t)job.wdense + kBitsThreads * kBitsPT - 1) /
                                (kBitsThreads * kBitsPT),
@@ -1416,46 +1464,70 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   uint32_t j0 = 1;
   out.outer_started = 1;
   while (true) {
-    for (uint32_t j = j0; j < j0 + kChain; j++) {
-      if (!kMulti) {
-        // after a full-volume detection j: its target list
-        k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
-            c.tbyte.p, c.tlist.p, c.ring, (j - 1) % kCtrRing, j % kCtrRing, false,
-            (uint32_t)job.groups, doms, ndom, detect_tag(j));
+    // the chain of sweeps j0 .. j0 + kChain - 1, enqueued on the context's stream
+    auto enqueue_chain = [&](uint32_t j0) -> int {
+      for (uint32_t j = j0; j < j0 + kChain; j++) {
+        if (!kMulti) {
+          // after a full-volume detection j: its target list
+          k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
+              c.tbyte.p, c.tlist.p, c.ring, (j - 1) % kCtrRing, j % kCtrRing, false,
+              (uint32_t)job.groups, doms, ndom, detect_tag(j));
+        }
+        k_apply<T, kMulti><<<g_apply, kApplyThreads, 0, c.stream>>>(
+            d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j,
+            job.s);
+        if (sync_every) {
+          k_pack<T><<<g_shell, kSyncThreads, 0, c.stream>>>(doms, ds, job.gn[0], job.gn[1],
+                                                            job.gn[2], c.ring, j, 1);
+          k_unpack<T><<<g_shell, kSyncThreads, 0, c.stream>>>(doms, ds, sx, sy, sz, job.ndir,
+                                                              c.wmark.p, c.wlist.p, c.ring, j, 1);
+        }
+        // the work list of detection j+1 from the bitmap that apply j (and a ghost
+        // sync) marked
+        k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
+            c.wmark.p, doms, ndom, (uint32_t)job.wdense, c.wlist.p, &c.ring[j % kCtrRing].nwork,
+            &c.ring[j % kCtrRing], kModeSparse);
+        // the tags start over: no mark of an earlier sweep may carry detection j+1's
+        if (detect_tag(j + 1) == 1u)
+          k_fill16<<<g_fill, 256, 0, c.stream>>>((uint4 *)c.tbyte.p, job.groups);
+        k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
+            d0, doms, ndom, ds, c.tbyte.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, (j + 1) & 1u,
+            c.run, detect_tag(j + 1));
+        if (kMulti)
+          k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
+              c.tbyte.p, c.tlist.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false,
+              (uint32_t)job.groups, doms, ndom, detect_tag(j + 1));
+        k_detect_list<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
+            d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j);
+        // the targets of a work-list detection j+1, from the bitmap
+        k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
+            c.tmark.p, doms, ndom, (uint32_t)job.wdense, c.tlist.p,
+            &c.ring[(j + 1) % kCtrRing].ntargets, &c.ring[j % kCtrRing], kModeSparse);
+        if (kMulti)
+          k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, j, 1, j + 1, kOpDetect,
+                                                     job.s, 1);
       }
-      k_apply<T, kMulti><<<g_apply, kApplyThreads, 0, c.stream>>>(
-          d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j,
-          job.s);
-      if (sync_every) {
-        k_pack<T><<<g_shell, kSyncThreads, 0, c.stream>>>(doms, ds, job.gn[0], job.gn[1],
-                                                          job.gn[2], c.ring, j, 1);
-        k_unpack<T><<<g_shell, kSyncThreads, 0, c.stream>>>(doms, ds, sx, sy, sz, job.ndir,
-                                                            c.wmark.p, c.wlist.p, c.ring, j, 1);
-      }
-      // the work list of detection j+1 from the bitmap that apply j (and a ghost
-      // sync) marked
-      k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
-          c.wmark.p, doms, ndom, (uint32_t)job.wdense, c.wlist.p, &c.ring[j % kCtrRing].nwork,
-          &c.ring[j % kCtrRing], kModeSparse);
-      // the tags start over: no mark of an earlier sweep may carry detection j+1's
-      if (detect_tag(j + 1) == 1u)
-        PMSZ_HIP(hipMemsetAsync(c.tbyte.p, 0, 16 * job.groups, c.stream));
-      k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
-          d0, doms, ndom, ds, c.tbyte.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, (j + 1) & 1u,
-          c.run, detect_tag(j + 1));
-      if (kMulti)
-        k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
-            c.tbyte.p, c.tlist.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false,
-            (uint32_t)job.groups, doms, ndom, detect_tag(j + 1));
-      k_detect_list<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
-          d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j);
-      // the targets of a work-list detection j+1, from the bitmap
-      k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
-          c.tmark.p, doms, ndom, (uint32_t)job.wdense, c.tlist.p,
-          &c.ring[(j + 1) % kCtrRing].ntargets, &c.ring[j % kCtrRing], kModeSparse);
-      if (kMulti)
-        k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, j, 1, j + 1, kOpDetect,
-                                                   job.s, 1);
+      PMSZ_HIP(hipGetLastError());
+      return PMSZ_OK;
+    };
+    if (kMulti && c.tune.graphs) {
+      // replay of a captured chain: one graph per job layout and first sweep,
+      // kept by the context while the layout recurs.  The key holds every value
+      // the chain's launches take (d0 is not read by multi-domain kernels).
+      const std::vector<uint64_t> key = {
+          sizeof(T), j0, ndom, job.shift, job.groups, job.wdense, (uint64_t)(uintptr_t)doms,
+          (uint64_t)(uintptr_t)ds, (uint64_t)(uintptr_t)c.tbyte.p, (uint64_t)(uintptr_t)c.tmark.p,
+          (uint64_t)(uintptr_t)c.tlist.p, (uint64_t)(uintptr_t)c.wmark.p,
+          (uint64_t)(uintptr_t)c.wlist.p, (uint64_t)(uintptr_t)c.ring, (uint64_t)(uintptr_t)c.run,
+          (uint64_t)job.s.mode, job.s.max_sweeps, job.s.dense_min_work, job.s.tail_ok,
+          job.s.tail_targets, g_compact, g_apply, g_bits, g_fill, g_dense, g_list, g_shell.x,
+          g_shell.y, sync_every ? 1u : 0u, (uint64_t)sx, (uint64_t)sy, (uint64_t)sz,
+          (uint64_t)job.ndir, (uint64_t)job.gn[0], (uint64_t)job.gn[1], (uint64_t)job.gn[2]};
+      int rc = chain_graph(c, key, j0, enqueue_chain);
+      if (rc) return rc;
+    } else {
+      int rc = enqueue_chain(j0);
+      if (rc) return rc;
     }
     if (job.s.tail_ok) {
       Dom a_d0 = d0;
@@ -2205,6 +2277,9 @@ int pmsz_autotune(pmsz_ctx *ctx, int n, const pmsz_params *params, const void *c
   if (rc == PMSZ_OK) sweep(&pmsz_tuning::dense_div, dd, 5);
   if (rc == PMSZ_OK) sweep(&pmsz_tuning::tail_targets, tt, 4);
   if (rc == PMSZ_OK) sweep(&pmsz_tuning::tail_blocks, tb, 4);
+  const int gr[] = {0, 1};  // captured chains: multi-domain runs only
+  if (rc == PMSZ_OK && (n > 1 || params[0].sub[0] * params[0].sub[1] * params[0].sub[2] != 1))
+    sweep(&pmsz_tuning::graphs, gr, 2);
   (void)hipFree(scratch);
   c.tune = rc == PMSZ_OK ? cur : start;
   if (rc) return rc;

@@ -16,6 +16,8 @@ TUNINGS = [
     {"tail_targets": 0},          # no tail: every sweep as separate launches
     {"tail_targets": 1 << 30, "tail_blocks": 128},  # the tail from the first sparse sweep
     {"dense_div": 1 << 12},       # full-volume detections only while targets are many
+    {"graphs": 1},                # multi-domain runs replay captured chains
+    {"graphs": 1, "zchunk": 32, "tail_targets": 4096},
 ]
 KEYS = ("edit_sweeps", "check_sweeps", "total_targets", "touched_voxels", "clamped_voxels",
         "final_cnt", "status", "trace_n")
@@ -70,7 +72,7 @@ def test_tunings_subdomains_and_batch(dev, mode):
     ref, st0 = _run(tf, tg0, eb, sub=(2, 2, 2), mode=mode)
     refb = [g.clone() for g in g0s]
     stb0 = engine.correct_batch(fs, refb, ebs, mode=mode)
-    for t in TUNINGS[:4]:
+    for t in TUNINGS[:4] + TUNINGS[-2:]:
         engine.set_tuning(t)
         g, st = _run(tf, tg0, eb, sub=(2, 2, 2), mode=mode)
         assert torch.equal(g, ref), t
@@ -115,3 +117,34 @@ def test_autotune_single_and_batch(dev):
     finally:
         ctx.close()
     engine.set_tuning({})
+
+
+def test_graph_replays_match_eager(dev):
+    # captured chains are replayed on later calls with the same layout: every
+    # replay, with eager work of the program in between, equals the eager result
+    from pmsz_anonymous_amd import engine, fieldio
+    cases = [fieldio.perlin_case((20 + 6 * k, 36, 44), 40 + k, 5, 3, 10 ** (-2 - k % 3), "sz3",
+                                 np.float32) for k in range(5)]
+    fs = [torch.from_numpy(c[0]).to(dev) for c in cases]
+    g0s = [torch.from_numpy(c[1]).to(dev) for c in cases]
+    ebs = [c[2] for c in cases]
+    ctx = engine.Context()
+    try:
+        engine.set_tuning({}, ctx)
+        ref = [g.clone() for g in g0s]
+        st_ref = ctx.correct_batch(fs, ref, ebs)
+        engine.set_tuning({"graphs": 1}, ctx)
+        x = torch.randn(1 << 20, device=dev)
+        for rep in range(4):
+            gs = [g.clone() for g in g0s]
+            st = ctx.correct_batch(fs, gs, ebs)
+            for k in range(len(gs)):
+                assert torch.equal(gs[k], ref[k]), (rep, k)
+                assert st[k]["edit_sweeps"] == st_ref[k]["edit_sweeps"], (rep, k)
+            # eager work between replays: torch kernels and a plain correction
+            x = torch.sort(x * 1.0001)[0]
+            gp = g0s[0].clone()
+            ctx.correct(fs[0], gp, ebs[0])
+            torch.cuda.synchronize()
+    finally:
+        ctx.close()
