@@ -9,7 +9,7 @@ serving run (Poisson arrivals, p50 / p99), the 768^3 float64 2x2x2 subdomain
 run (lazy vs per-iteration sync) and the oracle on the host cores.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 512] [--no-cpu]
-                  [--no-batch] [--no-serve] [--no-subdomains]
+                  [--no-batch] [--no-serve] [--no-subdomains] [--no-tune]
 """
 from __future__ import annotations
 
@@ -296,6 +296,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
     ap.add_argument("--no-subdomains", action="store_true",
                     help="skip the 768^3 f64 2x2x2 lazy-vs-sync run")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuned repeat of the headline workload")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the multi-GPU path is a design "
@@ -398,6 +400,22 @@ def main():
         "definition": "(N*(s+1)*(1+detections) + edits*(3s+2)) / time per correction",
         "detections": detections, "algorithmic_bytes": e2e_bytes,
         "achieved": e2e, "unit": "GB/s", "frac": e2e / HBM_PEAK_GBPS}
+    # the autotuner on this workload, then the same steps under the tuning it kept
+    # (the headline value above is the default tuning)
+    if not args.no_tune:
+        tuning, _ = engine.autotune([tf], [tg0], [eb], reps=2)
+        tcopies = [tg0.clone() for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for g in tcopies:
+            engine.correct(tf, g, eb)
+        torch.cuda.synchronize()
+        tms = (time.perf_counter() - t1) / args.steps * 1e3
+        same = torch.equal(tcopies[-1], copies[-1])
+        engine.set_tuning({})
+        del tcopies
+        res["autotuned"] = {"tuning": tuning, "ms_per_step": tms,
+                            "value": nvox / (tms * 1e-3) / 1e9, "identical": bool(same)}
     if not args.no_batch:
         res["batched"] = batched_vs_plain()
     if not args.no_serve:

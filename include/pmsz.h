@@ -140,9 +140,11 @@ int64_t pmsz_workspace_bytes(int n, const pmsz_params *params);
  * reference has no counterpart (its schedule is fixed: a full sweep per
  * iteration, c_loops pMSz.cu:1943-2060). */
 typedef struct {
-  int dense_div;     /* detection j+1 over every voxel when 15 x targets >= voxels / dense_div (16) */
-  int tail_targets;  /* the tail kernel takes over at or below this many targets (16384) */
-  int tail_blocks;   /* workgroups of the tail kernel, 1..256 (64) */
+  int dense_div;     /* detection j+1 over every voxel when 15 x targets >= voxels / dense_div (32) */
+  int tail_targets;  /* the tail kernel takes over at or below this many targets (4096) */
+  int tail_blocks;   /* workgroups (256 threads each) of the tail kernel, 1..256 (32) */
+  int solo_targets;  /* the tail continues in one workgroup, with workgroup barriers, once a
+                        sweep has at most this many targets (64; 0: never) */
   int zchunk;        /* planes per column unit of the full-volume kernel: 16, 32 or 64 (64) */
   int graphs;        /* 1: multi-domain runs (batches, subdomains) replay each chain of 8 sweeps
                         as a captured HIP graph, kept by the context per job layout (0) */
@@ -153,8 +155,8 @@ int pmsz_ctx_set_tuning(pmsz_ctx *ctx, const pmsz_tuning *t);
 int pmsz_ctx_get_tuning(pmsz_ctx *ctx, pmsz_tuning *t);
 /* Autotuner: corrects copies of the given fields (n == 1: one field, any sub;
  * n > 1: a batch, as pmsz_ctx_correct_batch) under candidate tunings -- a
- * coordinate descent over zchunk, dense_div, tail_targets, tail_blocks and (for
- * several domains) graphs, each
+ * coordinate descent over zchunk, dense_div, tail_targets, tail_blocks,
+ * solo_targets and (for several domains) graphs, each
  * candidate timed as the best of `reps` runs -- and keeps the fastest in the
  * context.  d_g0 are not modified; the library allocates the copies for the
  * duration of the call.  *best / *best_seconds (nullable) receive the result. */

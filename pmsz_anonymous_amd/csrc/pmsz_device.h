@@ -158,6 +158,7 @@ struct Sched {
   uint32_t dense_min_work;  // 15 * targets at or above this -> next detection dense
   uint32_t tail_ok;         // the cooperative tail may take over
   uint32_t tail_targets;    // the tail takes over at or below this many targets
+  uint32_t solo_targets;    // the tail continues in one workgroup at or below this many
 };
 
 // Run-wide state, written only by kernels that end a phase.

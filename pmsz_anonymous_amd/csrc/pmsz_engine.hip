@@ -68,7 +68,7 @@ constexpr int kTailThreads = 256;
 // the tail takes over once a sweep has at most this many targets: heavier work-list
 // sweeps run as separate launches, which bring more threads to their atomics
 // most blocks of the tail (its grid barriers cost about 45 ns per block on MI355X;
-// the tuning picks the grid, 64 by default)
+// the tuning picks the grid, 32 blocks of 256 threads by default)
 constexpr int kTailBlocks = 256;
 constexpr size_t kBarBytes = sizeof(uint32_t) * 16 * (kTailBlocks + 1);
 constexpr int kSyncThreads = 256;
@@ -641,7 +641,15 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
   uint32_t j = ld_agent(&run->tail_j);
   if (j == 0 || ld_agent(&run->done)) return;  // uniform: written by earlier kernels
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-  const uint32_t nthreads = gridDim.x * kTailThreads;
+  // once a sweep's targets are few, block 0 carries on alone: its phases are then
+  // separated by workgroup barriers instead of grid barriers
+  bool solo = gridDim.x == 1;
+  uint32_t nthreads = gridDim.x * kTailThreads;
+  uint32_t first = blockIdx.x * kTailThreads;
+  auto barrier = [&]() {
+    if (solo) __syncthreads();
+    else tail_barrier(bar, ++epoch);
+  };
   unsigned long long tt = 0, sv = 0;
   uint32_t touched = 0, clamped = 0;
   __shared__ uint32_t s_mode, s_done, s_nt;
@@ -667,10 +675,16 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
     }
     if (!kMulti && (done == 1 || (done == 2 && s.mode == PMSZ_SYNC))) break;
     if (kMulti && mode == kModeDone && nt == 0) break;
+    if (!solo && nt <= s.solo_targets) {  // uniform: every block read the same counters
+      if (blockIdx.x != 0) break;
+      solo = true;
+      nthreads = kTailThreads;
+      first = 0;
+    }
     if (lead && mode != kModeDone) zero_counters(&ring[(j + 1) % kCtrRing]);
     // apply sweep j, queueing the next work list unless this was the last sweep
     const bool queue = mode != kModeDone;
-    for (uint32_t base = blockIdx.x * kTailThreads; base < nt; base += nthreads) {
+    for (uint32_t base = first; base < nt; base += nthreads) {
       const uint32_t i = base + threadIdx.x;
       uint32_t u[kCand];
       int n = 0;
@@ -686,14 +700,14 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
       }
     }
     if (lead) tt += nt;
-    tail_barrier(bar, ++epoch);
+    barrier();
     if (!queue) break;  // sweep cap: sweep j's edits landed, no detection j+1
     // detection j+1 over the work list
     Counters &cn = ring[(j + 1) % kCtrRing];
     const uint32_t par = (j + 1) & 1u;
     const uint32_t nw = ld_agent(&cj.nwork);
     uint32_t lc[3] = {0, 0, 0};
-    for (uint32_t base = blockIdx.x * kTailThreads; base < nw; base += nthreads) {
+    for (uint32_t base = first; base < nw; base += nthreads) {
       const uint32_t i = base + threadIdx.x;
       uint32_t out[4];
       uint32_t vc[3] = {0, 0, 0};
@@ -713,7 +727,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
     }
     block_add3<kTailThreads>(lc, cn);
     if (lead) sv += nw;
-    tail_barrier(bar, ++epoch);
+    barrier();
     if (kMulti) {
       if (blockIdx.x == 0) {
         if (threadIdx.x == 0) {
@@ -733,7 +747,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
           cn.ncapped = s_cap;
         }
       }
-      tail_barrier(bar, ++epoch);
+      barrier();
     }
     j++;
   }
@@ -1116,9 +1130,11 @@ struct Buf {
 
 pmsz_tuning default_tuning() {
   pmsz_tuning t;
-  t.dense_div = 16;
-  t.tail_targets = 16384;
-  t.tail_blocks = 64;
+  // measured on the 512^3 and 64-field workloads (autotuner), MI355X
+  t.dense_div = 32;
+  t.tail_targets = 4096;
+  t.tail_blocks = 32;
+  t.solo_targets = 64;
   t.zchunk = kColZ;
   t.graphs = 0;
   return t;
@@ -1126,7 +1142,7 @@ pmsz_tuning default_tuning() {
 
 bool tuning_valid(const pmsz_tuning &t) {
   return t.dense_div >= 1 && t.dense_div <= 4096 && t.tail_targets >= 0 &&
-         t.tail_blocks >= 1 && t.tail_blocks <= kTailBlocks &&
+         t.tail_blocks >= 1 && t.tail_blocks <= kTailBlocks && t.solo_targets >= 0 &&
          (t.zchunk == 16 || t.zchunk == 32 || t.zchunk == 64) && (t.graphs == 0 || t.graphs == 1);
 }
 
@@ -1673,6 +1689,7 @@ Sched make_sched(const Ctx &c, const pmsz_params *p, uint64_t work_voxels, bool 
       std::max<uint64_t>(work_voxels / (uint64_t)c.tune.dense_div, 1), 0xffffffffull);
   s.tail_ok = tail_ok ? 1u : 0u;
   s.tail_targets = (uint32_t)c.tune.tail_targets;
+  s.solo_targets = (uint32_t)c.tune.solo_targets;
   return s;
 }
 
@@ -2257,9 +2274,10 @@ int pmsz_autotune(pmsz_ctx *ctx, int n, const pmsz_params *params, const void *c
   double cur_s = 0;
   rc = measure(cur, cur_s);
   const int zc[] = {64, 32, 16};
-  const int dd[] = {16, 8, 32, 4, 64};
-  const int tt[] = {16384, 4096, 65536, 1024};
-  const int tb[] = {64, 32, 128, 16};
+  const int dd[] = {32, 16, 64, 8, 128};
+  const int tt[] = {4096, 16384, 1024, 65536};
+  const int tb[] = {32, 64, 16, 128};
+  const int so[] = {64, 0, 256, 1024};
   auto sweep = [&](int pmsz_tuning::*field, const int *vals, int nv) {
     for (int v = 0; v < nv && rc == PMSZ_OK; v++) {
       if (cur.*field == vals[v]) continue;
@@ -2277,6 +2295,7 @@ int pmsz_autotune(pmsz_ctx *ctx, int n, const pmsz_params *params, const void *c
   if (rc == PMSZ_OK) sweep(&pmsz_tuning::dense_div, dd, 5);
   if (rc == PMSZ_OK) sweep(&pmsz_tuning::tail_targets, tt, 4);
   if (rc == PMSZ_OK) sweep(&pmsz_tuning::tail_blocks, tb, 4);
+  if (rc == PMSZ_OK) sweep(&pmsz_tuning::solo_targets, so, 4);
   const int gr[] = {0, 1};  // captured chains: multi-domain runs only
   if (rc == PMSZ_OK && (n > 1 || params[0].sub[0] * params[0].sub[1] * params[0].sub[2] != 1))
     sweep(&pmsz_tuning::graphs, gr, 2);

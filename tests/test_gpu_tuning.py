@@ -16,6 +16,8 @@ TUNINGS = [
     {"tail_targets": 0},          # no tail: every sweep as separate launches
     {"tail_targets": 1 << 30, "tail_blocks": 128},  # the tail from the first sparse sweep
     {"dense_div": 1 << 12},       # full-volume detections only while targets are many
+    {"solo_targets": 0},          # the tail never shrinks to one workgroup
+    {"solo_targets": 1 << 20, "tail_blocks": 4},  # ... or from its first sweep
     {"graphs": 1},                # multi-domain runs replay captured chains
     {"graphs": 1, "zchunk": 32, "tail_targets": 4096},
 ]
