@@ -159,6 +159,7 @@ struct Sched {
   uint32_t tail_ok;         // the cooperative tail may take over
   uint32_t tail_targets;    // the tail takes over at or below this many targets
   uint32_t solo_targets;    // the tail continues in one workgroup at or below this many
+  uint32_t dense_ok;        // 0: detections run over work lists only
 };
 
 // Run-wide state, written only by kernels that end a phase.
@@ -170,7 +171,7 @@ struct RunState {
   uint32_t err;        // bit 0: NaN or sentinel value in an input
   unsigned long long dense_passes;
   uint32_t bar_epoch;  // last epoch of the tail's grid barrier in this run
-  uint32_t pad0;
+  uint32_t last_mode;  // the mode the latest apply decided
   unsigned long long pad[4];
   // by stripe: edits applied, voxels edited at least once, voxels clamped,
   // voxels re-examined by work-list sweeps
@@ -405,6 +406,68 @@ __device__ __forceinline__ uint32_t block_reserve(uint32_t n, uint32_t *counter)
   __syncthreads();
   uint32_t pos = s_base + s_off[wave] + incl - n;
   __syncthreads();
+  return pos;
+}
+
+// A block's list entries staged in LDS across the iterations of a grid-stride
+// loop and copied to the global list in large pieces, so the block makes one
+// reservation (a same-address atomic) per kCap entries instead of one per
+// iteration.  Entries keep no order.
+template <int kBlock, int kCap>
+struct StageLds {
+  uint32_t buf[kCap];
+  uint32_t off[kBlock / 64];
+  uint32_t fill, total, base;
+};
+
+template <int kBlock, int kCap>
+__device__ __forceinline__ void stage_init(StageLds<kBlock, kCap> &st) {
+  if (threadIdx.x == 0) st.fill = 0;
+  __syncthreads();
+}
+
+// Copies the staged entries to list[atomicAdd(counter, fill) ...]; every thread
+// of the block calls it.
+template <int kBlock, int kCap>
+__device__ __forceinline__ void stage_flush(StageLds<kBlock, kCap> &st, uint32_t *list,
+                                            uint32_t *counter) {
+  __syncthreads();  // the staged writes are complete
+  const uint32_t fill = st.fill;
+  if (threadIdx.x == 0) st.base = fill ? atomicAdd(counter, fill) : 0u;
+  __syncthreads();
+  const uint32_t base = st.base;
+  for (uint32_t k = threadIdx.x; k < fill; k += kBlock) list[base + k] = st.buf[k];
+  __syncthreads();  // every thread has read fill, base and the entries
+  if (threadIdx.x == 0) st.fill = 0;
+  __syncthreads();
+}
+
+// Room for n entries of this thread (n * kBlock <= kCap); returns the first
+// buffer position.  Flushes first when the block's entries would not fit.
+// Every thread of the block calls it; the caller then writes st.buf[pos ...].
+template <int kBlock, int kCap>
+__device__ __forceinline__ uint32_t stage_reserve(StageLds<kBlock, kCap> &st, uint32_t n,
+                                                  uint32_t *list, uint32_t *counter) {
+  constexpr int kWaves = kBlock / 64;
+  const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
+  const uint32_t incl = wave_incl_scan(n);
+  if (lane == 63) st.off[wave] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int w = 0; w < kWaves; w++) {
+      const uint32_t v = st.off[w];
+      st.off[w] = run;
+      run += v;
+    }
+    st.total = run;
+  }
+  __syncthreads();
+  const uint32_t total = st.total;
+  if (st.fill + total > (uint32_t)kCap) stage_flush(st, list, counter);  // uniform
+  const uint32_t pos = st.fill + st.off[wave] + incl - n;
+  __syncthreads();  // every thread has read fill and off
+  if (threadIdx.x == 0) st.fill += total;
   return pos;
 }
 

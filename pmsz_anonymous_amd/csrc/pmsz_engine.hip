@@ -72,6 +72,10 @@ constexpr int kTailThreads = 256;
 constexpr int kTailBlocks = 256;
 constexpr size_t kBarBytes = sizeof(uint32_t) * 16 * (kTailBlocks + 1);
 constexpr int kSyncThreads = 256;
+// One domain: work-list sweeps build their lists by appending the bits that
+// returning atomics claimed (apply: the targets' neighbourhoods in wmark;
+// detection: targets in tmark), instead of compacting the bitmaps afterwards.
+constexpr bool kListAppend = true;
 constexpr uint32_t kMaxDomains = 1024;  // domain bookkeeping: one thread each
 static_assert(kMaxDomains <= (uint32_t)kCompactDoms, "compaction domain table");
 constexpr int kMaxContexts = 8;         // live contexts, the default one included
@@ -112,7 +116,7 @@ __device__ __forceinline__ uint32_t decide(const Counters &cj, uint32_t j, const
     *done = 2;
     return kModeDone;
   }
-  const bool dense = (uint64_t)nt * kCand >= s.dense_min_work;
+  const bool dense = s.dense_ok && (uint64_t)nt * kCand >= s.dense_min_work;
   if (tail_allowed && !dense && nt <= s.tail_targets) return kModeTail;
   if (s.mode == PMSZ_LAZY && j >= s.max_sweeps) {
     *done = 2;  // the edits of sweep j still land (they belong to sweep j)
@@ -132,7 +136,7 @@ __device__ __forceinline__ uint32_t decide_multi(const Counters &cj, const Sched
     return kModeDone;
   }
   const uint32_t nt = ld_agent(&cj.ntargets);
-  const bool dense = (uint64_t)nt * kCand >= s.dense_min_work;
+  const bool dense = s.dense_ok && (uint64_t)nt * kCand >= s.dense_min_work;
   if (tail_allowed && !dense && nt <= s.tail_targets) return kModeTail;
   return dense ? kModeDense : kModeSparse;
 }
@@ -282,6 +286,48 @@ __device__ __forceinline__ void mark_neighbourhood(uint32_t t, const Dom &D, uin
   mark_neighbourhood(t, x, y, z, D, wmark);
 }
 
+// The same row-wise marks with returning atomics.  Claims: per stencil row, the
+// bits of words w and w + 1 that this call set (claims of other threads
+// excluded).
+struct Claims {
+  uint32_t w[7], lo[7], hi[7];
+  __device__ __forceinline__ uint32_t count() const {
+    uint32_t n = 0;
+#pragma unroll
+    for (int k = 0; k < 7; k++) n += __popc(lo[k]) + __popc(hi[k]);
+    return n;
+  }
+  // the claimed ids, in row order, to list[pos ...]
+  __device__ __forceinline__ void emit(uint32_t *list, uint32_t pos) const {
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      for (uint32_t b = lo[k]; b; b &= b - 1) list[pos++] = w[k] * 32u + (uint32_t)(__ffs(b) - 1);
+      for (uint32_t b = hi[k]; b; b &= b - 1)
+        list[pos++] = (w[k] + 1) * 32u + (uint32_t)(__ffs(b) - 1);
+    }
+  }
+};
+__device__ __forceinline__ void claim_neighbourhood(uint32_t t, const Dom &D, uint32_t *wmark,
+                                                    Claims &c) {
+  int64_t x, y, z;
+  decompose(t, D.d, x, y, z);
+  const uint32_t vm = box_mask(x, y, z, D.lo, D.hi);
+  constexpr int kRowFirst[7] = {0, 2, 4, 6, 9, 11, 13};
+  constexpr int kRowLen[7] = {2, 2, 2, 3, 2, 2, 2};
+  // every claim is issued before any result is looked at, so the atomics overlap
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    const uint32_t rb = (vm >> kRowFirst[k]) & ((1u << kRowLen[k]) - 1u);
+    const uint32_t v0 = D.gbase + (uint32_t)((int64_t)t + rank_off(kRowFirst[k], D.d));
+    const uint64_t m = (uint64_t)rb << (v0 & 31u);
+    c.w[k] = v0 >> 5;
+    c.lo[k] = (uint32_t)m ? atomicOr(&wmark[c.w[k]], (uint32_t)m) : 0u;
+    c.hi[k] = (m >> 32) ? atomicOr(&wmark[c.w[k] + 1], (uint32_t)(m >> 32)) : 0u;
+    c.lo[k] = (uint32_t)m & ~c.lo[k];
+    c.hi[k] = (uint32_t)(m >> 32) & ~c.hi[k];
+  }
+}
+
 // Detection at one listed voxel (global loads); its targets are marked in tmark
 // without looking at the old bits (the list is compacted afterwards).  In a
 // padded block a neighbour holding the sentinel is missing (pMSz.cu:709); a
@@ -322,7 +368,7 @@ __device__ __forceinline__ void detect_mark(const Dom &D, uint32_t *tmark, uint3
 // takes kBitsPT words per step, interleaved across the block so loads coalesce;
 // a block makes one list reservation per step.
 constexpr int kBitsThreads = 256;
-constexpr int kBitsPT = 16;
+constexpr int kBitsPT = 64;
 __global__ __launch_bounds__(kBitsThreads) void k_compact_bits(
     uint32_t *__restrict__ bits, const Dom *__restrict__ doms, uint32_t ndom, uint32_t wdense,
     uint32_t *__restrict__ list, uint32_t *__restrict__ count, const Counters *__restrict__ gate,
@@ -462,6 +508,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
       kMulti ? decide_multi(cj, s, s.tail_ok != 0, &done) : decide<false>(cj, j, s, true, &done);
   if (lead) {
     cj.mode = mode;
+    run->last_mode = mode;
     if (mode == kModeTail) run->tail_j = j;
     if (done) {
       run->done = done;
@@ -476,7 +523,30 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
   const bool queue = mode == kModeSparse;
   uint32_t touched = 0, clamped = 0;
   const uint32_t stride = gridDim.x * kApplyThreads;
-  if (!kMulti && queue) {
+  // one domain: the targets of a work-list detection are claimed in tmark; every
+  // set bit is a target of this sweep, so their words are cleared with plain stores
+  const bool clear = !kMulti && kListAppend && pm == kModeSparse;
+  if (!kMulti && queue && kListAppend) {
+    // one domain, work-list detection next: the edit, then the neighbourhood's
+    // claims in wmark, listed at once (staged per block)
+    __shared__ StageLds<kApplyThreads, kApplyThreads * kCand> st;
+    stage_init(st);
+    for (uint32_t base = blockIdx.x * kApplyThreads; base < nt; base += stride) {
+      const uint32_t i = base + threadIdx.x;
+      Claims cl;
+      uint32_t n = 0;
+      if (i < nt) {
+        const uint32_t t = tlist[i];
+        apply_one<T>((T *)d0.g, (const T *)d0.f, d0.dc, t, d0.ep, touched, clamped);
+        if (clear) tmark[t >> 5] = 0u;
+        claim_neighbourhood(t, d0, wmark, cl);
+        n = cl.count();
+      }
+      const uint32_t pos = stage_reserve(st, n, wlist, &cj.nwork);
+      if (n) cl.emit(st.buf, pos);
+    }
+    stage_flush(st, wlist, &cj.nwork);
+  } else if (!kMulti && queue) {
     // one domain, work-list detection next: the edit, then the neighbourhood's
     // bits in wmark (the list is compacted from the bitmap afterwards)
     for (uint32_t i = blockIdx.x * kApplyThreads + threadIdx.x; i < nt; i += stride) {
@@ -512,6 +582,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
       for (int q = 0; q < kU; q++) {
         const uint32_t i = base + (uint32_t)q * kApplyThreads + threadIdx.x;
         t[q] = i < nt ? tlist[i] : 0xffffffffu;
+        if (clear && t[q] != 0xffffffffu) tmark[t[q] >> 5] = 0u;
       }
 #pragma unroll
       for (int q = 0; q < kU; q++) {
@@ -572,8 +643,26 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
   const uint32_t nw = cj.nwork;
   uint32_t lc[3] = {0, 0, 0};
   const uint32_t stride = gridDim.x * kApplyThreads;
-  // targets go to tmark bits, compacted into tlist afterwards
-  if (!kMulti) {
+  if (!kMulti && kListAppend) {
+    // targets claimed in tmark and listed at once; every set bit of wmark is a
+    // listed voxel of this sweep, so its words are cleared with plain stores
+    __shared__ StageLds<kApplyThreads, kApplyThreads * 4> st;
+    stage_init(st);
+    for (uint32_t base = blockIdx.x * kApplyThreads; base < nw; base += stride) {
+      const uint32_t i = base + threadIdx.x;
+      uint32_t out[4];
+      int n = 0;
+      if (i < nw) {
+        const uint32_t v = wlist[i];
+        wmark[v >> 5] = 0u;
+        n = detect_one<T>(d0, tmark, v, false, true, lc, out);
+      }
+      const uint32_t pos = stage_reserve(st, (uint32_t)n, tlist, &cn.ntargets);
+      for (int q = 0; q < n; q++) st.buf[pos + q] = out[q];
+    }
+    stage_flush(st, tlist, &cn.ntargets);
+  } else if (!kMulti) {
+    // targets go to tmark bits, compacted into tlist afterwards
     for (uint32_t i = blockIdx.x * kApplyThreads + threadIdx.x; i < nw; i += stride)
       detect_mark<T>(d0, tmark, wlist[i], false, true, lc);
   } else {
@@ -1479,11 +1568,17 @@ int drive(Ctx &c, Job &job, Outcome &out) {
 
   uint32_t j0 = 1;
   out.outer_started = 1;
+  // one domain: once a chain has ended in a work-list sweep, the next chain holds
+  // work-list sweeps only (its applies may not choose a full-volume detection,
+  // which gives the same results), so no launch of it waits for a dense phase
+  bool sparse_chain = false;
+  Sched s_sparse = job.s;
+  s_sparse.dense_ok = 0;
   while (true) {
     // the chain of sweeps j0 .. j0 + kChain - 1, enqueued on the context's stream
     auto enqueue_chain = [&](uint32_t j0) -> int {
       for (uint32_t j = j0; j < j0 + kChain; j++) {
-        if (!kMulti) {
+        if (!kMulti && !sparse_chain) {
           // after a full-volume detection j: its target list
           k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
               c.tbyte.p, c.tlist.p, c.ring, (j - 1) % kCtrRing, j % kCtrRing, false,
@@ -1491,7 +1586,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
         }
         k_apply<T, kMulti><<<g_apply, kApplyThreads, 0, c.stream>>>(
             d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j,
-            job.s);
+            sparse_chain ? s_sparse : job.s);
         if (sync_every) {
           k_pack<T><<<g_shell, kSyncThreads, 0, c.stream>>>(doms, ds, job.gn[0], job.gn[1],
                                                             job.gn[2], c.ring, j, 1);
@@ -1500,15 +1595,17 @@ int drive(Ctx &c, Job &job, Outcome &out) {
         }
         // the work list of detection j+1 from the bitmap that apply j (and a ghost
         // sync) marked
-        k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
+        if (kMulti || !kListAppend)
+          k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
             c.wmark.p, doms, ndom, (uint32_t)job.wdense, c.wlist.p, &c.ring[j % kCtrRing].nwork,
             &c.ring[j % kCtrRing], kModeSparse);
         // the tags start over: no mark of an earlier sweep may carry detection j+1's
         if (detect_tag(j + 1) == 1u)
           k_fill16<<<g_fill, 256, 0, c.stream>>>((uint4 *)c.tbyte.p, job.groups);
-        k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
-            d0, doms, ndom, ds, c.tbyte.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, (j + 1) & 1u,
-            c.run, detect_tag(j + 1));
+        if (!sparse_chain)
+          k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
+              d0, doms, ndom, ds, c.tbyte.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing,
+              (j + 1) & 1u, c.run, detect_tag(j + 1));
         if (kMulti)
           k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
               c.tbyte.p, c.tlist.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false,
@@ -1516,7 +1613,8 @@ int drive(Ctx &c, Job &job, Outcome &out) {
         k_detect_list<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
             d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j);
         // the targets of a work-list detection j+1, from the bitmap
-        k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
+        if (kMulti || !kListAppend)
+          k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
             c.tmark.p, doms, ndom, (uint32_t)job.wdense, c.tlist.p,
             &c.ring[(j + 1) % kCtrRing].ntargets, &c.ring[j % kCtrRing], kModeSparse);
         if (kMulti)
@@ -1536,7 +1634,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
           (uint64_t)(uintptr_t)c.tlist.p, (uint64_t)(uintptr_t)c.wmark.p,
           (uint64_t)(uintptr_t)c.wlist.p, (uint64_t)(uintptr_t)c.ring, (uint64_t)(uintptr_t)c.run,
           (uint64_t)job.s.mode, job.s.max_sweeps, job.s.dense_min_work, job.s.tail_ok,
-          job.s.tail_targets, g_compact, g_apply, g_bits, g_fill, g_dense, g_list, g_shell.x,
+          job.s.tail_targets, job.s.solo_targets, job.s.dense_ok, g_compact, g_apply, g_bits, g_fill, g_dense, g_list, g_shell.x,
           g_shell.y, sync_every ? 1u : 0u, (uint64_t)sx, (uint64_t)sy, (uint64_t)sz,
           (uint64_t)job.ndir, (uint64_t)job.gn[0], (uint64_t)job.gn[1], (uint64_t)job.gn[2]};
       int rc = chain_graph(c, key, j0, enqueue_chain);
@@ -1568,6 +1666,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
     if (!c.h_run->done) {
       // the chain ended before the tail took over: continue it where it stopped
       j0 += kChain;
+      sparse_chain = !kMulti && c.h_run->last_mode == kModeSparse;
       continue;
     }
     const uint32_t m = c.h_run->last_j;
@@ -1690,6 +1789,7 @@ Sched make_sched(const Ctx &c, const pmsz_params *p, uint64_t work_voxels, bool 
   s.tail_ok = tail_ok ? 1u : 0u;
   s.tail_targets = (uint32_t)c.tune.tail_targets;
   s.solo_targets = (uint32_t)c.tune.solo_targets;
+  s.dense_ok = 1;
   return s;
 }
 
