@@ -537,9 +537,9 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
         const int e = p - 1;  // plane completed at this step
         const bool emitp = e >= zs;
         const uint32_t tself0 = lane + 5u * (uint32_t)(p - zs);
-        // row extremes: A over (x, x+1), B over (x-1..x+1), C over (x-1, x) of
-        // rows k-2 (A only) and k-1
-        double a2x = 0, a2n = 0, a1x = 0, a1n = 0, b1x = 0, b1n = 0, c1x = 0, c1n = 0;
+        // row extremes: A over (x, x+1) and C over (x-1, x) of rows k-2 (A only)
+        // and k-1; the row's own key k1 and right neighbour r1 of row k-1
+        double a2x = 0, a2n = 0, a1x = 0, a1n = 0, c1x = 0, c1n = 0, k1 = 0, r1 = 0;
 #pragma unroll
         for (int k = 0; k < kColH; k++) {
           const bool ok = valid(pin && rok[k], rv[k]);
@@ -554,12 +554,15 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
           const double l = from_left(kk), r = from_right(kk);
           const double ax = kmax(kk, r), an = kmin(kk, r);
           const double cx = kmax(l, kk), cn = kmin(l, kk);
-          const double bx = kmax(ax, l), bn = kmin(an, l);
           if (k >= 2) {
             const int h = k - 1;  // voxel row completed by row k
+            // with y the voxel row: D = C(y) + C(y+1), the plane's share for the
+            // voxel in the previous plane; Q = A(y-1) + the cell right of the voxel
+            const double dx = kmax(c1x, cx), dn = kmin(c1n, cn);
+            const double qx = kmax(a2x, r1), qn = kmin(a2n, r1);
             if (emitp) {
-              const double wmx = kmax(pt_mx[h - 1], kmax(c1x, cx));
-              const double wmn = kmin(pt_mn[h - 1], kmin(c1n, cn));
+              const double wmx = kmax(pt_mx[h - 1], dx);
+              const double wmn = kmin(pt_mn[h - 1], dn);
               const uint32_t ts = tself0 + 3u * (uint32_t)h;
               if (kLabel) {
                 label(h - 1, e, key_rank(wmx, ts), key_rank(wmn, ts));
@@ -576,16 +579,17 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
                 });
               }
             }
-            // plane p's partials: mid for plane p, lo for plane p+1
-            pt_mx[h - 1] = kmax(lo_mx[h - 1], kmax(a2x, kmax(b1x, cx)));
-            pt_mn[h - 1] = kmin(lo_mn[h - 1], kmin(a2n, kmin(b1n, cn)));
-            lo_mx[h - 1] = kmax(a2x, a1x);
-            lo_mn[h - 1] = kmin(a2n, a1n);
+            // plane p's partials: mid for plane p (A(y-1), B(y) = C(y) + the right
+            // cell, C(y+1)), lo for plane p+1 (A(y-1) + A(y) = Q + the cell itself)
+            pt_mx[h - 1] = kmax(lo_mx[h - 1], kmax(dx, qx));
+            pt_mn[h - 1] = kmin(lo_mn[h - 1], kmin(dn, qn));
+            lo_mx[h - 1] = kmax(qx, k1);
+            lo_mn[h - 1] = kmin(qn, k1);
           }
           a2x = a1x; a2n = a1n;
           a1x = ax; a1n = an;
-          b1x = bx; b1n = bn;
           c1x = cx; c1n = cn;
+          k1 = kk; r1 = r;
         }
       };
       // two register sets for the planes, alternating: no copies between steps
