@@ -441,10 +441,6 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
       vis[h] = owned && y < D.hi[1];
       coff[h] = vis[h] ? (uint32_t)x + nx * (uint32_t)y : 0u;
     }
-    // the voxel rows this lane resolves, as wave masks
-    uint64_t vmask[kColRows];
-#pragma unroll
-    for (int h = 0; h < kColRows; h++) vmask[h] = __ballot(vis[h]);
     // raw values of plane pz into nv; the plane's validity is uniform over the
     // wave, so an out-of-domain plane is a branch, not a select
     auto fetch = [&](int pz, T (&nv)[kColH]) {
@@ -453,11 +449,11 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
 #pragma unroll
       for (int h = 0; h < kColH; h++) plane_load(rs, roff[h], nv[h]);
     };
-    auto fetch_codes = [&](int pz, uint8_t (&nc)[kColRows]) {  // pz lies in the domain
+    auto fetch_codes = [&](int pz, uint32_t (&nc)[kColRows]) {  // pz lies in the domain
       const auto rs = plane_rsrc(code + (uint64_t)pz * sxy);
 #pragma unroll
       for (int h = 0; h < kColRows; h++)
-        nc[h] = (uint8_t)__builtin_amdgcn_raw_buffer_load_b8(rs, coff[h], 0, 0);
+        nc[h] = __builtin_amdgcn_raw_buffer_load_b8(rs, coff[h], 0, 0);
     };
     // the value's validity: a padded block's sentinel cells carry no value; in
     // a field, NaN or the sentinel is an input error
@@ -477,8 +473,10 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
         const int oa = (int)(q.y & 15u), od = (int)((q.y >> 4) & 15u);
         int da, dd;
         if constexpr (kF32) {
-          da = diff_rank((int)((q.y >> 8) & 31u));
-          dd = diff_rank((int)((q.y >> 13) & 31u));
+          // bytes 1, 2: the winners' tag differences less the original winners',
+          // mod 256 (the differences themselves lie in 0..16)
+          da = diff_rank((int)min(((q.y >> 8) + tag_diff(oa)) & 255u, 16u));
+          dd = diff_rank((int)min(((q.y >> 16) + tag_diff(od)) & 255u, 16u));
         } else {
           da = (int)((q.y >> 8) & 15u);
           dd = (int)((q.y >> 12) & 15u);
@@ -494,10 +492,11 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
       }
       qh += count;
     };
-    // append the lanes of wave mask bm; `info` is evaluated only if bm is not 0
-    auto enqueue = [&](uint64_t bm, int h, int e, auto info) {
+    // append the lanes of wave mask bm (mine: this lane's bit); `info` is
+    // evaluated only if bm is not 0
+    auto enqueue = [&](uint64_t bm, bool mine, int h, int e, auto info) {
       if (bm == 0) return;  // uniform
-      if ((bm >> lane) & 1u) {
+      if (mine) {
         const uint32_t pos = qt + __builtin_amdgcn_mbcnt_hi(
                                       (uint32_t)(bm >> 32),
                                       __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
@@ -526,8 +525,8 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
       for (int h = 0; h < kColRows; h++) lo_mx[h] = lo_mn[h] = kNaN;
       // plane step p: rows rv of plane p are in registers, cc holds the codes of
       // plane p-1; plane p+1 is loaded into nv and the codes of plane p into nc
-      auto step = [&](int p, const T (&rv)[kColH], const uint8_t (&cc)[kColRows],
-                      T (&nv)[kColH], uint8_t (&nc)[kColRows]) {
+      auto step = [&](int p, const T (&rv)[kColH], const uint32_t (&cc)[kColRows],
+                      T (&nv)[kColH], uint32_t (&nc)[kColRows]) {
         const bool pin = p >= 0 && p < (int)nz;  // uniform
         if (p < ze) {
           fetch(p + 1, nv);
@@ -569,13 +568,13 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
               } else {
                 const uint32_t c = cc[h - 1];
                 const uint32_t ex = s_exp[c] + ts * 0x10001u;
-                const uint32_t wa = (uint32_t)__double2loint(wmx);
-                const uint32_t wd = (uint32_t)__double2loint(wmn);
-                const uint64_t bm =
-                    __ballot((((wa - (ex & 0xffffu)) | (wd - (ex >> 16))) & 511u) != 0u) &
-                    vmask[h - 1];
-                enqueue(bm, h - 1, e, [&] {
-                  return c | (((wa - ts + 8u) & 31u) << 8) | (((wd - ts + 8u) & 31u) << 13);
+                const uint32_t ra = (uint32_t)__double2loint(wmx) - (ex & 0xffffu);
+                const uint32_t rd = (uint32_t)__double2loint(wmn) - (ex >> 16);
+                const bool fail = ((ra | rd) & 511u) != 0u && vis[h - 1];
+                const uint64_t bm = __ballot(fail);
+                // info: the code byte, then the low bytes of ra and rd
+                enqueue(bm, fail, h - 1, e, [&] {
+                  return c | (__builtin_amdgcn_perm(rd, ra, 0x0c0c0400u) << 8);
                 });
               }
             }
@@ -594,7 +593,8 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
       };
       // two register sets for the planes, alternating: no copies between steps
       T va[kColH], vb[kColH];
-      uint8_t ca[kColRows] = {}, cb[kColRows] = {};
+      // the code bytes, one per register (packed bytes would cost an extract each)
+      uint32_t ca[kColRows] = {}, cb[kColRows] = {};
       fetch(zs - 1, va);
       int p = zs - 1;
       for (; p + 1 <= ze; p += 2) {
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
 #pragma unroll
       for (int h = 0; h < kColRows; h++) lo_e[h].first(kNaN, 0);
       T nv[kColH];
-      uint8_t ccode[kColRows], ncode[kColRows] = {};
+      uint32_t ccode[kColRows], ncode[kColRows] = {};
       fetch(zs - 1, nv);
       for (int p = zs - 1; p <= ze; p++) {
         ColPlane cp;
@@ -641,9 +641,9 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
             } else {
               const uint32_t c = ccode[h - 1];
               const uint32_t w = (uint32_t)da | ((uint32_t)dd << 4);
-              const uint64_t bm =
-                  __ballot(w != c || c == (uint32_t)(kSelf | (kSelf << 4))) & vmask[h - 1];
-              enqueue(bm, h - 1, e, [&] { return c | (w << 8); });
+              const bool fail = (w != c || c == (uint32_t)(kSelf | (kSelf << 4))) && vis[h - 1];
+              const uint64_t bm = __ballot(fail);
+              enqueue(bm, fail, h - 1, e, [&] { return c | (w << 8); });
             }
           }
           // partials of plane p for the voxels of planes p and p+1
