@@ -148,6 +148,9 @@ typedef struct {
   int zchunk;        /* planes per column unit of the full-volume kernel: 16, 32 or 64 (64) */
   int graphs;        /* 1: multi-domain runs (batches, subdomains) replay each chain of 8 sweeps
                         as a captured HIP graph, kept by the context per job layout (0) */
+  int stream_div;    /* one field: the targets of a full-volume detection with at least
+                        voxels / stream_div violations are edited by one pass over the target
+                        byte map instead of a target list (16; 0: never) */
 } pmsz_tuning;
 void pmsz_default_tuning(pmsz_tuning *t);
 /* ctx NULL: the default context. */
@@ -156,7 +159,7 @@ int pmsz_ctx_get_tuning(pmsz_ctx *ctx, pmsz_tuning *t);
 /* Autotuner: corrects copies of the given fields (n == 1: one field, any sub;
  * n > 1: a batch, as pmsz_ctx_correct_batch) under candidate tunings -- a
  * coordinate descent over zchunk, dense_div, tail_targets, tail_blocks,
- * solo_targets and (for several domains) graphs, each
+ * solo_targets, (for one field) stream_div and (for several domains) graphs, each
  * candidate timed as the best of `reps` runs -- and keeps the fastest in the
  * context.  d_g0 are not modified; the library allocates the copies for the
  * duration of the call.  *best / *best_seconds (nullable) receive the result. */

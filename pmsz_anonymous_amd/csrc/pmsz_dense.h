@@ -209,8 +209,9 @@ __device__ __forceinline__ uint32_t byte_bits(uint32_t t) {
 __global__ __launch_bounds__(kCompactThreads) void k_compact(
     const uint8_t *__restrict__ tbyte, uint32_t *__restrict__ tlist, Counters *__restrict__ ring,
     uint32_t jslot_prev, uint32_t jslot, bool always, uint32_t ngroups,
-    const Dom *__restrict__ doms, uint32_t ndom, uint32_t tag) {
+    const Dom *__restrict__ doms, uint32_t ndom, uint32_t tag, uint32_t stream_min) {
   if (!always && ring[jslot_prev].mode != kModeDense) return;  // uniform
+  if (stream_min && cnt_sum(ring[jslot]) >= stream_min) return;  // the apply streams the map
   // the domains' first cells and ids, for the cell -> id mapping (ndom <= 1024)
   __shared__ uint32_t s_cb[kCompactDoms], s_gb[kCompactDoms];
   if (ndom > 1) {

@@ -142,6 +142,13 @@ __device__ __forceinline__ uint32_t cnt_any(const Counters &c) {
   for (int k = 0; k < kStripes; k++) s |= c.cs[k][0] | c.cs[k][1] | c.cs[k][2];
   return s;
 }
+// Violations of a detection (plain loads: after a kernel boundary).
+__device__ __forceinline__ uint32_t cnt_sum(const Counters &c) {
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kStripes; k++) s += c.cs[k][0] + c.cs[k][1] + c.cs[k][2];
+  return s;
+}
 __device__ __forceinline__ uint32_t cnt_any_agent(const Counters &c) {
   return cnt_get(c, 0) | cnt_get(c, 1) | cnt_get(c, 2);
 }
@@ -160,6 +167,9 @@ struct Sched {
   uint32_t tail_targets;    // the tail takes over at or below this many targets
   uint32_t solo_targets;    // the tail continues in one workgroup at or below this many
   uint32_t dense_ok;        // 0: detections run over work lists only
+  uint32_t stream_min;      // one domain: the targets of a full-volume detection with at
+                            // least this many violations are edited by one pass over the
+                            // target byte map instead of a list
 };
 
 // Run-wide state, written only by kernels that end a phase.

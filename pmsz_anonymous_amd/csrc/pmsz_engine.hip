@@ -125,6 +125,22 @@ __device__ __forceinline__ uint32_t decide(const Counters &cj, uint32_t j, const
   return dense ? kModeDense : kModeSparse;
 }
 
+// One domain, the targets of detection j streamed from the byte map (their
+// number is counted while they are edited): detection j+1 runs over every voxel.
+__device__ __forceinline__ uint32_t decide_stream(const Counters &cj, uint32_t j, const Sched &s,
+                                                  uint32_t *done) {
+  *done = 0;
+  if (cnt_any(cj) == 0) {
+    *done = 1;
+    return kModeDone;
+  }
+  if ((s.mode == PMSZ_SYNC && j > s.max_sweeps) || (s.mode == PMSZ_LAZY && j >= s.max_sweeps)) {
+    *done = 2;  // lazy: the edits of sweep j still land
+    return kModeDone;
+  }
+  return kModeDense;
+}
+
 // Several domains: from the domain bookkeeping after detection j (the caps are
 // per domain, applied there).  The targets of detection j are always applied:
 // a domain that must not edit them emitted none.
@@ -487,6 +503,88 @@ __device__ __forceinline__ int apply_target(uint32_t t, const Dom &d0, const Dom
 }
 
 // ---------------------------------------------------------------------------
+// Edits of the targets of a full-volume detection, streamed from the target
+// byte map (one domain): a thread takes 16 cells at a time, loads their tags,
+// and where any cell carries the sweep's tag loads those cells' f, g and dc as
+// vectors (by halves of 8 cells, skipping a half without targets), edits the
+// tagged cells and stores the vectors back.  The 16 cells belong to that thread
+// alone, so the stores race with nothing.  Counts the targets (for the
+// statistics) into cj.ntargets.
+template <typename T>
+__device__ __forceinline__ void apply_stream(const Dom &d0, const uint8_t *__restrict__ tbyte,
+                                             uint32_t ngroups, uint32_t tag, Counters &cj,
+                                             RunState *run) {
+  constexpr int kVec = 8 * (int)sizeof(T) / 16;  // uint4 vectors per 8 cells of T
+  T *g = (T *)d0.g;
+  const T *f = (const T *)d0.f;
+  uint8_t *dc = d0.dc;
+  const uint64_t n = (uint64_t)d0.d.nx * (uint64_t)d0.d.ny * (uint64_t)d0.d.nz;
+  const uint64_t nfull = n / 16;  // groups whose 16 cells all lie in the field
+  const uint32_t rep = tag * 0x01010101u;
+  const uint4 *t16 = reinterpret_cast<const uint4 *>(tbyte);
+  uint32_t touched = 0, clamped = 0, count = 0;
+  const uint32_t stride = gridDim.x * kApplyThreads;
+  for (uint32_t gi = blockIdx.x * kApplyThreads + threadIdx.x; gi < ngroups; gi += stride) {
+    const uint4 tv = t16[gi];
+    const uint32_t m = byte_bits(tag_bytes(tv.x, rep)) | (byte_bits(tag_bytes(tv.y, rep)) << 4) |
+                       (byte_bits(tag_bytes(tv.z, rep)) << 8) |
+                       (byte_bits(tag_bytes(tv.w, rep)) << 12);
+    if (!m) continue;
+    count += __popc(m);
+    const uint64_t c0 = (uint64_t)gi * 16u;
+    if (gi < nfull) {
+      const uint4 cv = reinterpret_cast<const uint4 *>(dc)[gi];
+      uint32_t cw[4] = {cv.x, cv.y, cv.z, cv.w};
+#pragma unroll
+      for (int hf = 0; hf < 2; hf++) {
+        const uint32_t mh = (m >> (8 * hf)) & 0xffu;
+        if (!mh) continue;
+        union {
+          uint4 v[kVec];
+          T x[8];
+        } fv, gv;
+        const uint4 *fp = reinterpret_cast<const uint4 *>(f + c0 + 8 * hf);
+        uint4 *gp = reinterpret_cast<uint4 *>(g + c0 + 8 * hf);
+#pragma unroll
+        for (int k = 0; k < kVec; k++) {
+          fv.v[k] = fp[k];
+          gv.v[k] = gp[k];
+        }
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+          if ((mh >> b) & 1u) {
+            const int w = 2 * hf + (b >> 2);
+            const uint32_t sh = 8u * (uint32_t)(b & 3);
+            const uint8_t c0b = (uint8_t)(cw[w] >> sh);
+            uint8_t c = c0b;
+            edit_value<T>(fv.x[b], gv.x[b], c, d0.ep);
+            cw[w] = (cw[w] & ~(0xffu << sh)) | ((uint32_t)c << sh);
+            touched += c0b == 0;
+            clamped += (c == d0.ep.q && c0b != d0.ep.q);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < kVec; k++) gp[k] = gv.v[k];
+      }
+      reinterpret_cast<uint4 *>(dc)[gi] = make_uint4(cw[0], cw[1], cw[2], cw[3]);
+    } else {
+      // the field's last, partial group: cell by cell (tags lie on cells only)
+      for (uint32_t mm = m; mm; mm &= mm - 1) {
+        const uint32_t t = (uint32_t)(c0 + (uint32_t)(__ffs(mm) - 1));
+        apply_one<T>(g, f, dc, t, d0.ep, touched, clamped);
+      }
+    }
+  }
+  const uint32_t ts = wave_sum(touched), cs = wave_sum(clamped), ns = wave_sum(count);
+  if (lane_id() == 0) {
+    acc_add(run, kAccTouched, ts);
+    acc_add(run, kAccClamped, cs);
+    acc_add(run, kAccTargets, ns);
+    if (ns) atomicAdd(&cj.ntargets, ns);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Apply of sweep j (grid-stride over its targets), queueing the next work list
 // when detection j+1 is sparse.
 template <typename T, bool kMulti>
@@ -494,7 +592,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
     Dom d0, const Dom *__restrict__ doms, uint32_t shift, DomState *__restrict__ ds,
     uint32_t *__restrict__ tmark, const uint32_t *__restrict__ tlist,
     uint32_t *__restrict__ wmark, uint32_t *__restrict__ wlist, Counters *__restrict__ ring,
-    RunState *run, uint32_t j, Sched s) {
+    RunState *run, uint32_t j, Sched s, const uint8_t *__restrict__ tbyte, uint32_t ngroups) {
   Counters &prev = ring[(j - 1) % kCtrRing];
   Counters &cj = ring[j % kCtrRing];
   const uint32_t pm = prev.mode;
@@ -503,9 +601,13 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
     if (lead) cj.mode = pm == kModeUnset ? kModeDone : pm;
     return;
   }
+  // one domain, many targets of a full-volume detection: no list was made; the
+  // byte map is streamed (the same decision as the compaction's)
+  const bool stream = !kMulti && pm == kModeDense && s.stream_min && cnt_sum(cj) >= s.stream_min;
   uint32_t done;
-  const uint32_t mode =
-      kMulti ? decide_multi(cj, s, s.tail_ok != 0, &done) : decide<false>(cj, j, s, true, &done);
+  const uint32_t mode = kMulti   ? decide_multi(cj, s, s.tail_ok != 0, &done)
+                        : stream ? decide_stream(cj, j, s, &done)
+                                 : decide<false>(cj, j, s, true, &done);
   if (lead) {
     cj.mode = mode;
     run->last_mode = mode;
@@ -517,6 +619,11 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
   }
   if (mode == kModeTail) return;
   if (!kMulti && mode == kModeDone && (done == 1 || s.mode == PMSZ_SYNC)) return;
+  if (stream) {
+    if (lead && mode != kModeDone) zero_counters(&ring[(j + 1) % kCtrRing]);
+    apply_stream<T>(d0, tbyte, ngroups, detect_tag(j), cj, run);
+    return;
+  }
   const uint32_t nt = cj.ntargets;
   if (kMulti && mode == kModeDone && nt == 0) return;
   if (lead && mode != kModeDone) zero_counters(&ring[(j + 1) % kCtrRing]);
@@ -1226,13 +1333,15 @@ pmsz_tuning default_tuning() {
   t.solo_targets = 64;
   t.zchunk = kColZ;
   t.graphs = 0;
+  t.stream_div = 16;
   return t;
 }
 
 bool tuning_valid(const pmsz_tuning &t) {
   return t.dense_div >= 1 && t.dense_div <= 4096 && t.tail_targets >= 0 &&
          t.tail_blocks >= 1 && t.tail_blocks <= kTailBlocks && t.solo_targets >= 0 &&
-         (t.zchunk == 16 || t.zchunk == 32 || t.zchunk == 64) && (t.graphs == 0 || t.graphs == 1);
+         (t.zchunk == 16 || t.zchunk == 32 || t.zchunk == 64) && (t.graphs == 0 || t.graphs == 1) &&
+         t.stream_div >= 0 && t.stream_div <= (1 << 24);
 }
 
 struct Ctx {
@@ -1493,6 +1602,8 @@ int chain_graph(Ctx &c, const std::vector<uint64_t> &key, uint32_t j0, F &&enque
 
 template <typename T, bool kMulti>
 int drive(Ctx &c, Job &job, Outcome &out) {
+  // the streamed apply is for one domain whose f and g take 16-byte vector accesses
+  if (kMulti || (((uintptr_t)job.doms[0].f | (uintptr_t)job.doms[0].g) & 15u)) job.s.stream_min = 0;
   const uint32_t ndom = (uint32_t)job.doms.size();
   const Dom &d0 = job.doms[0];
   const Dom *doms = kMulti ? c.doms.p : nullptr;
@@ -1562,7 +1673,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   if (kMulti) {
     k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(c.tbyte.p, c.tlist.p, c.ring, 0, 1,
                                                            true, (uint32_t)job.groups, doms,
-                                                           ndom, detect_tag(1));
+                                                           ndom, detect_tag(1), 0u);
     k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, 0, 0, 1, kOpDetect, job.s, 1);
   }
 
@@ -1582,11 +1693,11 @@ int drive(Ctx &c, Job &job, Outcome &out) {
           // after a full-volume detection j: its target list
           k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
               c.tbyte.p, c.tlist.p, c.ring, (j - 1) % kCtrRing, j % kCtrRing, false,
-              (uint32_t)job.groups, doms, ndom, detect_tag(j));
+              (uint32_t)job.groups, doms, ndom, detect_tag(j), job.s.stream_min);
         }
         k_apply<T, kMulti><<<g_apply, kApplyThreads, 0, c.stream>>>(
             d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j,
-            sparse_chain ? s_sparse : job.s);
+            sparse_chain ? s_sparse : job.s, c.tbyte.p, (uint32_t)job.groups);
         if (sync_every) {
           k_pack<T><<<g_shell, kSyncThreads, 0, c.stream>>>(doms, ds, job.gn[0], job.gn[1],
                                                             job.gn[2], c.ring, j, 1);
@@ -1609,7 +1720,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
         if (kMulti)
           k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
               c.tbyte.p, c.tlist.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false,
-              (uint32_t)job.groups, doms, ndom, detect_tag(j + 1));
+              (uint32_t)job.groups, doms, ndom, detect_tag(j + 1), job.s.stream_min);
         k_detect_list<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
             d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j);
         // the targets of a work-list detection j+1, from the bitmap
@@ -1634,7 +1745,8 @@ int drive(Ctx &c, Job &job, Outcome &out) {
           (uint64_t)(uintptr_t)c.tlist.p, (uint64_t)(uintptr_t)c.wmark.p,
           (uint64_t)(uintptr_t)c.wlist.p, (uint64_t)(uintptr_t)c.ring, (uint64_t)(uintptr_t)c.run,
           (uint64_t)job.s.mode, job.s.max_sweeps, job.s.dense_min_work, job.s.tail_ok,
-          job.s.tail_targets, job.s.solo_targets, job.s.dense_ok, g_compact, g_apply, g_bits, g_fill, g_dense, g_list, g_shell.x,
+          job.s.tail_targets, job.s.solo_targets, job.s.dense_ok, job.s.stream_min, g_compact,
+          g_apply, g_bits, g_fill, g_dense, g_list, g_shell.x,
           g_shell.y, sync_every ? 1u : 0u, (uint64_t)sx, (uint64_t)sy, (uint64_t)sz,
           (uint64_t)job.ndir, (uint64_t)job.gn[0], (uint64_t)job.gn[1], (uint64_t)job.gn[2]};
       int rc = chain_graph(c, key, j0, enqueue_chain);
@@ -1790,6 +1902,11 @@ Sched make_sched(const Ctx &c, const pmsz_params *p, uint64_t work_voxels, bool 
   s.tail_targets = (uint32_t)c.tune.tail_targets;
   s.solo_targets = (uint32_t)c.tune.solo_targets;
   s.dense_ok = 1;
+  s.stream_min = c.tune.stream_div
+                     ? (uint32_t)std::min<uint64_t>(
+                           std::max<uint64_t>(work_voxels / (uint64_t)c.tune.stream_div, 1),
+                           0xffffffffull)
+                     : 0u;
   return s;
 }
 
@@ -2396,9 +2513,11 @@ int pmsz_autotune(pmsz_ctx *ctx, int n, const pmsz_params *params, const void *c
   if (rc == PMSZ_OK) sweep(&pmsz_tuning::tail_targets, tt, 4);
   if (rc == PMSZ_OK) sweep(&pmsz_tuning::tail_blocks, tb, 4);
   if (rc == PMSZ_OK) sweep(&pmsz_tuning::solo_targets, so, 4);
+  const bool multi = n > 1 || params[0].sub[0] * params[0].sub[1] * params[0].sub[2] != 1;
+  const int sd[] = {16, 8, 32, 0};  // the streamed apply: one-domain runs only
+  if (rc == PMSZ_OK && !multi) sweep(&pmsz_tuning::stream_div, sd, 4);
   const int gr[] = {0, 1};  // captured chains: multi-domain runs only
-  if (rc == PMSZ_OK && (n > 1 || params[0].sub[0] * params[0].sub[1] * params[0].sub[2] != 1))
-    sweep(&pmsz_tuning::graphs, gr, 2);
+  if (rc == PMSZ_OK && multi) sweep(&pmsz_tuning::graphs, gr, 2);
   (void)hipFree(scratch);
   c.tune = rc == PMSZ_OK ? cur : start;
   if (rc) return rc;

@@ -97,7 +97,8 @@ class Tuning(ctypes.Structure):
     """pmsz_tuning: schedule and tile parameters (speed only, never results)."""
     _fields_ = [("dense_div", ctypes.c_int), ("tail_targets", ctypes.c_int),
                 ("tail_blocks", ctypes.c_int), ("solo_targets", ctypes.c_int),
-                ("zchunk", ctypes.c_int), ("graphs", ctypes.c_int)]
+                ("zchunk", ctypes.c_int), ("graphs", ctypes.c_int),
+                ("stream_div", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

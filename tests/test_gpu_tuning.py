@@ -18,6 +18,8 @@ TUNINGS = [
     {"dense_div": 1 << 12},       # full-volume detections only while targets are many
     {"solo_targets": 0},          # the tail never shrinks to one workgroup
     {"solo_targets": 1 << 20, "tail_blocks": 4},  # ... or from its first sweep
+    {"stream_div": 0},            # targets of full-volume detections always listed
+    {"stream_div": 1 << 24},      # ... streamed from the byte map whenever any violation
     {"graphs": 1},                # multi-domain runs replay captured chains
     {"graphs": 1, "zchunk": 32, "tail_targets": 4096},
 ]
