@@ -384,7 +384,7 @@ __device__ __forceinline__ void detect_mark(const Dom &D, uint32_t *tmark, uint3
 // takes kBitsPT words per step, interleaved across the block so loads coalesce;
 // a block makes one list reservation per step.
 constexpr int kBitsThreads = 256;
-constexpr int kBitsPT = 64;
+constexpr int kBitsPT = 16;
 __global__ __launch_bounds__(kBitsThreads) void k_compact_bits(
     uint32_t *__restrict__ bits, const Dom *__restrict__ doms, uint32_t ndom, uint32_t wdense,
     uint32_t *__restrict__ list, uint32_t *__restrict__ count, const Counters *__restrict__ gate,
@@ -506,8 +506,8 @@ __device__ __forceinline__ int apply_target(uint32_t t, const Dom &d0, const Dom
 // Edits of the targets of a full-volume detection, streamed from the target
 // byte map (one domain): a thread takes 16 cells at a time, loads their tags,
 // and where any cell carries the sweep's tag loads those cells' f, g and dc as
-// vectors (by halves of 8 cells, skipping a half without targets), edits the
-// tagged cells and stores the vectors back.  The 16 cells belong to that thread
+// vectors (by halves of 8 cells), edits the tagged cells and stores the vectors
+// back.  The 16 cells belong to that thread
 // alone, so the stores race with nothing.  Counts the targets (for the
 // statistics) into cj.ntargets.
 template <typename T>
@@ -535,10 +535,11 @@ __device__ __forceinline__ void apply_stream(const Dom &d0, const uint8_t *__res
     if (gi < nfull) {
       const uint4 cv = reinterpret_cast<const uint4 *>(dc)[gi];
       uint32_t cw[4] = {cv.x, cv.y, cv.z, cv.w};
+      // both halves are loaded and stored even without targets: the group's
+      // lines are then written whole
 #pragma unroll
       for (int hf = 0; hf < 2; hf++) {
         const uint32_t mh = (m >> (8 * hf)) & 0xffu;
-        if (!mh) continue;
         union {
           uint4 v[kVec];
           T x[8];
