@@ -36,6 +36,10 @@ LABEL_BYTES_PER_VOXEL = 4 + 1 + 1
 # detection 1, each scaled by what the counter reads for a device copy of known
 # size.
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_columns_detect_512cube_f32_pmc.md")
+# VALU issue peak: one wave64 VALU instruction per CU and clock, 256 CUs at the
+# 2.4 GHz peak clock (G wave instructions/s).  The same summary holds the
+# kernel's SQ_INSTS_VALU per launch ("valu_instructions_per_launch").
+VALU_PEAK_GIPS = 256 * 2.4
 
 
 def log(*a):
@@ -64,8 +68,8 @@ def cpu_baseline(f, g0, eb, edge):
     }
 
 
-def pmc_traffic(nvox):
-    """HBM bytes per launch of the first pass from the committed --pmc summary
+def pmc_summary(nvox, key):
+    """One per-launch figure of detection 1 from the committed --pmc summary
     (None when the summary is absent or was taken on another size)."""
     try:
         with open(PMC_SUMMARY) as fh:
@@ -76,7 +80,7 @@ def pmc_traffic(nvox):
                     kv[k.strip()] = v.strip().split()[0]
         if int(kv.get("voxels", -1)) != nvox:
             return None
-        return int(kv["traffic_bytes_per_launch"])
+        return int(kv[key])
     except (OSError, KeyError, ValueError):
         return None
 
@@ -342,7 +346,8 @@ def main():
     dense_bytes = nvox * DETECT_BYTES_PER_VOXEL
     achieved = dense_bytes / dense_s / 1e9
     label_gbs = nvox * LABEL_BYTES_PER_VOXEL / label_s / 1e9
-    traffic = pmc_traffic(nvox)
+    traffic = pmc_summary(nvox, "traffic_bytes_per_launch")
+    valu = pmc_summary(nvox, "valu_instructions_per_launch")
     # full check of one result against the original's codes (topology preserved)
     ok = torch.equal(engine.codes(tf), engine.codes(copies[-1]))
     res = {
@@ -382,6 +387,14 @@ def main():
             "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if traffic else None,
             "kernel_ms": dense_s * 1e3,
             "bytes_per_launch": dense_bytes,
+            # the kernel is VALU-bound: its issue rate against the VALU peak
+            "valu": None if valu is None else {
+                "instructions_per_launch": valu,
+                "achieved": valu / dense_s / 1e9,
+                "peak": VALU_PEAK_GIPS,
+                "unit": "G wave64 VALU instructions/s",
+                "frac": valu / dense_s / 1e9 / VALU_PEAK_GIPS,
+                "source": os.path.relpath(PMC_SUMMARY, ROOT)},
             "labeling": {"kernel": "k_columns<float,true,false>", "kernel_ms": label_s * 1e3,
                          "achieved": label_gbs, "frac": label_gbs / HBM_PEAK_GBPS,
                          "bytes_per_launch": nvox * LABEL_BYTES_PER_VOXEL},
