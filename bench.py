@@ -196,9 +196,12 @@ def serving(n_req=400, pool=32, rate=200.0, lanes=4, seed=0):
     copies = [entries[k][1].clone() for k in picks]
     torch.cuda.synchronize()
     with serve.Server(lanes=lanes, hbm_budget=96 << 30, max_batch=16) as srv:
-        # warm-up: one job per lane shape class
-        for f, g0, eb in entries[:lanes]:
-            srv.submit(f, g0.clone(), eb).wait(120)
+        # warm-up: the largest jobs of the pool at once, one per lane, so that every
+        # lane's context holds buffers of the largest size before timing
+        largest = sorted(entries, key=lambda e: -e[0].numel())[:lanes]
+        warm = [srv.submit(f, g0.clone(), eb) for f, g0, eb in largest]
+        for j in warm:
+            j.wait(120)
         start = time.perf_counter()
         jobs = []
         for k in range(n_req):

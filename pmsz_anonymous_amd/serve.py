@@ -8,9 +8,11 @@ can admit at once -- same value type and loop mode, at most ``max_batch`` jobs,
 and only while the library-owned device memory of all running work (the
 workspace of each lane's run, pmsz_workspace_bytes, or what the lane's context
 already holds if that is more, idle lanes included) stays within ``hbm_budget`` -- and corrects them
-as one batched run (pmsz_ctx_correct_batch).  Runs on different lanes overlap
-on the device: the dense passes of one run fill the GPU while another run is in
-its latency-bound tail.
+as one batched run (pmsz_ctx_correct_batch).  A job of ``alone_voxels`` voxels or
+more runs by itself: its full-volume passes fill the GPU anyway, and a field
+alone takes the one-domain schedule (streamed applies, claimed work lists).
+Runs on different lanes overlap on the device: the dense passes of one run fill
+the GPU while another run is in its latency-bound tail.
 
 Every job's result equals the job corrected alone (the batched run is exact per
 field, and lanes share no state); tests/test_gpu_serve.py checks this.
@@ -63,13 +65,14 @@ class Job:
 
 
 class Server:
-    def __init__(self, lanes=4, hbm_budget=64 << 30, max_batch=32):
+    def __init__(self, lanes=4, hbm_budget=64 << 30, max_batch=32, alone_voxels=1 << 20):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("the correction server needs a HIP device")
         self.lanes = lanes
         self.hbm_budget = int(hbm_budget)
         self.max_batch = int(max_batch)
+        self.alone_voxels = int(alone_voxels)
         self._ctxs = [engine.Context() for _ in range(lanes)]
         self._reserved = [0] * lanes   # workspace of each lane's running group
         self._held = [0] * lanes       # bytes each lane's context holds (grow-only)
@@ -128,10 +131,13 @@ class Server:
         others = sum(max(r, h) for k, (r, h) in enumerate(zip(self._reserved, self._held))
                      if k != lane)
         have = self._held[lane]
-        key = self._queue[0].key
+        head = self._queue[0]
+        key = head.key
+        alone = head.voxels >= self.alone_voxels
         group, shapes = [], []
         for job in list(self._queue):
-            if job.key != key:
+            # the head, and with a small head the small jobs of its key
+            if job is not head and (alone or job.key != key or job.voxels >= self.alone_voxels):
                 continue
             trial = shapes + [(tuple(job.f.shape), job.f.dtype)]
             need = max(have, engine.workspace_bytes(trial))
@@ -139,7 +145,7 @@ class Server:
                 break
             group.append(job)
             shapes = trial
-            if len(group) >= self.max_batch:
+            if alone or len(group) >= self.max_batch:
                 break
         if group:
             for job in group:

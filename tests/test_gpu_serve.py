@@ -107,3 +107,25 @@ def test_budget_admission(dev):
         tg = torch.from_numpy(g0).to(dev)
         engine.correct(torch.from_numpy(f).to(dev), tg, eb)
         assert torch.equal(tg, j.g)
+
+
+def test_large_jobs_run_alone(dev):
+    # jobs at or above alone_voxels take a run of their own; smaller ones batch
+    from pmsz_anonymous_amd import engine, serve
+    cases = _jobs(12, 5)
+    fs = [torch.from_numpy(f).to(dev) for f, _, _ in cases]
+    gs = [torch.from_numpy(g0).to(dev) for _, g0, _ in cases]
+    sizes = sorted(f.numel() for f in fs)
+    cut = sizes[len(sizes) // 2]
+    torch.cuda.synchronize()
+    with serve.Server(lanes=2, max_batch=8, alone_voxels=cut) as srv:
+        jobs = [srv.submit(f, g, eb) for f, g, (_, _, eb) in zip(fs, gs, cases)]
+        for j in jobs:
+            j.wait(60)
+    for j, f in zip(jobs, fs):
+        if f.numel() >= cut:
+            assert j.batch_size == 1
+    for j, (f, g0, eb) in zip(jobs, cases):
+        tg = torch.from_numpy(g0).to(dev)
+        engine.correct(torch.from_numpy(f).to(dev), tg, eb)
+        assert torch.equal(tg, j.g)
