@@ -170,6 +170,7 @@ struct Sched {
   uint32_t stream_min;      // one domain: the targets of a full-volume detection with at
                             // least this many violations are edited by one pass over the
                             // target byte map instead of a list
+  uint32_t append;          // work-list sweeps list claimed bits (else bitmaps are compacted)
 };
 
 // Run-wide state, written only by kernels that end a phase.

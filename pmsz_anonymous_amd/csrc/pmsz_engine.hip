@@ -72,10 +72,6 @@ constexpr int kTailThreads = 256;
 constexpr int kTailBlocks = 256;
 constexpr size_t kBarBytes = sizeof(uint32_t) * 16 * (kTailBlocks + 1);
 constexpr int kSyncThreads = 256;
-// One domain: work-list sweeps build their lists by appending the bits that
-// returning atomics claimed (apply: the targets' neighbourhoods in wmark;
-// detection: targets in tmark), instead of compacting the bitmaps afterwards.
-constexpr bool kListAppend = true;
 constexpr uint32_t kMaxDomains = 1024;  // domain bookkeeping: one thread each
 static_assert(kMaxDomains <= (uint32_t)kCompactDoms, "compaction domain table");
 constexpr int kMaxContexts = 8;         // live contexts, the default one included
@@ -524,8 +520,12 @@ __device__ __forceinline__ void apply_stream(const Dom &d0, const uint8_t *__res
   const uint4 *t16 = reinterpret_cast<const uint4 *>(tbyte);
   uint32_t touched = 0, clamped = 0, count = 0;
   const uint32_t stride = gridDim.x * kApplyThreads;
-  for (uint32_t gi = blockIdx.x * kApplyThreads + threadIdx.x; gi < ngroups; gi += stride) {
-    const uint4 tv = t16[gi];
+  // the next group's tags are loaded while this group is edited
+  uint32_t gi = blockIdx.x * kApplyThreads + threadIdx.x;
+  uint4 tnext = gi < ngroups ? t16[gi] : make_uint4(0, 0, 0, 0);
+  for (; gi < ngroups; gi += stride) {
+    const uint4 tv = tnext;
+    if (gi + stride < ngroups) tnext = t16[gi + stride];
     const uint32_t m = byte_bits(tag_bytes(tv.x, rep)) | (byte_bits(tag_bytes(tv.y, rep)) << 4) |
                        (byte_bits(tag_bytes(tv.z, rep)) << 8) |
                        (byte_bits(tag_bytes(tv.w, rep)) << 12);
@@ -631,13 +631,14 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
   const bool queue = mode == kModeSparse;
   uint32_t touched = 0, clamped = 0;
   const uint32_t stride = gridDim.x * kApplyThreads;
-  // one domain: the targets of a work-list detection are claimed in tmark; every
-  // set bit is a target of this sweep, so their words are cleared with plain stores
-  const bool clear = !kMulti && kListAppend && pm == kModeSparse;
-  if (!kMulti && queue && kListAppend) {
+  // claimed lists (s.append): the targets of a work-list detection are claimed in
+  // tmark; every set bit is a target of this sweep, so their words are cleared with
+  // plain stores
+  const bool clear = s.append && pm == kModeSparse;
+  __shared__ StageLds<kApplyThreads, kApplyThreads * kCand> st;
+  if (!kMulti && queue && s.append) {
     // one domain, work-list detection next: the edit, then the neighbourhood's
     // claims in wmark, listed at once (staged per block)
-    __shared__ StageLds<kApplyThreads, kApplyThreads * kCand> st;
     stage_init(st);
     for (uint32_t base = blockIdx.x * kApplyThreads; base < nt; base += stride) {
       const uint32_t i = base + threadIdx.x;
@@ -662,6 +663,30 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
       apply_one<T>((T *)d0.g, (const T *)d0.f, d0.dc, t, d0.ep, touched, clamped);
       mark_neighbourhood(t, d0, wmark);
     }
+  } else if (kMulti && queue && s.append) {
+    // several domains without ghost syncs, work-list detection next: the edit,
+    // then the neighbourhood's claims, listed at once (staged per block)
+    stage_init(st);
+    for (uint32_t base = blockIdx.x * kApplyThreads; base < nt; base += stride) {
+      const uint32_t i = base + threadIdx.x;
+      uint32_t u[kCand];
+      uint32_t dk = 0, n = 0;
+      Claims cl;
+      const bool has = i < nt;
+      if (has) {
+        const uint32_t t = tlist[i];
+        apply_target<T, kMulti>(t, d0, doms, shift, ds, clear ? tmark : nullptr, wmark, false,
+                                false, dk, touched, clamped, u);
+        uint32_t local;
+        const Dom &D = dom_of<kMulti>(t, d0, doms, shift, dk, local);
+        claim_neighbourhood(local, D, wmark, cl);
+        n = cl.count();
+      }
+      dom_add_targets(has, dk, ds);
+      const uint32_t pos = stage_reserve(st, n, wlist, &cj.nwork);
+      if (n) cl.emit(st.buf, pos);
+    }
+    stage_flush(st, wlist, &cj.nwork);
   } else if (kMulti) {
     // several domains: the edit (and the ghost-sync flag), the neighbourhood's
     // bits in wmark when detection j+1 runs over a work list
@@ -671,8 +696,8 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
       uint32_t dk = 0;
       const bool has = i < nt;
       if (has)
-        apply_target<T, kMulti>(tlist[i], d0, doms, shift, ds, nullptr, wmark, false, queue, dk,
-                                touched, clamped, u);
+        apply_target<T, kMulti>(tlist[i], d0, doms, shift, ds, clear ? tmark : nullptr, wmark,
+                                false, queue, dk, touched, clamped, u);
       dom_add_targets(has, dk, ds);
     }
   } else {
@@ -743,7 +768,8 @@ template <typename T, bool kMulti>
 __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
     Dom d0, const Dom *__restrict__ doms, uint32_t shift, DomState *__restrict__ ds,
     uint32_t *__restrict__ tmark, uint32_t *__restrict__ tlist, uint32_t *__restrict__ wmark,
-    const uint32_t *__restrict__ wlist, Counters *__restrict__ ring, RunState *run, uint32_t j) {
+    const uint32_t *__restrict__ wlist, Counters *__restrict__ ring, RunState *run, uint32_t j,
+    uint32_t append) {
   const Counters &cj = ring[j % kCtrRing];
   if (cj.mode != kModeSparse) return;
   Counters &cn = ring[(j + 1) % kCtrRing];
@@ -751,20 +777,26 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
   const uint32_t nw = cj.nwork;
   uint32_t lc[3] = {0, 0, 0};
   const uint32_t stride = gridDim.x * kApplyThreads;
-  if (!kMulti && kListAppend) {
+  __shared__ StageLds<kApplyThreads, kApplyThreads * 4> st;
+  if (append) {
     // targets claimed in tmark and listed at once; every set bit of wmark is a
     // listed voxel of this sweep, so its words are cleared with plain stores
-    __shared__ StageLds<kApplyThreads, kApplyThreads * 4> st;
     stage_init(st);
     for (uint32_t base = blockIdx.x * kApplyThreads; base < nw; base += stride) {
       const uint32_t i = base + threadIdx.x;
       uint32_t out[4];
+      uint32_t vc[3] = {0, 0, 0};
+      uint32_t dk = 0;
+      bool has = false;
       int n = 0;
       if (i < nw) {
         const uint32_t v = wlist[i];
         wmark[v >> 5] = 0u;
-        n = detect_one<T>(d0, tmark, v, false, true, lc, out);
+        n = detect_listed<T, kMulti>(v, d0, doms, shift, ds, tmark, dk, has, vc, out);
       }
+      if (kMulti) dom_add3(has, dk, vc, ds, par);
+#pragma unroll
+      for (int q = 0; q < 3; q++) lc[q] += vc[q];
       const uint32_t pos = stage_reserve(st, (uint32_t)n, tlist, &cn.ntargets);
       for (int q = 0; q < n; q++) st.buf[pos + q] = out[q];
     }
@@ -1183,21 +1215,48 @@ __global__ __launch_bounds__(kSyncThreads) void k_finish(const Dom *__restrict__
   const uint32_t rows = by * bz;
   const uint8_t q = (uint8_t)D.ep.q;
   uint32_t touched = 0, clamped = 0;
-  // one x row per block step (coalesced), rows dealt grid-stride
-  for (uint32_t r = blockIdx.x; r < rows; r += gridDim.x) {
-    const uint32_t y = r % by, z = r / by;  // uniform
-    const int64_t l0 = (int64_t)D.lo[0] +
-                       D.d.nx * ((int64_t)(y + D.lo[1]) + D.d.ny * (int64_t)(z + D.lo[2]));
-    const int64_t g0 = G ? (int64_t)D.off[0] + gnx * ((int64_t)(y + D.off[1]) +
-                                                       gny * (int64_t)(z + D.off[2]))
-                         : 0;
-    for (uint32_t x = threadIdx.x; x < bx; x += kSyncThreads) {
-      const uint8_t c = D.dc[l0 + x];
+  if (!G && !D.padded) {
+    // a field of a batch: its delta counters are one run of bytes, counted 16 at a
+    // time (an unaligned head and the tail byte by byte)
+    const uint8_t *dcp = D.dc;
+    const uint64_t n = (uint64_t)D.d.nx * (uint64_t)D.d.ny * (uint64_t)D.d.nz;
+    const uint64_t head = std::min<uint64_t>(n, (16u - ((uintptr_t)dcp & 15u)) & 15u);
+    const uint64_t nv = (n - head) / 16u, tail0 = head + nv * 16u;
+    const uint64_t tid = (uint64_t)blockIdx.x * kSyncThreads + threadIdx.x;
+    const uint64_t nthr = (uint64_t)gridDim.x * kSyncThreads;
+    const uint32_t qrep = (uint32_t)q * 0x01010101u;
+    const uint4 *v16 = reinterpret_cast<const uint4 *>(dcp + head);
+    for (uint64_t v = tid; v < nv; v += nthr) {
+      const uint4 w = v16[v];
+      const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int k2 = 0; k2 < 4; k2++) {
+        touched += 4u - (uint32_t)__popc(byte_bits(tag_bytes(ws[k2], 0u)));
+        clamped += (uint32_t)__popc(byte_bits(tag_bytes(ws[k2], qrep)));
+      }
+    }
+    for (uint64_t i = tid; i < head + (n - tail0); i += nthr) {
+      const uint8_t c = dcp[i < head ? i : tail0 + (i - head)];
       touched += c > 0;
       clamped += c == q;
-      if (G) {
-        G[g0 + x] = ((const T *)D.g)[l0 + x];
-        if (DC) DC[g0 + x] = c;
+    }
+  } else {
+    // one x row per block step (coalesced), rows dealt grid-stride
+    for (uint32_t r = blockIdx.x; r < rows; r += gridDim.x) {
+      const uint32_t y = r % by, z = r / by;  // uniform
+      const int64_t l0 = (int64_t)D.lo[0] +
+                         D.d.nx * ((int64_t)(y + D.lo[1]) + D.d.ny * (int64_t)(z + D.lo[2]));
+      const int64_t g0 = G ? (int64_t)D.off[0] + gnx * ((int64_t)(y + D.off[1]) +
+                                                         gny * (int64_t)(z + D.off[2]))
+                           : 0;
+      for (uint32_t x = threadIdx.x; x < bx; x += kSyncThreads) {
+        const uint8_t c = D.dc[l0 + x];
+        touched += c > 0;
+        clamped += c == q;
+        if (G) {
+          G[g0 + x] = ((const T *)D.g)[l0 + x];
+          if (DC) DC[g0 + x] = c;
+        }
       }
     }
   }
@@ -1605,6 +1664,8 @@ template <typename T, bool kMulti>
 int drive(Ctx &c, Job &job, Outcome &out) {
   // the streamed apply is for one domain whose f and g take 16-byte vector accesses
   if (kMulti || (((uintptr_t)job.doms[0].f | (uintptr_t)job.doms[0].g) & 15u)) job.s.stream_min = 0;
+  // work lists from claims, except where ghost syncs mark the bitmaps too
+  job.s.append = job.subdomains ? 0u : 1u;
   const uint32_t ndom = (uint32_t)job.doms.size();
   const Dom &d0 = job.doms[0];
   const Dom *doms = kMulti ? c.doms.p : nullptr;
@@ -1707,7 +1768,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
         }
         // the work list of detection j+1 from the bitmap that apply j (and a ghost
         // sync) marked
-        if (kMulti || !kListAppend)
+        if (!job.s.append)
           k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
             c.wmark.p, doms, ndom, (uint32_t)job.wdense, c.wlist.p, &c.ring[j % kCtrRing].nwork,
             &c.ring[j % kCtrRing], kModeSparse);
@@ -1723,9 +1784,10 @@ int drive(Ctx &c, Job &job, Outcome &out) {
               c.tbyte.p, c.tlist.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false,
               (uint32_t)job.groups, doms, ndom, detect_tag(j + 1), job.s.stream_min);
         k_detect_list<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
-            d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j);
+            d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j,
+            job.s.append);
         // the targets of a work-list detection j+1, from the bitmap
-        if (kMulti || !kListAppend)
+        if (!job.s.append)
           k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
             c.tmark.p, doms, ndom, (uint32_t)job.wdense, c.tlist.p,
             &c.ring[(j + 1) % kCtrRing].ntargets, &c.ring[j % kCtrRing], kModeSparse);
@@ -1746,7 +1808,8 @@ int drive(Ctx &c, Job &job, Outcome &out) {
           (uint64_t)(uintptr_t)c.tlist.p, (uint64_t)(uintptr_t)c.wmark.p,
           (uint64_t)(uintptr_t)c.wlist.p, (uint64_t)(uintptr_t)c.ring, (uint64_t)(uintptr_t)c.run,
           (uint64_t)job.s.mode, job.s.max_sweeps, job.s.dense_min_work, job.s.tail_ok,
-          job.s.tail_targets, job.s.solo_targets, job.s.dense_ok, job.s.stream_min, g_compact,
+          job.s.tail_targets, job.s.solo_targets, job.s.dense_ok, job.s.stream_min, job.s.append,
+          g_compact,
           g_apply, g_bits, g_fill, g_dense, g_list, g_shell.x,
           g_shell.y, sync_every ? 1u : 0u, (uint64_t)sx, (uint64_t)sy, (uint64_t)sz,
           (uint64_t)job.ndir, (uint64_t)job.gn[0], (uint64_t)job.gn[1], (uint64_t)job.gn[2]};
@@ -1803,7 +1866,8 @@ int drive(Ctx &c, Job &job, Outcome &out) {
     k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, 0, 0, m + 1, kOpPreCheck, job.s,
                                                emit_ok);
     k_detect_list<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
-        d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, m);
+        d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, m,
+        job.s.append);
     k_compact_bits<<<g_bits, kBitsThreads, 0, c.stream>>>(
         c.tmark.p, doms, ndom, (uint32_t)job.wdense, c.tlist.p,
         &c.ring[(m + 1) % kCtrRing].ntargets, &c.ring[m % kCtrRing], kModeSparse);
@@ -1903,6 +1967,7 @@ Sched make_sched(const Ctx &c, const pmsz_params *p, uint64_t work_voxels, bool 
   s.tail_targets = (uint32_t)c.tune.tail_targets;
   s.solo_targets = (uint32_t)c.tune.solo_targets;
   s.dense_ok = 1;
+  s.append = 1;
   s.stream_min = c.tune.stream_div
                      ? (uint32_t)std::min<uint64_t>(
                            std::max<uint64_t>(work_voxels / (uint64_t)c.tune.stream_div, 1),
