@@ -1741,9 +1741,9 @@ int drive(Ctx &c, Job &job, Outcome &out) {
 
   uint32_t j0 = 1;
   out.outer_started = 1;
-  // one domain: once a chain has ended in a work-list sweep, the next chain holds
-  // work-list sweeps only (its applies may not choose a full-volume detection,
-  // which gives the same results), so no launch of it waits for a dense phase
+  // once a chain has ended in a work-list sweep, the next chain holds work-list
+  // sweeps only (its applies may not choose a full-volume detection, which gives
+  // the same results), so no launch of it waits for a dense phase
   bool sparse_chain = false;
   Sched s_sparse = job.s;
   s_sparse.dense_ok = 0;
@@ -1779,7 +1779,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
           k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
               d0, doms, ndom, ds, c.tbyte.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing,
               (j + 1) & 1u, c.run, detect_tag(j + 1));
-        if (kMulti)
+        if (kMulti && !sparse_chain)
           k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
               c.tbyte.p, c.tlist.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false,
               (uint32_t)job.groups, doms, ndom, detect_tag(j + 1), job.s.stream_min);
@@ -1809,6 +1809,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
           (uint64_t)(uintptr_t)c.wlist.p, (uint64_t)(uintptr_t)c.ring, (uint64_t)(uintptr_t)c.run,
           (uint64_t)job.s.mode, job.s.max_sweeps, job.s.dense_min_work, job.s.tail_ok,
           job.s.tail_targets, job.s.solo_targets, job.s.dense_ok, job.s.stream_min, job.s.append,
+          sparse_chain ? 1u : 0u,
           g_compact,
           g_apply, g_bits, g_fill, g_dense, g_list, g_shell.x,
           g_shell.y, sync_every ? 1u : 0u, (uint64_t)sx, (uint64_t)sy, (uint64_t)sz,
@@ -1842,7 +1843,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
     if (!c.h_run->done) {
       // the chain ended before the tail took over: continue it where it stopped
       j0 += kChain;
-      sparse_chain = !kMulti && c.h_run->last_mode == kModeSparse;
+      sparse_chain = c.h_run->last_mode == kModeSparse;
       continue;
     }
     const uint32_t m = c.h_run->last_j;
@@ -1895,6 +1896,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
     // apply the check's targets (the first sweep of the new loops); when every
     // subdomain is capped after it, that apply ends the run
     j0 = m + 1;
+    sparse_chain = false;
   }
   PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
   PMSZ_HIP(hipMemcpyAsync(c.h_ring, c.ring, sizeof(Counters) * kCtrRing, hipMemcpyDeviceToHost,
