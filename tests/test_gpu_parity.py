@@ -204,3 +204,25 @@ def test_c2_512cube_properties(dev):
     g = tg.cpu().numpy()
     assert fieldio.bound_violations(f, g, eb)[1] == 0
     assert (g <= g0).all()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_unaligned_fields_parity(dev, dtype):
+    # f and g one element into larger buffers: no 16-byte vector access of them is
+    # allowed (the streamed apply then gives way to the list apply); same results
+    from pmsz_anonymous_amd import engine
+    f, g0, eb = _case((37, 52, 45), 7, dtype=dtype, eb_rel=1e-2, freq=5)
+    go, dco, sto, _, _ = O.correct(f, g0, eb)
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    fb = torch.zeros(f.size + 1, dtype=tdt, device=dev)
+    gb = torch.zeros(f.size + 1, dtype=tdt, device=dev)
+    tf = fb[1:].view(f.shape)
+    tg = gb[1:].view(f.shape)
+    tf.copy_(torch.from_numpy(f))
+    tg.copy_(torch.from_numpy(g0))
+    assert tf.data_ptr() % 16 != 0
+    dc = torch.zeros(f.shape, dtype=torch.uint8, device=dev)
+    _, st = engine.correct(tf, tg, eb, dc=dc)
+    assert tg.cpu().numpy().tobytes() == go.tobytes()
+    assert np.array_equal(dc.cpu().numpy(), dco)
+    assert st["total_targets"] == sto["total_targets"]
