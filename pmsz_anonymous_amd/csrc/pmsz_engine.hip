@@ -1745,12 +1745,13 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   // sweeps only (its applies may not choose a full-volume detection, which gives
   // the same results), so no launch of it waits for a dense phase
   bool sparse_chain = false;
+  uint32_t chain = kChain;  // sweeps of the next chain
   Sched s_sparse = job.s;
   s_sparse.dense_ok = 0;
   while (true) {
-    // the chain of sweeps j0 .. j0 + kChain - 1, enqueued on the context's stream
+    // the chain of sweeps j0 .. j0 + chain - 1, enqueued on the context's stream
     auto enqueue_chain = [&](uint32_t j0) -> int {
-      for (uint32_t j = j0; j < j0 + kChain; j++) {
+      for (uint32_t j = j0; j < j0 + chain; j++) {
         if (!kMulti && !sparse_chain) {
           // after a full-volume detection j: its target list
           k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
@@ -1809,7 +1810,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
           (uint64_t)(uintptr_t)c.wlist.p, (uint64_t)(uintptr_t)c.ring, (uint64_t)(uintptr_t)c.run,
           (uint64_t)job.s.mode, job.s.max_sweeps, job.s.dense_min_work, job.s.tail_ok,
           job.s.tail_targets, job.s.solo_targets, job.s.dense_ok, job.s.stream_min, job.s.append,
-          sparse_chain ? 1u : 0u,
+          sparse_chain ? 1u : 0u, chain,
           g_compact,
           g_apply, g_bits, g_fill, g_dense, g_list, g_shell.x,
           g_shell.y, sync_every ? 1u : 0u, (uint64_t)sx, (uint64_t)sy, (uint64_t)sz,
@@ -1842,7 +1843,8 @@ int drive(Ctx &c, Job &job, Outcome &out) {
     if (c.h_run->err) return PMSZ_EINVAL;
     if (!c.h_run->done) {
       // the chain ended before the tail took over: continue it where it stopped
-      j0 += kChain;
+      j0 += chain;
+      chain = kChain;
       sparse_chain = c.h_run->last_mode == kModeSparse;
       continue;
     }
@@ -1897,6 +1899,9 @@ int drive(Ctx &c, Job &job, Outcome &out) {
     // subdomain is capped after it, that apply ends the run
     j0 = m + 1;
     sparse_chain = false;
+    // few targets: the first apply hands over to the tail at once, so the chain
+    // before the tail launch holds that one sweep
+    chain = job.s.tail_ok && cc.ntargets <= job.s.tail_targets ? 1u : kChain;
   }
   PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
   PMSZ_HIP(hipMemcpyAsync(c.h_ring, c.ring, sizeof(Counters) * kCtrRing, hipMemcpyDeviceToHost,
