@@ -226,3 +226,11 @@ def test_unaligned_fields_parity(dev, dtype):
     assert tg.cpu().numpy().tobytes() == go.tobytes()
     assert np.array_equal(dc.cpu().numpy(), dco)
     assert st["total_targets"] == sto["total_targets"]
+
+
+@pytest.mark.parametrize("q", [3, 17, 200])
+@pytest.mark.parametrize("clamp", [O.CLAMP_REFERENCE, O.CLAMP_BOUND_SAFE])
+def test_large_q_parity(dev, q, clamp):
+    # delta counters well above 15 (and near the byte's range) in both clamp modes
+    f, g0, eb = _case((36, 41, 47), 9, eb_rel=3e-3, freq=5)
+    _compare(f, g0, eb, dev, q=q, clamp_mode=clamp)
