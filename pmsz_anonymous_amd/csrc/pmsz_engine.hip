@@ -292,11 +292,6 @@ __device__ __forceinline__ void mark_neighbourhood(uint32_t t, int64_t x, int64_
     if (m >> 32) atomicOr(&wmark[(v0 >> 5) + 1], (uint32_t)(m >> 32));
   }
 }
-__device__ __forceinline__ void mark_neighbourhood(uint32_t t, const Dom &D, uint32_t *wmark) {
-  int64_t x, y, z;
-  decompose(t, D.d, x, y, z);
-  mark_neighbourhood(t, x, y, z, D, wmark);
-}
 
 // The same row-wise marks with returning atomics.  Claims: per stencil row, the
 // bits of words w and w + 1 that this call set (claims of other threads
@@ -634,9 +629,9 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
   // claimed lists (s.append): the targets of a work-list detection are claimed in
   // tmark; every set bit is a target of this sweep, so their words are cleared with
   // plain stores
-  const bool clear = s.append && pm == kModeSparse;
+  const bool clear = (!kMulti || s.append) && pm == kModeSparse;
   __shared__ StageLds<kApplyThreads, kApplyThreads * kCand> st;
-  if (!kMulti && queue && s.append) {
+  if (!kMulti && queue) {
     // one domain, work-list detection next: the edit, then the neighbourhood's
     // claims in wmark, listed at once (staged per block)
     stage_init(st);
@@ -655,14 +650,6 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
       if (n) cl.emit(st.buf, pos);
     }
     stage_flush(st, wlist, &cj.nwork);
-  } else if (!kMulti && queue) {
-    // one domain, work-list detection next: the edit, then the neighbourhood's
-    // bits in wmark (the list is compacted from the bitmap afterwards)
-    for (uint32_t i = blockIdx.x * kApplyThreads + threadIdx.x; i < nt; i += stride) {
-      const uint32_t t = tlist[i];
-      apply_one<T>((T *)d0.g, (const T *)d0.f, d0.dc, t, d0.ep, touched, clamped);
-      mark_neighbourhood(t, d0, wmark);
-    }
   } else if (kMulti && queue && s.append) {
     // several domains without ghost syncs, work-list detection next: the edit,
     // then the neighbourhood's claims, listed at once (staged per block)
@@ -778,7 +765,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
   uint32_t lc[3] = {0, 0, 0};
   const uint32_t stride = gridDim.x * kApplyThreads;
   __shared__ StageLds<kApplyThreads, kApplyThreads * 4> st;
-  if (append) {
+  if (!kMulti || append) {  // one field always lists its claims
     // targets claimed in tmark and listed at once; every set bit of wmark is a
     // listed voxel of this sweep, so its words are cleared with plain stores
     stage_init(st);
@@ -801,10 +788,6 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
       for (int q = 0; q < n; q++) st.buf[pos + q] = out[q];
     }
     stage_flush(st, tlist, &cn.ntargets);
-  } else if (!kMulti) {
-    // targets go to tmark bits, compacted into tlist afterwards
-    for (uint32_t i = blockIdx.x * kApplyThreads + threadIdx.x; i < nw; i += stride)
-      detect_mark<T>(d0, tmark, wlist[i], false, true, lc);
   } else {
     for (uint32_t base = blockIdx.x * kApplyThreads; base < nw; base += stride) {
       const uint32_t i = base + threadIdx.x;
