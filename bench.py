@@ -216,21 +216,26 @@ def serving(n_req=400, pool=32, rate=200.0, lanes=4, seed=0):
         lat = np.array([j.t_done - (start + a) for j, a in zip(jobs, arrivals)])
         vox = sum(entries[k][0].numel() for k in picks)
         batch = float(np.mean([j.batch_size for j in jobs]))
-        # saturation: every job of one pool pass queued at once
-        sat_copies = [g0.clone() for _, g0, _ in entries]
+        # saturation: every job of one pool pass queued at once (best of 3 passes)
+        t_sat = float("inf")
+        for _ in range(3):
+            sat_copies = [g0.clone() for _, g0, _ in entries]
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            sj = [srv.submit(f, g, eb) for (f, _, eb), g in zip(entries, sat_copies)]
+            for j in sj:
+                j.wait(600)
+            t_sat = min(t_sat, time.perf_counter() - ts)
+    # the same jobs one after another (best of 3 passes)
+    t_seq = float("inf")
+    for _ in range(3):
+        seq_copies = [g0.clone() for _, g0, _ in entries]
         torch.cuda.synchronize()
         ts = time.perf_counter()
-        sj = [srv.submit(f, g, eb) for (f, _, eb), g in zip(entries, sat_copies)]
-        for j in sj:
-            j.wait(600)
-        t_sat = time.perf_counter() - ts
-    seq_copies = [g0.clone() for _, g0, _ in entries]
-    torch.cuda.synchronize()
-    ts = time.perf_counter()
-    for (f, _, eb), g in zip(entries, seq_copies):
-        engine.correct(f, g, eb)
-    torch.cuda.synchronize()
-    t_seq = time.perf_counter() - ts
+        for (f, _, eb), g in zip(entries, seq_copies):
+            engine.correct(f, g, eb)
+        torch.cuda.synchronize()
+        t_seq = min(t_seq, time.perf_counter() - ts)
     identical = all(torch.equal(a, b) for a, b in zip(sat_copies, seq_copies))
     pool_vox = sum(f.numel() for f, _, _ in entries)
     return {
