@@ -255,6 +255,40 @@ def serving(n_req=400, pool=32, rate=200.0, lanes=4, seed=0):
     }
 
 
+def edit_channel(tf, tg0, tg, eb, reps=10):
+    """The edit side channel of one corrected field, as the reference's
+    gather_cost_sparse produces it (pMSz.cu:2429-2543): pmsz_encode_edits on
+    the device (packed symbols + compacted clamp values; wall clock around the
+    synchronous call), then one zstd frame of each on the host (levels 5 / 3)."""
+    import tempfile
+
+    import torch
+    from pmsz_anonymous_amd import engine, fieldio
+    engine.encode_edits(tf, tg0, tg, eb)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        pk, vals, bl, info = engine.encode_edits(tf, tg0, tg, eb)
+    ms = (time.perf_counter() - t) / reps * 1e3
+    s = tf.element_size()
+    nvox = tf.numel()
+    # read g and g0 everywhere, f where g < g0; write the packed stream and the values
+    alg = nvox * 2 * s + info["edited"] * s + bl[0]["packed_bytes"] + info["values"] * s
+    res = {"kernel": "k_edit_encode<float,true>", "ms_per_call": ms,
+           "algorithmic_bytes": alg, "achieved": alg / ms / 1e6, "unit": "GB/s",
+           "frac": alg / ms / 1e6 / HBM_PEAK_GBPS, "nbits": info["nbits"],
+           "edited": info["edited"], "clamp_values": info["values"]}
+    if fieldio.zstd_version() is not None:
+        pkh, vh = pk.cpu().numpy(), vals.cpu().numpy()
+        with tempfile.TemporaryDirectory() as d:
+            t = time.perf_counter()
+            nd, ne = fieldio.write_edits(os.path.join(d, "bench"), np.float32, bl, pkh, vh)
+            zs = time.perf_counter() - t
+        res.update({"zstd_host_ms": zs * 1e3, "deltas_bytes": nd, "edits_bytes": ne,
+                    "overhead_bits_per_voxel": (nd + ne) * 8 / nvox})
+    return res
+
+
 def subdomains_lazy_vs_sync(edge=768, reps=2):
     """The subdomain ghost path on one GPU: a 768^3 float64 Perlin field (seed 2,
     base frequency 16, 6 octaves) at abs eb = 1e-4 x range, split 2x2x2 into
@@ -308,6 +342,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
     ap.add_argument("--no-subdomains", action="store_true",
                     help="skip the 768^3 f64 2x2x2 lazy-vs-sync run")
+    ap.add_argument("--no-edits", action="store_true",
+                    help="skip the edit side-channel encoding of the corrected field")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuned repeat of the headline workload")
     args = ap.parse_args()
@@ -421,6 +457,9 @@ def main():
         "definition": "(N*(s+1)*(1+detections) + edits*(3s+2)) / time per correction",
         "detections": detections, "algorithmic_bytes": e2e_bytes,
         "achieved": e2e, "unit": "GB/s", "frac": e2e / HBM_PEAK_GBPS}
+    if not args.no_edits:
+        res["edit_channel"] = edit_channel(tf, tg0, copies[-1], eb)
+        log("edit channel done")
     # the autotuner on this workload, then the same steps under the tuning it kept
     # (the headline value above is the default tuning)
     if not args.no_tune:

@@ -192,6 +192,45 @@ int pmsz_labels(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_fie
 int pmsz_edit_symbols(const pmsz_params *p, const void *d_f, const void *d_g0,
                       const void *d_g, uint8_t *d_symbols, int64_t *edited);
 
+/* The edit side channel of one corrected field, per rank block: the blocks of
+ * sub = {PX, PY, PZ} (pMSz.cu:2690-2706) in the reference's rank order, which
+ * MPI_Cart_create without reordering numbers row-major over {PX, PY, PZ}
+ * (pMSz.cu:2673-2682): the z coordinate varies fastest. */
+typedef struct {
+  int64_t voxels;        /* core voxels of the block */
+  int64_t packed_offset; /* byte offset of its packed symbol stream in d_packed */
+  int64_t packed_bytes;  /* ceil(voxels * nbits / 8) */
+  int64_t value_offset;  /* element offset of its clamp values in d_values */
+  int64_t values;        /* clamp values kept */
+} pmsz_edit_block;
+
+typedef struct {
+  int32_t nbits;          /* bits per symbol: bits_for_max(q) (pMSz.cu:2352-2356) */
+  int32_t blocks;         /* PX * PY * PZ */
+  int64_t edited;         /* voxels with g < g0 (h_count, pMSz.cu:2108) */
+  int64_t values;         /* clamp values kept, all blocks */
+  int64_t packed_capacity; /* bytes d_packed must hold (pmsz_edit_packed_bytes) */
+} pmsz_edit_info;
+
+/* Bytes the packed symbol streams of pmsz_encode_edits take at d_packed: each
+ * block's stream starts 16-byte aligned. */
+int pmsz_edit_packed_bytes(const pmsz_params *p, int64_t *bytes);
+
+/* Replaces run_extract_edits (pMSz.cu:2141-2226) with what gather_cost_sparse
+ * then does on the host before zstd (pMSz.cu:2469-2483): for each rank block,
+ * in core order (x fastest), the delta symbols of extract_edits_kernel
+ * (pMSz.cu:2090-2138) packed LSB first at nbits bits (pack_nbits,
+ * pMSz.cu:2328-2349), and the clamp values of its symbol-q voxels without the
+ * values equal to zero (compact_edits_dense_to_sparse, pMSz.cu:227-250).
+ * d_packed: pmsz_edit_packed_bytes bytes, 16-byte aligned; d_values: room for
+ * nx*ny*nz values of the dtype; blocks: PX*PY*PZ entries (host, nullable); info
+ * nullable.  A symbol wider than nbits bits (g0 is not the field the correction
+ * started from) gives PMSZ_EINVAL, as pack_nbits throws out_of_range.  A voxel
+ * with g > g0 gets symbol 0; the reference leaves its byte unwritten. */
+int pmsz_encode_edits(const pmsz_params *p, const void *d_f, const void *d_g0, const void *d_g,
+                      uint8_t *d_packed, void *d_values, pmsz_edit_block *blocks,
+                      pmsz_edit_info *info);
+
 /* Average device times of the original labeling pass (avg_seconds[0]) and of
  * detection 1, a full-volume detection sweep on g (avg_seconds[1]), over
  * `iters` launches each, timed with HIP events on the library stream.

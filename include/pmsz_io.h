@@ -1,7 +1,7 @@
 /*
  * pmsz_io.h -- C-ABI of libpmsz_io.so, the host-side front end that feeds the
  * correction: synthetic Perlin fields, the error-bounded codecs that produce the
- * decompressed field, and raw field I/O.  None of this is on the correction's
+ * decompressed field, the zstd files of the edit side channel, and raw field I/O.  None of this is on the correction's
  * hot path; it replaces the reference's reader and compressor adapters so the
  * CLI and the benchmarks can build their inputs.
  *
@@ -10,6 +10,8 @@
 #ifndef PMSZ_IO_H
 #define PMSZ_IO_H
 #include <stdint.h>
+
+#include "pmsz.h" /* pmsz_edit_block */
 
 #ifdef __cplusplus
 extern "C" {
@@ -49,6 +51,35 @@ int pmsz_io_zfp(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *f,
 /* Largest |a - b| and the number of voxels where |a - b| <= bound fails exactly. */
 int pmsz_io_max_abs_err(int dtype, int64_t n, const void *a, const void *b, double bound,
                         double *max_err, int64_t *violations);
+
+/* zstd, from the system libzstd loaded at run time (the reference links zstd:
+ * compress_zstd / zstd_compress, pMSz.cu:2261-2288).  Returns
+ * ZSTD_versionNumber(), or PMSZ_IO_EUNAVAILABLE when the library is absent. */
+int pmsz_io_zstd_version(void);
+
+/* The edit files of gather_cost_sparse (pMSz.cu:2469-2523): for each rank
+ * block in rank order, one zstd frame of its packed symbol stream at level
+ * level_deltas (5 in the reference, compress_symbols_0_N) appended to
+ * <prefix>_deltas.bin, and one frame of its clamp values at level level_edits
+ * (3, compress_zstd) appended to <prefix>_edits.bin; a block without clamp
+ * values adds nothing there (compress_zstd returns no bytes for an empty
+ * vector).  packed, values and blocks are pmsz_encode_edits' outputs, copied to
+ * the host.  The files are truncated first (the reference opens them without
+ * MPI_MODE_TRUNC).  deltas_bytes / edits_bytes (nullable) receive the file
+ * sizes; their sum is the reference's storage overhead (pMSz.cu:2495). */
+int pmsz_io_write_edits(const char *prefix, int dtype, int nblocks, const pmsz_edit_block *blocks,
+                        const uint8_t *packed, const void *values, int level_deltas,
+                        int level_edits, int64_t *deltas_bytes, int64_t *edits_bytes);
+
+/* Reads the edit files back (validateResult, pMSz.cu:2545-2590): decompresses
+ * every frame of <prefix>_deltas.bin and unpacks each block's stream
+ * (unpack_nbits, pMSz.cu:2393-2408) into symbols (sum of block_voxels bytes:
+ * the blocks' symbols in rank order, each block x fastest), and decompresses
+ * <prefix>_edits.bin into values (room for values_cap values), setting
+ * *nvalues.  PMSZ_IO_EFAIL on a missing file, a zstd error or a size mismatch. */
+int pmsz_io_read_edits(const char *prefix, int dtype, int q, int nblocks,
+                       const int64_t *block_voxels, uint8_t *symbols, void *values,
+                       int64_t values_cap, int64_t *nvalues);
 
 /* Raw little-endian arrays (the reference reads raw f64, pMSz.cu:274-313). */
 int pmsz_io_read_raw(const char *path, int dtype, int64_t n, void *out);

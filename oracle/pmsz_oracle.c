@@ -12,6 +12,7 @@
  *   ghost values / sync .... pMSz.cu:320-608, 909-1272
  *   path compression ....... pMSz.cu:255-271
  *   edit symbols ........... pMSz.cu:2090-2138
+ *   edit side channel ...... pMSz.cu:223-250, 2328-2356, 2469-2483, 2673-2682
  *   block split ............ pMSz.cu:2690-2711
  *
  * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
@@ -177,4 +178,50 @@ int orc_edit_symbols(const orc_params *p, const void *f, const void *g0, const v
     edit_symbols_f64(p, (const double *)f, (const double *)g0, (const double *)g, symbols,
                      edited);
   return ORC_OK;
+}
+
+unsigned orc_bits_for_max(unsigned q) {
+  /* ceil(log2(q + 1)), 1 for q = 0 (pMSz.cu:2352-2356), counted in integers */
+  unsigned b = 0;
+  while (b < 32 && (1ull << b) <= q) b++;
+  return b ? b : 1;
+}
+
+int orc_encode_edits(const orc_params *p, const void *f, const void *g0, const void *g,
+                     uint8_t *packed, void *vals, orc_edit_block *blocks, int64_t *edited) {
+  int rc = orc_check(p);
+  if (rc) return rc;
+  const unsigned nbits = orc_bits_for_max((unsigned)p->q);
+  const int px = p->sub[0], py = p->sub[1], pz = p->sub[2];
+  const size_t es = p->dtype == ORC_F32 ? 4 : 8;
+  int64_t poff = 0, voff = 0, ne = 0;
+  int bad = 0;
+  /* MPI_Cart_create(..., reorder = 0) over dims {PX, PY, PZ} (pMSz.cu:2673-2682)
+   * numbers the ranks row-major: rank = (cx * PY + cy) * PZ + cz. */
+  for (int r = 0; r < px * py * pz; r++) {
+    int cx = r / (py * pz), cy = (r / pz) % py, cz = r % pz;
+    int64_t l[3], o[3], nv = 0;
+    orc_split(p->nx, px, cx, &l[0], &o[0]);
+    orc_split(p->ny, py, cy, &l[1], &o[1]);
+    orc_split(p->nz, pz, cz, &l[2], &o[2]);
+    char *vp = (char *)vals + voff * es;
+    if (p->dtype == ORC_F32)
+      bad |= encode_block_f32(p, (const float *)f, (const float *)g0, (const float *)g, l, o,
+                              nbits, packed + poff, (float *)vp, &nv, &ne);
+    else
+      bad |= encode_block_f64(p, (const double *)f, (const double *)g0, (const double *)g, l, o,
+                              nbits, packed + poff, (double *)vp, &nv, &ne);
+    int64_t vox = l[0] * l[1] * l[2], pb = (vox * nbits + 7) / 8;
+    if (blocks) {
+      blocks[r].voxels = vox;
+      blocks[r].packed_offset = poff;
+      blocks[r].packed_bytes = pb;
+      blocks[r].value_offset = voff;
+      blocks[r].values = nv;
+    }
+    poff += (pb + 15) & ~(int64_t)15;
+    voff += nv;
+  }
+  if (edited) *edited = ne;
+  return bad ? ORC_EINVAL : ORC_OK;
 }

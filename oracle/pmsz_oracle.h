@@ -102,6 +102,22 @@ int orc_correct(const orc_params *p, const void *f, void *g, uint8_t *dc,
 int orc_edit_symbols(const orc_params *p, const void *f, const void *g0,
                      const void *g, uint8_t *symbols, int64_t *edited);
 
+/* One rank block of the edit side channel: the same fields, layout and rank
+ * order as pmsz_edit_block / pmsz_encode_edits of the product. */
+typedef struct {
+  int64_t voxels, packed_offset, packed_bytes, value_offset, values;
+} orc_edit_block;
+
+/* bits_for_max (pMSz.cu:2352-2356). */
+unsigned orc_bits_for_max(unsigned q);
+
+/* The edit side channel per rank block, in rank order: packed symbol streams
+ * (each starting 16-byte aligned in packed) and the nonzero clamp values
+ * (pMSz.cu:227-250, 2090-2138, 2328-2356).  packed must hold the padded total,
+ * vals nx*ny*nz values.  ORC_EINVAL when a symbol exceeds bits_for_max(q) bits. */
+int orc_encode_edits(const orc_params *p, const void *f, const void *g0, const void *g,
+                     uint8_t *packed, void *vals, orc_edit_block *blocks, int64_t *edited);
+
 /* Balanced block split of one dimension (pMSz.cu:2690-2706). */
 void orc_split(int64_t n, int parts, int coord, int64_t *local, int64_t *offset);
 

@@ -16,6 +16,7 @@ _LIB_PATH = os.path.join(_HERE, "libpmsz_oracle.so")
 
 LAZY, SYNC = 0, 1
 CLAMP_REFERENCE, CLAMP_BOUND_SAFE = 0, 1
+OK, EINVAL, ENOMEM = 0, -1, -2
 GHOST_REFERENCE, GHOST_FULL = 0, 1
 NOT_CONVERGED = -3
 
@@ -84,6 +85,10 @@ def lib():
         L.orc_correct.argtypes = [ctypes.POINTER(Params), vp, vp, vp, ctypes.POINTER(Stats),
                                   ctypes.POINTER(Trace)]
         L.orc_edit_symbols.argtypes = [ctypes.POINTER(Params), vp, vp, vp, vp, vp]
+        L.orc_bits_for_max.argtypes = [ctypes.c_uint]
+        L.orc_bits_for_max.restype = ctypes.c_uint
+        L.orc_encode_edits.argtypes = [ctypes.POINTER(Params), vp, vp, vp, vp, vp,
+                                       ctypes.POINTER(EditBlock), vp]
         L.orc_within.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_within.restype = i32
         L.orc_split.argtypes = [i64, i32, i32, ctypes.POINTER(i64), ctypes.POINTER(i64)]
@@ -185,6 +190,35 @@ def edit_symbols(f, g0, g, eb, **kw):
                                 ctypes.cast(ctypes.byref(ed), ctypes.c_void_p))
     assert rc == 0, rc
     return sym, ed.value
+
+
+class EditBlock(ctypes.Structure):
+    _fields_ = [("voxels", ctypes.c_int64), ("packed_offset", ctypes.c_int64),
+                ("packed_bytes", ctypes.c_int64), ("value_offset", ctypes.c_int64),
+                ("values", ctypes.c_int64)]
+
+
+def bits_for_max(q):
+    return int(lib().orc_bits_for_max(int(q)))
+
+
+def encode_edits(f, g0, g, eb, **kw):
+    """(packed bytes, clamp values, blocks, edited, rc) of the edit side channel:
+    per rank block in rank order, as pmsz_encode_edits lays them out."""
+    p = make_params(f.shape, f.dtype, eb, **kw)
+    nb = p.sub[0] * p.sub[1] * p.sub[2]
+    nbits = bits_for_max(p.q)
+    packed = np.zeros(f.size * nbits // 8 + 32 * nb, np.uint8)
+    vals = np.zeros(f.size, f.dtype)
+    blocks = (EditBlock * nb)()
+    ed = ctypes.c_int64(0)
+    rc = lib().orc_encode_edits(ctypes.byref(p), _ptr(f), _ptr(g0), _ptr(g), _ptr(packed),
+                                _ptr(vals), blocks,
+                                ctypes.cast(ctypes.byref(ed), ctypes.c_void_p))
+    bl = [{k: getattr(b, k) for k, _ in EditBlock._fields_} for b in blocks]
+    nv = sum(b["values"] for b in bl)
+    end = bl[-1]["packed_offset"] + bl[-1]["packed_bytes"]
+    return packed[:end], vals[:nv], bl, ed.value, rc
 
 
 def clamp(fv, eb, clamp_mode, dtype=np.float32):

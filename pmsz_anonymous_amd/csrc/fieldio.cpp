@@ -16,15 +16,20 @@
 //   stored verbatim if it fails, so g0 always honours the bound.
 // * ZFP: fixed-accuracy mode of the system libzfp (0.5.5 in this image), loaded
 //   at run time; absent library => PMSZ_IO_EUNAVAILABLE, never a fake result.
+// * Edit files: zstd frames of the packed symbols and clamp values, written and
+//   read as gather_cost_sparse / validateResult do (pMSz.cu:2429-2590), with the
+//   system libzstd loaded at run time the same way.
 #include "../../include/pmsz_io.h"
 
 #include <dlfcn.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace {
@@ -207,6 +212,68 @@ ZfpApi &zfp_api() {
   return api;
 }
 
+// The zstd entry points the edit files need (zstd.h declares them with these
+// C signatures); the image ships the library without its header.
+struct ZstdApi {
+  void *h = nullptr;
+  size_t (*compress)(void *, size_t, const void *, size_t, int) = nullptr;
+  size_t (*bound)(size_t) = nullptr;
+  size_t (*decompress)(void *, size_t, const void *, size_t) = nullptr;
+  unsigned (*is_error)(size_t) = nullptr;
+  unsigned (*version)(void) = nullptr;
+  bool ok = false;
+  ZstdApi() {
+    const char *names[] = {"libzstd.so.1", "libzstd.so"};
+    for (const char *n : names) {
+      h = dlopen(n, RTLD_NOW | RTLD_LOCAL);
+      if (h) break;
+    }
+    if (!h) return;
+#define PMSZ_SYM(dst, name) *(void **)(&dst) = dlsym(h, name)
+    PMSZ_SYM(compress, "ZSTD_compress");
+    PMSZ_SYM(bound, "ZSTD_compressBound");
+    PMSZ_SYM(decompress, "ZSTD_decompress");
+    PMSZ_SYM(is_error, "ZSTD_isError");
+    PMSZ_SYM(version, "ZSTD_versionNumber");
+#undef PMSZ_SYM
+    ok = compress && bound && decompress && is_error && version;
+  }
+};
+
+ZstdApi &zstd_api() {
+  static ZstdApi api;
+  return api;
+}
+
+// One zstd frame of n bytes appended to fp; nothing for n = 0.
+int zstd_append(FILE *fp, const void *src, size_t n, int level, int64_t &written) {
+  if (n == 0) return PMSZ_IO_OK;
+  ZstdApi &z = zstd_api();
+  std::vector<uint8_t> out(z.bound(n));
+  size_t c = z.compress(out.data(), out.size(), src, n, level);
+  if (z.is_error(c)) return PMSZ_IO_EFAIL;
+  if (std::fwrite(out.data(), 1, c, fp) != c) return PMSZ_IO_EFAIL;
+  written += (int64_t)c;
+  return PMSZ_IO_OK;
+}
+
+bool read_file(const std::string &path, std::vector<uint8_t> &buf) {
+  FILE *fp = std::fopen(path.c_str(), "rb");
+  if (!fp) return false;
+  buf.clear();
+  uint8_t tmp[1 << 16];
+  size_t k;
+  while ((k = std::fread(tmp, 1, sizeof(tmp), fp)) > 0) buf.insert(buf.end(), tmp, tmp + k);
+  std::fclose(fp);
+  return true;
+}
+
+unsigned bits_for_max(unsigned q) {
+  unsigned b = 0;
+  while (b < 32 && (1ull << b) <= q) b++;
+  return b ? b : 1;
+}
+
 }  // namespace
 
 extern "C" {
@@ -316,6 +383,96 @@ int pmsz_io_max_abs_err(int dtype, int64_t n, const void *a, const void *b, doub
   }
   if (max_err) *max_err = m;
   if (violations) *violations = v;
+  return PMSZ_IO_OK;
+}
+
+int pmsz_io_zstd_version(void) {
+  ZstdApi &z = zstd_api();
+  return z.ok ? (int)z.version() : PMSZ_IO_EUNAVAILABLE;
+}
+
+int pmsz_io_write_edits(const char *prefix, int dtype, int nblocks, const pmsz_edit_block *blocks,
+                        const uint8_t *packed, const void *values, int level_deltas,
+                        int level_edits, int64_t *deltas_bytes, int64_t *edits_bytes) {
+  if (!prefix || nblocks < 1 || !blocks || !packed || (dtype != 0 && dtype != 1))
+    return PMSZ_IO_EINVAL;
+  if (!zstd_api().ok) return PMSZ_IO_EUNAVAILABLE;
+  const size_t es = dtype == 0 ? 4 : 8;
+  for (int b = 0; b < nblocks; b++)
+    if (blocks[b].packed_bytes < 0 || blocks[b].values < 0 || (blocks[b].values && !values))
+      return PMSZ_IO_EINVAL;
+  const std::string pre(prefix);
+  FILE *fd = std::fopen((pre + "_deltas.bin").c_str(), "wb");
+  if (!fd) return PMSZ_IO_EFAIL;
+  FILE *fe = std::fopen((pre + "_edits.bin").c_str(), "wb");
+  if (!fe) {
+    std::fclose(fd);
+    return PMSZ_IO_EFAIL;
+  }
+  int64_t nd = 0, ne = 0;
+  int rc = PMSZ_IO_OK;
+  for (int b = 0; b < nblocks && rc == PMSZ_IO_OK; b++) {
+    rc = zstd_append(fd, packed + blocks[b].packed_offset, (size_t)blocks[b].packed_bytes,
+                     level_deltas, nd);
+    if (rc == PMSZ_IO_OK)
+      rc = zstd_append(fe, (const uint8_t *)values + blocks[b].value_offset * es,
+                       (size_t)blocks[b].values * es, level_edits, ne);
+  }
+  if (std::fclose(fd) != 0) rc = PMSZ_IO_EFAIL;
+  if (std::fclose(fe) != 0) rc = PMSZ_IO_EFAIL;
+  if (rc) return rc;
+  if (deltas_bytes) *deltas_bytes = nd;
+  if (edits_bytes) *edits_bytes = ne;
+  return PMSZ_IO_OK;
+}
+
+int pmsz_io_read_edits(const char *prefix, int dtype, int q, int nblocks,
+                       const int64_t *block_voxels, uint8_t *symbols, void *values,
+                       int64_t values_cap, int64_t *nvalues) {
+  if (!prefix || nblocks < 1 || !block_voxels || !symbols || q < 0 || values_cap < 0 ||
+      (values_cap && !values) || (dtype != 0 && dtype != 1))
+    return PMSZ_IO_EINVAL;
+  ZstdApi &z = zstd_api();
+  if (!z.ok) return PMSZ_IO_EUNAVAILABLE;
+  const unsigned nbits = bits_for_max((unsigned)q);
+  if (nbits > 16) return PMSZ_IO_EINVAL;
+  const std::string pre(prefix);
+  std::vector<uint8_t> comp;
+  // deltas: the concatenated frames decode to the blocks' packed streams in order
+  if (!read_file(pre + "_deltas.bin", comp)) return PMSZ_IO_EFAIL;
+  size_t need = 0;
+  for (int b = 0; b < nblocks; b++) {
+    if (block_voxels[b] < 0) return PMSZ_IO_EINVAL;
+    need += ((size_t)block_voxels[b] * nbits + 7) / 8;
+  }
+  std::vector<uint8_t> pk(std::max<size_t>(need, 1));
+  size_t got = need ? z.decompress(pk.data(), need, comp.data(), comp.size()) : 0;
+  if (z.is_error(got) || got != need) return PMSZ_IO_EFAIL;
+  const uint64_t mask = (1ull << nbits) - 1ull;
+  size_t in = 0, o = 0;
+  for (int b = 0; b < nblocks; b++) {
+    uint64_t buf = 0;
+    unsigned cnt = 0;
+    for (int64_t i = 0; i < block_voxels[b]; i++) {
+      while (cnt < nbits) {
+        buf |= (uint64_t)pk[in++] << cnt;
+        cnt += 8;
+      }
+      symbols[o++] = (uint8_t)(buf & mask);
+      buf >>= nbits;
+      cnt -= nbits;
+    }
+  }
+  // edits: the concatenated frames of clamp values
+  if (!read_file(pre + "_edits.bin", comp)) return PMSZ_IO_EFAIL;
+  const size_t es = dtype == 0 ? 4 : 8;
+  size_t vb = 0;
+  if (!comp.empty()) {
+    if (!values_cap) return PMSZ_IO_EFAIL;
+    vb = z.decompress(values, (size_t)values_cap * es, comp.data(), comp.size());
+    if (z.is_error(vb) || vb % es) return PMSZ_IO_EFAIL;
+  }
+  if (nvalues) *nvalues = (int64_t)(vb / es);
   return PMSZ_IO_OK;
 }
 

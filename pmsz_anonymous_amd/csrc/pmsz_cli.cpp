@@ -6,14 +6,19 @@
 // little-endian field (x fastest), the absolute error bound, the compressor id
 // and the subdomain grid.  The reference reads float64 only (pMSz.cu:2821);
 // --dtype f32 reads float32.  Output lines keep the reference's keys
-// ("iteration is:", pMSz.cu:2073; "overall edit ratio is: ... total_edited:",
-// pMSz.cu:3144).
+// ("iteration is:", pMSz.cu:2073; "total storage overhead is:", "original CR
+// is:", "overall CR is:", "overall edit ratio is: ... total_edited:",
+// pMSz.cu:3140-3144).  Like the reference (gather_cost_sparse,
+// pMSz.cu:2429-2543) it writes the edit side channel to <prefix>_deltas.bin and
+// <prefix>_edits.bin, prefix "" by default (the working directory).
 //
 // options: --dtype f32|f64   --mode lazy|sync   --max-sweeps N
 //          --clamp reference|bound-safe   --ghost reference|full   --q N
 //          --out <path> (corrected field, raw)   --symbols <path> (delta symbols)
+//          --edits-prefix <prefix>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -32,7 +37,7 @@ int usage() {
                "usage: pmsz \"<path>,<W>,<H>,<D>\" <abs_eb> <sz3|zfp> <PX> <PY> <PZ> "
                "[--dtype f32|f64] [--mode lazy|sync] [--max-sweeps N] "
                "[--clamp reference|bound-safe] [--ghost reference|full] [--q N] "
-               "[--out path] [--symbols path]\n");
+               "[--out path] [--symbols path] [--edits-prefix prefix]\n");
   return 2;
 }
 
@@ -67,7 +72,7 @@ int main(int argc, char **argv) {
   p.sub[0] = std::atoi(argv[4]);
   p.sub[1] = std::atoi(argv[5]);
   p.sub[2] = std::atoi(argv[6]);
-  std::string out_path, sym_path;
+  std::string out_path, sym_path, edits_prefix;
   for (int i = 7; i < argc; i++) {
     std::string o = argv[i];
     auto val = [&](void) -> std::string {
@@ -94,6 +99,8 @@ int main(int argc, char **argv) {
       out_path = val();
     } else if (o == "--symbols") {
       sym_path = val();
+    } else if (o == "--edits-prefix") {
+      edits_prefix = val();
     } else {
       return usage();
     }
@@ -149,8 +156,43 @@ int main(int argc, char **argv) {
     return 1;
   }
   std::printf("iteration is: %lld\n", (long long)st.ite);
-  int64_t edited = 0;
-  if (pmsz_edit_symbols(&p, df, dg0, dg, dsym, &edited) != PMSZ_OK) {
+  // the edit side channel: packed symbols and clamp values on the device, zstd
+  // frames per rank block on the host (gather_cost_sparse, pMSz.cu:2447-2523)
+  auto t4 = std::chrono::steady_clock::now();
+  int64_t cap = 0;
+  void *dpk = nullptr, *dvals = nullptr;
+  if (pmsz_edit_packed_bytes(&p, &cap) != PMSZ_OK || hipMalloc(&dpk, cap + 16) != hipSuccess ||
+      hipMalloc(&dvals, n * es) != hipSuccess) {
+    std::fprintf(stderr, "edit buffers failed\n");
+    return 1;
+  }
+  const int nblocks = p.sub[0] * p.sub[1] * p.sub[2];
+  std::vector<pmsz_edit_block> blocks(nblocks);
+  pmsz_edit_info info;
+  rc = pmsz_encode_edits(&p, df, dg0, dg, (uint8_t *)dpk, dvals, blocks.data(), &info);
+  if (rc != PMSZ_OK) {
+    std::fprintf(stderr, "pmsz_encode_edits: %s (%d)\n", pmsz_strerror(rc), rc);
+    return 1;
+  }
+  std::vector<uint8_t> packed(cap + 16);
+  std::vector<unsigned char> vals((size_t)std::max<int64_t>(info.values, 1) * es);
+  if (hipMemcpy(packed.data(), dpk, cap, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(vals.data(), dvals, info.values * es, hipMemcpyDeviceToHost) != hipSuccess)
+    return 1;
+  auto t5 = std::chrono::steady_clock::now();
+  std::printf("[Time] extract_edits: %.6f s\n", std::chrono::duration<double>(t5 - t4).count());
+  const int64_t edited = info.edited;
+  int64_t nd = 0, ne = 0;
+  int wrc = pmsz_io_write_edits(edits_prefix.c_str(), p.dtype, nblocks, blocks.data(),
+                                packed.data(), vals.data(), 5, 3, &nd, &ne);
+  auto t6 = std::chrono::steady_clock::now();
+  if (wrc != PMSZ_IO_OK) {
+    std::fprintf(stderr, "cannot write the edit files (%d)\n", wrc);
+    return 1;
+  }
+  std::printf("[Time] compress: %.6f s\n", std::chrono::duration<double>(t6 - t5).count());
+  std::printf("compressed deltas: %lld, %lld\n", (long long)nd, (long long)ne);
+  if (!sym_path.empty() && pmsz_edit_symbols(&p, df, dg0, dg, dsym, nullptr) != PMSZ_OK) {
     std::fprintf(stderr, "edit extraction failed\n");
     return 1;
   }
@@ -178,6 +220,11 @@ int main(int argc, char **argv) {
               rc == PMSZ_OK ? "converged" : "NOT CONVERGED");
   std::printf("max abs error: %.17g (bound %.17g, violations %lld)\n", max_err, p.abs_eb,
               (long long)viol);
+  // pMSz.cu:3133-3144: ratios over the input file's size
+  const double orig = (double)n * (double)es, overhead = (double)(nd + ne);
+  std::printf("total storage overhead is: %lld\n", (long long)(nd + ne));
+  std::printf("original CR is: %.17g\n", cbytes > 0 ? orig / (double)cbytes : 0.0);
+  std::printf("overall CR is: %.17g\n", orig / ((double)cbytes + overhead));
   std::printf("overall edit ratio is: %.17g total_edited:%lld\n", (double)edited / (double)n,
               (long long)edited);
   std::printf("throughput: %.3f GVoxels/s (device %.3f ms)\n", n / secs / 1e9,
@@ -186,5 +233,7 @@ int main(int argc, char **argv) {
   (void)hipFree(dg0);
   (void)hipFree(dg);
   (void)hipFree(dsym);
+  (void)hipFree(dpk);
+  (void)hipFree(dvals);
   return rc == PMSZ_OK ? 0 : 3;
 }

@@ -78,15 +78,20 @@ struct EditParams {
 // from f - eb formed in double (within one unit in the last place of T of it)
 // and settled with the exact predicate, one step each way at most.
 template <typename T>
+__host__ __device__ inline T clamp_search(T fv, const EditParams &ep) {
+  T c = (T)((double)fv - ep.eb);
+  for (int k = 0; k < 8 && !within_exact((double)fv, (double)c, ep.eb); k++) c = next_up(c);
+  for (int k = 0; k < 8 && within_exact((double)fv, (double)next_down(c), ep.eb); k++)
+    c = next_down(c);
+  return c;
+}
+
+template <typename T>
 __host__ __device__ inline T clamp_value(T fv, const EditParams &ep) {
   T eps = (T)ep.eb;
   T c = fv - eps;
-  if (ep.clamp_mode == 1 && c < fv && !within_exact((double)fv, (double)c, ep.eb)) {
-    c = (T)((double)fv - ep.eb);
-    for (int k = 0; k < 8 && !within_exact((double)fv, (double)c, ep.eb); k++) c = next_up(c);
-    for (int k = 0; k < 8 && within_exact((double)fv, (double)next_down(c), ep.eb); k++)
-      c = next_down(c);
-  }
+  if (ep.clamp_mode == 1 && c < fv && !within_exact((double)fv, (double)c, ep.eb))
+    c = clamp_search(fv, ep);
   return c;
 }
 

@@ -59,6 +59,7 @@
 #include "../../include/pmsz.h"
 #include "pmsz_dense.h"
 #include "pmsz_device.h"
+#include "pmsz_edits.h"
 
 namespace pmsz {
 
@@ -1533,6 +1534,14 @@ Grid3 grid_of(int64_t nx, int64_t ny, int64_t nz) {
   return d;
 }
 
+// Block split of one axis (pMSz.cu:2690-2706): the first n % parts blocks hold
+// one more cell.
+void block_split(int64_t n, int parts, int coord, int64_t &len, int64_t &off) {
+  const int64_t base = n / parts, rem = n % parts;
+  len = base + (coord < rem ? 1 : 0);
+  off = (int64_t)coord * base + (coord < rem ? coord : rem);
+}
+
 EditParams edit_params(const pmsz_params *p) {
   EditParams ep;
   ep.eb = p->abs_eb;
@@ -2049,22 +2058,15 @@ int correct_subdomains(Ctx &c, const pmsz_params *p, const T *F, T *G, uint8_t *
   job.gn[0] = p->nx;
   job.gn[1] = p->ny;
   job.gn[2] = p->nz;
-  // block split of each axis (pMSz.cu:2690-2706): the first n % parts blocks
-  // hold one more cell
-  auto split = [](int64_t n, int parts, int coord, int64_t &len, int64_t &off) {
-    const int64_t base = n / parts, rem = n % parts;
-    len = base + (coord < rem ? 1 : 0);
-    off = (int64_t)coord * base + (coord < rem ? coord : rem);
-  };
   uint64_t cells = 0, maxcells = 0, core = 0;
   std::vector<int64_t> offs;
   for (int cz = 0; cz < sz; cz++)
     for (int cy = 0; cy < sy; cy++)
       for (int cx = 0; cx < sx; cx++) {
         int64_t l[3], o[3];
-        split(p->nx, sx, cx, l[0], o[0]);
-        split(p->ny, sy, cy, l[1], o[1]);
-        split(p->nz, sz, cz, l[2], o[2]);
+        block_split(p->nx, sx, cx, l[0], o[0]);
+        block_split(p->ny, sy, cy, l[1], o[1]);
+        block_split(p->nz, sz, cz, l[2], o[2]);
         Dom D;
         std::memset(&D, 0, sizeof(D));
         D.d = grid_of(l[0] + 2, l[1] + 2, l[2] + 2);
@@ -2329,6 +2331,112 @@ int correct_dispatch(Ctx &c, const pmsz_params *p, const void *d_f, void *d_g, u
   if (split)
     return correct_subdomains<double>(c, p, (const double *)d_f, (double *)d_g, d_dc, stats);
   return correct_single<double>(c, p, (const double *)d_f, (double *)d_g, d_dc, stats);
+}
+
+// bits_for_max (pMSz.cu:2352-2356): ceil(log2(q + 1)), the bit length of q.
+int edit_nbits(int q) {
+  int b = 0;
+  while (b < 31 && (1 << b) <= q) b++;
+  return b ? b : 1;
+}
+
+// The rank blocks of p in rank order (row-major over {PX, PY, PZ}: z fastest),
+// their tiles and their 16-byte aligned packed streams.
+void edit_layout(const pmsz_params *p, int nbits, std::vector<EncBlock> &bl, int64_t &packed,
+                 uint32_t &ntiles) {
+  bl.clear();
+  packed = 0;
+  ntiles = 0;
+  const int px = p->sub[0], py = p->sub[1], pz = p->sub[2];
+  for (int r = 0; r < px * py * pz; r++) {
+    const int cx = r / (py * pz), cy = (r / pz) % py, cz = r % pz;
+    int64_t l[3], o[3];
+    block_split(p->nx, px, cx, l[0], o[0]);
+    block_split(p->ny, py, cy, l[1], o[1]);
+    block_split(p->nz, pz, cz, l[2], o[2]);
+    EncBlock B;
+    B.lx = (uint32_t)l[0];
+    B.ly = (uint32_t)l[1];
+    B.n = (uint32_t)(l[0] * l[1] * l[2]);
+    B.tile0 = ntiles;
+    B.base = (o[2] * p->ny + o[1]) * p->nx + o[0];
+    B.pk_off = packed;
+    B.pk_bytes = ((int64_t)B.n * nbits + 7) / 8;
+    packed += (B.pk_bytes + 15) & ~(int64_t)15;
+    ntiles += (B.n + kEncTile - 1) / kEncTile;
+    bl.push_back(B);
+  }
+}
+
+template <typename T>
+int encode_edits(Ctx &c, const pmsz_params *p, const T *f, const T *g0, const T *g,
+                 uint8_t *packed, T *vals, pmsz_edit_block *blocks, pmsz_edit_info *info) {
+  const int nbits = edit_nbits(p->q);
+  std::vector<EncBlock> bl;
+  int64_t packed_bytes;
+  uint32_t ntiles;
+  edit_layout(p, nbits, bl, packed_bytes, ntiles);
+  const uint32_t nb = (uint32_t)bl.size();
+  // scratch in the context's byte map: blocks | look-back words | per-tile edited
+  // counts | ticket, error | totals
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t o_look = up(nb * sizeof(EncBlock));
+  const size_t o_edit = o_look + up((size_t)ntiles * 8);
+  const size_t o_tick = o_edit + up((size_t)ntiles * 4);
+  const size_t o_tot = o_tick + 256;
+  const size_t total = o_tot + up((size_t)(nb + 2) * 8);
+  int rc = c.tbyte.ensure(total);
+  if (rc) return rc;
+  uint8_t *s = c.tbyte.p;
+  EncBlock *d_bl = (EncBlock *)s;
+  unsigned long long *look = (unsigned long long *)(s + o_look);
+  uint32_t *tile_edited = (uint32_t *)(s + o_edit);
+  uint32_t *ticket = (uint32_t *)(s + o_tick), *err = ticket + 1;
+  unsigned long long *tot = (unsigned long long *)(s + o_tot);
+  PMSZ_HIP(hipMemcpyAsync(d_bl, bl.data(), nb * sizeof(EncBlock), hipMemcpyHostToDevice,
+                          c.stream));
+  PMSZ_HIP(hipMemsetAsync(s + o_look, 0, total - o_look, c.stream));
+  const EditParams ep = edit_params(p);
+  const bool flat = nb == 1 && ((uintptr_t)f & 15) == 0 && ((uintptr_t)g0 & 15) == 0 &&
+                    ((uintptr_t)g & 15) == 0;
+  if (flat)
+    k_edit_encode<T, true><<<ntiles, kEncThreads, 0, c.stream>>>(
+        f, g0, g, d_bl, nb, p->nx, p->nx * p->ny, nbits, ep, packed, vals, look, tile_edited,
+        ticket, err);
+  else
+    k_edit_encode<T, false><<<ntiles, kEncThreads, 0, c.stream>>>(
+        f, g0, g, d_bl, nb, p->nx, p->nx * p->ny, nbits, ep, packed, vals, look, tile_edited,
+        ticket, err);
+  PMSZ_HIP(hipGetLastError());
+  k_edit_totals<<<std::min<uint32_t>((ntiles + 255) / 256, 128u), 256, 0, c.stream>>>(
+      d_bl, nb, ntiles, look, tile_edited, tot);
+  PMSZ_HIP(hipGetLastError());
+  std::vector<unsigned long long> h(nb + 2);
+  uint32_t herr = 0;
+  PMSZ_HIP(hipMemcpyAsync(h.data(), tot, (nb + 2) * 8, hipMemcpyDeviceToHost, c.stream));
+  PMSZ_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c.stream));
+  PMSZ_HIP(hipStreamSynchronize(c.stream));
+  if (herr & kEncErrLook || h[nb + 1]) return PMSZ_EDEVICE;
+  if (herr & kEncErrSymbol) return PMSZ_EINVAL;
+  int64_t voff = 0;
+  for (uint32_t b = 0; b < nb; b++) {
+    if (blocks) {
+      blocks[b].voxels = bl[b].n;
+      blocks[b].packed_offset = bl[b].pk_off;
+      blocks[b].packed_bytes = bl[b].pk_bytes;
+      blocks[b].value_offset = voff;
+      blocks[b].values = (int64_t)h[1 + b];
+    }
+    voff += (int64_t)h[1 + b];
+  }
+  if (info) {
+    info->nbits = nbits;
+    info->blocks = (int32_t)nb;
+    info->edited = (int64_t)h[0];
+    info->values = voff;
+    info->packed_capacity = packed_bytes;
+  }
+  return PMSZ_OK;
 }
 }  // namespace pmsz
 
@@ -2744,6 +2852,35 @@ int pmsz_edit_symbols(const pmsz_params *p, const void *d_f, const void *d_g0, c
   PMSZ_HIP(hipStreamSynchronize(c.stream));
   if (edited) *edited = (int64_t)h;
   return PMSZ_OK;
+}
+
+int pmsz_edit_packed_bytes(const pmsz_params *p, int64_t *bytes) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (!bytes || (uint32_t)(p->sub[0] * p->sub[1] * p->sub[2]) > kMaxDomains) return PMSZ_EINVAL;
+  std::vector<EncBlock> bl;
+  uint32_t ntiles;
+  edit_layout(p, edit_nbits(p->q), bl, *bytes, ntiles);
+  return PMSZ_OK;
+}
+
+int pmsz_encode_edits(const pmsz_params *p, const void *d_f, const void *d_g0, const void *d_g,
+                      uint8_t *d_packed, void *d_values, pmsz_edit_block *blocks,
+                      pmsz_edit_info *info) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  const size_t es = p->dtype == PMSZ_F32 ? 4 : 8;
+  if (!d_f || !d_g0 || !d_g || !d_packed || !d_values) return PMSZ_EINVAL;
+  if (((uintptr_t)d_packed & 15) || ((uintptr_t)d_values % es)) return PMSZ_EINVAL;
+  if ((uint32_t)(p->sub[0] * p->sub[1] * p->sub[2]) > kMaxDomains) return PMSZ_EINVAL;
+  Ctx &c = default_ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  if ((rc = ctx_device(c))) return rc;
+  if (p->dtype == PMSZ_F32)
+    return encode_edits<float>(c, p, (const float *)d_f, (const float *)d_g0,
+                               (const float *)d_g, d_packed, (float *)d_values, blocks, info);
+  return encode_edits<double>(c, p, (const double *)d_f, (const double *)d_g0,
+                              (const double *)d_g, d_packed, (double *)d_values, blocks, info);
 }
 
 const char *pmsz_strerror(int code) {

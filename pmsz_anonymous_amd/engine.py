@@ -29,7 +29,8 @@ EXPORTED = [
     "pmsz_default_params", "pmsz_correct", "pmsz_correct_batch", "pmsz_correct_host",
     "pmsz_ctx_create", "pmsz_ctx_destroy", "pmsz_ctx_correct", "pmsz_ctx_correct_batch",
     "pmsz_ctx_bytes", "pmsz_ctx_trim", "pmsz_workspace_bytes",
-    "pmsz_codes", "pmsz_labels", "pmsz_edit_symbols", "pmsz_time_first_pass", "pmsz_strerror",
+    "pmsz_codes", "pmsz_labels", "pmsz_edit_symbols", "pmsz_edit_packed_bytes",
+    "pmsz_encode_edits", "pmsz_time_first_pass", "pmsz_strerror",
     "pmsz_version", "pmsz_default_tuning", "pmsz_ctx_set_tuning", "pmsz_ctx_get_tuning",
     "pmsz_autotune",
 ]
@@ -104,6 +105,20 @@ class Tuning(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class EditBlock(ctypes.Structure):
+    """pmsz_edit_block: one rank block of the edit side channel."""
+    _fields_ = [("voxels", ctypes.c_int64), ("packed_offset", ctypes.c_int64),
+                ("packed_bytes", ctypes.c_int64), ("value_offset", ctypes.c_int64),
+                ("values", ctypes.c_int64)]
+
+
+class EditInfo(ctypes.Structure):
+    """pmsz_edit_info: totals of pmsz_encode_edits."""
+    _fields_ = [("nbits", ctypes.c_int32), ("blocks", ctypes.c_int32),
+                ("edited", ctypes.c_int64), ("values", ctypes.c_int64),
+                ("packed_capacity", ctypes.c_int64)]
+
+
 def _tuning(d):
     t = Tuning()
     lib().pmsz_default_tuning(ctypes.byref(t))
@@ -173,6 +188,9 @@ def lib():
         L.pmsz_codes.argtypes = [i32, i64, i64, i64, vp, vp]
         L.pmsz_labels.argtypes = [i32, i64, i64, i64, vp, vp, vp]
         L.pmsz_edit_symbols.argtypes = [P, vp, vp, vp, vp, ctypes.POINTER(i64)]
+        L.pmsz_edit_packed_bytes.argtypes = [P, ctypes.POINTER(i64)]
+        L.pmsz_encode_edits.argtypes = [P, vp, vp, vp, vp, vp, ctypes.POINTER(EditBlock),
+                                        ctypes.POINTER(EditInfo)]
         L.pmsz_time_first_pass.argtypes = [P, vp, vp, i32, ctypes.POINTER(ctypes.c_double)]
         L.pmsz_strerror.argtypes = [i32]
         L.pmsz_strerror.restype = ctypes.c_char_p
@@ -427,6 +445,35 @@ def edit_symbols(f, g0, g, abs_eb, **kw):
                                  ctypes.c_void_p(sym.data_ptr()), ctypes.byref(ed))
     _check(rc, "pmsz_edit_symbols")
     return sym, ed.value
+
+
+def encode_edits(f, g0, g, abs_eb, **kw):
+    """Edit side channel of a correction on CUDA tensors (pmsz_encode_edits).
+
+    Returns (packed uint8 tensor, values tensor of f's dtype, blocks, info):
+    blocks lists, per rank block in rank order, voxels / packed_offset /
+    packed_bytes / value_offset / values; info holds nbits, blocks, edited,
+    values and packed_capacity."""
+    import torch
+    p = make_params(f.shape, f.dtype, abs_eb, **kw)
+    cap = ctypes.c_int64(0)
+    _check(lib().pmsz_edit_packed_bytes(ctypes.byref(p), ctypes.byref(cap)),
+           "pmsz_edit_packed_bytes")
+    nb = p.sub[0] * p.sub[1] * p.sub[2]
+    packed = torch.empty(max(cap.value, 16), dtype=torch.uint8, device=f.device)
+    vals = torch.empty(f.numel(), dtype=f.dtype, device=f.device)
+    blocks = (EditBlock * nb)()
+    info = EditInfo()
+    torch.cuda.current_stream().synchronize()
+    rc = lib().pmsz_encode_edits(ctypes.byref(p), ctypes.c_void_p(f.contiguous().data_ptr()),
+                                 ctypes.c_void_p(g0.contiguous().data_ptr()),
+                                 ctypes.c_void_p(g.contiguous().data_ptr()),
+                                 ctypes.c_void_p(packed.data_ptr()),
+                                 ctypes.c_void_p(vals.data_ptr()), blocks, ctypes.byref(info))
+    _check(rc, "pmsz_encode_edits")
+    bl = [{k: getattr(b, k) for k, _ in EditBlock._fields_} for b in blocks]
+    inf = {k: getattr(info, k) for k, _ in EditInfo._fields_}
+    return packed[:cap.value], vals[:info.values], bl, inf
 
 
 def time_first_pass(f, g, abs_eb, iters=5, **kw):

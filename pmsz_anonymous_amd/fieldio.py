@@ -1,5 +1,5 @@
 """Python view of libpmsz_io.so: synthetic Perlin fields, the error-bounded codecs
-that produce the decompressed field, and raw I/O.  Front end of the correction
+that produce the decompressed field, the zstd edit files, and raw I/O.  Front end of the correction
 path (the reference's reader / compressor adapters), not part of the hot path."""
 from __future__ import annotations
 
@@ -32,6 +32,11 @@ def lib():
         L.pmsz_io_zfp.argtypes = [i32, i64, i64, i64, vp, dbl, vp, ctypes.POINTER(i64)]
         L.pmsz_io_max_abs_err.argtypes = [i32, i64, vp, vp, dbl, ctypes.POINTER(dbl),
                                           ctypes.POINTER(i64)]
+        L.pmsz_io_zstd_version.argtypes = []
+        L.pmsz_io_write_edits.argtypes = [ctypes.c_char_p, i32, i32, vp, vp, vp, i32, i32,
+                                          ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.pmsz_io_read_edits.argtypes = [ctypes.c_char_p, i32, i32, i32, vp, vp, vp, i64,
+                                         ctypes.POINTER(i64)]
         L.pmsz_io_read_raw.argtypes = [ctypes.c_char_p, i32, i64, vp]
         L.pmsz_io_write_raw.argtypes = [ctypes.c_char_p, i32, i64, vp]
         _lib = L
@@ -128,3 +133,45 @@ def perlin_case(shape, seed, freq, octaves, eb_rel, compressor="sz3", dtype=np.f
     eb = eb_rel * (hi - lo)
     g0, _ = compress(f, eb, compressor)
     return f, g0, eb
+
+
+def zstd_version():
+    """ZSTD_versionNumber() of the system libzstd, or None when it is absent."""
+    v = lib().pmsz_io_zstd_version()
+    return None if v < 0 else v
+
+
+_BLOCK_KEYS = ("voxels", "packed_offset", "packed_bytes", "value_offset", "values")
+
+
+def write_edits(prefix, dtype, blocks, packed, values, level_deltas=5, level_edits=3):
+    """Writes <prefix>_deltas.bin and <prefix>_edits.bin from host copies of the
+    edit side channel (numpy packed bytes and values, the blocks list of
+    engine.encode_edits); returns (deltas bytes, edits bytes)."""
+    bl = np.array([[b[k] for k in _BLOCK_KEYS] for b in blocks], np.int64)
+    packed = np.ascontiguousarray(packed, np.uint8)
+    if packed.size == 0:
+        packed = np.zeros(1, np.uint8)
+    values = np.ascontiguousarray(values, np.dtype(dtype))
+    if values.size == 0:
+        values = np.zeros(1, np.dtype(dtype))
+    nd, ne = ctypes.c_int64(0), ctypes.c_int64(0)
+    rc = lib().pmsz_io_write_edits(str(prefix).encode(), _dt(dtype), len(blocks), _p(bl),
+                                   _p(packed), _p(values), int(level_deltas), int(level_edits),
+                                   ctypes.byref(nd), ctypes.byref(ne))
+    if rc != 0:
+        raise ValueError(f"pmsz_io_write_edits failed ({rc})")
+    return nd.value, ne.value
+
+
+def read_edits(prefix, dtype, q, block_voxels, values_cap):
+    """(symbols uint8 in rank-block order, values) read back from the edit files."""
+    bv = np.ascontiguousarray(block_voxels, np.int64)
+    sym = np.zeros(max(int(bv.sum()), 1), np.uint8)
+    vals = np.zeros(max(int(values_cap), 1), np.dtype(dtype))
+    nv = ctypes.c_int64(0)
+    rc = lib().pmsz_io_read_edits(str(prefix).encode(), _dt(dtype), int(q), bv.size, _p(bv),
+                                  _p(sym), _p(vals), int(values_cap), ctypes.byref(nv))
+    if rc != 0:
+        raise ValueError(f"pmsz_io_read_edits failed ({rc})")
+    return sym[:int(bv.sum())], vals[:nv.value]

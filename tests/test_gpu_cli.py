@@ -1,0 +1,63 @@
+"""The pmsz executable end to end, as the reference is run (pMSz.cu:2596-3188):
+raw f64 input, SZ-style compression, correction over a 2x1x2 rank grid, the
+corrected field and the edit files checked against the oracle, and the report
+lines (iteration, storage overhead, CR, overall CR, edit ratio)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from editcheck import rank_blocks
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "pmsz_anonymous_amd", "pmsz")
+
+
+def test_cli_sz3_subdomains(tmp_path):
+    from pmsz_anonymous_amd import fieldio
+    if fieldio.zstd_version() is None:
+        pytest.skip("libzstd absent")
+    nz, ny, nx = 30, 36, 40
+    f = fieldio.perlin((nz, ny, nx), 3, 5, 4, dtype=np.float64)
+    lo, hi = fieldio.value_range(f)
+    eb = 2e-3 * (hi - lo)
+    src = tmp_path / "field.raw"
+    f.tofile(src)
+    out = tmp_path / "corrected.raw"
+    cmd = [EXE, f"{src},{nx},{ny},{nz}", repr(eb), "sz3", "2", "1", "2", "--out", str(out),
+           "--mode", "sync"]
+    print("run", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    text = r.stdout
+
+    g0, info = fieldio.szlike(f, eb)
+    go, _, sto, _, rc = O.correct(f, g0, eb, sub=(2, 1, 2), mode=O.SYNC)
+    assert rc == 0
+    g = np.fromfile(out, np.float64).reshape(f.shape)
+    assert g.tobytes() == go.tobytes()
+    assert f"iteration is: {sto['ite']}" in text
+
+    # the edit files in the working directory, prefix ""
+    sym, vals = fieldio.read_edits(str(tmp_path) + "/", np.float64, 5,
+                                   [f[sl].size for sl in rank_blocks(f.shape, (2, 1, 2))],
+                                   f.size)
+    want, edited = O.edit_symbols(f, g0, go, eb)
+    assert sym.tobytes() == np.concatenate(
+        [want[sl].ravel() for sl in rank_blocks(f.shape, (2, 1, 2))]).tobytes()
+    _, ovals, _, _, _ = O.encode_edits(f, g0, go, eb, sub=(2, 1, 2))
+    assert vals.tobytes() == ovals.tobytes()
+
+    nd = os.path.getsize(tmp_path / "_deltas.bin")
+    ne = os.path.getsize(tmp_path / "_edits.bin")
+    assert re.search(rf"total storage overhead is: {nd + ne}\n", text)
+    m = re.search(r"compression time: \S+ s \(sz3, (\d+) bytes\)", text)
+    cmp_bytes = int(m.group(1))
+    ocr = float(re.search(r"overall CR is: (\S+)", text).group(1))
+    assert ocr == pytest.approx(f.size * 8 / (cmp_bytes + nd + ne), rel=1e-12)
+    assert f"total_edited:{edited}" in text
