@@ -2344,6 +2344,7 @@ int edit_nbits(int q) {
 // their tiles and their 16-byte aligned packed streams.
 void edit_layout(const pmsz_params *p, int nbits, std::vector<EncBlock> &bl, int64_t &packed,
                  uint32_t &ntiles) {
+  const uint32_t tile = p->dtype == PMSZ_F32 ? enc_tile<float>() : enc_tile<double>();
   bl.clear();
   packed = 0;
   ntiles = 0;
@@ -2363,7 +2364,7 @@ void edit_layout(const pmsz_params *p, int nbits, std::vector<EncBlock> &bl, int
     B.pk_off = packed;
     B.pk_bytes = ((int64_t)B.n * nbits + 7) / 8;
     packed += (B.pk_bytes + 15) & ~(int64_t)15;
-    ntiles += (B.n + kEncTile - 1) / kEncTile;
+    ntiles += (B.n + tile - 1) / tile;
     bl.push_back(B);
   }
 }
@@ -2378,7 +2379,7 @@ int encode_edits(Ctx &c, const pmsz_params *p, const T *f, const T *g0, const T 
   edit_layout(p, nbits, bl, packed_bytes, ntiles);
   const uint32_t nb = (uint32_t)bl.size();
   // scratch in the context's byte map: blocks | look-back words | per-tile edited
-  // counts | ticket, error | totals
+  // counts | error flags | totals
   auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t o_look = up(nb * sizeof(EncBlock));
   const size_t o_edit = o_look + up((size_t)ntiles * 8);
@@ -2391,22 +2392,23 @@ int encode_edits(Ctx &c, const pmsz_params *p, const T *f, const T *g0, const T 
   EncBlock *d_bl = (EncBlock *)s;
   unsigned long long *look = (unsigned long long *)(s + o_look);
   uint32_t *tile_edited = (uint32_t *)(s + o_edit);
-  uint32_t *ticket = (uint32_t *)(s + o_tick), *err = ticket + 1;
+  uint32_t *err = (uint32_t *)(s + o_tick);
   unsigned long long *tot = (unsigned long long *)(s + o_tot);
   PMSZ_HIP(hipMemcpyAsync(d_bl, bl.data(), nb * sizeof(EncBlock), hipMemcpyHostToDevice,
                           c.stream));
   PMSZ_HIP(hipMemsetAsync(s + o_look, 0, total - o_look, c.stream));
   const EditParams ep = edit_params(p);
-  const bool flat = nb == 1 && ((uintptr_t)f & 15) == 0 && ((uintptr_t)g0 & 15) == 0 &&
-                    ((uintptr_t)g & 15) == 0;
+  const bool aligned = ((uintptr_t)f & 15) == 0 && ((uintptr_t)g0 & 15) == 0 &&
+                       ((uintptr_t)g & 15) == 0;
+  const bool flat = nb == 1 && aligned;
   if (flat)
     k_edit_encode<T, true><<<ntiles, kEncThreads, 0, c.stream>>>(
-        f, g0, g, d_bl, nb, p->nx, p->nx * p->ny, nbits, ep, packed, vals, look, tile_edited,
-        ticket, err);
+        f, g0, g, d_bl, nb, p->nx, p->nx * p->ny, nbits, aligned, ep, packed, vals, look,
+        tile_edited, err);
   else
     k_edit_encode<T, false><<<ntiles, kEncThreads, 0, c.stream>>>(
-        f, g0, g, d_bl, nb, p->nx, p->nx * p->ny, nbits, ep, packed, vals, look, tile_edited,
-        ticket, err);
+        f, g0, g, d_bl, nb, p->nx, p->nx * p->ny, nbits, aligned, ep, packed, vals, look,
+        tile_edited, err);
   PMSZ_HIP(hipGetLastError());
   k_edit_totals<<<std::min<uint32_t>((ntiles + 255) / 256, 128u), 256, 0, c.stream>>>(
       d_bl, nb, ntiles, look, tile_edited, tot);
