@@ -289,6 +289,26 @@ def edit_channel(tf, tg0, tg, eb, reps=10):
     return res
 
 
+def naive_baseline(tf, tg0, eb, lazy_ms):
+    """The naive-MSz baseline (mode NAIVE, naive_MSz.cu:1887-2219) on the same
+    field: extrema-only sweeps plus path re-tracing, one run, against the
+    headline (lazy) correction's time per step."""
+    import torch
+    from pmsz_anonymous_amd import engine
+    g = tg0.clone()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, st = engine.correct(tf, g, eb, mode=engine.NAIVE)
+    ms = (time.perf_counter() - t) * 1e3
+    a0, d0 = engine.labels(tf)
+    a1, d1 = engine.labels(g)
+    ok = torch.equal(a0, a1) and torch.equal(d0, d1)
+    return {"ms": ms, "gvoxels_per_s": tf.numel() / ms / 1e6, "edit_sweeps": st["edit_sweeps"],
+            "outer_iters": st["outer_iters"], "total_targets": st["total_targets"],
+            "labels_preserved": bool(ok), "lazy_ms": lazy_ms,
+            "lazy_speedup": ms / lazy_ms}
+
+
 def subdomains_lazy_vs_sync(edge=768, reps=2):
     """The subdomain ghost path on one GPU: a 768^3 float64 Perlin field (seed 2,
     base frequency 16, 6 octaves) at abs eb = 1e-4 x range, split 2x2x2 into
@@ -344,6 +364,8 @@ def main():
                     help="skip the 768^3 f64 2x2x2 lazy-vs-sync run")
     ap.add_argument("--no-edits", action="store_true",
                     help="skip the edit side-channel encoding of the corrected field")
+    ap.add_argument("--no-naive", action="store_true",
+                    help="skip the naive-MSz baseline run on the same field")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuned repeat of the headline workload")
     args = ap.parse_args()
@@ -460,6 +482,9 @@ def main():
     if not args.no_edits:
         res["edit_channel"] = edit_channel(tf, tg0, copies[-1], eb)
         log("edit channel done")
+    if not args.no_naive:
+        res["naive"] = naive_baseline(tf, tg0, eb, ms)
+        log("naive baseline done")
     # the autotuner on this workload, then the same steps under the tuning it kept
     # (the headline value above is the default tuning)
     if not args.no_tune:

@@ -30,7 +30,10 @@ extern "C" {
 
 typedef enum { PMSZ_F32 = 0, PMSZ_F64 = 1 } pmsz_dtype;
 /* lazy ghost sync (pMSz.cu) or per-iteration sync (sync_pMSz.cu) */
-typedef enum { PMSZ_LAZY = 0, PMSZ_SYNC = 1 } pmsz_sync_mode;
+/* PMSZ_NAIVE: the naive-MSz baseline (naive_MSz.cu:1887-2219): extrema-only
+ * sweeps, then path re-tracing against the labels, per outer iteration; one
+ * domain only (sub = 1,1,1), and not in batches or the autotuner. */
+typedef enum { PMSZ_LAZY = 0, PMSZ_SYNC = 1, PMSZ_NAIVE = 2 } pmsz_sync_mode;
 /* clamp of the edit primitive: the reference's g = f - eb (pMSz.cu:636) in the
  * value type, or -- when that value misses |f - g| <= eb exactly -- the smallest
  * value of the type that meets it */

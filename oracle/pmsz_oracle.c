@@ -13,6 +13,7 @@
  *   path compression ....... pMSz.cu:255-271
  *   edit symbols ........... pMSz.cu:2090-2138
  *   edit side channel ...... pMSz.cu:223-250, 2328-2356, 2469-2483, 2673-2682
+ *   naive-MSz loop ......... naive_MSz.cu:644-850, 1767-1864, 1887-2219
  *   block split ............ pMSz.cu:2690-2711
  *
  * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
@@ -92,6 +93,23 @@ static void orc_compress_pointers(int64_t *lab, int64_t n) {
   }
 }
 
+/* Steps of a naive-MSz path walk before it is abandoned (a walk that follows
+ * f's steepest path never revisits a voxel; the cap only bounds a malformed one). */
+#define ORC_NAIVE_WALK_CAP (1 << 20)
+
+/* f's and g's pointers as or_types / de_direction hold them (naive_MSz.cu:740-760,
+ * 830-842): slot 0 of f is -1 at a maximum, -2 at a minimum, else the descent;
+ * slot 1 is the descent at a maximum, else the ascent. */
+static int64_t orc_naive_f0(const orc_params *p, const uint8_t *cf, int64_t c) {
+  if ((cf[c] & 15) == 14) return -1;
+  if ((cf[c] >> 4) == 14) return -2;
+  return orc_neighbor_global(p->nx, p->ny, c, cf[c] >> 4);
+}
+static int64_t orc_naive_f1(const orc_params *p, const uint8_t *cf, int64_t c) {
+  if ((cf[c] & 15) == 14) return orc_neighbor_global(p->nx, p->ny, c, cf[c] >> 4);
+  return orc_neighbor_global(p->nx, p->ny, c, cf[c] & 15);
+}
+
 static float next_up_f32(float c) { return nextafterf(c, INFINITY); }
 static double next_up_f64(double c) { return nextafter(c, INFINITY); }
 static float next_down_f32(float c) { return nextafterf(c, -INFINITY); }
@@ -159,6 +177,12 @@ int orc_correct(const orc_params *p, const void *f, void *g, uint8_t *dc, orc_st
   orc_stats tmp;
   if (!st) st = &tmp;
   if (trace) trace->n = 0;
+  if (p->mode == ORC_NAIVE) {
+    if (p->sub[0] * p->sub[1] * p->sub[2] != 1) return ORC_EINVAL;
+    if (p->dtype == ORC_F32)
+      return correct_naive_f32(p, (const float *)f, (float *)g, dc, st, trace);
+    return correct_naive_f64(p, (const double *)f, (double *)g, dc, st, trace);
+  }
   if (p->dtype == ORC_F32) {
     rc = correct_f32(p, (const float *)f, (float *)g, dc, st, trace);
   } else {

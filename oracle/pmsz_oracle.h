@@ -28,7 +28,7 @@ extern "C" {
 #endif
 
 enum { ORC_F32 = 0, ORC_F64 = 1 };
-enum { ORC_LAZY = 0, ORC_SYNC = 1 };
+enum { ORC_LAZY = 0, ORC_SYNC = 1, ORC_NAIVE = 2 };
 enum { ORC_CLAMP_REFERENCE = 0, ORC_CLAMP_BOUND_SAFE = 1 };
 enum { ORC_GHOST_REFERENCE = 0, ORC_GHOST_FULL = 1 };
 
@@ -45,7 +45,8 @@ typedef struct {
   double abs_eb;        /* absolute error bound (pMSz.cu:2628) */
   int q;                /* decrements before clamping, 5 in the reference (pMSz.cu:2630) */
   int sub[3];           /* subdomain grid PX PY PZ (pMSz.cu:2631-2633) */
-  int mode;             /* ORC_LAZY (pMSz.cu) or ORC_SYNC (sync_pMSz.cu) */
+  int mode;             /* ORC_LAZY (pMSz.cu), ORC_SYNC (sync_pMSz.cu) or ORC_NAIVE
+                           (naive_MSz.cu, one domain) */
   int max_sweeps;       /* cap on edit sweeps; the reference has none */
   int clamp_mode;       /* ORC_CLAMP_* */
   int ghost_policy;     /* ORC_GHOST_* */

@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libpmsz_oracle.so")
 
-LAZY, SYNC = 0, 1
+LAZY, SYNC, NAIVE = 0, 1, 2
 CLAMP_REFERENCE, CLAMP_BOUND_SAFE = 0, 1
 OK, EINVAL, ENOMEM = 0, -1, -2
 GHOST_REFERENCE, GHOST_FULL = 0, 1

@@ -12,7 +12,7 @@
 // pMSz.cu:2429-2543) it writes the edit side channel to <prefix>_deltas.bin and
 // <prefix>_edits.bin, prefix "" by default (the working directory).
 //
-// options: --dtype f32|f64   --mode lazy|sync   --max-sweeps N
+// options: --dtype f32|f64   --mode lazy|sync|naive   --max-sweeps N
 //          --clamp reference|bound-safe   --ghost reference|full   --q N
 //          --out <path> (corrected field, raw)   --symbols <path> (delta symbols)
 //          --edits-prefix <prefix>
@@ -35,7 +35,7 @@ namespace {
 int usage() {
   std::fprintf(stderr,
                "usage: pmsz \"<path>,<W>,<H>,<D>\" <abs_eb> <sz3|zfp> <PX> <PY> <PZ> "
-               "[--dtype f32|f64] [--mode lazy|sync] [--max-sweeps N] "
+               "[--dtype f32|f64] [--mode lazy|sync|naive] [--max-sweeps N] "
                "[--clamp reference|bound-safe] [--ghost reference|full] [--q N] "
                "[--out path] [--symbols path] [--edits-prefix prefix]\n");
   return 2;
@@ -86,7 +86,8 @@ int main(int argc, char **argv) {
       std::string v = val();
       p.dtype = v == "f32" ? PMSZ_F32 : PMSZ_F64;
     } else if (o == "--mode") {
-      p.mode = val() == "sync" ? PMSZ_SYNC : PMSZ_LAZY;
+      const std::string v = val();
+      p.mode = v == "sync" ? PMSZ_SYNC : v == "naive" ? PMSZ_NAIVE : PMSZ_LAZY;
     } else if (o == "--max-sweeps") {
       p.max_sweeps = std::atoi(val().c_str());
     } else if (o == "--clamp") {

@@ -60,6 +60,7 @@
 #include "pmsz_dense.h"
 #include "pmsz_device.h"
 #include "pmsz_edits.h"
+#include "pmsz_naive.h"
 
 namespace pmsz {
 
@@ -1511,7 +1512,7 @@ int check_params(const pmsz_params *p) {
   if (!(p->abs_eb > 0) || std::isinf(p->abs_eb)) return PMSZ_EINVAL;
   if (p->q < 1 || p->q > 255) return PMSZ_EINVAL;
   if (p->max_sweeps < 1) return PMSZ_EINVAL;
-  if (p->mode != PMSZ_LAZY && p->mode != PMSZ_SYNC) return PMSZ_EINVAL;
+  if (p->mode != PMSZ_LAZY && p->mode != PMSZ_SYNC && p->mode != PMSZ_NAIVE) return PMSZ_EINVAL;
   if (p->clamp_mode != 0 && p->clamp_mode != 1) return PMSZ_EINVAL;
   if (p->ghost_policy != 0 && p->ghost_policy != 1) return PMSZ_EINVAL;
   for (int a = 0; a < 3; a++)
@@ -2318,11 +2319,204 @@ struct pmsz_ctx {
 };
 
 namespace pmsz {
+// The codes of one unpadded field (the labeling pass of the full-volume kernel).
+template <typename T>
+int launch_codes(Ctx &c, const pmsz_params *p, const T *field, uint8_t *codes) {
+  Dom D = field_dom(p, field, nullptr, codes, nullptr);
+  static const int occ = occupancy(k_columns<T, true, false, false>, kColThreads);
+  k_columns<T, true, false, false><<<col_blocks(c, dom_units(D), occ), kColThreads, 0,
+                                     c.stream>>>(D, nullptr, 1, nullptr, (uint8_t *)nullptr,
+                                                 c.ring, 0, 0, 0u, c.run, 0u);
+  PMSZ_HIP(hipGetLastError());
+  return PMSZ_OK;
+}
+
+// Labels (pointer roots) from codes: pointer jumping in batches of 4 rounds until
+// a batch moves nothing.  settled holds sbytes of wave bits; changed one word.
+int labels_from_codes(Ctx &c, const uint8_t *codes, int32_t *asc, int32_t *desc, const Grid3 &d,
+                      uint32_t *settled, size_t sbytes, uint32_t *changed) {
+  const int64_t n = d.n;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  const unsigned jblocks = (unsigned)((2 * n + 255) / 256);
+  hipError_t e = hipMemsetAsync(settled, 0, sbytes, c.stream);
+  if (e == hipSuccess) {
+    k_init_pointers<<<blocks, 256, 0, c.stream>>>(codes, asc, desc, d);
+    // rounds in batches of 4; the batch's last round tells whether anything moved
+    // (a chain is shorter than n < 2^31, so 32 rounds settle any field)
+    bool settled_all = false;
+    for (int round = 0; round < 36 && e == hipSuccess && !settled_all; round += 4) {
+      for (int k = 0; k < 3; k++)
+        k_jump<<<jblocks, 256, 0, c.stream>>>(asc, desc, n, settled, changed);
+      e = hipMemsetAsync(changed, 0, sizeof(uint32_t), c.stream);
+      k_jump<<<jblocks, 256, 0, c.stream>>>(asc, desc, n, settled, changed);
+      uint32_t h = 1;
+      if (e == hipSuccess) e = hipGetLastError();
+      if (e == hipSuccess) e = hipMemcpyAsync(&h, changed, sizeof(h), hipMemcpyDeviceToHost, c.stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+      settled_all = e == hipSuccess && h == 0;
+    }
+    if (e == hipSuccess && !settled_all) e = hipErrorUnknown;  // a pointer field with a cycle
+  }
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e != hipSuccess) {
+    last_hip_error() = e;
+    return PMSZ_EDEVICE;
+  }
+  return PMSZ_OK;
+}
+
+// Bytes of the wave bits k_jump keeps for n voxels (two label fields).
+size_t settled_bytes(int64_t n) {
+  return sizeof(uint32_t) * (size_t)(((2 * n + 63) / 64 + 31) / 32 + 1);
+}
+
+// naive_MSz's correction (c_loops, naive_MSz.cu:1887-2219) on one field, under
+// snapshot sweeps.  Per outer iteration: extrema sweeps until no false maximum
+// or minimum is left (naive_MSz.cu:2000-2033); the labels of g and one path
+// sweep (naive_MSz.cu:2045-2058); a check of extrema and labels
+// (naive_MSz.cu:2077-2165; the reference runs it twice around the ghost
+// exchange, which one domain does not need).  The host reads the counters after
+// every step: the naive schedule is a baseline, not a tuned path.
+template <typename T>
+int correct_naive(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *d_dc,
+                  pmsz_stats *st) {
+  std::memset(st, 0, sizeof(*st));
+  const int64_t n = p->nx * p->ny * p->nz;
+  if (n >= ((int64_t)1 << 31)) return PMSZ_ETOOBIG;
+  const Grid3 d = grid_of(p->nx, p->ny, p->nz);
+  const size_t nb = ((size_t)n + 255) & ~(size_t)255;
+  const size_t sbytes = settled_bytes(n);
+  int rc;
+  if ((rc = c.code.ensure(n)) || (rc = c.snap.ensure(n)) || (rc = c.dc.ensure(n)) ||
+      (rc = c.tbyte.ensure(nb + 256 + sbytes + 256)) || (rc = c.tmark.ensure(n)) ||
+      (rc = c.wmark.ensure(n)) || (rc = c.tlist.ensure(n)) || (rc = c.wlist.ensure(n)))
+    return rc;
+  uint8_t *cf = c.code.p, *cg = c.snap.p, *dc = d_dc ? d_dc : c.dc.p;
+  uint8_t *mark = c.tbyte.p;
+  unsigned *cnt = (unsigned *)(c.tbyte.p + nb);
+  uint32_t *settled = (uint32_t *)(c.tbyte.p + nb + 256);
+  uint32_t *changed = (uint32_t *)((uint8_t *)settled + sbytes);
+  int32_t *fa = (int32_t *)c.tmark.p, *fd = (int32_t *)c.wmark.p;
+  int32_t *ga = (int32_t *)c.tlist.p, *gd = (int32_t *)c.wlist.p;
+  const EditParams ep = edit_params(p);
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  unsigned h[8];
+  auto read_cnt = [&]() -> int {
+    PMSZ_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+    PMSZ_HIP(hipStreamSynchronize(c.stream));
+    return PMSZ_OK;
+  };
+  auto clear_cnt = [&]() -> int {
+    PMSZ_HIP(hipMemsetAsync(cnt, 0, sizeof(h), c.stream));
+    return PMSZ_OK;
+  };
+  auto trace = [&](uint32_t a, uint32_t b, uint32_t dir, uint32_t targets) {
+    if (st->trace_n < 64) {
+      st->trace_cnt[st->trace_n][0] = a;
+      st->trace_cnt[st->trace_n][1] = b;
+      st->trace_cnt[st->trace_n][2] = dir;
+      st->trace_targets[st->trace_n] = targets;
+      st->trace_n++;
+    }
+  };
+
+  PMSZ_HIP(hipEventRecord(c.ev0, c.stream));
+  PMSZ_HIP(hipMemsetAsync(c.run, 0, sizeof(RunState), c.stream));
+  PMSZ_HIP(hipMemsetAsync(dc, 0, (size_t)n, c.stream));
+  PMSZ_HIP(hipMemsetAsync(mark, 0, (size_t)n, c.stream));
+  if ((rc = launch_codes<T>(c, p, f, cf)) || (rc = launch_codes<T>(c, p, g, cg))) return rc;
+  // NaN or sentinel values in f or g (flagged by the labeling pass)
+  PMSZ_HIP(hipMemcpyAsync(c.h_run, c.run, sizeof(RunState), hipMemcpyDeviceToHost, c.stream));
+  PMSZ_HIP(hipStreamSynchronize(c.stream));
+  if (c.h_run->err) return PMSZ_EINVAL;
+  if ((rc = labels_from_codes(c, cf, fa, fd, d, settled, sbytes, changed))) return rc;
+  st->dense_passes = 2;
+
+  uint32_t hmax = 1, hmin = 1, hdir = 0;
+  int64_t sweeps = 0;
+  int status = PMSZ_OK;
+  for (;;) {
+    st->outer_iters++;
+    st->ite += 2;  // naive_MSz.cu:1966, 1979
+    while (hmax || hmin) {
+      if (sweeps >= p->max_sweeps) {
+        status = PMSZ_NOT_CONVERGED;
+        break;
+      }
+      st->ite++;  // naive_MSz.cu:1982
+      if ((rc = launch_codes<T>(c, p, g, cg)) || (rc = clear_cnt())) return rc;
+      k_naive_extrema<<<blocks, 256, 0, c.stream>>>(cf, cg, d, mark, cnt);
+      k_naive_apply<T><<<blocks, 256, 0, c.stream>>>(f, g, dc, mark, n, ep, cnt);
+      PMSZ_HIP(hipGetLastError());
+      if ((rc = read_cnt())) return rc;
+      hmax = h[kNvMax];
+      hmin = h[kNvMin];
+      trace(hmax, hmin, 0, h[kNvTargets]);
+      st->total_targets += h[kNvTargets];
+      st->edit_sweeps++;
+      st->dense_passes++;
+      sweeps++;
+    }
+    if (status) break;
+    if (sweeps >= p->max_sweeps) {
+      status = PMSZ_NOT_CONVERGED;
+      break;
+    }
+    // g is unchanged since cg was taken: the last extrema sweep found nothing,
+    // or the previous check classified this g
+    if ((rc = labels_from_codes(c, cg, ga, gd, d, settled, sbytes, changed))) return rc;
+    if ((rc = clear_cnt())) return rc;
+    k_naive_paths<<<blocks, 256, 0, c.stream>>>(cf, cg, fa, fd, ga, gd, d, mark, cnt);
+    k_naive_apply<T><<<blocks, 256, 0, c.stream>>>(f, g, dc, mark, n, ep, cnt);
+    PMSZ_HIP(hipGetLastError());
+    if ((rc = read_cnt())) return rc;
+    trace(0, 0, h[kNvDir], h[kNvTargets]);
+    st->total_targets += h[kNvTargets];
+    st->edit_sweeps++;
+    sweeps++;
+    // the check
+    if ((rc = launch_codes<T>(c, p, g, cg)) || (rc = clear_cnt())) return rc;
+    k_naive_extrema<<<blocks, 256, 0, c.stream>>>(cf, cg, d, nullptr, cnt);
+    PMSZ_HIP(hipGetLastError());
+    if ((rc = labels_from_codes(c, cg, ga, gd, d, settled, sbytes, changed))) return rc;
+    k_naive_paths<<<blocks, 256, 0, c.stream>>>(cf, cg, fa, fd, ga, gd, d, nullptr, cnt);
+    PMSZ_HIP(hipGetLastError());
+    if ((rc = read_cnt())) return rc;
+    hmax = h[kNvMax];
+    hmin = h[kNvMin];
+    hdir = h[kNvDir];
+    st->check_sweeps += 2;
+    st->dense_passes++;
+    st->final_cnt[0] = hmax;
+    st->final_cnt[1] = hmin;
+    st->final_cnt[2] = hdir;
+    if (!hmax && !hmin && !hdir) break;
+  }
+  if ((rc = clear_cnt())) return rc;
+  k_naive_dc_stats<<<blocks, 256, 0, c.stream>>>(dc, n, p->q, cnt);
+  PMSZ_HIP(hipGetLastError());
+  PMSZ_HIP(hipEventRecord(c.ev1, c.stream));
+  if ((rc = read_cnt())) return rc;
+  float ms = 0;
+  PMSZ_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  st->t_total_s = ms * 1e-3;
+  st->touched_voxels = h[kNvTouched];
+  st->clamped_voxels = h[kNvClamped];
+  st->status = status;
+  return status;
+}
+
 // One field (whole or split into subdomains), by value type; the caller holds
 // the context's lock.
 int correct_dispatch(Ctx &c, const pmsz_params *p, const void *d_f, void *d_g, uint8_t *d_dc,
                      pmsz_stats *stats) {
   const bool split = p->sub[0] * p->sub[1] * p->sub[2] != 1;
+  if (p->mode == PMSZ_NAIVE) {
+    if (split) return PMSZ_EINVAL;
+    if (p->dtype == PMSZ_F32)
+      return correct_naive<float>(c, p, (const float *)d_f, (float *)d_g, d_dc, stats);
+    return correct_naive<double>(c, p, (const double *)d_f, (double *)d_g, d_dc, stats);
+  }
   if (p->dtype == PMSZ_F32) {
     if (split)
       return correct_subdomains<float>(c, p, (const float *)d_f, (float *)d_g, d_dc, stats);
@@ -2540,7 +2734,7 @@ int64_t pmsz_workspace_bytes(int n, const pmsz_params *params) {
   if (n < 1 || !params) return PMSZ_EINVAL;
   uint64_t cells = 0, maxn = 0;
   for (int k = 0; k < n; k++) {
-    if (check_params(&params[k])) return PMSZ_EINVAL;
+    if (check_params(&params[k]) || params[k].mode == PMSZ_NAIVE) return PMSZ_EINVAL;
     const uint64_t nk = (uint64_t)(params[k].nx * params[k].ny * params[k].nz);
     cells += nk;
     maxn = std::max(maxn, nk);
@@ -2562,6 +2756,7 @@ int pmsz_ctx_correct_batch(pmsz_ctx *ctx, int n, const pmsz_params *params,
   if (!ctx || n < 1 || !params || !d_f || !d_g || !stats) return PMSZ_EINVAL;
   for (int k = 0; k < n; k++) {
     int rc = check_params(&params[k]);
+    if (!rc && params[k].mode == PMSZ_NAIVE) rc = PMSZ_EINVAL;
     if (rc) return rc;
     if (!d_f[k] || !d_g[k]) return PMSZ_EINVAL;
     if (params[k].sub[0] * params[k].sub[1] * params[k].sub[2] != 1) return PMSZ_EINVAL;
@@ -2602,6 +2797,7 @@ int pmsz_autotune(pmsz_ctx *ctx, int n, const pmsz_params *params, const void *c
   if (n < 1 || !params || !d_f || !d_g0 || reps < 1) return PMSZ_EINVAL;
   for (int k = 0; k < n; k++) {
     int rc = check_params(&params[k]);
+    if (!rc && params[k].mode == PMSZ_NAIVE) rc = PMSZ_EINVAL;
     if (rc) return rc;
     if (!d_f[k] || !d_g0[k]) return PMSZ_EINVAL;
     if (n > 1 && (params[k].sub[0] * params[k].sub[1] * params[k].sub[2] != 1 ||
@@ -2787,44 +2983,18 @@ int pmsz_labels(int dtype, int64_t nx, int64_t ny, int64_t nz, const void *d_fie
   Ctx &c = default_ctx();
   std::lock_guard<std::mutex> lk(c.mu);
   const Grid3 d = grid_of(nx, ny, nz);
-  const unsigned blocks = (unsigned)((n + 255) / 256);
-  const unsigned jblocks = (unsigned)((2 * n + 255) / 256);
-  const int64_t waves = (2 * n + 63) / 64;
-  const size_t sbytes = sizeof(uint32_t) * (size_t)((waves + 31) / 32 + 1);
+  const size_t sbytes = settled_bytes(n);
   uint32_t *settled = nullptr;
   if (hipMalloc(&settled, sbytes + sizeof(uint32_t)) != hipSuccess) {
     (void)hipFree(codes);
     return PMSZ_ENOMEM;
   }
   uint32_t *changed = settled + sbytes / sizeof(uint32_t);
-  hipError_t e = hipMemsetAsync(settled, 0, sbytes, c.stream);
-  if (e == hipSuccess) {
-    k_init_pointers<<<blocks, 256, 0, c.stream>>>(codes, d_asc, d_desc, d);
-    // rounds in batches of 4; the batch's last round tells whether anything moved
-    // (a chain is shorter than n < 2^31, so 32 rounds settle any field)
-    bool settled_all = false;
-    for (int round = 0; round < 36 && e == hipSuccess && !settled_all; round += 4) {
-      for (int k = 0; k < 3; k++)
-        k_jump<<<jblocks, 256, 0, c.stream>>>(d_asc, d_desc, n, settled, changed);
-      e = hipMemsetAsync(changed, 0, sizeof(uint32_t), c.stream);
-      k_jump<<<jblocks, 256, 0, c.stream>>>(d_asc, d_desc, n, settled, changed);
-      uint32_t h = 1;
-      if (e == hipSuccess) e = hipGetLastError();
-      if (e == hipSuccess) e = hipMemcpyAsync(&h, changed, sizeof(h), hipMemcpyDeviceToHost, c.stream);
-      if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
-      settled_all = e == hipSuccess && h == 0;
-    }
-    if (e == hipSuccess && !settled_all) e = hipErrorUnknown;  // a pointer field with a cycle
-  }
-  if (e == hipSuccess) e = hipGetLastError();
-  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  rc = labels_from_codes(c, codes, d_asc, d_desc, d, settled, sbytes, changed);
+  if (rc == PMSZ_OK && hipStreamSynchronize(c.stream) != hipSuccess) rc = PMSZ_EDEVICE;
   (void)hipFree(codes);
   (void)hipFree(settled);
-  if (e != hipSuccess) {
-    last_hip_error() = e;
-    return PMSZ_EDEVICE;
-  }
-  return PMSZ_OK;
+  return rc;
 }
 
 int pmsz_edit_symbols(const pmsz_params *p, const void *d_f, const void *d_g0, const void *d_g,

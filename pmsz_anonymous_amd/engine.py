@@ -19,7 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libpmsz.so")
 
 F32, F64 = 0, 1
-LAZY, SYNC = 0, 1
+LAZY, SYNC, NAIVE = 0, 1, 2
 CLAMP_REFERENCE, CLAMP_BOUND_SAFE = 0, 1
 GHOST_REFERENCE, GHOST_FULL = 0, 1
 

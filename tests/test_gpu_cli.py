@@ -61,3 +61,26 @@ def test_cli_sz3_subdomains(tmp_path):
     ocr = float(re.search(r"overall CR is: (\S+)", text).group(1))
     assert ocr == pytest.approx(f.size * 8 / (cmp_bytes + nd + ne), rel=1e-12)
     assert f"total_edited:{edited}" in text
+
+
+def test_cli_naive_mode(tmp_path):
+    from pmsz_anonymous_amd import fieldio
+    if fieldio.zstd_version() is None:
+        pytest.skip("libzstd absent")
+    nz, ny, nx = 20, 24, 28
+    f = fieldio.perlin((nz, ny, nx), 4, 5, 4, dtype=np.float64)
+    lo, hi = fieldio.value_range(f)
+    eb = 5e-3 * (hi - lo)
+    src = tmp_path / "field.raw"
+    f.tofile(src)
+    out = tmp_path / "corrected.raw"
+    cmd = [EXE, f"{src},{nx},{ny},{nz}", repr(eb), "sz3", "1", "1", "1", "--out", str(out),
+           "--mode", "naive"]
+    print("run", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    g0, _ = fieldio.szlike(f, eb)
+    go, _, sto, _, rc = O.correct(f, g0, eb, mode=O.NAIVE)
+    assert rc == 0
+    assert np.fromfile(out, np.float64).tobytes() == go.tobytes()
+    assert f"iteration is: {sto['ite']}" in r.stdout
