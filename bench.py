@@ -289,6 +289,31 @@ def edit_channel(tf, tg0, tg, eb, reps=10):
     return res
 
 
+def gpu_codec(f, tf, eb, reps=10):
+    """The GPU compressor front end (pmsz_codec_dq, dual-quantisation Lorenzo)
+    on the headline field, against the host SZ-like codec that builds g0 for the
+    other legs.  Algorithmic bytes: read f, write g0, write and read the
+    quanta (4 B each)."""
+    import torch
+    from pmsz_anonymous_amd import engine, fieldio
+    engine.codec_dq(tf, eb)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        _, hist, info = engine.codec_dq(tf, eb)
+    ms = (time.perf_counter() - t) / reps * 1e3
+    t = time.perf_counter()
+    _, hinfo = fieldio.szlike(f, eb)
+    host_ms = (time.perf_counter() - t) * 1e3
+    alg = tf.numel() * (2 * tf.element_size() + 8)
+    return {"ms_per_call": ms, "algorithmic_bytes": alg, "achieved": alg / ms / 1e6,
+            "unit": "GB/s", "frac": alg / ms / 1e6 / HBM_PEAK_GBPS,
+            "compressed_bytes": info["compressed_bytes"],
+            "ratio": tf.numel() * tf.element_size() / info["compressed_bytes"],
+            "host_szlike_ms": host_ms, "host_szlike_est_bytes": hinfo["est_bytes"],
+            "speedup_vs_host": host_ms / ms}
+
+
 def naive_baseline(tf, tg0, eb, lazy_ms):
     """The naive-MSz baseline (mode NAIVE, naive_MSz.cu:1887-2219) on the same
     field: extrema-only sweeps plus path re-tracing, one run, against the
@@ -364,6 +389,8 @@ def main():
                     help="skip the 768^3 f64 2x2x2 lazy-vs-sync run")
     ap.add_argument("--no-edits", action="store_true",
                     help="skip the edit side-channel encoding of the corrected field")
+    ap.add_argument("--no-codec", action="store_true",
+                    help="skip the GPU compressor front end against the host codec")
     ap.add_argument("--no-naive", action="store_true",
                     help="skip the naive-MSz baseline run on the same field")
     ap.add_argument("--no-tune", action="store_true",
@@ -482,6 +509,9 @@ def main():
     if not args.no_edits:
         res["edit_channel"] = edit_channel(tf, tg0, copies[-1], eb)
         log("edit channel done")
+    if not args.no_codec:
+        res["gpu_codec"] = gpu_codec(f, tf, eb)
+        log("gpu codec done")
     if not args.no_naive:
         res["naive"] = naive_baseline(tf, tg0, eb, ms)
         log("naive baseline done")

@@ -234,6 +234,27 @@ int pmsz_encode_edits(const pmsz_params *p, const void *d_f, const void *d_g0, c
                       uint8_t *d_packed, void *d_values, pmsz_edit_block *blocks,
                       pmsz_edit_info *info);
 
+/* The GPU compressor front end: an SZ-style Lorenzo codec with dual
+ * quantisation (cuSZ, J. Tian et al., PACT 2020), standing in for SZ3 in
+ * compressLocalData_coreOnly (pMSz.cu:1536-1624; SZ3 is absent).  Every voxel
+ * is quantised on its own, k = round(f * (1 / (2 abs_eb))) in double (0 when
+ * that product's magnitude is >= 2^30), and decompressed to g0 = k * 2 abs_eb in the value
+ * type, or to f itself (verbatim) when that value misses |f - g0| <= abs_eb
+ * exactly.  The Lorenzo residuals of k over the 7 lower neighbours, core-only
+ * per rank block of sub (rank order as for pmsz_encode_edits), are histogrammed
+ * per block.  hist (host, nullable): blocks x 1026 counts, residual d in slot
+ * d + 512 for |d| < 512, then the escapes and the verbatim voxels.  The
+ * estimated compressed size sums over blocks the entropy of the residual
+ * symbols (the escapes as one symbol), 32 bits per escape and the value of
+ * every verbatim voxel. */
+typedef struct {
+  int64_t compressed_bytes; /* estimated size of the compressed stream */
+  int64_t escapes;          /* residuals with |d| >= 512 */
+  int64_t verbatim;         /* voxels stored verbatim */
+} pmsz_codec_info;
+int pmsz_codec_dq(const pmsz_params *p, const void *d_f, void *d_g0, uint64_t *hist,
+                  pmsz_codec_info *info);
+
 /* Average device times of the original labeling pass (avg_seconds[0]) and of
  * detection 1, a full-volume detection sweep on g (avg_seconds[1]), over
  * `iters` launches each, timed with HIP events on the library stream.

@@ -14,6 +14,7 @@
  *   edit symbols ........... pMSz.cu:2090-2138
  *   edit side channel ...... pMSz.cu:223-250, 2328-2356, 2469-2483, 2673-2682
  *   naive-MSz loop ......... naive_MSz.cu:644-850, 1767-1864, 1887-2219
+ *   GPU codec stand-in ..... pMSz.cu:1536-1624 (SZ3 absent: cuSZ-style dual-quant Lorenzo)
  *   block split ............ pMSz.cu:2690-2711
  *
  * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
@@ -248,4 +249,11 @@ int orc_encode_edits(const orc_params *p, const void *f, const void *g0, const v
   }
   if (edited) *edited = ne;
   return bad ? ORC_EINVAL : ORC_OK;
+}
+
+int orc_codec_dq(const orc_params *p, const void *f, void *g0, uint64_t *hist) {
+  int rc = orc_check(p);
+  if (rc) return rc;
+  if (p->dtype == ORC_F32) return codec_dq_f32(p, (const float *)f, (float *)g0, hist);
+  return codec_dq_f64(p, (const double *)f, (double *)g0, hist);
 }

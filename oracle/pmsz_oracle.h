@@ -119,6 +119,13 @@ unsigned orc_bits_for_max(unsigned q);
 int orc_encode_edits(const orc_params *p, const void *f, const void *g0, const void *g,
                      uint8_t *packed, void *vals, orc_edit_block *blocks, int64_t *edited);
 
+/* The dual-quantisation Lorenzo codec (the GPU front end's stand-in for SZ3,
+ * pMSz.cu:1536-1624): g0 and, per rank block, ORC_DQ_BINS residual bins
+ * (residual d in bin d + ORC_DQ_RADIUS), the escapes and the verbatim voxels. */
+#define ORC_DQ_RADIUS 512
+#define ORC_DQ_BINS (2 * ORC_DQ_RADIUS)
+int orc_codec_dq(const orc_params *p, const void *f, void *g0, uint64_t *hist);
+
 /* Balanced block split of one dimension (pMSz.cu:2690-2706). */
 void orc_split(int64_t n, int parts, int coord, int64_t *local, int64_t *offset);
 

@@ -85,6 +85,7 @@ def lib():
         L.orc_correct.argtypes = [ctypes.POINTER(Params), vp, vp, vp, ctypes.POINTER(Stats),
                                   ctypes.POINTER(Trace)]
         L.orc_edit_symbols.argtypes = [ctypes.POINTER(Params), vp, vp, vp, vp, vp]
+        L.orc_codec_dq.argtypes = [ctypes.POINTER(Params), vp, vp, vp]
         L.orc_bits_for_max.argtypes = [ctypes.c_uint]
         L.orc_bits_for_max.restype = ctypes.c_uint
         L.orc_encode_edits.argtypes = [ctypes.POINTER(Params), vp, vp, vp, vp, vp,
@@ -219,6 +220,22 @@ def encode_edits(f, g0, g, eb, **kw):
     nv = sum(b["values"] for b in bl)
     end = bl[-1]["packed_offset"] + bl[-1]["packed_bytes"]
     return packed[:end], vals[:nv], bl, ed.value, rc
+
+
+DQ_RADIUS = 512
+DQ_BINS = 2 * DQ_RADIUS
+
+
+def codec_dq(f, eb, sub=(1, 1, 1)):
+    """(g0, hist) of the dual-quantisation Lorenzo codec: hist[b] holds block b's
+    DQ_BINS residual bins, then its escapes and verbatim voxels."""
+    p = make_params(f.shape, f.dtype, eb, sub=sub)
+    nb = sub[0] * sub[1] * sub[2]
+    g0 = np.empty_like(f)
+    hist = np.zeros((nb, DQ_BINS + 2), np.uint64)
+    rc = lib().orc_codec_dq(ctypes.byref(p), _ptr(f), _ptr(g0), _ptr(hist))
+    assert rc == 0, rc
+    return g0, hist
 
 
 def clamp(fv, eb, clamp_mode, dtype=np.float32):

@@ -1,6 +1,6 @@
 // pmsz_cli.cpp -- the command-line drop-in for the reference executable.
 //
-//   pmsz "<path>,<W>,<H>,<D>" <abs_eb> <sz3|zfp> <PX> <PY> <PZ> [options]
+//   pmsz "<path>,<W>,<H>,<D>" <abs_eb> <sz3|zfp|dq> <PX> <PY> <PZ> [options]
 //
 // The positional contract is the one main() parses (pMSz.cu:2625-2667): a raw
 // little-endian field (x fastest), the absolute error bound, the compressor id
@@ -34,7 +34,7 @@ namespace {
 
 int usage() {
   std::fprintf(stderr,
-               "usage: pmsz \"<path>,<W>,<H>,<D>\" <abs_eb> <sz3|zfp> <PX> <PY> <PZ> "
+               "usage: pmsz \"<path>,<W>,<H>,<D>\" <abs_eb> <sz3|zfp|dq> <PX> <PY> <PZ> "
                "[--dtype f32|f64] [--mode lazy|sync|naive] [--max-sweeps N] "
                "[--clamp reference|bound-safe] [--ghost reference|full] [--q N] "
                "[--out path] [--symbols path] [--edits-prefix prefix]\n");
@@ -106,8 +106,8 @@ int main(int argc, char **argv) {
       return usage();
     }
   }
-  if (comp != "sz3" && comp != "zfp") {
-    std::fprintf(stderr, "unknown compressor %s (sz3|zfp)\n", comp.c_str());
+  if (comp != "sz3" && comp != "zfp" && comp != "dq") {
+    std::fprintf(stderr, "unknown compressor %s (sz3|zfp|dq)\n", comp.c_str());
     return 1;
   }
   const int64_t n = W * H * D;
@@ -117,11 +117,32 @@ int main(int argc, char **argv) {
     std::fprintf(stderr, "cannot read %lld values from %s\n", (long long)n, path.c_str());
     return 1;
   }
+  void *df = nullptr, *dg0 = nullptr, *dg = nullptr;
+  uint8_t *dsym = nullptr;
+  if (hipMalloc(&df, f.size()) != hipSuccess || hipMalloc(&dg0, g.size()) != hipSuccess ||
+      hipMalloc(&dg, g.size()) != hipSuccess || hipMalloc((void **)&dsym, n) != hipSuccess) {
+    std::fprintf(stderr, "device allocation failed\n");
+    return 1;
+  }
+  if (hipMemcpy(df, f.data(), f.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    std::fprintf(stderr, "host to device copy failed\n");
+    return 1;
+  }
   auto t0 = std::chrono::steady_clock::now();
   int rc;
   int64_t cbytes = 0;
   double est = 0;
-  if (comp == "sz3") {
+  if (comp == "dq") {
+    // the GPU front end: dual-quantisation Lorenzo, core-only per rank block
+    pmsz_codec_info ci;
+    rc = pmsz_codec_dq(&p, df, dg0, nullptr, &ci);
+    if (rc != PMSZ_OK) {
+      std::fprintf(stderr, "pmsz_codec_dq: %s (%d)\n", pmsz_strerror(rc), rc);
+      return 1;
+    }
+    cbytes = ci.compressed_bytes;
+    if (hipMemcpy(g0.data(), dg0, g0.size(), hipMemcpyDeviceToHost) != hipSuccess) return 1;
+  } else if (comp == "sz3") {
     int64_t un = 0;
     rc = pmsz_io_szlike(p.dtype, W, H, D, f.data(), p.abs_eb, g0.data(), &un, &est);
     cbytes = (int64_t)est;
@@ -134,15 +155,7 @@ int main(int argc, char **argv) {
   }
   auto t1 = std::chrono::steady_clock::now();
   std::memcpy(g.data(), g0.data(), g.size());
-  void *df = nullptr, *dg0 = nullptr, *dg = nullptr;
-  uint8_t *dsym = nullptr;
-  if (hipMalloc(&df, f.size()) != hipSuccess || hipMalloc(&dg0, g.size()) != hipSuccess ||
-      hipMalloc(&dg, g.size()) != hipSuccess || hipMalloc((void **)&dsym, n) != hipSuccess) {
-    std::fprintf(stderr, "device allocation failed\n");
-    return 1;
-  }
-  if (hipMemcpy(df, f.data(), f.size(), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(dg0, g0.data(), g0.size(), hipMemcpyHostToDevice) != hipSuccess ||
+  if (hipMemcpy(dg0, g0.data(), g0.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dg, g0.data(), g0.size(), hipMemcpyHostToDevice) != hipSuccess) {
     std::fprintf(stderr, "host to device copy failed\n");
     return 1;

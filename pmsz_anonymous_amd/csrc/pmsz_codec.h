@@ -1,0 +1,118 @@
+// pmsz_codec.h -- the GPU compressor front end: an SZ-style Lorenzo codec with
+// dual quantisation (cuSZ: J. Tian et al., PACT 2020), standing in for SZ3
+// (compressLocalData_coreOnly, pMSz.cu:1536-1624), which is absent here.
+//
+// Dual quantisation removes the serial dependence of SZ's Lorenzo predictor on
+// reconstructed values: every voxel is first quantised on its own,
+//   k = round(f * (1 / (2 eb)))  (in double, the reciprocal rounded once;
+//                                 0 when the product's magnitude is >= 2^30),
+//   g0 = (T)(k * 2 eb)      (f itself, "verbatim", when that misses the bound),
+// and the Lorenzo residual d = k - pred(k) over the 7 lower neighbours is then
+// lossless integer work.  Only the residuals' histogram is needed here: it gives
+// the entropy-coded size of the compressed stream.  Prediction is core-only per
+// rank block (neighbours outside the block count as 0), as the reference
+// compresses each rank's core separately.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pmsz_common.h"
+
+namespace pmsz {
+
+constexpr int kDqRadius = 512;
+constexpr int kDqBins = 2 * kDqRadius;            // residual d in bin d + kDqRadius
+constexpr int kDqSlots = kDqBins + 2;             // bins, escapes, verbatim voxels
+constexpr int kDqRows = 64;                       // rows of a rank block per workgroup
+// residuals within +-kDqHot are counted by wave ballots (0: the zero residual
+// only; ballots for +-1..3 as well cost more VALU than their atomics, 725 vs
+// 854 us at 512^3)
+constexpr int kDqHot = 0;
+constexpr double kDqLimit = 1073741824.0;         // 2^30
+
+// One rank block of the codec, in rank order.
+struct DqBlock {
+  int32_t x0, y0, z0;     // first voxel
+  int32_t lx, ly, lz;     // extent
+  uint32_t chunk0;        // first workgroup of the block
+  uint32_t pad;
+};
+
+// Pass 1: the quantum of every voxel, stored as 2k + verbatim, and g0.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dq_quant(const T *__restrict__ f, T *__restrict__ g0,
+                                                  int32_t *__restrict__ kq, int64_t n, double eb,
+                                                  double inv_q2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T fv = f[i];
+  const double q2 = 2.0 * eb;
+  const double r = (double)fv * inv_q2;
+  const int32_t k = fabs(r) < kDqLimit ? (int32_t)round(r) : 0;
+  const T v = (T)((double)k * q2);
+  const bool keep = within_exact((double)fv, (double)v, eb);
+  g0[i] = keep ? v : fv;
+  kq[i] = 2 * k + (keep ? 0 : 1);
+}
+
+// Pass 2: Lorenzo residuals of one rank block's rows into an LDS histogram, then
+// into the block's global histogram (one atomic per nonzero slot).
+__global__ __launch_bounds__(256) void k_dq_hist(const int32_t *__restrict__ kq,
+                                                 const DqBlock *__restrict__ blocks,
+                                                 uint32_t nblocks, int64_t nx, int64_t sxy,
+                                                 unsigned long long *__restrict__ hist) {
+  __shared__ uint32_t s_h[kDqSlots];
+  for (int b = threadIdx.x; b < kDqSlots; b += blockDim.x) s_h[b] = 0;
+  uint32_t lo = 0, hi = nblocks - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (blocks[mid].chunk0 <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const DqBlock B = blocks[lo];
+  __syncthreads();
+  const int64_t rows = (int64_t)B.ly * B.lz;
+  const int64_t r0 = (int64_t)(blockIdx.x - B.chunk0) * kDqRows;
+  const int64_t r1 = r0 + kDqRows < rows ? r0 + kDqRows : rows;
+  const uint32_t lane = threadIdx.x & 63u;
+  for (int64_t row = r0; row < r1; row++) {
+    const int32_t y = (int32_t)(row % B.ly), z = (int32_t)(row / B.ly);
+    const int64_t base = (int64_t)(B.z0 + z) * sxy + (int64_t)(B.y0 + y) * nx + B.x0;
+    const bool hy = y > 0, hz = z > 0;
+    // whole waves step through the row together, so the ballot below is uniform
+    for (int32_t x0 = 0; x0 < B.lx; x0 += (int32_t)blockDim.x) {
+      const int32_t x = x0 + (int32_t)threadIdx.x;
+      const bool in = x < B.lx;
+      int64_t d = 0;
+      bool verb = false;
+      if (in) {
+        const int64_t i = base + x;
+        const bool hx = x > 0;
+        auto K = [&](bool ok, int64_t j) -> int64_t { return ok ? (int64_t)(kq[j] >> 1) : 0; };
+        const int32_t c = kq[i];
+        verb = c & 1;
+        const int64_t pred = K(hx, i - 1) + K(hy, i - nx) + K(hz, i - sxy) -
+                             K(hx && hy, i - 1 - nx) - K(hx && hz, i - 1 - sxy) -
+                             K(hy && hz, i - nx - sxy) + K(hx && hy && hz, i - 1 - nx - sxy);
+        d = (int64_t)(c >> 1) - pred;
+      }
+      // the zero residual is most of a smooth field: one LDS atomic per wave
+#pragma unroll
+      for (int v = -kDqHot; v <= kDqHot; v++) {
+        const uint64_t m = __ballot(in && d == v);
+        if (lane == 0 && m) atomicAdd(&s_h[kDqRadius + v], (uint32_t)__popcll(m));
+      }
+      if (in && (d < -kDqHot || d > kDqHot)) {
+        const int slot = (d > -kDqRadius && d < kDqRadius) ? (int)d + kDqRadius : kDqBins;
+        atomicAdd(&s_h[slot], 1u);
+      }
+      if (in && verb) atomicAdd(&s_h[kDqBins + 1], 1u);
+    }
+  }
+  __syncthreads();
+  unsigned long long *h = hist + (size_t)lo * kDqSlots;
+  for (int b = threadIdx.x; b < kDqSlots; b += blockDim.x)
+    if (s_h[b]) atomicAdd(&h[b], (unsigned long long)s_h[b]);
+}
+
+}  // namespace pmsz

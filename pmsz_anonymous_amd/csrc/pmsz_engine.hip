@@ -61,6 +61,7 @@
 #include "pmsz_device.h"
 #include "pmsz_edits.h"
 #include "pmsz_naive.h"
+#include "pmsz_codec.h"
 
 namespace pmsz {
 
@@ -2506,6 +2507,75 @@ int correct_naive(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *d_dc,
   return status;
 }
 
+// Entropy-coded size of one rank block's residual stream: every residual (the
+// escapes as one more symbol) at -log2 of its frequency, plus 32 bits per escaped
+// residual and the value of every verbatim voxel.
+int64_t dq_block_bytes(const unsigned long long *h, size_t value_bytes) {
+  double total = 0, bits = 0;
+  for (int s = 0; s <= kDqBins; s++) total += (double)h[s];
+  for (int s = 0; s <= kDqBins; s++)
+    if (h[s]) bits += (double)h[s] * std::log2(total / (double)h[s]);
+  bits += 32.0 * (double)h[kDqBins] + 8.0 * (double)value_bytes * (double)h[kDqBins + 1];
+  return (int64_t)std::ceil(bits / 8.0);
+}
+
+template <typename T>
+int codec_dq(Ctx &c, const pmsz_params *p, const T *f, T *g0, uint64_t *hist_out,
+             pmsz_codec_info *info) {
+  const int64_t n = p->nx * p->ny * p->nz;
+  const int px = p->sub[0], py = p->sub[1], pz = p->sub[2];
+  const uint32_t nb = (uint32_t)(px * py * pz);
+  std::vector<DqBlock> bl;
+  uint32_t chunks = 0;
+  for (uint32_t r = 0; r < nb; r++) {
+    const int cx = (int)r / (py * pz), cy = ((int)r / pz) % py, cz = (int)r % pz;
+    int64_t l[3], o[3];
+    block_split(p->nx, px, cx, l[0], o[0]);
+    block_split(p->ny, py, cy, l[1], o[1]);
+    block_split(p->nz, pz, cz, l[2], o[2]);
+    DqBlock B;
+    B.x0 = (int32_t)o[0];
+    B.y0 = (int32_t)o[1];
+    B.z0 = (int32_t)o[2];
+    B.lx = (int32_t)l[0];
+    B.ly = (int32_t)l[1];
+    B.lz = (int32_t)l[2];
+    B.chunk0 = chunks;
+    B.pad = 0;
+    chunks += (uint32_t)((l[1] * l[2] + kDqRows - 1) / kDqRows);
+    bl.push_back(B);
+  }
+  const size_t o_hist = ((nb * sizeof(DqBlock)) + 255) & ~(size_t)255;
+  const size_t hbytes = (size_t)nb * kDqSlots * sizeof(unsigned long long);
+  int rc;
+  if ((rc = c.tmark.ensure((size_t)n)) || (rc = c.tbyte.ensure(o_hist + hbytes))) return rc;
+  DqBlock *d_bl = (DqBlock *)c.tbyte.p;
+  unsigned long long *d_hist = (unsigned long long *)(c.tbyte.p + o_hist);
+  int32_t *kq = (int32_t *)c.tmark.p;
+  PMSZ_HIP(hipMemcpyAsync(d_bl, bl.data(), nb * sizeof(DqBlock), hipMemcpyHostToDevice,
+                          c.stream));
+  PMSZ_HIP(hipMemsetAsync(d_hist, 0, hbytes, c.stream));
+  k_dq_quant<T><<<(unsigned)((n + 255) / 256), 256, 0, c.stream>>>(f, g0, kq, n, p->abs_eb,
+                                                                    1.0 / (2.0 * p->abs_eb));
+  PMSZ_HIP(hipGetLastError());
+  k_dq_hist<<<chunks, 256, 0, c.stream>>>(kq, d_bl, nb, p->nx, p->nx * p->ny, d_hist);
+  PMSZ_HIP(hipGetLastError());
+  std::vector<unsigned long long> h((size_t)nb * kDqSlots);
+  PMSZ_HIP(hipMemcpyAsync(h.data(), d_hist, hbytes, hipMemcpyDeviceToHost, c.stream));
+  PMSZ_HIP(hipStreamSynchronize(c.stream));
+  if (hist_out) std::memcpy(hist_out, h.data(), hbytes);
+  if (info) {
+    std::memset(info, 0, sizeof(*info));
+    for (uint32_t b = 0; b < nb; b++) {
+      const unsigned long long *hb = h.data() + (size_t)b * kDqSlots;
+      info->compressed_bytes += dq_block_bytes(hb, sizeof(T));
+      info->escapes += (int64_t)hb[kDqBins];
+      info->verbatim += (int64_t)hb[kDqBins + 1];
+    }
+  }
+  return PMSZ_OK;
+}
+
 // One field (whole or split into subdomains), by value type; the caller holds
 // the context's lock.
 int correct_dispatch(Ctx &c, const pmsz_params *p, const void *d_f, void *d_g, uint8_t *d_dc,
@@ -3024,6 +3094,20 @@ int pmsz_edit_symbols(const pmsz_params *p, const void *d_f, const void *d_g0, c
   PMSZ_HIP(hipStreamSynchronize(c.stream));
   if (edited) *edited = (int64_t)h;
   return PMSZ_OK;
+}
+
+int pmsz_codec_dq(const pmsz_params *p, const void *d_f, void *d_g0, uint64_t *hist,
+                  pmsz_codec_info *info) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (!d_f || !d_g0 || (uint32_t)(p->sub[0] * p->sub[1] * p->sub[2]) > kMaxDomains)
+    return PMSZ_EINVAL;
+  Ctx &c = default_ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  if ((rc = ctx_device(c))) return rc;
+  if (p->dtype == PMSZ_F32)
+    return codec_dq<float>(c, p, (const float *)d_f, (float *)d_g0, hist, info);
+  return codec_dq<double>(c, p, (const double *)d_f, (double *)d_g0, hist, info);
 }
 
 int pmsz_edit_packed_bytes(const pmsz_params *p, int64_t *bytes) {

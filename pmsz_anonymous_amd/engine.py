@@ -30,7 +30,7 @@ EXPORTED = [
     "pmsz_ctx_create", "pmsz_ctx_destroy", "pmsz_ctx_correct", "pmsz_ctx_correct_batch",
     "pmsz_ctx_bytes", "pmsz_ctx_trim", "pmsz_workspace_bytes",
     "pmsz_codes", "pmsz_labels", "pmsz_edit_symbols", "pmsz_edit_packed_bytes",
-    "pmsz_encode_edits", "pmsz_time_first_pass", "pmsz_strerror",
+    "pmsz_encode_edits", "pmsz_codec_dq", "pmsz_time_first_pass", "pmsz_strerror",
     "pmsz_version", "pmsz_default_tuning", "pmsz_ctx_set_tuning", "pmsz_ctx_get_tuning",
     "pmsz_autotune",
 ]
@@ -189,6 +189,7 @@ def lib():
         L.pmsz_labels.argtypes = [i32, i64, i64, i64, vp, vp, vp]
         L.pmsz_edit_symbols.argtypes = [P, vp, vp, vp, vp, ctypes.POINTER(i64)]
         L.pmsz_edit_packed_bytes.argtypes = [P, ctypes.POINTER(i64)]
+        L.pmsz_codec_dq.argtypes = [P, vp, vp, vp, ctypes.POINTER(i64 * 3)]
         L.pmsz_encode_edits.argtypes = [P, vp, vp, vp, vp, vp, ctypes.POINTER(EditBlock),
                                         ctypes.POINTER(EditInfo)]
         L.pmsz_time_first_pass.argtypes = [P, vp, vp, i32, ctypes.POINTER(ctypes.c_double)]
@@ -474,6 +475,28 @@ def encode_edits(f, g0, g, abs_eb, **kw):
     bl = [{k: getattr(b, k) for k, _ in EditBlock._fields_} for b in blocks]
     inf = {k: getattr(info, k) for k, _ in EditInfo._fields_}
     return packed[:cap.value], vals[:info.values], bl, inf
+
+
+DQ_SLOTS = 1026
+
+
+def codec_dq(f, abs_eb, sub=(1, 1, 1)):
+    """GPU dual-quantisation Lorenzo codec (pmsz_codec_dq) on a CUDA tensor f.
+    Returns (g0 tensor, per-block histograms as a (blocks, 1026) uint64 array,
+    info dict with compressed_bytes, escapes, verbatim)."""
+    import numpy as np
+    import torch
+    p = make_params(f.shape, f.dtype, abs_eb, sub=sub)
+    nb = p.sub[0] * p.sub[1] * p.sub[2]
+    g0 = torch.empty_like(f)
+    hist = np.zeros((nb, DQ_SLOTS), np.uint64)
+    info = (ctypes.c_int64 * 3)()
+    torch.cuda.current_stream().synchronize()
+    rc = lib().pmsz_codec_dq(ctypes.byref(p), ctypes.c_void_p(f.contiguous().data_ptr()),
+                             ctypes.c_void_p(g0.data_ptr()),
+                             hist.ctypes.data_as(ctypes.c_void_p), ctypes.byref(info))
+    _check(rc, "pmsz_codec_dq")
+    return g0, hist, {"compressed_bytes": info[0], "escapes": info[1], "verbatim": info[2]}
 
 
 def time_first_pass(f, g, abs_eb, iters=5, **kw):

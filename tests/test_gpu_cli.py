@@ -84,3 +84,26 @@ def test_cli_naive_mode(tmp_path):
     assert rc == 0
     assert np.fromfile(out, np.float64).tobytes() == go.tobytes()
     assert f"iteration is: {sto['ite']}" in r.stdout
+
+
+def test_cli_gpu_codec(tmp_path):
+    from pmsz_anonymous_amd import fieldio
+    if fieldio.zstd_version() is None:
+        pytest.skip("libzstd absent")
+    nz, ny, nx = 22, 26, 30
+    f = fieldio.perlin((nz, ny, nx), 6, 5, 4, dtype=np.float64)
+    lo, hi = fieldio.value_range(f)
+    eb = 2e-3 * (hi - lo)
+    src = tmp_path / "field.raw"
+    f.tofile(src)
+    out = tmp_path / "corrected.raw"
+    cmd = [EXE, f"{src},{nx},{ny},{nz}", repr(eb), "dq", "2", "1", "1", "--out", str(out)]
+    print("run", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    g0, hist = O.codec_dq(f, eb, (2, 1, 1))
+    go, _, sto, _, rc = O.correct(f, g0, eb, sub=(2, 1, 1))
+    assert rc == 0
+    assert np.fromfile(out, np.float64).tobytes() == go.tobytes()
+    m = re.search(r"compression time: \S+ s \(dq, (\d+) bytes\)", r.stdout)
+    assert m and int(m.group(1)) > 0
