@@ -166,7 +166,9 @@ int main(int argc, char **argv) {
   rc = pmsz_correct(&p, df, dg, nullptr, &st);
   auto t3 = std::chrono::steady_clock::now();
   if (rc != PMSZ_OK && rc != PMSZ_NOT_CONVERGED) {
-    std::fprintf(stderr, "pmsz_correct: %s (%d)\n", pmsz_strerror(rc), rc);
+    std::fprintf(stderr, "pmsz_correct: %s (%d)", pmsz_strerror(rc), rc);
+    if (rc == PMSZ_EDEVICE) std::fprintf(stderr, ": %s", hipGetErrorString(hipGetLastError()));
+    std::fprintf(stderr, "\n");
     return 1;
   }
   std::printf("iteration is: %lld\n", (long long)st.ite);

@@ -1828,8 +1828,18 @@ int drive(Ctx &c, Job &job, Outcome &out) {
       uint32_t *a_bar = c.bar;
       void *args[] = {&a_d0,    &a_doms,  &a_shift, &a_ndom, &a_ds,  &a_tmark,
                       &a_tlist, &a_wmark, &a_wlist, &a_ring, &a_run, &a_s, &a_bar};
-      PMSZ_HIP(hipLaunchCooperativeKernel((const void *)k_tail<T, kMulti>, dim3(g_tail),
-                                          dim3(kTailThreads), args, 0, c.stream));
+      hipError_t e = hipLaunchCooperativeKernel((const void *)k_tail<T, kMulti>, dim3(g_tail),
+                                                dim3(kTailThreads), args, 0, c.stream);
+      if (e != hipSuccess) {
+        // the runtime can refuse a cooperative launch while another process holds
+        // the device's cooperative resources.  The tail's grid is sized to fit on
+        // the device beside every other context's, so as a regular launch its
+        // blocks are all resident as well
+        (void)hipGetLastError();
+        e = hipLaunchKernel((const void *)k_tail<T, kMulti>, dim3(g_tail), dim3(kTailThreads),
+                            args, 0, c.stream);
+      }
+      PMSZ_HIP(e);
     }
     PMSZ_HIP(hipGetLastError());
     PMSZ_HIP(hipMemcpyAsync(c.h_run, c.run, sizeof(RunState), hipMemcpyDeviceToHost, c.stream));
