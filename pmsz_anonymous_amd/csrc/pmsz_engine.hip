@@ -2410,7 +2410,9 @@ int correct_naive(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *d_dc,
   int32_t *fa = (int32_t *)c.tmark.p, *fd = (int32_t *)c.wmark.p;
   int32_t *ga = (int32_t *)c.tlist.p, *gd = (int32_t *)c.wlist.p;
   const EditParams ep = edit_params(p);
-  const unsigned blocks = (unsigned)((n + 255) / 256);
+  // grid-stride passes: a few blocks per CU, one counter atomic per block
+  const unsigned blocks = (unsigned)std::max<int64_t>(
+      1, std::min<int64_t>((n + 255) / 256, (int64_t)c.cus * 8));
   unsigned h[8];
   auto read_cnt = [&]() -> int {
     PMSZ_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, c.stream));
