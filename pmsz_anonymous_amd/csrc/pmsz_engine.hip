@@ -1308,19 +1308,24 @@ __global__ void k_jump(int32_t *__restrict__ lab0, int32_t *__restrict__ lab1, i
   }
 }
 
-// extract_edits_kernel (pMSz.cu:2090-2138).
+// extract_edits_kernel (pMSz.cu:2090-2138).  A grid of a few blocks per CU
+// strides over the voxels and adds its count once per block (same-address
+// device atomics serialise).
 template <typename T>
-__global__ void k_edit_symbols(const T *__restrict__ f, const T *__restrict__ g0,
-                               const T *__restrict__ g, uint8_t *__restrict__ sym,
-                               unsigned long long *__restrict__ count, int64_t n,
-                               EditParams ep) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool edited = false;
-  if (i < n) {
+__global__ __launch_bounds__(256) void k_edit_symbols(const T *__restrict__ f,
+                                                      const T *__restrict__ g0,
+                                                      const T *__restrict__ g,
+                                                      uint8_t *__restrict__ sym,
+                                                      unsigned long long *__restrict__ count,
+                                                      int64_t n, EditParams ep) {
+  __shared__ unsigned long long s_c[4];
+  unsigned long long edited = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
     T a = g[i], b = g0[i];
     uint8_t s = 0;
     if (a != b && a < b) {
-      edited = true;
+      edited++;
       if (a != clamp_value<T>(f[i], ep)) {
         T delta = (T)ep.delta_d;
         s = (uint8_t)(int)std::round((b - a) / delta);
@@ -1330,8 +1335,13 @@ __global__ void k_edit_symbols(const T *__restrict__ f, const T *__restrict__ g0
     }
     sym[i] = s;
   }
-  uint64_t m = __ballot(edited);
-  if (m && __lane_id() == 0) atomicAdd(count, (unsigned long long)__popcll(m));
+  edited = wave_sum_u64(edited);
+  if (__lane_id() == 0) s_c[threadIdx.x >> 6] = edited;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+    if (t) atomicAdd(count, t);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -3091,7 +3101,8 @@ int pmsz_edit_symbols(const pmsz_params *p, const void *d_f, const void *d_g0, c
   const EditParams ep = edit_params(p);
   unsigned long long *dcount = &c.run->acc[0][kAccTargets];
   PMSZ_HIP(hipMemsetAsync(dcount, 0, sizeof(unsigned long long), c.stream));
-  unsigned blocks = (unsigned)((n + 255) / 256);
+  const unsigned blocks =
+      (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)c.cus * 8));
   if (p->dtype == PMSZ_F32)
     k_edit_symbols<float><<<blocks, 256, 0, c.stream>>>((const float *)d_f, (const float *)d_g0,
                                                         (const float *)d_g, d_symbols, dcount,
