@@ -52,7 +52,7 @@ def build_engine():
 def build_io():
     out = os.path.join(HERE, "libpmsz_io.so")
     _run(["g++", *HOST_FLAGS, "-shared", "-o", out + ".tmp",
-          os.path.join(CSRC, "fieldio.cpp"), "-ldl"])
+          os.path.join(CSRC, "fieldio.cpp"), "-ldl", "-pthread"])
     os.replace(out + ".tmp", out)
     return out
 

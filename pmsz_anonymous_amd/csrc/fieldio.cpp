@@ -29,7 +29,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -246,14 +248,15 @@ ZstdApi &zstd_api() {
 }
 
 // One zstd frame of n bytes appended to fp; nothing for n = 0.
-int zstd_append(FILE *fp, const void *src, size_t n, int level, int64_t &written) {
+// One zstd frame of n bytes into out (empty for n = 0).
+int zstd_frame(const void *src, size_t n, int level, std::vector<uint8_t> &out) {
+  out.clear();
   if (n == 0) return PMSZ_IO_OK;
   ZstdApi &z = zstd_api();
-  std::vector<uint8_t> out(z.bound(n));
+  out.resize(z.bound(n));
   size_t c = z.compress(out.data(), out.size(), src, n, level);
   if (z.is_error(c)) return PMSZ_IO_EFAIL;
-  if (std::fwrite(out.data(), 1, c, fp) != c) return PMSZ_IO_EFAIL;
-  written += (int64_t)c;
+  out.resize(c);
   return PMSZ_IO_OK;
 }
 
@@ -409,14 +412,36 @@ int pmsz_io_write_edits(const char *prefix, int dtype, int nblocks, const pmsz_e
     std::fclose(fd);
     return PMSZ_IO_EFAIL;
   }
+  // the blocks' frames are independent: compress them on up to 16 threads (the
+  // system libzstd has no threads of its own), then write them in rank order
+  std::vector<std::vector<uint8_t>> fdel(nblocks), fval(nblocks);
+  std::vector<int> brc(nblocks, PMSZ_IO_OK);
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    for (int b; (b = next.fetch_add(1)) < nblocks;) {
+      brc[b] = zstd_frame(packed + blocks[b].packed_offset, (size_t)blocks[b].packed_bytes,
+                          level_deltas, fdel[b]);
+      if (brc[b] == PMSZ_IO_OK)
+        brc[b] = zstd_frame((const uint8_t *)values + blocks[b].value_offset * es,
+                            (size_t)blocks[b].values * es, level_edits, fval[b]);
+    }
+  };
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int nt = std::min(nblocks, (int)std::min(16u, hw ? hw : 1u));
+  std::vector<std::thread> pool;
+  for (int k = 1; k < nt; k++) pool.emplace_back(work);
+  work();
+  for (std::thread &th : pool) th.join();
   int64_t nd = 0, ne = 0;
   int rc = PMSZ_IO_OK;
   for (int b = 0; b < nblocks && rc == PMSZ_IO_OK; b++) {
-    rc = zstd_append(fd, packed + blocks[b].packed_offset, (size_t)blocks[b].packed_bytes,
-                     level_deltas, nd);
-    if (rc == PMSZ_IO_OK)
-      rc = zstd_append(fe, (const uint8_t *)values + blocks[b].value_offset * es,
-                       (size_t)blocks[b].values * es, level_edits, ne);
+    rc = brc[b];
+    if (rc == PMSZ_IO_OK && std::fwrite(fdel[b].data(), 1, fdel[b].size(), fd) != fdel[b].size())
+      rc = PMSZ_IO_EFAIL;
+    if (rc == PMSZ_IO_OK && std::fwrite(fval[b].data(), 1, fval[b].size(), fe) != fval[b].size())
+      rc = PMSZ_IO_EFAIL;
+    nd += (int64_t)fdel[b].size();
+    ne += (int64_t)fval[b].size();
   }
   if (std::fclose(fd) != 0) rc = PMSZ_IO_EFAIL;
   if (std::fclose(fe) != 0) rc = PMSZ_IO_EFAIL;
