@@ -24,10 +24,6 @@ constexpr int kDqRadius = 512;
 constexpr int kDqBins = 2 * kDqRadius;            // residual d in bin d + kDqRadius
 constexpr int kDqSlots = kDqBins + 2;             // bins, escapes, verbatim voxels
 constexpr int kDqRows = 64;                       // rows of a rank block per workgroup
-// residuals within +-kDqHot are counted by wave ballots (0: the zero residual
-// only; ballots for +-1..3 as well cost more VALU than their atomics, 725 vs
-// 854 us at 512^3)
-constexpr int kDqHot = 0;
 constexpr double kDqLimit = 1073741824.0;         // 2^30
 
 // One rank block of the codec, in rank order.
@@ -75,8 +71,9 @@ __global__ __launch_bounds__(256) void k_dq_hist(const int32_t *__restrict__ kq,
   const int64_t r0 = (int64_t)(blockIdx.x - B.chunk0) * kDqRows;
   const int64_t r1 = r0 + kDqRows < rows ? r0 + kDqRows : rows;
   const uint32_t lane = threadIdx.x & 63u;
+  // (y, z) of the first row, then stepped: no division per row
+  int32_t y = (int32_t)(r0 % B.ly), z = (int32_t)(r0 / B.ly);
   for (int64_t row = r0; row < r1; row++) {
-    const int32_t y = (int32_t)(row % B.ly), z = (int32_t)(row / B.ly);
     const int64_t base = (int64_t)(B.z0 + z) * sxy + (int64_t)(B.y0 + y) * nx + B.x0;
     const bool hy = y > 0, hz = z > 0;
     // whole waves step through the row together, so the ballot below is uniform
@@ -97,16 +94,17 @@ __global__ __launch_bounds__(256) void k_dq_hist(const int32_t *__restrict__ kq,
         d = (int64_t)(c >> 1) - pred;
       }
       // the zero residual is most of a smooth field: one LDS atomic per wave
-#pragma unroll
-      for (int v = -kDqHot; v <= kDqHot; v++) {
-        const uint64_t m = __ballot(in && d == v);
-        if (lane == 0 && m) atomicAdd(&s_h[kDqRadius + v], (uint32_t)__popcll(m));
-      }
-      if (in && (d < -kDqHot || d > kDqHot)) {
+      const uint64_t zm = __ballot(in && d == 0);
+      if (lane == 0 && zm) atomicAdd(&s_h[kDqRadius], (uint32_t)__popcll(zm));
+      if (in && d != 0) {
         const int slot = (d > -kDqRadius && d < kDqRadius) ? (int)d + kDqRadius : kDqBins;
         atomicAdd(&s_h[slot], 1u);
       }
       if (in && verb) atomicAdd(&s_h[kDqBins + 1], 1u);
+    }
+    if (++y == B.ly) {
+      y = 0;
+      z++;
     }
   }
   __syncthreads();
