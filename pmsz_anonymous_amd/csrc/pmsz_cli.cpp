@@ -1,6 +1,7 @@
 // pmsz_cli.cpp -- the command-line drop-in for the reference executable.
 //
 //   pmsz "<path>,<W>,<H>,<D>" <abs_eb> <sz3|zfp|dq> <PX> <PY> <PZ> [options]
+//   pmsz <path> <W> <H> <D> <abs_eb> <sz3|zfp|dq> <PX> <PY> <PZ> [options]
 //
 // The positional contract is the one main() parses (pMSz.cu:2625-2667): a raw
 // little-endian field (x fastest), the absolute error bound, the compressor id
@@ -53,11 +54,32 @@ bool parse_dims(const std::string &s, std::string &path, int64_t &w, int64_t &h,
 
 }  // namespace
 
+// The README's form, "<path> <W> <H> <D> <abs_eb> <compressor> <PX> <PY> <PZ>"
+// (README.md:68-74): the first argument names a file without the ",W,H,D"
+// suffix and the next three are whole numbers.
+bool readme_form(int argc, char **argv) {
+  if (argc < 10 || std::strchr(argv[1], ',')) return false;
+  for (int k = 2; k <= 4; k++) {
+    char *end = nullptr;
+    const long long v = std::strtoll(argv[k], &end, 10);
+    if (!end || *end || v <= 0) return false;
+  }
+  return true;
+}
+
 int main(int argc, char **argv) {
   if (argc < 7) return usage();
   std::string path;
   int64_t W = 0, H = 0, D = 0;
-  if (!parse_dims(argv[1], path, W, H, D)) {
+  // the code's form (pMSz.cu:2625-2667) or the README's, told apart as above
+  const bool rf = readme_form(argc, argv);
+  const int a = rf ? 3 : 0;  // shift of the arguments after the extent
+  if (rf) {
+    path = argv[1];
+    W = std::atoll(argv[2]);
+    H = std::atoll(argv[3]);
+    D = std::atoll(argv[4]);
+  } else if (!parse_dims(argv[1], path, W, H, D)) {
     std::fprintf(stderr, "Parsing error for dimensions\n");
     return 1;
   }
@@ -67,13 +89,13 @@ int main(int argc, char **argv) {
   p.nx = W;
   p.ny = H;
   p.nz = D;
-  p.abs_eb = std::strtod(argv[2], nullptr);
-  std::string comp = argv[3];
-  p.sub[0] = std::atoi(argv[4]);
-  p.sub[1] = std::atoi(argv[5]);
-  p.sub[2] = std::atoi(argv[6]);
+  p.abs_eb = std::strtod(argv[2 + a], nullptr);
+  std::string comp = argv[3 + a];
+  p.sub[0] = std::atoi(argv[4 + a]);
+  p.sub[1] = std::atoi(argv[5 + a]);
+  p.sub[2] = std::atoi(argv[6 + a]);
   std::string out_path, sym_path, edits_prefix;
-  for (int i = 7; i < argc; i++) {
+  for (int i = 7 + a; i < argc; i++) {
     std::string o = argv[i];
     auto val = [&](void) -> std::string {
       if (i + 1 >= argc) {

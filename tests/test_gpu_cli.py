@@ -135,3 +135,27 @@ def test_cli_lazy_after_parent_cooperative_tail(tmp_path):
     go, _, _, _, rc = O.correct(f, g0, eb, sub=(2, 2, 1))
     assert rc == 0
     assert np.fromfile(out, np.float64).tobytes() == go.tobytes()
+
+
+def test_cli_readme_argument_form(tmp_path):
+    # README.md:68-74: <path> <W> <H> <D> <abs_eb> <compressor> <PX> <PY> <PZ>
+    from pmsz_anonymous_amd import fieldio
+    if fieldio.zstd_version() is None:
+        pytest.skip("libzstd absent")
+    nz, ny, nx = 18, 20, 22
+    f = fieldio.perlin((nz, ny, nx), 9, 5, 4, dtype=np.float64)
+    lo, hi = fieldio.value_range(f)
+    eb = 4e-3 * (hi - lo)
+    src = tmp_path / "data.bin"
+    f.tofile(src)
+    out = tmp_path / "corrected.raw"
+    cmd = [EXE, str(src), str(nx), str(ny), str(nz), repr(eb), "sz3", "1", "2", "1",
+           "--out", str(out)]
+    print("run", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    g0, _ = fieldio.szlike(f, eb)
+    go, _, sto, _, rc = O.correct(f, g0, eb, sub=(1, 2, 1))
+    assert rc == 0
+    assert np.fromfile(out, np.float64).tobytes() == go.tobytes()
+    assert f"iteration is: {sto['ite']}" in r.stdout
