@@ -278,6 +278,21 @@ def edit_channel(tf, tg0, tg, eb, reps=10):
            "algorithmic_bytes": alg, "achieved": alg / ms / 1e6, "unit": "GB/s",
            "frac": alg / ms / 1e6 / HBM_PEAK_GBPS, "nbits": info["nbits"],
            "edited": info["edited"], "clamp_values": info["values"]}
+    # the oracle's encoder on the first 64 planes, one host core (the reference
+    # extracts on the GPU and packs on the host: pMSz.cu:2141-2226, 2328-2349)
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        planes = min(64, tf.shape[0])
+        fs, g0s, gs = (a[:planes].cpu().numpy() for a in (tf, tg0, tg))
+        t = time.perf_counter()
+        pyoracle.encode_edits(fs, g0s, gs, eb)
+        cs = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": fs.size / cs / 1e9, "unit": "GVoxels/s", "cores": 1,
+                               "kind": "port", "sample": f"first {planes} planes, oracle encoder",
+                               "gpu_gvoxels_per_s": nvox / ms / 1e6}
+    except ImportError:
+        pass
     if fieldio.zstd_version() is not None:
         pkh, vh = pk.cpu().numpy(), vals.cpu().numpy()
         with tempfile.TemporaryDirectory() as d:
