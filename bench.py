@@ -413,7 +413,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the multi-GPU path is a design "
-              "(one subdomain per GPU), not something this build runs", file=sys.stderr)
+              "(one subdomain per GPU), not something this benchmark runs", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -511,7 +511,7 @@ def main():
     # End to end against the reference algorithm's own schedule: the labeling
     # pass plus one full-volume detection per detection of the loop (read g and
     # the code: s + 1 bytes per voxel each), plus 3s + 2 bytes per edit (read f,
-    # g, dc; write g, dc).  This build skips most of that work (work-list sweeps),
+    # g, dc; write g, dc).  The engine skips most of that work (work-list sweeps),
     # so the figure is what the reference's dense schedule would have to move.
     s_bytes = 4
     detections = s0["edit_sweeps"] + 1
