@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "pmsz_common.h"
 
 namespace pmsz {
@@ -34,21 +36,47 @@ struct DqBlock {
   uint32_t pad;
 };
 
-// Pass 1: the quantum of every voxel, stored as 2k + verbatim, and g0.
+// Pass 1: the quantum of every voxel, stored as 2k + verbatim, and g0.  A
+// thread takes 16 bytes of f (vector loads and stores when f, g0 and the quanta
+// are 16-byte aligned).
 template <typename T>
-__global__ __launch_bounds__(256) void k_dq_quant(const T *__restrict__ f, T *__restrict__ g0,
-                                                  int32_t *__restrict__ kq, int64_t n, double eb,
-                                                  double inv_q2) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const T fv = f[i];
+__device__ __forceinline__ int32_t dq_quantum(T fv, T &gv, double eb, double inv_q2) {
   const double q2 = 2.0 * eb;
   const double r = (double)fv * inv_q2;
   const int32_t k = fabs(r) < kDqLimit ? (int32_t)round(r) : 0;
   const T v = (T)((double)k * q2);
   const bool keep = within_exact((double)fv, (double)v, eb);
-  g0[i] = keep ? v : fv;
-  kq[i] = 2 * k + (keep ? 0 : 1);
+  gv = keep ? v : fv;
+  return 2 * k + (keep ? 0 : 1);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dq_quant(const T *__restrict__ f, T *__restrict__ g0,
+                                                  int32_t *__restrict__ kq, int64_t n, double eb,
+                                                  double inv_q2, bool aligned) {
+  constexpr int kV = 16 / sizeof(T);
+  using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+  using K = typename std::conditional<sizeof(T) == 4, int4, int2>::type;
+  const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kV;
+  if (i0 >= n) return;
+  if (aligned && i0 + kV <= n) {
+    const V a = *reinterpret_cast<const V *>(f + i0);
+    V b;
+    K k;
+    const T *pa = reinterpret_cast<const T *>(&a);
+    T *pb = reinterpret_cast<T *>(&b);
+    int32_t *pk = reinterpret_cast<int32_t *>(&k);
+#pragma unroll
+    for (int u = 0; u < kV; u++) pk[u] = dq_quantum<T>(pa[u], pb[u], eb, inv_q2);
+    *reinterpret_cast<V *>(g0 + i0) = b;
+    *reinterpret_cast<K *>(kq + i0) = k;
+  } else {
+    for (int64_t i = i0; i < i0 + kV && i < n; i++) {
+      T gv;
+      kq[i] = dq_quantum<T>(f[i], gv, eb, inv_q2);
+      g0[i] = gv;
+    }
+  }
 }
 
 // Pass 2: Lorenzo residuals of one rank block's rows into an LDS histogram, then

@@ -2577,8 +2577,11 @@ int codec_dq(Ctx &c, const pmsz_params *p, const T *f, T *g0, uint64_t *hist_out
   PMSZ_HIP(hipMemcpyAsync(d_bl, bl.data(), nb * sizeof(DqBlock), hipMemcpyHostToDevice,
                           c.stream));
   PMSZ_HIP(hipMemsetAsync(d_hist, 0, hbytes, c.stream));
-  k_dq_quant<T><<<(unsigned)((n + 255) / 256), 256, 0, c.stream>>>(f, g0, kq, n, p->abs_eb,
-                                                                    1.0 / (2.0 * p->abs_eb));
+  constexpr int kV = 16 / sizeof(T);
+  const bool aligned = ((uintptr_t)f & 15) == 0 && ((uintptr_t)g0 & 15) == 0;  // kq: hipMalloc
+  const int64_t nthreads = (n + kV - 1) / kV;
+  k_dq_quant<T><<<(unsigned)((nthreads + 255) / 256), 256, 0, c.stream>>>(
+      f, g0, kq, n, p->abs_eb, 1.0 / (2.0 * p->abs_eb), aligned);
   PMSZ_HIP(hipGetLastError());
   k_dq_hist<<<chunks, 256, 0, c.stream>>>(kq, d_bl, nb, p->nx, p->nx * p->ny, d_hist);
   PMSZ_HIP(hipGetLastError());

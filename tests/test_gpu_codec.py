@@ -63,6 +63,21 @@ def test_codec_ragged(dev, shape):
     _compare(f, 1e-3, dev)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_codec_unaligned(dev, dtype):
+    # f one element into a larger buffer: the voxel-by-voxel path
+    from pmsz_anonymous_amd import engine, fieldio
+    f = fieldio.perlin((13, 17, 19), 4, 5, 4, dtype=dtype)
+    og0, oh = O.codec_dq(f, 1e-3)
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    buf = torch.zeros(f.size + 1, dtype=tdt, device=dev)
+    tf = buf[1:].view(f.shape)
+    tf.copy_(torch.from_numpy(f))
+    assert tf.data_ptr() % 16 != 0
+    g0, h, _ = engine.codec_dq(tf, 1e-3)
+    assert g0.cpu().numpy().tobytes() == og0.tobytes() and np.array_equal(h, oh)
+
+
 def test_codec_escapes_and_verbatim(dev):
     rng = np.random.default_rng(2)
     f = (rng.standard_normal((9, 10, 11)) * 50).astype(np.float64)
