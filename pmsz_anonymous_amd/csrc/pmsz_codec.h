@@ -141,4 +141,87 @@ __global__ __launch_bounds__(256) void k_dq_hist(const int32_t *__restrict__ kq,
     if (s_h[b]) atomicAdd(&h[b], (unsigned long long)s_h[b]);
 }
 
+// Pass 2 with 4 consecutive voxels of a row per thread, when every block's x
+// offset and extent and the row pitch are multiples of 4: the quanta of the
+// voxel, its x-1 neighbour and the three lower rows come as 4 vector loads and 4
+// scalar loads per 4 voxels instead of 32 scalar loads.
+__global__ __launch_bounds__(256) void k_dq_hist4(const int32_t *__restrict__ kq,
+                                                  const DqBlock *__restrict__ blocks,
+                                                  uint32_t nblocks, int64_t nx, int64_t sxy,
+                                                  unsigned long long *__restrict__ hist) {
+  __shared__ uint32_t s_h[kDqSlots];
+  for (int b = threadIdx.x; b < kDqSlots; b += blockDim.x) s_h[b] = 0;
+  uint32_t lo = 0, hi = nblocks - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (blocks[mid].chunk0 <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const DqBlock B = blocks[lo];
+  __syncthreads();
+  const int64_t rows = (int64_t)B.ly * B.lz;
+  const int64_t r0 = (int64_t)(blockIdx.x - B.chunk0) * kDqRows;
+  const int64_t r1 = r0 + kDqRows < rows ? r0 + kDqRows : rows;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t quads = (uint32_t)((r1 - r0) * B.lx / 4);  // groups of 4 in the chunk
+  const uint32_t qrow = (uint32_t)B.lx / 4;
+  // whole waves step together (the bound is uniform), so the ballots are uniform
+  for (uint32_t q0 = 0; q0 < quads; q0 += blockDim.x) {
+    const uint32_t q = q0 + threadIdx.x;
+    const bool in = q < quads;
+    int64_t d[4] = {0, 0, 0, 0};
+    uint32_t verb = 0;
+    if (in) {
+      const uint32_t rr = q / qrow;
+      const int32_t x = (int32_t)(q - rr * qrow) * 4;
+      const uint32_t row = (uint32_t)r0 + rr;  // rows of a block < 2^32
+      const int32_t y = (int32_t)(row % (uint32_t)B.ly), z = (int32_t)(row / (uint32_t)B.ly);
+      const int64_t i = (int64_t)(B.z0 + z) * sxy + (int64_t)(B.y0 + y) * nx + B.x0 + x;
+      const bool hx = x > 0, hy = y > 0, hz = z > 0;
+      // rows: the voxel's (0), y-1 (1), z-1 (2), y-1 and z-1 (3); lane 0 of each is x-1
+      int32_t v[4][5];
+      const int64_t off[4] = {0, -nx, -sxy, -nx - sxy};
+      const bool ok[4] = {true, hy, hz, hy && hz};
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int4 w = make_int4(0, 0, 0, 0);
+        int32_t left = 0;
+        if (ok[r]) {
+          w = *reinterpret_cast<const int4 *>(kq + i + off[r]);
+          if (hx) left = kq[i + off[r] - 1];
+        }
+        v[r][0] = left;
+        v[r][1] = w.x;
+        v[r][2] = w.y;
+        v[r][3] = w.z;
+        v[r][4] = w.w;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        auto K = [&](int r, int c) -> int64_t { return (int64_t)(v[r][c] >> 1); };
+        const int c = u + 1;  // the voxel's column; c - 1 is its x-1 neighbour (0 at x = 0)
+        const int64_t pred = K(0, c - 1) + K(1, c) + K(2, c) - K(1, c - 1) - K(2, c - 1) -
+                             K(3, c) + K(3, c - 1);
+        d[u] = K(0, c) - pred;
+        verb |= (uint32_t)(v[0][c] & 1) << u;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      // the zero residual is most of a smooth field: one LDS atomic per wave
+      const uint64_t zm = __ballot(in && d[u] == 0);
+      if (lane == 0 && zm) atomicAdd(&s_h[kDqRadius], (uint32_t)__popcll(zm));
+      if (in && d[u] != 0) {
+        const int slot = (d[u] > -kDqRadius && d[u] < kDqRadius) ? (int)d[u] + kDqRadius : kDqBins;
+        atomicAdd(&s_h[slot], 1u);
+      }
+      if (in && ((verb >> u) & 1u)) atomicAdd(&s_h[kDqBins + 1], 1u);
+    }
+  }
+  __syncthreads();
+  unsigned long long *h = hist + (size_t)lo * kDqSlots;
+  for (int b = threadIdx.x; b < kDqSlots; b += blockDim.x)
+    if (s_h[b]) atomicAdd(&h[b], (unsigned long long)s_h[b]);
+}
+
 }  // namespace pmsz

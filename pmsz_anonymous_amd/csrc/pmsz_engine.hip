@@ -2583,7 +2583,12 @@ int codec_dq(Ctx &c, const pmsz_params *p, const T *f, T *g0, uint64_t *hist_out
   k_dq_quant<T><<<(unsigned)((nthreads + 255) / 256), 256, 0, c.stream>>>(
       f, g0, kq, n, p->abs_eb, 1.0 / (2.0 * p->abs_eb), aligned);
   PMSZ_HIP(hipGetLastError());
-  k_dq_hist<<<chunks, 256, 0, c.stream>>>(kq, d_bl, nb, p->nx, p->nx * p->ny, d_hist);
+  bool quads = p->nx % 4 == 0;
+  for (const DqBlock &B : bl) quads = quads && B.x0 % 4 == 0 && B.lx % 4 == 0;
+  if (quads)
+    k_dq_hist4<<<chunks, 256, 0, c.stream>>>(kq, d_bl, nb, p->nx, p->nx * p->ny, d_hist);
+  else
+    k_dq_hist<<<chunks, 256, 0, c.stream>>>(kq, d_bl, nb, p->nx, p->nx * p->ny, d_hist);
   PMSZ_HIP(hipGetLastError());
   std::vector<unsigned long long> h((size_t)nb * kDqSlots);
   PMSZ_HIP(hipMemcpyAsync(h.data(), d_hist, hbytes, hipMemcpyDeviceToHost, c.stream));

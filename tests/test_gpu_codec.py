@@ -56,6 +56,18 @@ def test_codec_parity(dev, dtype, sub):
         _compare(f, eb, dev, sub)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("sub", [(1, 1, 1), (2, 2, 2), (4, 1, 3)])
+def test_codec_parity_row_quads(dev, dtype, sub):
+    # rows whose x offsets and extents are multiples of 4: 4 voxels per thread
+    from pmsz_anonymous_amd import fieldio
+    f = fieldio.perlin((24, 20, 32), 5, 6, 4, dtype=dtype)
+    rng = np.random.default_rng(3)
+    f.reshape(-1)[rng.integers(0, f.size, 7)] = 1e12  # escapes and verbatim voxels
+    for eb in (1e-4, 1e-2):
+        _compare(f, eb, dev, sub)
+
+
 @pytest.mark.parametrize("shape", [(1, 1, 2), (1, 3, 5), (2, 70, 3), (65, 2, 2)])
 def test_codec_ragged(dev, shape):
     rng = np.random.default_rng(sum(shape))
