@@ -238,9 +238,9 @@ int pmsz_encode_edits(const pmsz_params *p, const void *d_f, const void *d_g0, c
  * quantisation (cuSZ, J. Tian et al., PACT 2020), standing in for SZ3 in
  * compressLocalData_coreOnly (pMSz.cu:1536-1624; SZ3 is absent).  Every voxel
  * is quantised on its own, k = round(f * (1 / (2 abs_eb))) in double (0 when
- * that product's magnitude is >= 2^30), and decompressed to g0 = k * 2 abs_eb in the value
- * type, or to f itself (verbatim) when that value misses |f - g0| <= abs_eb
- * exactly.  The Lorenzo residuals of k over the 7 lower neighbours, core-only
+ * the rounded product's magnitude is >= 2^30, or f is NaN), and decompressed to
+ * g0 = k * 2 abs_eb in the value type, or to f itself (verbatim) when that value
+ * misses |f - g0| <= abs_eb exactly.  The Lorenzo residuals of k over the 7 lower neighbours, core-only
  * per rank block of sub (rank order as for pmsz_encode_edits), are histogrammed
  * per block.  hist (host, nullable): blocks x 1026 counts, residual d in slot
  * d + 512 for |d| < 512, then the escapes and the verbatim voxels.  The

@@ -42,8 +42,10 @@ struct DqBlock {
 template <typename T>
 __device__ __forceinline__ int32_t dq_quantum(T fv, T &gv, double eb, double inv_q2) {
   const double q2 = 2.0 * eb;
-  const double r = (double)fv * inv_q2;
-  const int32_t k = fabs(r) < kDqLimit ? (int32_t)round(r) : 0;
+  // the rounded quantum must stay below 2^30 in magnitude, so that 2k + 1 fits
+  // an int32; NaN fails the test (k = 0) and is then stored verbatim
+  const double r = round((double)fv * inv_q2);
+  const int32_t k = fabs(r) < kDqLimit ? (int32_t)r : 0;
   const T v = (T)((double)k * q2);
   const bool keep = within_exact((double)fv, (double)v, eb);
   gv = keep ? v : fv;

@@ -173,13 +173,16 @@ struct Sched {
   uint32_t append;          // work-list sweeps list claimed bits (else bitmaps are compacted)
 };
 
+constexpr uint32_t kErrInput = 1u, kErrBarrier = 2u;
+
 // Run-wide state, written only by kernels that end a phase.
 enum : int { kAccTargets = 0, kAccTouched = 1, kAccClamped = 2, kAccSparse = 3 };
 struct RunState {
   uint32_t tail_j;     // sweep whose apply the tail kernel starts with (0: none)
   uint32_t done;       // 0 running, 1 converged, 2 sweep cap reached
   uint32_t last_j;     // detections performed when done
-  uint32_t err;        // bit 0: NaN or sentinel value in an input
+  uint32_t err;        // bit 0 (kErrInput): NaN or sentinel value in an input;
+                       // bit 1 (kErrBarrier): a grid barrier of the tail timed out
   unsigned long long dense_passes;
   uint32_t bar_epoch;  // last epoch of the tail's grid barrier in this run
   uint32_t last_mode;  // the mode the latest apply decided

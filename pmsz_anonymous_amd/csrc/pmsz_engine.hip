@@ -819,32 +819,53 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
 // (acquire) and publishes the epoch in the generation word; the others wait for
 // it.  Epochs only grow within a run (the state is cleared before each run), so
 // no flag needs resetting.  bar[16 * b]: flag of block b; bar[16 * kTailBlocks]:
-// the generation.
-__device__ __forceinline__ void tail_barrier(uint32_t *bar, uint32_t epoch) {
+// the generation.  Every wait is bounded: after kBarrierTicks of the 100 MHz
+// wall clock (2 s; a barrier of the tail takes microseconds) the block sets
+// kErrBarrier in run->err and the barrier returns false, so a grid whose blocks
+// are not all resident ends with an error instead of hanging.
+constexpr long long kBarrierTicks = 200000000ll;
+__device__ __forceinline__ bool bar_wait(const uint32_t *w, uint32_t epoch) {
+  const long long t0 = wall_clock64();
+  while (__hip_atomic_load(w, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+    if (wall_clock64() - t0 > kBarrierTicks) return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+__device__ __forceinline__ bool tail_barrier(uint32_t *bar, uint32_t epoch, RunState *run) {
+  __shared__ uint32_t s_ok;
   __syncthreads();
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
+    s_ok = 1u;
     __hip_atomic_store(&bar[16 * blockIdx.x], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
   if (blockIdx.x == 0) {
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-      while (__hip_atomic_load(&bar[16 * b], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch)
-        __builtin_amdgcn_s_sleep(1);
+    bool ok = true;
+    for (uint32_t b = threadIdx.x; b < gridDim.x && ok; b += blockDim.x)
+      ok = bar_wait(&bar[16 * b], epoch);
+    if (!ok) s_ok = 0u;
     __syncthreads();
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0 && s_ok)
       __hip_atomic_store(&bar[16 * kTailBlocks], epoch, __ATOMIC_RELEASE,
                          __HIP_MEMORY_SCOPE_AGENT);
   } else if (threadIdx.x == 0) {
-    while (__hip_atomic_load(&bar[16 * kTailBlocks], __ATOMIC_ACQUIRE,
-                             __HIP_MEMORY_SCOPE_AGENT) != epoch)
-      __builtin_amdgcn_s_sleep(1);
+    if (!bar_wait(&bar[16 * kTailBlocks], epoch)) s_ok = 0u;
   }
   __syncthreads();
+  const bool ok = s_ok != 0u;
+  if (!ok && threadIdx.x == 0) atomicOr(&run->err, kErrBarrier);
+  __syncthreads();  // s_ok is rewritten by the next barrier
+  return ok;
 }
 
 // The tail: once a sweep's targets are few and detection j+1 would run over a
 // work list, one cooperative kernel runs the remaining sweeps -- apply, grid barrier, work-list detection,
 // grid barrier -- instead of several launches per sweep.  Every block decides
-// alike from counters read after a barrier.  Launched only through
-// hipLaunchCooperativeKernel, which guarantees that all blocks are resident.
+// alike from counters read after a barrier.  A grid of several blocks is
+// launched only through hipLaunchCooperativeKernel, which guarantees that all
+// blocks are resident; where the runtime refuses that, the host launches one
+// block, which runs in solo mode (workgroup barriers only) from the start.
 template <typename T, bool kMulti>
 __global__ __launch_bounds__(kTailThreads) void k_tail(
     Dom d0, const Dom *__restrict__ doms, uint32_t shift, uint32_t ndom,
@@ -861,9 +882,13 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
   bool solo = gridDim.x == 1;
   uint32_t nthreads = gridDim.x * kTailThreads;
   uint32_t first = blockIdx.x * kTailThreads;
-  auto barrier = [&]() {
-    if (solo) __syncthreads();
-    else tail_barrier(bar, ++epoch);
+  // false: a grid barrier timed out (kErrBarrier is set); the block leaves
+  auto barrier = [&]() -> bool {
+    if (solo) {
+      __syncthreads();
+      return true;
+    }
+    return tail_barrier(bar, ++epoch, run);
   };
   unsigned long long tt = 0, sv = 0;
   uint32_t touched = 0, clamped = 0;
@@ -915,7 +940,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
       }
     }
     if (lead) tt += nt;
-    barrier();
+    if (!barrier()) return;
     if (!queue) break;  // sweep cap: sweep j's edits landed, no detection j+1
     // detection j+1 over the work list
     Counters &cn = ring[(j + 1) % kCtrRing];
@@ -942,7 +967,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
     }
     block_add3<kTailThreads>(lc, cn);
     if (lead) sv += nw;
-    barrier();
+    if (!barrier()) return;
     if (kMulti) {
       if (blockIdx.x == 0) {
         if (threadIdx.x == 0) {
@@ -962,7 +987,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
           cn.ncapped = s_cap;
         }
       }
-      barrier();
+      if (!barrier()) return;
     }
     j++;
   }
@@ -1423,7 +1448,20 @@ struct Ctx {
   RunState *h_run = nullptr;   // pinned mirror of the run state
   DomState *h_ds = nullptr;    // pinned mirror of the domain states
   size_t h_ds_cap = 0;
+  // the runtime refused a cooperative launch of the tail in this context: its
+  // tails run as one workgroup from then on
+  bool coop_refused = false;
 };
+
+// Records a refused cooperative launch: the context falls back to one-workgroup
+// tails, and the HIP error is reported on stderr once per process.
+void note_coop_refused(Ctx &c, hipError_t e) {
+  c.coop_refused = true;
+  static std::atomic<int> said{0};
+  if (said.exchange(1) == 0)
+    fprintf(stderr, "pmsz: cooperative launch of the tail refused (%s: %s); tails run as one "
+                    "workgroup in this context\n", hipGetErrorName(e), hipGetErrorString(e));
+}
 
 void clear_graphs(Ctx &c) {
   for (auto &kv : c.graphs) (void)hipGraphExecDestroy(kv.second);
@@ -1687,9 +1725,10 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   static const int occ_dlist = occupancy(k_detect_list<T, kMulti>, kApplyThreads);
   const unsigned g_apply = (unsigned)std::max(1, c.cus * std::min(occ_apply, 8));
   const unsigned g_list = (unsigned)std::max(1, c.cus * std::min(occ_dlist, 8));
-  // the tail: its grid barriers need every block resident, also while the tails of
-  // other contexts run -- each tail takes at most 1/(kMaxContexts+1) of what the
-  // device can hold of it, and at most kMaxContexts contexts exist
+  // the tail: a grid of several blocks is launched cooperatively (the runtime
+  // checks that every block can be resident); it takes at most
+  // 1/(kMaxContexts+1) of what the device can hold of it, so the tails of all
+  // contexts of this process fit together
   static const int occ_tail = occupancy(k_tail<T, kMulti>, kTailThreads);
   const unsigned g_tail = (unsigned)std::max(
       1, std::min(std::min(kTailBlocks, c.tune.tail_blocks), c.cus * occ_tail / (kMaxContexts + 1)));
@@ -1838,22 +1877,27 @@ int drive(Ctx &c, Job &job, Outcome &out) {
       uint32_t *a_bar = c.bar;
       void *args[] = {&a_d0,    &a_doms,  &a_shift, &a_ndom, &a_ds,  &a_tmark,
                       &a_tlist, &a_wmark, &a_wlist, &a_ring, &a_run, &a_s, &a_bar};
-      hipError_t e = hipLaunchCooperativeKernel((const void *)k_tail<T, kMulti>, dim3(g_tail),
-                                                dim3(kTailThreads), args, 0, c.stream);
-      if (e != hipSuccess) {
-        // the runtime can refuse a cooperative launch while another process holds
-        // the device's cooperative resources.  The tail's grid is sized to fit on
-        // the device beside every other context's, so as a regular launch its
-        // blocks are all resident as well
-        (void)hipGetLastError();
-        e = hipLaunchKernel((const void *)k_tail<T, kMulti>, dim3(g_tail), dim3(kTailThreads),
-                            args, 0, c.stream);
+      hipError_t e = hipSuccess;
+      if (g_tail > 1 && !c.coop_refused) {
+        e = hipLaunchCooperativeKernel((const void *)k_tail<T, kMulti>, dim3(g_tail),
+                                       dim3(kTailThreads), args, 0, c.stream);
+        if (e != hipSuccess) {
+          // the runtime can refuse a cooperative launch (another process may hold
+          // the device's cooperative resources); the context then launches the
+          // tail as one workgroup, which needs no co-residency
+          note_coop_refused(c, e);
+          (void)hipGetLastError();
+        }
       }
+      if (g_tail == 1 || c.coop_refused)
+        e = hipLaunchKernel((const void *)k_tail<T, kMulti>, dim3(1), dim3(kTailThreads), args, 0,
+                            c.stream);
       PMSZ_HIP(e);
     }
     PMSZ_HIP(hipGetLastError());
     PMSZ_HIP(hipMemcpyAsync(c.h_run, c.run, sizeof(RunState), hipMemcpyDeviceToHost, c.stream));
     PMSZ_HIP(hipStreamSynchronize(c.stream));
+    if (c.h_run->err & kErrBarrier) return PMSZ_EDEVICE;
     if (c.h_run->err) return PMSZ_EINVAL;
     if (!c.h_run->done) {
       // the chain ended before the tail took over: continue it where it stopped

@@ -21,20 +21,29 @@ R = O.DQ_RADIUS
 
 
 def quantise(f, eb):
-    """k = round half away from zero of f * (1 / (2 eb)), exactly; 0 beyond 2^30."""
+    """k = round half away from zero of f * (1 / (2 eb)), exactly; 0 where the
+    rounded value reaches 2^30 in magnitude, and for NaN."""
     r = f.astype(np.float64) * (1.0 / (2.0 * eb))
-    a = np.abs(r)
-    fl = np.floor(a)
-    k = (fl + (a - fl >= 0.5)) * np.sign(r)
-    return np.where(a < 2.0 ** 30, k, 0).astype(np.int64)
+    with np.errstate(invalid="ignore"):
+        a = np.abs(r)
+        fl = np.floor(a)
+        k = (fl + (a - fl >= 0.5)) * np.sign(r)
+        ok = np.abs(k) < 2.0 ** 30
+    return np.where(ok, np.nan_to_num(k), 0).astype(np.int64)
+
+
+def within(a, b, e):
+    """|a - b| <= e in rationals; False when either value is NaN."""
+    if a != a or b != b:
+        return False
+    return abs(Fraction(float(a)) - Fraction(float(b))) <= e
 
 
 def dq_numpy(f, eb, sub):
     k = quantise(f, eb)
     v = (k.astype(np.float64) * (2.0 * eb)).astype(f.dtype)
     e = Fraction(float(eb))
-    keep = np.array([abs(Fraction(float(a)) - Fraction(float(b))) <= e
-                     for a, b in zip(f.ravel(), v.ravel())]).reshape(f.shape)
+    keep = np.array([within(a, b, e) for a, b in zip(f.ravel(), v.ravel())]).reshape(f.shape)
     g0 = np.where(keep, v, f).astype(f.dtype)
     hists = []
     for sl in rank_blocks(f.shape, sub):
@@ -74,6 +83,22 @@ def test_codec_escapes_and_verbatim():
     assert g0.tobytes() == wg0.tobytes() and np.array_equal(h, wh)
     assert h[0, O.DQ_BINS] > 0 and h[0, O.DQ_BINS + 1] >= 1
     assert g0[2, 3, 4] == f[2, 3, 4]
+
+
+def test_codec_quantum_limit_and_nan():
+    # f / (2 eb) = 2^30 - 0.25 rounds to 2^30: k = 0 and the voxel is verbatim (2k + 1
+    # would overflow an int32); 2^30 - 0.75 rounds to 2^30 - 1 and is kept; NaN is
+    # verbatim and counted as such
+    f = np.zeros((4, 5, 6), np.float64)
+    f[1, 2, 3] = 2.0 ** 30 - 0.25
+    f[2, 2, 2] = 2.0 ** 30 - 0.75
+    f[3, 4, 5] = np.nan
+    g0, h = O.codec_dq(f, 0.5)
+    wg0, wh = dq_numpy(f, 0.5, (1, 1, 1))
+    assert g0.tobytes() == wg0.tobytes() and np.array_equal(h, wh)
+    assert g0[1, 2, 3] == f[1, 2, 3] and g0[2, 2, 2] == 2.0 ** 30 - 1
+    assert np.isnan(g0[3, 4, 5])
+    assert h[0, O.DQ_BINS + 1] == 2
 
 
 @pytest.mark.parametrize("sub", [(1, 1, 1), (3, 2, 1)])

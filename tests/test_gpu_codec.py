@@ -98,6 +98,19 @@ def test_codec_escapes_and_verbatim(dev):
     assert info["escapes"] > 0 and info["verbatim"] >= 1
 
 
+def test_codec_quantum_limit_and_nan(dev):
+    # f / (2 eb) = 2^30 - 0.25 rounds to 2^30 (k = 0, verbatim); NaN is verbatim
+    f = np.zeros((8, 9, 10), np.float64)
+    f[1, 2, 3] = 2.0 ** 30 - 0.25
+    f[2, 2, 2] = 2.0 ** 30 - 0.75
+    f[5, 4, 3] = np.nan
+    from pmsz_anonymous_amd import engine
+    og0, oh = O.codec_dq(f, 0.5)
+    g0, h, info = engine.codec_dq(torch.from_numpy(f).to(dev), 0.5)
+    assert g0.cpu().numpy().tobytes() == og0.tobytes()
+    assert np.array_equal(h, oh) and info["verbatim"] == 2
+
+
 def test_codec_constant_field(dev):
     f = np.full((20, 21, 22), 7 * 0.5, np.float32)
     info = _compare(f, 0.25, dev, sub=(2, 3, 1))
