@@ -130,6 +130,7 @@ def test_cli_lazy_after_parent_cooperative_tail(tmp_path):
     cmd = [EXE, f"{src},{nx},{ny},{nz}", repr(eb), "sz3", "2", "2", "1", "--out", str(out)]
     print("run", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stderr, flush=True)  # a refused cooperative launch is reported here
     assert r.returncode == 0, r.stdout + r.stderr
     g0, _ = fieldio.szlike(f, eb)
     go, _, _, _, rc = O.correct(f, g0, eb, sub=(2, 2, 1))
