@@ -234,6 +234,21 @@ int pmsz_encode_edits(const pmsz_params *p, const void *d_f, const void *d_g0, c
                       uint8_t *d_packed, void *d_values, pmsz_edit_block *blocks,
                       pmsz_edit_info *info);
 
+/* pmsz_encode_edits without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same inputs,
+ * outputs and results as pmsz_encode_edits; the encoder's scratch (block table,
+ * look-back words, per-tile counts, totals) is a fixed 4 MiB device array of the
+ * library, one per device, held until the process ends.  Limits: at most 256
+ * rank blocks, and tiles of 4096 (float) / 2048 (double) voxels whose 12 bytes
+ * each fit that array -- about 1.4 G float or 0.7 G double voxels; larger
+ * requests return PMSZ_ETOOBIG before touching the device.  Runs on `stream` (a
+ * hipStream_t; NULL: the null stream) and returns when the results are there.
+ * Calls on one device share the array: calls that overlap on one device race. */
+int pmsz_encode_edits_noalloc(const pmsz_params *p, const void *d_f, const void *d_g0,
+                              const void *d_g, uint8_t *d_packed, void *d_values,
+                              pmsz_edit_block *blocks, pmsz_edit_info *info, void *stream);
+
 /* The GPU compressor front end: an SZ-style Lorenzo codec with dual
  * quantisation (cuSZ, J. Tian et al., PACT 2020), standing in for SZ3 in
  * compressLocalData_coreOnly (pMSz.cu:1536-1624; SZ3 is absent).  Every voxel

@@ -2679,11 +2679,11 @@ int edit_nbits(int q) {
 }
 
 // The rank blocks of p in rank order (row-major over {PX, PY, PZ}: z fastest),
-// their tiles and their 16-byte aligned packed streams.
-void edit_layout(const pmsz_params *p, int nbits, std::vector<EncBlock> &bl, int64_t &packed,
-                 uint32_t &ntiles) {
+// their tiles and their 16-byte aligned packed streams, into bl (room for
+// PX * PY * PZ entries).
+void edit_layout_into(const pmsz_params *p, int nbits, EncBlock *bl, int64_t &packed,
+                      uint32_t &ntiles) {
   const uint32_t tile = p->dtype == PMSZ_F32 ? enc_tile<float>() : enc_tile<double>();
-  bl.clear();
   packed = 0;
   ntiles = 0;
   const int px = p->sub[0], py = p->sub[1], pz = p->sub[2];
@@ -2693,7 +2693,7 @@ void edit_layout(const pmsz_params *p, int nbits, std::vector<EncBlock> &bl, int
     block_split(p->nx, px, cx, l[0], o[0]);
     block_split(p->ny, py, cy, l[1], o[1]);
     block_split(p->nz, pz, cz, l[2], o[2]);
-    EncBlock B;
+    EncBlock &B = bl[r];
     B.lx = (uint32_t)l[0];
     B.ly = (uint32_t)l[1];
     B.n = (uint32_t)(l[0] * l[1] * l[2]);
@@ -2703,59 +2703,76 @@ void edit_layout(const pmsz_params *p, int nbits, std::vector<EncBlock> &bl, int
     B.pk_bytes = ((int64_t)B.n * nbits + 7) / 8;
     packed += (B.pk_bytes + 15) & ~(int64_t)15;
     ntiles += (B.n + tile - 1) / tile;
-    bl.push_back(B);
   }
 }
+void edit_layout(const pmsz_params *p, int nbits, std::vector<EncBlock> &bl, int64_t &packed,
+                 uint32_t &ntiles) {
+  bl.resize((size_t)(p->sub[0] * p->sub[1] * p->sub[2]));
+  edit_layout_into(p, nbits, bl.data(), packed, ntiles);
+}
 
-template <typename T>
-int encode_edits(Ctx &c, const pmsz_params *p, const T *f, const T *g0, const T *g,
-                 uint8_t *packed, T *vals, pmsz_edit_block *blocks, pmsz_edit_info *info) {
-  const int nbits = edit_nbits(p->q);
-  std::vector<EncBlock> bl;
-  int64_t packed_bytes;
-  uint32_t ntiles;
-  edit_layout(p, nbits, bl, packed_bytes, ntiles);
-  const uint32_t nb = (uint32_t)bl.size();
-  // scratch in the context's byte map: blocks | look-back words | per-tile edited
-  // counts | error flags | totals
+// Scratch of the encoder in one device buffer: blocks | look-back words |
+// per-tile edited counts | error flags | totals (each part 256-byte aligned).
+struct EncScratch {
+  size_t o_look, o_edit, o_tick, o_tot, total;
+};
+EncScratch enc_scratch(uint32_t nb, uint32_t ntiles) {
   auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  const size_t o_look = up(nb * sizeof(EncBlock));
-  const size_t o_edit = o_look + up((size_t)ntiles * 8);
-  const size_t o_tick = o_edit + up((size_t)ntiles * 4);
-  const size_t o_tot = o_tick + 256;
-  const size_t total = o_tot + up((size_t)(nb + 2) * 8);
-  int rc = c.tbyte.ensure(total);
-  if (rc) return rc;
-  uint8_t *s = c.tbyte.p;
+  EncScratch e;
+  e.o_look = up(nb * sizeof(EncBlock));
+  e.o_edit = e.o_look + up((size_t)ntiles * 8);
+  e.o_tick = e.o_edit + up((size_t)ntiles * 4);
+  e.o_tot = e.o_tick + 256;
+  e.total = e.o_tot + up((size_t)(nb + 2) * 8);
+  return e;
+}
+
+// The encoder's launches on stream st with scratch s (enc_scratch bytes, 256-byte
+// aligned): the block table is copied in from bl, the rest of the scratch is
+// cleared by a fill kernel, and the totals (edited, values per block, look-back
+// failures: nb + 2 words) and the error flags are copied back to h_tot / h_err.
+// Returns when they are there.
+template <typename T>
+int encode_edits_on(hipStream_t st, uint8_t *s, const pmsz_params *p, const EncBlock *bl,
+                    uint32_t nb, uint32_t ntiles, int nbits, const T *f, const T *g0, const T *g,
+                    uint8_t *packed, T *vals, unsigned long long *h_tot, uint32_t *h_err) {
+  const EncScratch sc = enc_scratch(nb, ntiles);
   EncBlock *d_bl = (EncBlock *)s;
-  unsigned long long *look = (unsigned long long *)(s + o_look);
-  uint32_t *tile_edited = (uint32_t *)(s + o_edit);
-  uint32_t *err = (uint32_t *)(s + o_tick);
-  unsigned long long *tot = (unsigned long long *)(s + o_tot);
-  PMSZ_HIP(hipMemcpyAsync(d_bl, bl.data(), nb * sizeof(EncBlock), hipMemcpyHostToDevice,
-                          c.stream));
-  PMSZ_HIP(hipMemsetAsync(s + o_look, 0, total - o_look, c.stream));
+  unsigned long long *look = (unsigned long long *)(s + sc.o_look);
+  uint32_t *tile_edited = (uint32_t *)(s + sc.o_edit);
+  uint32_t *err = (uint32_t *)(s + sc.o_tick);
+  unsigned long long *tot = (unsigned long long *)(s + sc.o_tot);
+  PMSZ_HIP(hipMemcpyAsync(d_bl, bl, nb * sizeof(EncBlock), hipMemcpyHostToDevice, st));
+  const uint64_t n16 = (sc.total - sc.o_look) / 16;
+  k_fill16<<<(unsigned)std::min<uint64_t>((n16 + 255) / 256, 1024), 256, 0, st>>>(
+      (uint4 *)(s + sc.o_look), n16);
+  PMSZ_HIP(hipGetLastError());
   const EditParams ep = edit_params(p);
   const bool aligned = ((uintptr_t)f & 15) == 0 && ((uintptr_t)g0 & 15) == 0 &&
                        ((uintptr_t)g & 15) == 0;
   const bool flat = nb == 1 && aligned;
   if (flat)
-    k_edit_encode<T, true><<<ntiles, kEncThreads, 0, c.stream>>>(
+    k_edit_encode<T, true><<<ntiles, kEncThreads, 0, st>>>(
         f, g0, g, d_bl, nb, p->nx, p->nx * p->ny, nbits, aligned, ep, packed, vals, look,
         tile_edited, err);
   else
-    k_edit_encode<T, false><<<ntiles, kEncThreads, 0, c.stream>>>(
+    k_edit_encode<T, false><<<ntiles, kEncThreads, 0, st>>>(
         f, g0, g, d_bl, nb, p->nx, p->nx * p->ny, nbits, aligned, ep, packed, vals, look,
         tile_edited, err);
   PMSZ_HIP(hipGetLastError());
-  k_edit_totals<<<std::min<uint32_t>((ntiles + 255) / 256, 128u), 256, 0, c.stream>>>(
+  k_edit_totals<<<std::min<uint32_t>((ntiles + 255) / 256, 128u), 256, 0, st>>>(
       d_bl, nb, ntiles, look, tile_edited, tot);
   PMSZ_HIP(hipGetLastError());
-  std::vector<unsigned long long> h(nb + 2);
-  uint32_t herr = 0;
-  PMSZ_HIP(hipMemcpyAsync(h.data(), tot, (nb + 2) * 8, hipMemcpyDeviceToHost, c.stream));
-  PMSZ_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c.stream));
-  PMSZ_HIP(hipStreamSynchronize(c.stream));
+  PMSZ_HIP(hipMemcpyAsync(h_tot, tot, (nb + 2) * 8, hipMemcpyDeviceToHost, st));
+  PMSZ_HIP(hipMemcpyAsync(h_err, err, 4, hipMemcpyDeviceToHost, st));
+  PMSZ_HIP(hipStreamSynchronize(st));
+  return PMSZ_OK;
+}
+
+// The caller's outputs from the totals of encode_edits_on.
+int encode_edits_report(const EncBlock *bl, uint32_t nb, int nbits, int64_t packed_bytes,
+                        const unsigned long long *h, uint32_t herr, pmsz_edit_block *blocks,
+                        pmsz_edit_info *info) {
   if (herr & kEncErrLook || h[nb + 1]) return PMSZ_EDEVICE;
   if (herr & kEncErrSymbol) return PMSZ_EINVAL;
   int64_t voff = 0;
@@ -2777,6 +2794,84 @@ int encode_edits(Ctx &c, const pmsz_params *p, const T *f, const T *g0, const T 
     info->packed_capacity = packed_bytes;
   }
   return PMSZ_OK;
+}
+
+template <typename T>
+int encode_edits(Ctx &c, const pmsz_params *p, const T *f, const T *g0, const T *g,
+                 uint8_t *packed, T *vals, pmsz_edit_block *blocks, pmsz_edit_info *info) {
+  const int nbits = edit_nbits(p->q);
+  std::vector<EncBlock> bl;
+  int64_t packed_bytes;
+  uint32_t ntiles;
+  edit_layout(p, nbits, bl, packed_bytes, ntiles);
+  const uint32_t nb = (uint32_t)bl.size();
+  // the scratch lives in the context's byte map
+  int rc = c.tbyte.ensure(enc_scratch(nb, ntiles).total);
+  if (rc) return rc;
+  std::vector<unsigned long long> h(nb + 2);
+  uint32_t herr = 0;
+  rc = encode_edits_on<T>(c.stream, c.tbyte.p, p, bl.data(), nb, ntiles, nbits, f, g0, g, packed,
+                          vals, h.data(), &herr);
+  if (rc) return rc;
+  return encode_edits_report(bl.data(), nb, nbits, packed_bytes, h.data(), herr, blocks, info);
+}
+
+// ---------------------------------------------------------------------------
+// The encoder without allocation (pmsz_encode_edits_noalloc).  Its scratch is
+// this file-scope device array: each device that runs the library holds its own
+// copy until the process ends.  Host code never passes `scratch` itself (in host
+// code the name is a host-side stand-in, not a device address): the device
+// address comes from hipGetSymbolAddress, once per device, with that device
+// current, and is kept in g_scratch_ptr.  Every call on a device shares that
+// device's array and leaves its contents behind; the parts that must start at
+// zero are cleared by each call, and calls that overlap on one device race.
+constexpr size_t kNoAllocScratchBytes = (size_t)4 << 20;  // 4 MiB
+constexpr int kNoAllocBlocks = 256;                        // most rank blocks per call
+constexpr int kNoAllocDevices = 64;
+__device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kNoAllocDevices];
+
+// Device address of the current device's scratch array (looked up on the
+// device's first call).
+int scratch_on_device(uint8_t **out) {
+  int dev = -1;
+  PMSZ_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices) return PMSZ_EDEVICE;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      last_hip_error() = e;
+      return PMSZ_EDEVICE;
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = (uint8_t *)g_scratch_ptr[dev];
+  return PMSZ_OK;
+}
+
+template <typename T>
+int encode_edits_noalloc(hipStream_t st, const pmsz_params *p, const T *f, const T *g0,
+                         const T *g, uint8_t *packed, T *vals, pmsz_edit_block *blocks,
+                         pmsz_edit_info *info) {
+  const int nbits = edit_nbits(p->q);
+  const uint32_t nb = (uint32_t)(p->sub[0] * p->sub[1] * p->sub[2]);
+  if (nb > (uint32_t)kNoAllocBlocks) return PMSZ_ETOOBIG;
+  EncBlock bl[kNoAllocBlocks];
+  int64_t packed_bytes;
+  uint32_t ntiles;
+  edit_layout_into(p, nbits, bl, packed_bytes, ntiles);
+  if (enc_scratch(nb, ntiles).total > kNoAllocScratchBytes) return PMSZ_ETOOBIG;
+  uint8_t *s = nullptr;
+  int rc = scratch_on_device(&s);
+  if (rc) return rc;
+  unsigned long long h[kNoAllocBlocks + 2];
+  uint32_t herr = 0;
+  rc = encode_edits_on<T>(st, s, p, bl, nb, ntiles, nbits, f, g0, g, packed, vals, h, &herr);
+  if (rc) return rc;
+  return encode_edits_report(bl, nb, nbits, packed_bytes, h, herr, blocks, info);
 }
 }  // namespace pmsz
 
@@ -3214,6 +3309,24 @@ int pmsz_encode_edits(const pmsz_params *p, const void *d_f, const void *d_g0, c
                               (const double *)d_g, d_packed, (double *)d_values, blocks, info);
 }
 
+int pmsz_encode_edits_noalloc(const pmsz_params *p, const void *d_f, const void *d_g0,
+                              const void *d_g, uint8_t *d_packed, void *d_values,
+                              pmsz_edit_block *blocks, pmsz_edit_info *info, void *stream) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  const size_t es = p->dtype == PMSZ_F32 ? 4 : 8;
+  if (!d_f || !d_g0 || !d_g || !d_packed || !d_values) return PMSZ_EINVAL;
+  if (((uintptr_t)d_packed & 15) || ((uintptr_t)d_values % es)) return PMSZ_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (p->dtype == PMSZ_F32)
+    return encode_edits_noalloc<float>(st, p, (const float *)d_f, (const float *)d_g0,
+                                       (const float *)d_g, d_packed, (float *)d_values, blocks,
+                                       info);
+  return encode_edits_noalloc<double>(st, p, (const double *)d_f, (const double *)d_g0,
+                                      (const double *)d_g, d_packed, (double *)d_values, blocks,
+                                      info);
+}
+
 const char *pmsz_strerror(int code) {
   switch (code) {
     case PMSZ_OK: return "ok";
@@ -3221,7 +3334,8 @@ const char *pmsz_strerror(int code) {
     case PMSZ_ENOMEM: return "device allocation failed";
     case PMSZ_NOT_CONVERGED: return "not converged within max_sweeps";
     case PMSZ_EDEVICE: return "HIP runtime error";
-    case PMSZ_ETOOBIG: return "field too large for 32-bit voxel indices";
+    case PMSZ_ETOOBIG: return "field too large (32-bit voxel indices, or the fixed scratch of a "
+                              "no-allocation entry point)";
     case PMSZ_EBUSY: return "capacity exhausted";
     default: return "unknown error";
   }

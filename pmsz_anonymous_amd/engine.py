@@ -192,6 +192,8 @@ def lib():
         L.pmsz_codec_dq.argtypes = [P, vp, vp, vp, ctypes.POINTER(i64 * 3)]
         L.pmsz_encode_edits.argtypes = [P, vp, vp, vp, vp, vp, ctypes.POINTER(EditBlock),
                                         ctypes.POINTER(EditInfo)]
+        L.pmsz_encode_edits_noalloc.argtypes = [P, vp, vp, vp, vp, vp, ctypes.POINTER(EditBlock),
+                                                ctypes.POINTER(EditInfo), vp]
         L.pmsz_time_first_pass.argtypes = [P, vp, vp, i32, ctypes.POINTER(ctypes.c_double)]
         L.pmsz_strerror.argtypes = [i32]
         L.pmsz_strerror.restype = ctypes.c_char_p
@@ -472,6 +474,36 @@ def encode_edits(f, g0, g, abs_eb, **kw):
                                  ctypes.c_void_p(packed.data_ptr()),
                                  ctypes.c_void_p(vals.data_ptr()), blocks, ctypes.byref(info))
     _check(rc, "pmsz_encode_edits")
+    bl = [{k: getattr(b, k) for k, _ in EditBlock._fields_} for b in blocks]
+    inf = {k: getattr(info, k) for k, _ in EditInfo._fields_}
+    return packed[:cap.value], vals[:info.values], bl, inf
+
+
+def encode_edits_noalloc(f, g0, g, abs_eb, packed=None, vals=None, **kw):
+    """The edit side channel through pmsz_encode_edits_noalloc, which allocates no
+    device memory: its scratch is the library's fixed per-device array.  packed /
+    vals (nullable) are the caller's output tensors (packed: uint8, at least
+    pmsz_edit_packed_bytes; vals: f's dtype, f.numel() elements).  Runs on torch's
+    current stream.  Returns as encode_edits."""
+    import torch
+    p = make_params(f.shape, f.dtype, abs_eb, **kw)
+    cap = ctypes.c_int64(0)
+    _check(lib().pmsz_edit_packed_bytes(ctypes.byref(p), ctypes.byref(cap)),
+           "pmsz_edit_packed_bytes")
+    nb = p.sub[0] * p.sub[1] * p.sub[2]
+    if packed is None:
+        packed = torch.empty(max(cap.value, 16), dtype=torch.uint8, device=f.device)
+    if vals is None:
+        vals = torch.empty(f.numel(), dtype=f.dtype, device=f.device)
+    blocks = (EditBlock * nb)()
+    info = EditInfo()
+    stream = torch.cuda.current_stream(f.device).cuda_stream
+    rc = lib().pmsz_encode_edits_noalloc(
+        ctypes.byref(p), ctypes.c_void_p(f.contiguous().data_ptr()),
+        ctypes.c_void_p(g0.contiguous().data_ptr()), ctypes.c_void_p(g.contiguous().data_ptr()),
+        ctypes.c_void_p(packed.data_ptr()), ctypes.c_void_p(vals.data_ptr()), blocks,
+        ctypes.byref(info), ctypes.c_void_p(stream))
+    _check(rc, "pmsz_encode_edits_noalloc")
     bl = [{k: getattr(b, k) for k, _ in EditBlock._fields_} for b in blocks]
     inf = {k: getattr(info, k) for k, _ in EditInfo._fields_}
     return packed[:cap.value], vals[:info.values], bl, inf
