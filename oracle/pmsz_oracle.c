@@ -257,3 +257,18 @@ int orc_codec_dq(const orc_params *p, const void *f, void *g0, uint64_t *hist) {
   if (p->dtype == ORC_F32) return codec_dq_f32(p, (const float *)f, (float *)g0, hist);
   return codec_dq_f64(p, (const double *)f, (double *)g0, hist);
 }
+
+int orc_blocks(const orc_params *p, const void *f, const void *g, void *fo, void *go) {
+  int rc = orc_check(p);
+  if (rc) return rc;
+  if (p->dtype == ORC_F32)
+    return blocks_hook_f32(p, (const float *)f, (const float *)g, (float *)fo, (float *)go);
+  return blocks_hook_f64(p, (const double *)f, (const double *)g, (double *)fo, (double *)go);
+}
+
+int orc_sync_blocks(const orc_params *p, void *fb, void *gb, uint8_t *dcb, const int *send) {
+  int rc = orc_check(p);
+  if (rc) return rc;
+  if (p->dtype == ORC_F32) return sync_hook_f32(p, (float *)fb, (float *)gb, dcb, send);
+  return sync_hook_f64(p, (double *)fb, (double *)gb, dcb, send);
+}

@@ -138,6 +138,16 @@ int orc_within(double a, double b, double eb);
 double orc_clamp_f32(double fv, double eb, int clamp_mode);
 double orc_clamp_f64(double fv, double eb, int clamp_mode);
 
+/* Test hooks of the subdomain path.  orc_blocks: the padded blocks of every
+ * subdomain after set-up (core values and the initial ghost values of the ghost
+ * policy, exchange_ghost_layers, pMSz.cu:320-608), f into fo and g into go,
+ * blocks back to back in the order cx fastest, then cy, then cz, each
+ * (lz+2)(ly+2)(lx+2) values.  orc_sync_blocks: one delta-counter ghost sync
+ * (pMSz.cu:909-1272) on padded blocks in that layout: gb and dcb are updated in
+ * place; send[k] is block k's send_flag. */
+int orc_blocks(const orc_params *p, const void *f, const void *g, void *fo, void *go);
+int orc_sync_blocks(const orc_params *p, void *fb, void *gb, uint8_t *dcb, const int *send);
+
 #ifdef __cplusplus
 }
 #endif

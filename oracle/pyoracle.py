@@ -94,6 +94,8 @@ def lib():
         L.orc_within.restype = i32
         L.orc_split.argtypes = [i64, i32, i32, ctypes.POINTER(i64), ctypes.POINTER(i64)]
         L.orc_split.restype = None
+        L.orc_blocks.argtypes = [ctypes.POINTER(Params), vp, vp, vp, vp]
+        L.orc_sync_blocks.argtypes = [ctypes.POINTER(Params), vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -255,3 +257,60 @@ def split(n, parts, coord):
     lo, off = ctypes.c_int64(0), ctypes.c_int64(0)
     lib().orc_split(n, parts, coord, ctypes.byref(lo), ctypes.byref(off))
     return lo.value, off.value
+
+
+def block_shapes(shape, sub):
+    """Padded (lz+2, ly+2, lx+2) shapes of the subdomain blocks, in the order cx
+    fastest, then cy, then cz, with their (oz, oy, ox) core offsets."""
+    nz, ny, nx = shape
+    out = []
+    for cz in range(sub[2]):
+        for cy in range(sub[1]):
+            for cx in range(sub[0]):
+                lx, ox = split(nx, sub[0], cx)
+                ly, oy = split(ny, sub[1], cy)
+                lz, oz = split(nz, sub[2], cz)
+                out.append(((lz + 2, ly + 2, lx + 2), (oz, oy, ox)))
+    return out
+
+
+def blocks(f, g, sub, ghost_policy=GHOST_FULL, eb=1.0):
+    """Padded blocks (f, g) of every subdomain after set-up: the cores and the
+    initial ghost values.  Returns two lists of (lz+2, ly+2, lx+2) arrays."""
+    p = make_params(f.shape, f.dtype, eb, sub=sub, ghost_policy=ghost_policy)
+    shp = block_shapes(f.shape, sub)
+    tot = sum(int(np.prod(s)) for s, _ in shp)
+    fo = np.empty(tot, f.dtype)
+    go = np.empty(tot, f.dtype)
+    rc = lib().orc_blocks(ctypes.byref(p), _ptr(np.ascontiguousarray(f)),
+                          _ptr(np.ascontiguousarray(g)), _ptr(fo), _ptr(go))
+    assert rc == 0, rc
+    fb, gb, off = [], [], 0
+    for s, _ in shp:
+        n = int(np.prod(s))
+        fb.append(fo[off:off + n].reshape(s).copy())
+        gb.append(go[off:off + n].reshape(s).copy())
+        off += n
+    return fb, gb
+
+
+def sync_blocks(shape, eb, sub, fb, gb, dcb, send, q=5, ghost_policy=GHOST_FULL,
+                clamp_mode=CLAMP_BOUND_SAFE):
+    """One delta-counter ghost sync on the padded blocks (lists as from blocks(),
+    dc blocks uint8); returns the new (g blocks, dc blocks)."""
+    dtype = fb[0].dtype
+    p = make_params(shape, dtype, eb, q=q, sub=sub, ghost_policy=ghost_policy,
+                    clamp_mode=clamp_mode)
+    fcat = np.concatenate([b.ravel() for b in fb]).astype(dtype)
+    gcat = np.concatenate([b.ravel() for b in gb]).astype(dtype)
+    dcat = np.concatenate([b.ravel() for b in dcb]).astype(np.uint8)
+    flags = np.ascontiguousarray(send, dtype=np.int32)
+    rc = lib().orc_sync_blocks(ctypes.byref(p), _ptr(fcat), _ptr(gcat), _ptr(dcat), _ptr(flags))
+    assert rc == 0, rc
+    go, do, off = [], [], 0
+    for b in fb:
+        n = b.size
+        go.append(gcat[off:off + n].reshape(b.shape).copy())
+        do.append(dcat[off:off + n].reshape(b.shape).copy())
+        off += n
+    return go, do

@@ -2220,7 +2220,8 @@ int correct_subdomains(Ctx &c, const pmsz_params *p, const T *F, T *G, uint8_t *
     st->clamped_voxels += (int64_t)s.clamped;
     for (int q = 0; q < 3; q++)
       if (!conv) st->final_cnt[q] = std::max(st->final_cnt[q], s.last_cnt[q]);
-    ite = std::max<int64_t>(ite, o.outer_started + (int64_t)s.sweeps);
+    // "iteration is:" is printed by rank 0 (pMSz.cu:2073): the block at (0, 0, 0)
+    if (k == 0) ite = o.outer_started + (int64_t)s.sweeps;
   }
   if (p->mode == PMSZ_LAZY) {
     st->edit_sweeps = sweeps;
