@@ -85,32 +85,14 @@ def pmc_summary(nvox, key):
         return None
 
 
-def c4_fields(n=64, seed0=100, rng_seed=0):
-    """The batched workload: n independent f32 Perlin fields (seeds seed0..), each
-    edge log-uniform in [32, 256] (non-cubic), eb log-uniform in [1e-4, 1e-2] x
-    range, SZ-like and ZFP alternating by seed parity (ZFP only if libzfp loads),
-    base frequency 4, 4 octaves."""
-    from pmsz_anonymous_amd import fieldio
-    rng = np.random.default_rng(rng_seed)
-    out = []
-    for k in range(n):
-        edges = np.exp(rng.uniform(np.log(32), np.log(256), size=3)).round().astype(int)
-        eb_rel = float(10 ** rng.uniform(-4, -2))
-        seed = seed0 + k
-        comp = "zfp" if seed % 2 and fieldio.zfp_available() else "sz3"
-        shape = (int(edges[2]), int(edges[1]), int(edges[0]))
-        out.append(fieldio.perlin_case(shape, seed, 4, 4, eb_rel, comp, np.float32) + (comp,))
-    return out
-
-
 def batched_vs_plain(reps=3):
     """C4: the 64-field batch corrected in one run vs the same fields one
     pmsz_correct after another (fresh copies each time, staged before timing);
     results must be identical per field."""
     import torch
-    from pmsz_anonymous_amd import engine
+    from pmsz_anonymous_amd import engine, workloads
     t0 = time.perf_counter()
-    cases = c4_fields()
+    cases = workloads.c4_fields()
     log(f"c4: {len(cases)} fields generated in {time.perf_counter() - t0:.1f} s")
     dev = torch.device("cuda:0")
     fs = [torch.from_numpy(f).to(dev) for f, _, _, _ in cases]
@@ -178,20 +160,13 @@ def serving(n_req=400, pool=32, rate=200.0, lanes=4, seed=0):
     saturated throughput (every job queued at once) against the same jobs
     corrected one after another."""
     import torch
-    from pmsz_anonymous_amd import engine, fieldio, serve
-    rng = np.random.default_rng(seed)
+    from pmsz_anonymous_amd import engine, serve, workloads
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
-    entries = []
-    for i in range(pool):
-        edges = np.exp(rng.uniform(np.log(64), np.log(512), size=3)).round().astype(int)
-        dt = np.float64 if rng.uniform() < 0.2 else np.float32
-        eb_rel = float(rng.choice([1e-2, 1e-3, 1e-4]))
-        f, g0, eb = fieldio.perlin_case((int(edges[2]), int(edges[1]), int(edges[0])), i, 8, 5,
-                                        eb_rel, "sz3", dt)
-        entries.append((torch.from_numpy(f).to(dev), torch.from_numpy(g0).to(dev), eb))
+    entries = [(torch.from_numpy(f).to(dev), torch.from_numpy(g0).to(dev), eb)
+               for f, g0, eb in workloads.c5_jobs(pool, rng_seed=seed)]
     log(f"c5: pool of {pool} jobs generated in {time.perf_counter() - t0:.1f} s")
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+    arrivals = workloads.poisson_arrivals(n_req, rate, rng_seed=seed)
     picks = [k % pool for k in range(n_req)]
     copies = [entries[k][1].clone() for k in picks]
     torch.cuda.synchronize()
@@ -356,9 +331,9 @@ def subdomains_lazy_vs_sync(edge=768, reps=2):
     per-iteration sync (full ghost policy).  Each result is checked for the
     original's codes over the whole field."""
     import torch
-    from pmsz_anonymous_amd import engine, fieldio
+    from pmsz_anonymous_amd import engine, workloads
     t0 = time.perf_counter()
-    f, g0, eb = fieldio.perlin_case((edge,) * 3, 2, 16.0, 6, 1e-4, "sz3", np.float64)
+    f, g0, eb = workloads.c3_case(edge)
     log(f"subdomain workload generated in {time.perf_counter() - t0:.1f} s")
     dev = torch.device("cuda:0")
     tf = torch.from_numpy(f).to(dev)
@@ -417,12 +392,12 @@ def main():
         sys.exit(2)
 
     import torch
-    from pmsz_anonymous_amd import engine, fieldio
+    from pmsz_anonymous_amd import engine, workloads
 
     n = args.size
     shape = (n, n, n)
     t0 = time.perf_counter()
-    f, g0, eb = fieldio.perlin_case(shape, 1, 8.0, 5, 1e-3, "sz3", np.float32)
+    f, g0, eb = workloads.c2_case("sz3", n)
     log(f"workload generated in {time.perf_counter() - t0:.1f} s, eb={eb:.6e}")
     dev = torch.device("cuda:0")
     tf = torch.from_numpy(f).to(dev)
