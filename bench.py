@@ -19,6 +19,11 @@ import os
 import sys
 import time
 
+# Hardware queues of the process (HIP's default is 4): the serving leg keeps up to
+# 7 correction contexts, each with its own stream, and streams beyond the queue
+# count share queues and serialise.  Set before the HIP runtime starts.
+os.environ["GPU_MAX_HW_QUEUES"] = "8"
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -151,7 +156,7 @@ def batched_vs_plain(reps=3):
     }
 
 
-def serving(n_req=400, pool=32, rate=200.0, lanes=4, seed=0):
+def serving(n_req=400, pool=32, rate=200.0, lanes=7, seed=0, alone_voxels=1 << 22):
     """C5: Poisson arrivals (mean `rate` per second) of correction jobs, edges
     log-uniform in [64, 512] per axis, f32 80% / f64 20%, eb in {1e-2, 1e-3,
     1e-4} x range, Perlin seed = job counter (pool entry = counter mod `pool`,
@@ -170,7 +175,8 @@ def serving(n_req=400, pool=32, rate=200.0, lanes=4, seed=0):
     picks = [k % pool for k in range(n_req)]
     copies = [entries[k][1].clone() for k in picks]
     torch.cuda.synchronize()
-    with serve.Server(lanes=lanes, hbm_budget=96 << 30, max_batch=16) as srv:
+    with serve.Server(lanes=lanes, hbm_budget=96 << 30, max_batch=16,
+                      alone_voxels=alone_voxels) as srv:
         # warm-up: the largest jobs of the pool at once, one per lane, so that every
         # lane's context holds buffers of the largest size before timing
         largest = sorted(entries, key=lambda e: -e[0].numel())[:lanes]
@@ -217,6 +223,7 @@ def serving(n_req=400, pool=32, rate=200.0, lanes=4, seed=0):
         "workload": f"{n_req} jobs, Poisson {rate:g}/s, edges log-uniform [64,512], f32 80% / "
                     f"f64 20%, eb {{1e-2,1e-3,1e-4}} x range, pool of {pool} Perlin fields",
         "lanes": lanes,
+        "alone_voxels": alone_voxels,
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                        "p99": float(np.percentile(lat, 99) * 1e3),
                        "max": float(lat.max() * 1e3)},

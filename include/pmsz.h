@@ -138,6 +138,54 @@ int pmsz_ctx_trim(pmsz_ctx *ctx);
  * admit work by HBM budget. */
 int64_t pmsz_workspace_bytes(int n, const pmsz_params *params);
 
+/* The correction server: continuous batching of correction jobs over lanes,
+ * admitted by an HBM budget.  The reference corrects one field per process run
+ * (main, pMSz.cu:2596-3188); this is its served form.  A server keeps `lanes`
+ * contexts (pmsz_ctx) and one host thread per lane.  A free lane takes the oldest
+ * queued job it can admit -- and, when that job has fewer than alone_voxels
+ * voxels, the further queued small jobs of the same dtype, mode and max_sweeps,
+ * up to max_batch -- while the library-owned device bytes of all lanes (each
+ * running lane's workspace, pmsz_workspace_bytes, or what its context holds if
+ * more, idle lanes included) stay within hbm_budget, and corrects them as one
+ * batched run (pmsz_ctx_correct_batch); idle lanes free their buffers when they
+ * block an admission.  Jobs submitted while others run are admitted as lanes and
+ * memory free up, and runs on different lanes overlap on the device.  Each
+ * job's result equals pmsz_correct on that job alone.  The server's contexts
+ * count against the 8 contexts of the process. */
+typedef struct pmsz_server pmsz_server;
+typedef struct {
+  int lanes;            /* contexts, one host thread each: 1..7 (4) */
+  int max_batch;        /* most jobs in one batched run (32) */
+  int64_t hbm_budget;   /* library-owned device bytes of all lanes together (64 GiB) */
+  int64_t alone_voxels; /* a job of this many voxels or more runs by itself (2^20) */
+} pmsz_server_config;
+typedef struct {
+  int status;           /* the job's return code: PMSZ_OK, PMSZ_NOT_CONVERGED or an error */
+  int batch_size;       /* jobs of the run that corrected it */
+  int lane;             /* the lane that ran it */
+  double t_submit, t_start, t_done; /* CLOCK_MONOTONIC seconds */
+} pmsz_job_info;
+void pmsz_default_server_config(pmsz_server_config *c);
+/* The lanes' contexts belong to the device current at this call. */
+int pmsz_server_create(const pmsz_server_config *c, pmsz_server **out);
+/* Corrects every queued job, then frees the server (unpolled jobs are dropped). */
+int pmsz_server_destroy(pmsz_server *s);
+/* Queues the correction of d_g (device, in place) against d_f with parameters p
+ * (sub = 1,1,1; not PMSZ_NAIVE); d_dc (nullable) receives the delta counters.
+ * The inputs are read once `stream` (a hipStream_t of the caller; NULL: the null
+ * stream) has reached this call.  *job identifies the job.  PMSZ_EBUSY: the
+ * job's workspace alone exceeds the budget, or the server is closing. */
+int pmsz_submit(pmsz_server *s, const pmsz_params *p, const void *d_f, void *d_g,
+                uint8_t *d_dc, void *stream, uint64_t *job);
+/* Waits up to timeout_ms (0: no wait, negative: no limit) for the job.
+ * PMSZ_OK: it is done, its results are in d_g / d_dc, *stats and *info
+ * (nullable) are filled, and the job id is released; PMSZ_EBUSY: not done yet;
+ * PMSZ_EINVAL: unknown job. */
+int pmsz_poll(pmsz_server *s, uint64_t job, int timeout_ms, pmsz_stats *stats,
+              pmsz_job_info *info);
+/* Runs made, jobs corrected and jobs queued so far (each nullable). */
+int pmsz_server_stats(pmsz_server *s, int64_t *runs, int64_t *jobs_done, int64_t *queued);
+
 /* Tuning of a context: parameters of the schedule and of the full-volume
  * kernel's tiles.  They change the speed of a run, never its result.  The
  * reference has no counterpart (its schedule is fixed: a full sweep per

@@ -27,7 +27,7 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall"]
 
-ENGINE_SRC = ["pmsz_engine.hip"]
+ENGINE_SRC = ["pmsz_engine.hip", "pmsz_serve.cpp"]
 ENGINE_DEPS = ["pmsz_common.h", os.path.join("..", "..", "include", "pmsz.h")]
 
 

@@ -495,9 +495,9 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
     };
     // append the lanes of wave mask bm (mine: this lane's bit); `info` is
     // evaluated only if bm is not 0
-    auto enqueue = [&](uint64_t bm, bool mine, int h, int e, auto info) {
+    auto enqueue = [&](uint64_t bm, int h, int e, auto info) {
       if (bm == 0) return;  // uniform
-      if (mine) {
+      if ((bm >> lane) & 1u) {
         const uint32_t pos = qt + __builtin_amdgcn_mbcnt_hi(
                                       (uint32_t)(bm >> 32),
                                       __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
@@ -525,14 +525,16 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
 #pragma unroll
       for (int h = 0; h < kColRows; h++) lo_mx[h] = lo_mn[h] = kNaN;
       // plane step p: rows rv of plane p are in registers, cc holds the codes of
-      // plane p-1; plane p+1 is loaded into nv and the codes of plane p into nc
+      // plane p-1.  The labeling loads plane p+2 into nv (two steps ahead: it has
+      // registers to spare, and a load then has two steps of every wave to
+      // arrive); a detection loads plane p+1 into nv and the codes of plane p into
+      // nc (one step ahead: measured faster, the detection is issue-bound)
+      constexpr int kAhead = kLabel ? 2 : 1;
       auto step = [&](int p, const T (&rv)[kColH], const uint32_t (&cc)[kColRows],
                       T (&nv)[kColH], uint32_t (&nc)[kColRows]) {
         const bool pin = p >= 0 && p < (int)nz;  // uniform
-        if (p < ze) {
-          fetch(p + 1, nv);
-          if (kDetect && p >= zs) fetch_codes(p, nc);
-        }
+        if (p + kAhead <= ze) fetch(p + kAhead, nv);
+        if (kDetect && p >= zs && p < ze) fetch_codes(p, nc);
         const uint32_t tplane = lane + 5u * (uint32_t)(p - zs + 1);
         const int e = p - 1;  // plane completed at this step
         const bool emitp = e >= zs;
@@ -571,10 +573,9 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
                 const uint32_t ex = s_exp[c] + ts * 0x10001u;
                 const uint32_t ra = (uint32_t)__double2loint(wmx) - (ex & 0xffffu);
                 const uint32_t rd = (uint32_t)__double2loint(wmn) - (ex >> 16);
-                const bool fail = ((ra | rd) & 511u) != 0u && vis[h - 1];
-                const uint64_t bm = __ballot(fail);
+                const uint64_t bm = __ballot(((ra | rd) & 511u) != 0u && vis[h - 1]);
                 // info: the code byte, then the low bytes of ra and rd
-                enqueue(bm, fail, h - 1, e, [&] {
+                enqueue(bm, h - 1, e, [&] {
                   return c | (__builtin_amdgcn_perm(rd, ra, 0x0c0c0400u) << 8);
                 });
               }
@@ -592,17 +593,33 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
           k1 = kk; r1 = r;
         }
       };
-      // two register sets for the planes, alternating: no copies between steps
-      T va[kColH], vb[kColH];
       // the code bytes, one per register (packed bytes would cost an extract each)
       uint32_t ca[kColRows] = {}, cb[kColRows] = {};
-      fetch(zs - 1, va);
       int p = zs - 1;
-      for (; p + 1 <= ze; p += 2) {
-        step(p, va, ca, vb, cb);
-        step(p + 1, vb, cb, va, ca);
+      if constexpr (kAhead == 2) {
+        // three register sets for the planes in rotation (no copies between
+        // steps): step q reads set q mod 3 and loads plane q + 2 into set
+        // (q + 2) mod 3
+        T va[kColH], vb[kColH], vc[kColH];
+        fetch(zs - 1, va);
+        fetch(zs, vb);
+        for (; p + 2 <= ze; p += 3) {
+          step(p, va, ca, vc, cb);
+          step(p + 1, vb, ca, va, cb);
+          step(p + 2, vc, ca, vb, cb);
+        }
+        if (p <= ze) step(p, va, ca, vc, cb);
+        if (p + 1 <= ze) step(p + 1, vb, ca, va, cb);
+      } else {
+        // two register sets for the planes, alternating
+        T va[kColH], vb[kColH];
+        fetch(zs - 1, va);
+        for (; p + 1 <= ze; p += 2) {
+          step(p, va, ca, vb, cb);
+          step(p + 1, vb, cb, va, ca);
+        }
+        if (p <= ze) step(p, va, ca, vb, cb);
       }
-      if (p <= ze) step(p, va, ca, vb, cb);
     } else {
       Ext<T> lo_e[kColRows], part[kColRows];
 #pragma unroll
@@ -642,9 +659,9 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
             } else {
               const uint32_t c = ccode[h - 1];
               const uint32_t w = (uint32_t)da | ((uint32_t)dd << 4);
-              const bool fail = (w != c || c == (uint32_t)(kSelf | (kSelf << 4))) && vis[h - 1];
-              const uint64_t bm = __ballot(fail);
-              enqueue(bm, fail, h - 1, e, [&] { return c | (w << 8); });
+              const uint64_t bm =
+                  __ballot((w != c || c == (uint32_t)(kSelf | (kSelf << 4))) && vis[h - 1]);
+              enqueue(bm, h - 1, e, [&] { return c | (w << 8); });
             }
           }
           // partials of plane p for the voxels of planes p and p+1
