@@ -318,6 +318,28 @@ typedef struct {
 int pmsz_codec_dq(const pmsz_params *p, const void *d_f, void *d_g0, uint64_t *hist,
                   pmsz_codec_info *info);
 
+/* The compressed stream of the same codec: per rank block, a canonical Huffman
+ * code (at most 24 bits) over the Lorenzo residual symbols -- d + 512 for
+ * |d| < 512, else an escape symbol with d in an escape list -- built from that
+ * block's histogram, plus the verbatim voxels' indices and values.  The stream
+ * replaces SZ_compress's byte output (pMSz.cu:1568-1572), whose size gives the
+ * reference's CR (pMSz.cu:3135-3144); its layout is given in
+ * csrc/pmsz_dqstream.h.  Encoding and decoding run on the device.
+ * pmsz_codec_dq_stream_bound: a capacity that holds the stream of any field of
+ * these parameters.  pmsz_codec_dq_compress: quantises d_f as pmsz_codec_dq
+ * (g0 into d_g0) and writes the stream to d_stream (device, 16-byte aligned,
+ * `capacity` bytes); *stream_bytes receives its size (PMSZ_ETOOBIG, with the
+ * size needed, when the capacity is short); info as pmsz_codec_dq (nullable).
+ * pmsz_codec_dq_stream_params: dtype, extent, sub and abs_eb of a stream (the
+ * other fields their defaults).  pmsz_codec_dq_decompress: g0 from a stream
+ * (d_g0: nx*ny*nz values of the stream's dtype), bit-exactly the g0 that
+ * compression produced; PMSZ_EINVAL for a stream that fails its checks. */
+int pmsz_codec_dq_stream_bound(const pmsz_params *p, int64_t *bytes);
+int pmsz_codec_dq_compress(const pmsz_params *p, const void *d_f, void *d_g0, uint8_t *d_stream,
+                           int64_t capacity, int64_t *stream_bytes, pmsz_codec_info *info);
+int pmsz_codec_dq_stream_params(const uint8_t *d_stream, int64_t bytes, pmsz_params *p);
+int pmsz_codec_dq_decompress(const uint8_t *d_stream, int64_t bytes, void *d_g0);
+
 /* Average device times of the original labeling pass (avg_seconds[0]) and of
  * detection 1, a full-volume detection sweep on g (avg_seconds[1]), over
  * `iters` launches each, timed with HIP events on the library stream.

@@ -118,6 +118,84 @@ static double next_down_f64(double c) { return nextafter(c, -INFINITY); }
 static float round_half_away_f32(float r) { return roundf(r); }
 static double round_half_away_f64(double r) { return round(r); }
 
+/* ---- the compressed stream of the dual-quantisation codec (untyped parts) ---- */
+#define ORC_DQS_SYMBOLS (ORC_DQ_BINS + 1)
+#define ORC_DQS_ESCAPE ORC_DQ_BINS
+#define ORC_DQS_CHUNK 2048
+#define ORC_DQS_MAXLEN 24
+#define ORC_DQS_MAGIC 0x31514450u
+
+/* Huffman code lengths for counts[0..ns) (0: unused), at most ORC_DQS_MAXLEN:
+ * leaves in (count, symbol) order, two queues with the leaf taken on equal
+ * weights, depths as lengths; longer than the limit: every count halved
+ * (rounded up) and again.  One used symbol: length 1. */
+static void orc_dqs_lengths(const uint64_t *counts, int ns, uint8_t *len) {
+  uint64_t *cnt = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)ns);
+  int *sym = (int *)malloc(sizeof(int) * (size_t)ns);
+  uint64_t *w = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(2 * ns));
+  int *par = (int *)malloc(sizeof(int) * (size_t)(2 * ns));
+  int *dep = (int *)malloc(sizeof(int) * (size_t)(2 * ns));
+  memcpy(cnt, counts, sizeof(uint64_t) * (size_t)ns);
+  for (;;) {
+    memset(len, 0, (size_t)ns);
+    int m = 0;
+    for (int s = 0; s < ns; s++)
+      if (cnt[s]) sym[m++] = s;
+    if (m == 0) break;
+    if (m == 1) { len[sym[0]] = 1; break; }
+    /* stable insertion sort by count (symbols start in increasing order) */
+    for (int a = 1; a < m; a++) {
+      int v = sym[a], b = a - 1;
+      while (b >= 0 && cnt[sym[b]] > cnt[v]) { sym[b + 1] = sym[b]; b--; }
+      sym[b + 1] = v;
+    }
+    for (int k = 0; k < m; k++) w[k] = cnt[sym[k]];
+    int i = 0, j = m;
+    for (int k = m; k < 2 * m - 1; k++) {
+      int pick[2];
+      for (int t = 0; t < 2; t++) {
+        if (i < m && (j >= k || w[i] <= w[j])) pick[t] = i++;
+        else pick[t] = j++;
+      }
+      w[k] = w[pick[0]] + w[pick[1]];
+      par[pick[0]] = par[pick[1]] = k;
+    }
+    dep[2 * m - 2] = 0;
+    int maxd = 0;
+    for (int k = 2 * m - 3; k >= 0; k--) {
+      dep[k] = dep[par[k]] + 1;
+      if (k < m && dep[k] > maxd) maxd = dep[k];
+    }
+    if (maxd <= ORC_DQS_MAXLEN) {
+      for (int k = 0; k < m; k++) len[sym[k]] = (uint8_t)dep[k];
+      break;
+    }
+    for (int s = 0; s < ns; s++)
+      if (cnt[s]) cnt[s] = (cnt[s] + 1) / 2;
+  }
+  free(cnt); free(sym); free(w); free(par); free(dep);
+}
+
+/* Canonical codes (DEFLATE's assignment) of the lengths. */
+static void orc_dqs_codes(const uint8_t *len, int ns, uint32_t *code) {
+  uint32_t bl[ORC_DQS_MAXLEN + 2] = {0}, next[ORC_DQS_MAXLEN + 2] = {0};
+  for (int s = 0; s < ns; s++) if (len[s]) bl[len[s]]++;
+  uint32_t c = 0;
+  for (int b = 1; b <= ORC_DQS_MAXLEN; b++) { c = (c + bl[b - 1]) << 1; next[b] = c; }
+  for (int s = 0; s < ns; s++) code[s] = len[s] ? next[len[s]]++ : 0;
+}
+
+/* LSB-first bit writer into 32-bit words; a code goes out first bit first. */
+typedef struct { uint32_t *w; uint64_t pos; } orc_bitw;
+static void orc_put_code(orc_bitw *bw, uint32_t code, int len) {
+  for (int k = len - 1; k >= 0; k--) {
+    if ((code >> k) & 1u) bw->w[bw->pos >> 5] |= 1u << (bw->pos & 31);
+    bw->pos++;
+  }
+}
+
+static uint64_t orc_up8(uint64_t x) { return (x + 7) & ~(uint64_t)7; }
+
 #define T float
 #define SFX(x) x##_f32
 #include "oracle_impl.inc"
@@ -271,4 +349,22 @@ int orc_sync_blocks(const orc_params *p, void *fb, void *gb, uint8_t *dcb, const
   if (rc) return rc;
   if (p->dtype == ORC_F32) return sync_hook_f32(p, (float *)fb, (float *)gb, dcb, send);
   return sync_hook_f64(p, (double *)fb, (double *)gb, dcb, send);
+}
+
+int orc_dq_compress(const orc_params *p, const void *f, void *g0, uint8_t *stream,
+                    int64_t capacity, int64_t *bytes) {
+  int rc = orc_check(p);
+  if (rc) return rc;
+  if (p->dtype == ORC_F32)
+    return dq_compress_f32(p, (const float *)f, (float *)g0, stream, capacity, bytes);
+  return dq_compress_f64(p, (const double *)f, (double *)g0, stream, capacity, bytes);
+}
+
+int orc_dq_decompress(const uint8_t *stream, int64_t bytes, void *g0) {
+  if (!stream || bytes < 64) return ORC_EINVAL;
+  uint32_t dtype;
+  memcpy(&dtype, stream + 4, 4);
+  if (dtype == ORC_F32) return dq_decompress_f32(stream, bytes, (float *)g0);
+  if (dtype == ORC_F64) return dq_decompress_f64(stream, bytes, (double *)g0);
+  return ORC_EINVAL;
 }

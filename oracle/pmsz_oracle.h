@@ -126,6 +126,14 @@ int orc_encode_edits(const orc_params *p, const void *f, const void *g0, const v
 #define ORC_DQ_BINS (2 * ORC_DQ_RADIUS)
 int orc_codec_dq(const orc_params *p, const void *f, void *g0, uint64_t *hist);
 
+/* The compressed stream of that codec (the layout of the product's
+ * csrc/pmsz_dqstream.h): orc_dq_compress writes it to stream (host, capacity
+ * bytes; *bytes its size, ORC_EINVAL when short) and g0 as orc_codec_dq;
+ * orc_dq_decompress gives g0 back from it. */
+int orc_dq_compress(const orc_params *p, const void *f, void *g0, uint8_t *stream,
+                    int64_t capacity, int64_t *bytes);
+int orc_dq_decompress(const uint8_t *stream, int64_t bytes, void *g0);
+
 /* Balanced block split of one dimension (pMSz.cu:2690-2706). */
 void orc_split(int64_t n, int parts, int coord, int64_t *local, int64_t *offset);
 

@@ -94,6 +94,9 @@ def lib():
         L.orc_within.restype = i32
         L.orc_split.argtypes = [i64, i32, i32, ctypes.POINTER(i64), ctypes.POINTER(i64)]
         L.orc_split.restype = None
+        L.orc_dq_compress.argtypes = [ctypes.POINTER(Params), vp, vp, vp, i64,
+                                      ctypes.POINTER(i64)]
+        L.orc_dq_decompress.argtypes = [vp, i64, vp]
         L.orc_blocks.argtypes = [ctypes.POINTER(Params), vp, vp, vp, vp]
         L.orc_sync_blocks.argtypes = [ctypes.POINTER(Params), vp, vp, vp, vp]
         _lib = L
@@ -238,6 +241,36 @@ def codec_dq(f, eb, sub=(1, 1, 1)):
     rc = lib().orc_codec_dq(ctypes.byref(p), _ptr(f), _ptr(g0), _ptr(hist))
     assert rc == 0, rc
     return g0, hist
+
+
+def dq_compress(f, eb, sub=(1, 1, 1)):
+    """(g0, stream) of the codec's compressed stream: stream is a uint8 array."""
+    f = np.ascontiguousarray(f)
+    p = make_params(f.shape, f.dtype, eb, sub=sub)
+    g0 = np.empty_like(f)
+    need = ctypes.c_int64(0)
+    lib().orc_dq_compress(ctypes.byref(p), _ptr(f), _ptr(g0), None, 0, ctypes.byref(need))
+    buf = np.zeros(max(need.value, 64), np.uint8)
+    rc = lib().orc_dq_compress(ctypes.byref(p), _ptr(f), _ptr(g0), _ptr(buf), buf.size,
+                               ctypes.byref(need))
+    assert rc == 0, rc
+    return g0, buf[:need.value]
+
+
+def dq_stream_shape(stream):
+    """(shape (nz, ny, nx), dtype, sub, abs_eb) from a stream's header."""
+    import struct
+    _, dt, nx, ny, nz, s0, s1, s2, nb, eb = struct.unpack_from("<IIqqqiiiId", bytes(stream[:56]))
+    return (nz, ny, nx), (np.float32 if dt == 0 else np.float64), (s0, s1, s2), eb
+
+
+def dq_decompress(stream):
+    stream = np.ascontiguousarray(stream, dtype=np.uint8)
+    shape, dt, _, _ = dq_stream_shape(stream)
+    g0 = np.empty(shape, dt)
+    rc = lib().orc_dq_decompress(_ptr(stream), stream.size, _ptr(g0))
+    assert rc == 0, rc
+    return g0
 
 
 def clamp(fv, eb, clamp_mode, dtype=np.float32):

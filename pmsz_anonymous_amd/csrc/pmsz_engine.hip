@@ -62,6 +62,7 @@
 #include "pmsz_edits.h"
 #include "pmsz_naive.h"
 #include "pmsz_codec.h"
+#include "pmsz_dqstream.h"
 
 namespace pmsz {
 
@@ -2586,14 +2587,13 @@ int64_t dq_block_bytes(const unsigned long long *h, size_t value_bytes) {
   return (int64_t)std::ceil(bits / 8.0);
 }
 
-template <typename T>
-int codec_dq(Ctx &c, const pmsz_params *p, const T *f, T *g0, uint64_t *hist_out,
-             pmsz_codec_info *info) {
-  const int64_t n = p->nx * p->ny * p->nz;
+// The rank blocks of the codec in rank order (row-major over {PX, PY, PZ}: z
+// fastest), each with its first histogram workgroup.
+void dq_blocks(const pmsz_params *p, std::vector<DqBlock> &bl, uint32_t &chunks) {
   const int px = p->sub[0], py = p->sub[1], pz = p->sub[2];
   const uint32_t nb = (uint32_t)(px * py * pz);
-  std::vector<DqBlock> bl;
-  uint32_t chunks = 0;
+  bl.clear();
+  chunks = 0;
   for (uint32_t r = 0; r < nb; r++) {
     const int cx = (int)r / (py * pz), cy = ((int)r / pz) % py, cz = (int)r % pz;
     int64_t l[3], o[3];
@@ -2612,6 +2612,18 @@ int codec_dq(Ctx &c, const pmsz_params *p, const T *f, T *g0, uint64_t *hist_out
     chunks += (uint32_t)((l[1] * l[2] + kDqRows - 1) / kDqRows);
     bl.push_back(B);
   }
+}
+
+// Quantisation (g0, and the quanta 2k + verbatim in c.tmark) and the residual
+// histograms of every rank block (h: blocks x kDqSlots, host).
+template <typename T>
+int dq_quant_hist(Ctx &c, const pmsz_params *p, const T *f, T *g0,
+                  std::vector<unsigned long long> &h) {
+  const int64_t n = p->nx * p->ny * p->nz;
+  std::vector<DqBlock> bl;
+  uint32_t chunks = 0;
+  dq_blocks(p, bl, chunks);
+  const uint32_t nb = (uint32_t)bl.size();
   const size_t o_hist = ((nb * sizeof(DqBlock)) + 255) & ~(size_t)255;
   const size_t hbytes = (size_t)nb * kDqSlots * sizeof(unsigned long long);
   int rc;
@@ -2635,20 +2647,411 @@ int codec_dq(Ctx &c, const pmsz_params *p, const T *f, T *g0, uint64_t *hist_out
   else
     k_dq_hist<<<chunks, 256, 0, c.stream>>>(kq, d_bl, nb, p->nx, p->nx * p->ny, d_hist);
   PMSZ_HIP(hipGetLastError());
-  std::vector<unsigned long long> h((size_t)nb * kDqSlots);
+  h.assign((size_t)nb * kDqSlots, 0);
   PMSZ_HIP(hipMemcpyAsync(h.data(), d_hist, hbytes, hipMemcpyDeviceToHost, c.stream));
   PMSZ_HIP(hipStreamSynchronize(c.stream));
-  if (hist_out) std::memcpy(hist_out, h.data(), hbytes);
-  if (info) {
-    std::memset(info, 0, sizeof(*info));
-    for (uint32_t b = 0; b < nb; b++) {
-      const unsigned long long *hb = h.data() + (size_t)b * kDqSlots;
-      info->compressed_bytes += dq_block_bytes(hb, sizeof(T));
-      info->escapes += (int64_t)hb[kDqBins];
-      info->verbatim += (int64_t)hb[kDqBins + 1];
+  return PMSZ_OK;
+}
+
+void dq_info(const std::vector<unsigned long long> &h, size_t value_bytes,
+             pmsz_codec_info *info) {
+  std::memset(info, 0, sizeof(*info));
+  for (size_t b = 0; b * kDqSlots < h.size(); b++) {
+    const unsigned long long *hb = h.data() + b * kDqSlots;
+    info->compressed_bytes += dq_block_bytes(hb, value_bytes);
+    info->escapes += (int64_t)hb[kDqBins];
+    info->verbatim += (int64_t)hb[kDqBins + 1];
+  }
+}
+
+template <typename T>
+int codec_dq(Ctx &c, const pmsz_params *p, const T *f, T *g0, uint64_t *hist_out,
+             pmsz_codec_info *info) {
+  std::vector<unsigned long long> h;
+  int rc = dq_quant_hist<T>(c, p, f, g0, h);
+  if (rc) return rc;
+  if (hist_out) std::memcpy(hist_out, h.data(), h.size() * sizeof(unsigned long long));
+  if (info) dq_info(h, sizeof(T), info);
+  return PMSZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The compressed stream (pmsz_dqstream.h).
+
+// Huffman code lengths for the counts of ns symbols (0: unused symbol), at most
+// kDqsMaxLen bits.  Leaves in (count, symbol) order; two queues, the leaf taken
+// on equal weights; a code longer than the limit halves every count (rounding
+// up) and starts over.  A single used symbol gets length 1.
+void dqs_lengths(const unsigned long long *counts, int ns, uint8_t *len) {
+  std::vector<unsigned long long> cnt(counts, counts + ns);
+  for (;;) {
+    std::memset(len, 0, (size_t)ns);
+    std::vector<int> sym;
+    for (int s = 0; s < ns; s++)
+      if (cnt[s]) sym.push_back(s);
+    if (sym.empty()) return;
+    if (sym.size() == 1) {
+      len[sym[0]] = 1;
+      return;
+    }
+    std::stable_sort(sym.begin(), sym.end(), [&](int a, int b) { return cnt[a] < cnt[b]; });
+    const int m = (int)sym.size();
+    std::vector<unsigned long long> w(2 * m - 1);
+    std::vector<int> parent(2 * m - 1, -1), depth(2 * m - 1, 0);
+    for (int k = 0; k < m; k++) w[k] = cnt[sym[k]];
+    int i = 0, j = m;
+    for (int k = m; k < 2 * m - 1; k++) {
+      int pick[2];
+      for (int t = 0; t < 2; t++) {
+        if (i < m && (j >= k || w[i] <= w[j])) pick[t] = i++;
+        else pick[t] = j++;
+      }
+      w[k] = w[pick[0]] + w[pick[1]];
+      parent[pick[0]] = parent[pick[1]] = k;
+    }
+    int maxd = 0;
+    for (int k = 2 * m - 3; k >= 0; k--) {
+      depth[k] = depth[parent[k]] + 1;
+      if (k < m) maxd = std::max(maxd, depth[k]);
+    }
+    if (maxd <= kDqsMaxLen) {
+      for (int k = 0; k < m; k++) len[sym[k]] = (uint8_t)depth[k];
+      return;
+    }
+    for (int s = 0; s < ns; s++)
+      if (cnt[s]) cnt[s] = (cnt[s] + 1) / 2;
+  }
+}
+
+// Canonical codes of the lengths (DEFLATE's assignment: codes of one length are
+// consecutive in symbol order), as entries (code reversed over its length) << 5
+// | length; 0 for an unused symbol.
+void dqs_codes(const uint8_t *len, int ns, uint32_t *entry) {
+  uint32_t bl[kDqsMaxLen + 2] = {}, next[kDqsMaxLen + 2] = {};
+  for (int s = 0; s < ns; s++)
+    if (len[s]) bl[len[s]]++;
+  uint32_t code = 0;
+  for (int b = 1; b <= kDqsMaxLen; b++) {
+    code = (code + bl[b - 1]) << 1;
+    next[b] = code;
+  }
+  for (int s = 0; s < ns; s++) {
+    entry[s] = 0;
+    if (!len[s]) continue;
+    const uint32_t cd = next[len[s]]++;
+    uint32_t rev = 0;
+    for (int k = 0; k < len[s]; k++) rev |= ((cd >> k) & 1u) << (len[s] - 1 - k);
+    entry[s] = (rev << 5) | len[s];
+  }
+}
+
+// Decoder tables of one block's code lengths (pmsz_dqstream.h).
+void dqs_dec_table(const uint8_t *len, int ns, DqsDecTable &t) {
+  std::memset(&t, 0, sizeof(t));
+  std::vector<uint32_t> entry((size_t)ns);
+  dqs_codes(len, ns, entry.data());
+  for (int s = 0; s < ns; s++)
+    if (len[s]) t.count[len[s]]++;
+  uint32_t code = 0, idx = 0;
+  for (int b = 1; b <= kDqsMaxLen; b++) {
+    code = (code + t.count[b - 1]) << 1;
+    t.first[b] = code;
+    t.index[b] = idx;
+    idx += t.count[b];
+  }
+  uint32_t fill[kDqsMaxLen + 1] = {};
+  for (int s = 0; s < ns; s++) {  // symbols of each length in symbol order
+    if (!len[s]) continue;
+    t.sorted[t.index[len[s]] + fill[len[s]]++] = (uint16_t)s;
+    if (len[s] <= kDqsFastBits) {
+      const uint32_t rev = entry[s] >> 5;
+      for (uint32_t i = rev; i < (1u << kDqsFastBits); i += 1u << len[s])
+        t.fast[i] = (uint32_t)s | ((uint32_t)len[s] << 16);
     }
   }
+}
+
+// Byte offsets of the parts of a stream.
+struct DqsLayout {
+  uint64_t blk, len, cwo, ceo, bits, esc, vidx, vval, end;
+};
+DqsLayout dqs_layout(uint32_t nb, uint64_t chunks, uint64_t words, uint64_t nesc,
+                     uint64_t nverb, size_t vsize) {
+  auto up8 = [](uint64_t x) { return (x + 7) & ~(uint64_t)7; };
+  DqsLayout L;
+  L.blk = sizeof(DqsHeader);
+  L.len = L.blk + (uint64_t)nb * sizeof(DqsBlock);
+  L.cwo = up8(L.len + (uint64_t)nb * kDqsSymbols);
+  L.ceo = L.cwo + 8 * (chunks + 1);
+  L.bits = L.ceo + 8 * chunks;
+  L.esc = L.bits + 4 * words;
+  L.vidx = L.esc + 4 * nesc;
+  L.vval = up8(L.vidx + 4 * nverb);
+  L.end = L.vval + vsize * nverb;
+  return L;
+}
+
+// Rank blocks of the stream with their chunks (and decoder workgroups).
+void dqs_geo(const pmsz_params *p, std::vector<DqsGeo> &geo, uint32_t &chunks, uint32_t &wgs) {
+  std::vector<DqBlock> bl;
+  uint32_t hc = 0;
+  dq_blocks(p, bl, hc);
+  geo.clear();
+  chunks = 0;
+  wgs = 0;
+  for (const DqBlock &B : bl) {
+    DqsGeo G;
+    std::memset(&G, 0, sizeof(G));
+    G.x0 = B.x0;
+    G.y0 = B.y0;
+    G.z0 = B.z0;
+    G.lx = B.lx;
+    G.ly = B.ly;
+    G.lz = B.lz;
+    G.voxels = (uint64_t)B.lx * B.ly * B.lz;
+    G.first_chunk = chunks;
+    G.nchunks = (uint32_t)((G.voxels + kDqsChunk - 1) / kDqsChunk);
+    G.wg0 = wgs;
+    chunks += G.nchunks;
+    wgs += (G.nchunks + kDqsDecThreads - 1) / kDqsDecThreads;
+    geo.push_back(G);
+  }
+}
+
+uint64_t dqs_bound(const pmsz_params *p) {
+  std::vector<DqsGeo> geo;
+  uint32_t chunks, wgs;
+  dqs_geo(p, geo, chunks, wgs);
+  const uint64_t n = (uint64_t)(p->nx * p->ny * p->nz);
+  return dqs_layout((uint32_t)geo.size(), chunks, (uint64_t)chunks * kDqsChunkWords, n, n,
+                    p->dtype == PMSZ_F32 ? 4 : 8)
+      .end;
+}
+
+template <typename T>
+int codec_dq_compress(Ctx &c, const pmsz_params *p, const T *f, T *g0, uint8_t *stream,
+                      int64_t capacity, int64_t *stream_bytes, pmsz_codec_info *info) {
+  std::vector<unsigned long long> h;
+  int rc = dq_quant_hist<T>(c, p, f, g0, h);
+  if (rc) return rc;
+  if (info) dq_info(h, sizeof(T), info);
+  std::vector<DqsGeo> geo;
+  uint32_t chunks, wgs;
+  dqs_geo(p, geo, chunks, wgs);
+  const uint32_t nb = (uint32_t)geo.size();
+  // code lengths and codes of every block (the escape symbol's count is the
+  // histogram's escape slot)
+  std::vector<uint8_t> lens((size_t)nb * kDqsSymbols);
+  std::vector<uint32_t> codes((size_t)nb * kDqsSymbols);
+  uint64_t nesc = 0, nverb = 0;
+  for (uint32_t b = 0; b < nb; b++) {
+    const unsigned long long *hb = h.data() + (size_t)b * kDqSlots;
+    dqs_lengths(hb, kDqsSymbols, &lens[(size_t)b * kDqsSymbols]);
+    dqs_codes(&lens[(size_t)b * kDqsSymbols], kDqsSymbols, &codes[(size_t)b * kDqsSymbols]);
+    geo[b].esc_offset = nesc;
+    geo[b].verb_offset = nverb;
+    nesc += hb[kDqBins];
+    nverb += hb[kDqBins + 1];
+  }
+  // scratch (c.wlist): geo | codes | chunk bits, escapes, verbatim | verbatim
+  // offsets | totals
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t o_codes = up(nb * sizeof(DqsGeo));
+  const size_t o_cb = o_codes + up((size_t)nb * kDqsSymbols * 4);
+  const size_t o_ce = o_cb + up((size_t)chunks * 4);
+  const size_t o_cv = o_ce + up((size_t)chunks * 4);
+  const size_t o_cvo = o_cv + up((size_t)chunks * 4);
+  const size_t o_tot = o_cvo + up((size_t)chunks * 8);
+  const size_t sbytes = o_tot + 256;
+  if ((rc = c.wlist.ensure((sbytes + 3) / 4))) return rc;
+  uint8_t *sc = (uint8_t *)c.wlist.p;
+  DqsGeo *d_geo = (DqsGeo *)sc;
+  uint32_t *d_codes = (uint32_t *)(sc + o_codes);
+  uint32_t *d_cb = (uint32_t *)(sc + o_cb), *d_ce = (uint32_t *)(sc + o_ce),
+           *d_cv = (uint32_t *)(sc + o_cv);
+  uint64_t *d_cvo = (uint64_t *)(sc + o_cvo), *d_tot = (uint64_t *)(sc + o_tot);
+  PMSZ_HIP(hipMemcpyAsync(d_geo, geo.data(), nb * sizeof(DqsGeo), hipMemcpyHostToDevice,
+                          c.stream));
+  PMSZ_HIP(hipMemcpyAsync(d_codes, codes.data(), codes.size() * 4, hipMemcpyHostToDevice,
+                          c.stream));
+  const DqsLayout L0 = dqs_layout(nb, chunks, 0, 0, 0, sizeof(T));
+  if ((int64_t)L0.bits > capacity) {
+    *stream_bytes = (int64_t)L0.bits;
+    PMSZ_HIP(hipStreamSynchronize(c.stream));
+    return PMSZ_ETOOBIG;
+  }
+  const int32_t *kq = (const int32_t *)c.tmark.p;
+  const int64_t nx = p->nx, sxy = p->nx * p->ny;
+  k_dqs_count<<<chunks, kDqsThreads, 0, c.stream>>>(kq, d_geo, nb, nx, sxy, d_codes, d_cb, d_ce,
+                                                   d_cv);
+  PMSZ_HIP(hipGetLastError());
+  uint64_t *cwoff = (uint64_t *)(stream + L0.cwo), *ceoff = (uint64_t *)(stream + L0.ceo);
+  k_dqs_scan<<<1, 1024, 0, c.stream>>>(d_cb, d_ce, d_cv, chunks, cwoff, ceoff, d_cvo, d_tot);
+  PMSZ_HIP(hipGetLastError());
+  uint64_t words = 0, tot[2] = {0, 0};
+  PMSZ_HIP(hipMemcpyAsync(&words, cwoff + chunks, 8, hipMemcpyDeviceToHost, c.stream));
+  PMSZ_HIP(hipMemcpyAsync(tot, d_tot, 16, hipMemcpyDeviceToHost, c.stream));
+  PMSZ_HIP(hipStreamSynchronize(c.stream));
+  if (tot[0] != nesc || tot[1] != nverb) return PMSZ_EDEVICE;  // histogram and count disagree
+  const DqsLayout L = dqs_layout(nb, chunks, words, nesc, nverb, sizeof(T));
+  *stream_bytes = (int64_t)L.end;
+  if ((int64_t)L.end > capacity) return PMSZ_ETOOBIG;
+  // header, block records and code lengths
+  std::vector<uint8_t> head(L.cwo, 0);
+  DqsHeader H;
+  std::memset(&H, 0, sizeof(H));
+  H.magic = kDqsMagic;
+  H.dtype = (uint32_t)p->dtype;
+  H.nx = p->nx;
+  H.ny = p->ny;
+  H.nz = p->nz;
+  for (int a = 0; a < 3; a++) H.sub[a] = p->sub[a];
+  H.nblocks = nb;
+  H.abs_eb = p->abs_eb;
+  H.chunk = kDqsChunk;
+  std::memcpy(head.data(), &H, sizeof(H));
+  for (uint32_t b = 0; b < nb; b++) {
+    DqsBlock R;
+    std::memset(&R, 0, sizeof(R));
+    const unsigned long long *hb = h.data() + (size_t)b * kDqSlots;
+    R.voxels = geo[b].voxels;
+    R.escapes = hb[kDqBins];
+    R.verbatim = hb[kDqBins + 1];
+    R.esc_offset = geo[b].esc_offset;
+    R.verb_offset = geo[b].verb_offset;
+    R.first_chunk = geo[b].first_chunk;
+    R.nchunks = geo[b].nchunks;
+    std::memcpy(head.data() + L.blk + (size_t)b * sizeof(DqsBlock), &R, sizeof(R));
+  }
+  std::memcpy(head.data() + L.len, lens.data(), lens.size());
+  PMSZ_HIP(hipMemcpyAsync(stream, head.data(), head.size(), hipMemcpyHostToDevice, c.stream));
+  k_dqs_encode<T><<<chunks, kDqsThreads, 0, c.stream>>>(
+      kq, g0, d_geo, nb, nx, sxy, d_codes, cwoff, ceoff, d_cvo, (uint32_t *)(stream + L.bits),
+      (uint32_t *)(stream + L.esc), (uint32_t *)(stream + L.vidx), (T *)(stream + L.vval));
+  PMSZ_HIP(hipGetLastError());
+  if (L.vval > L.vidx + 4 * nverb)  // the padding word
+    PMSZ_HIP(hipMemsetAsync(stream + L.vidx + 4 * nverb, 0, L.vval - (L.vidx + 4 * nverb),
+                            c.stream));
+  PMSZ_HIP(hipStreamSynchronize(c.stream));
   return PMSZ_OK;
+}
+
+// The header of a stream in device memory, checked against its size.
+int dqs_read_head(const uint8_t *stream, int64_t bytes, DqsHeader &H, std::vector<uint8_t> &head,
+                  hipStream_t st) {
+  if (bytes < (int64_t)sizeof(DqsHeader)) return PMSZ_EINVAL;
+  PMSZ_HIP(hipMemcpyAsync(&H, stream, sizeof(H), hipMemcpyDeviceToHost, st));
+  PMSZ_HIP(hipStreamSynchronize(st));
+  if (H.magic != kDqsMagic || H.chunk != (uint32_t)kDqsChunk || H.dtype > 1) return PMSZ_EINVAL;
+  if (H.nblocks < 1 || H.nblocks > kMaxDomains) return PMSZ_EINVAL;
+  const uint64_t need = sizeof(DqsHeader) + (uint64_t)H.nblocks * (sizeof(DqsBlock) + kDqsSymbols);
+  if ((uint64_t)bytes < need) return PMSZ_EINVAL;
+  head.resize(need);
+  PMSZ_HIP(hipMemcpyAsync(head.data(), stream, need, hipMemcpyDeviceToHost, st));
+  PMSZ_HIP(hipStreamSynchronize(st));
+  return PMSZ_OK;
+}
+
+pmsz_params dqs_params(const DqsHeader &H) {
+  pmsz_params q;
+  pmsz_default_params(&q);
+  q.dtype = (int)H.dtype;
+  q.nx = H.nx;
+  q.ny = H.ny;
+  q.nz = H.nz;
+  for (int a = 0; a < 3; a++) q.sub[a] = H.sub[a];
+  q.abs_eb = H.abs_eb;
+  return q;
+}
+
+template <typename T>
+int codec_dq_decompress(Ctx &c, const uint8_t *stream, int64_t bytes, const DqsHeader &H,
+                        const std::vector<uint8_t> &head, T *g0) {
+  const pmsz_params q = dqs_params(H);
+  int rc = check_params(&q);
+  if (rc) return rc;
+  std::vector<DqsGeo> geo;
+  uint32_t chunks, wgs;
+  dqs_geo(&q, geo, chunks, wgs);
+  const uint32_t nb = (uint32_t)geo.size();
+  if (nb != H.nblocks) return PMSZ_EINVAL;
+  uint64_t nesc = 0, nverb = 0;
+  std::vector<DqsDecTable> tabs(nb);
+  for (uint32_t b = 0; b < nb; b++) {
+    DqsBlock R;
+    std::memcpy(&R, head.data() + sizeof(DqsHeader) + (size_t)b * sizeof(DqsBlock), sizeof(R));
+    if (R.voxels != geo[b].voxels || R.first_chunk != geo[b].first_chunk ||
+        R.nchunks != geo[b].nchunks || R.esc_offset != nesc || R.verb_offset != nverb)
+      return PMSZ_EINVAL;
+    geo[b].esc_offset = nesc;
+    geo[b].verb_offset = nverb;
+    nesc += R.escapes;
+    nverb += R.verbatim;
+    const uint8_t *len = head.data() + sizeof(DqsHeader) + (size_t)nb * sizeof(DqsBlock) +
+                         (size_t)b * kDqsSymbols;
+    for (int s = 0; s < kDqsSymbols; s++)
+      if (len[s] > kDqsMaxLen) return PMSZ_EINVAL;
+    dqs_dec_table(len, kDqsSymbols, tabs[b]);
+  }
+  // the chunk offsets, checked before any kernel reads through them
+  const DqsLayout L0 = dqs_layout(nb, chunks, 0, 0, 0, sizeof(T));
+  if ((uint64_t)bytes < L0.bits) return PMSZ_EINVAL;
+  std::vector<uint64_t> cwo((size_t)chunks + 1), ceo((size_t)chunks);
+  PMSZ_HIP(hipMemcpyAsync(cwo.data(), stream + L0.cwo, 8 * (chunks + 1), hipMemcpyDeviceToHost,
+                          c.stream));
+  PMSZ_HIP(hipMemcpyAsync(ceo.data(), stream + L0.ceo, 8 * (size_t)chunks, hipMemcpyDeviceToHost,
+                          c.stream));
+  PMSZ_HIP(hipStreamSynchronize(c.stream));
+  if (cwo[0] != 0) return PMSZ_EINVAL;
+  for (uint32_t k = 0; k < chunks; k++)
+    if (cwo[k + 1] < cwo[k] || cwo[k + 1] - cwo[k] > (uint64_t)kDqsChunkWords ||
+        ceo[k] > nesc || (k && ceo[k] < ceo[k - 1]))
+      return PMSZ_EINVAL;
+  const uint64_t words = cwo[chunks];
+  const DqsLayout L = dqs_layout(nb, chunks, words, nesc, nverb, sizeof(T));
+  if ((uint64_t)bytes < L.end) return PMSZ_EINVAL;
+  // scratch: the residuals / quanta in c.tmark, tables and geo in c.wlist
+  const int64_t n = q.nx * q.ny * q.nz;
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t o_tab = up(nb * sizeof(DqsGeo));
+  const size_t o_err = o_tab + up(nb * sizeof(DqsDecTable));
+  if ((rc = c.tmark.ensure((size_t)n)) || (rc = c.wlist.ensure((o_err + 256) / 4))) return rc;
+  uint8_t *sc = (uint8_t *)c.wlist.p;
+  DqsGeo *d_geo = (DqsGeo *)sc;
+  DqsDecTable *d_tab = (DqsDecTable *)(sc + o_tab);
+  uint32_t *d_err = (uint32_t *)(sc + o_err);
+  uint32_t *kd = c.tmark.p;
+  PMSZ_HIP(hipMemcpyAsync(d_geo, geo.data(), nb * sizeof(DqsGeo), hipMemcpyHostToDevice,
+                          c.stream));
+  PMSZ_HIP(hipMemcpyAsync(d_tab, tabs.data(), nb * sizeof(DqsDecTable), hipMemcpyHostToDevice,
+                          c.stream));
+  PMSZ_HIP(hipMemsetAsync(d_err, 0, 4, c.stream));
+  const int64_t nx = q.nx, sxy = q.nx * q.ny;
+  k_dqs_decode<<<wgs, kDqsDecThreads, 0, c.stream>>>(
+      (const uint32_t *)(stream + L.bits), (const uint64_t *)(stream + L.cwo),
+      (const uint64_t *)(stream + L.ceo), (const uint32_t *)(stream + L.esc), nesc, d_geo, nb,
+      nx, sxy, d_tab, kd, d_err);
+  PMSZ_HIP(hipGetLastError());
+  for (uint32_t b = 0; b < nb; b++) {
+    const uint64_t rows = (uint64_t)geo[b].ly * geo[b].lz;
+    k_dqs_prefix_x<<<(unsigned)((rows + 3) / 4), 256, 0, c.stream>>>(kd, d_geo, b, nx, sxy);
+    const uint64_t ly = (uint64_t)geo[b].lx * geo[b].lz, lz = (uint64_t)geo[b].lx * geo[b].ly;
+    k_dqs_prefix_yz<<<(unsigned)((ly + 255) / 256), 256, 0, c.stream>>>(kd, d_geo, b, nx, sxy, 1);
+    k_dqs_prefix_yz<<<(unsigned)((lz + 255) / 256), 256, 0, c.stream>>>(kd, d_geo, b, nx, sxy, 2);
+  }
+  PMSZ_HIP(hipGetLastError());
+  const unsigned gv = (unsigned)std::min<int64_t>((n + 255) / 256, (int64_t)c.cus * 8);
+  k_dqs_values<T><<<gv, 256, 0, c.stream>>>(kd, g0, n, 2.0 * q.abs_eb);
+  if (nverb)
+    k_dqs_verbatim<T><<<(unsigned)std::min<uint64_t>((nverb + 255) / 256, 1024), 256, 0,
+                        c.stream>>>((const uint32_t *)(stream + L.vidx),
+                                    (const T *)(stream + L.vval), d_geo, nb, nverb, nx, sxy, g0,
+                                    d_err);
+  PMSZ_HIP(hipGetLastError());
+  uint32_t err = 0;
+  PMSZ_HIP(hipMemcpyAsync(&err, d_err, 4, hipMemcpyDeviceToHost, c.stream));
+  PMSZ_HIP(hipStreamSynchronize(c.stream));
+  return err ? PMSZ_EINVAL : PMSZ_OK;
 }
 
 // One field (whole or split into subdomains), by value type; the caller holds
@@ -3279,6 +3682,58 @@ int pmsz_codec_dq(const pmsz_params *p, const void *d_f, void *d_g0, uint64_t *h
   if (p->dtype == PMSZ_F32)
     return codec_dq<float>(c, p, (const float *)d_f, (float *)d_g0, hist, info);
   return codec_dq<double>(c, p, (const double *)d_f, (double *)d_g0, hist, info);
+}
+
+int pmsz_codec_dq_stream_bound(const pmsz_params *p, int64_t *bytes) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (!bytes || (uint32_t)(p->sub[0] * p->sub[1] * p->sub[2]) > kMaxDomains) return PMSZ_EINVAL;
+  *bytes = (int64_t)dqs_bound(p);
+  return PMSZ_OK;
+}
+
+int pmsz_codec_dq_compress(const pmsz_params *p, const void *d_f, void *d_g0, uint8_t *d_stream,
+                           int64_t capacity, int64_t *stream_bytes, pmsz_codec_info *info) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (!d_f || !d_g0 || !d_stream || !stream_bytes || ((uintptr_t)d_stream & 15) ||
+      (uint32_t)(p->sub[0] * p->sub[1] * p->sub[2]) > kMaxDomains)
+    return PMSZ_EINVAL;
+  Ctx &c = default_ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  if ((rc = ctx_device(c))) return rc;
+  if (p->dtype == PMSZ_F32)
+    return codec_dq_compress<float>(c, p, (const float *)d_f, (float *)d_g0, d_stream, capacity,
+                                    stream_bytes, info);
+  return codec_dq_compress<double>(c, p, (const double *)d_f, (double *)d_g0, d_stream, capacity,
+                                   stream_bytes, info);
+}
+
+int pmsz_codec_dq_stream_params(const uint8_t *d_stream, int64_t bytes, pmsz_params *p) {
+  if (!d_stream || !p) return PMSZ_EINVAL;
+  Ctx &c = default_ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  int rc = ctx_device(c);
+  if (rc) return rc;
+  DqsHeader H;
+  std::vector<uint8_t> head;
+  if ((rc = dqs_read_head(d_stream, bytes, H, head, c.stream))) return rc;
+  *p = dqs_params(H);
+  return check_params(p);
+}
+
+int pmsz_codec_dq_decompress(const uint8_t *d_stream, int64_t bytes, void *d_g0) {
+  if (!d_stream || !d_g0 || ((uintptr_t)d_stream & 15)) return PMSZ_EINVAL;
+  Ctx &c = default_ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  int rc = ctx_device(c);
+  if (rc) return rc;
+  DqsHeader H;
+  std::vector<uint8_t> head;
+  if ((rc = dqs_read_head(d_stream, bytes, H, head, c.stream))) return rc;
+  if (H.dtype == PMSZ_F32)
+    return codec_dq_decompress<float>(c, d_stream, bytes, H, head, (float *)d_g0);
+  return codec_dq_decompress<double>(c, d_stream, bytes, H, head, (double *)d_g0);
 }
 
 int pmsz_edit_packed_bytes(const pmsz_params *p, int64_t *bytes) {

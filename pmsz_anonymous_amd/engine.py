@@ -192,6 +192,11 @@ def lib():
         L.pmsz_codec_dq.argtypes = [P, vp, vp, vp, ctypes.POINTER(i64 * 3)]
         L.pmsz_encode_edits.argtypes = [P, vp, vp, vp, vp, vp, ctypes.POINTER(EditBlock),
                                         ctypes.POINTER(EditInfo)]
+        L.pmsz_codec_dq_stream_bound.argtypes = [P, ctypes.POINTER(i64)]
+        L.pmsz_codec_dq_compress.argtypes = [P, vp, vp, vp, i64, ctypes.POINTER(i64),
+                                             ctypes.POINTER(i64 * 3)]
+        L.pmsz_codec_dq_stream_params.argtypes = [vp, i64, P]
+        L.pmsz_codec_dq_decompress.argtypes = [vp, i64, vp]
         L.pmsz_encode_edits_noalloc.argtypes = [P, vp, vp, vp, vp, vp, ctypes.POINTER(EditBlock),
                                                 ctypes.POINTER(EditInfo), vp]
         L.pmsz_time_first_pass.argtypes = [P, vp, vp, i32, ctypes.POINTER(ctypes.c_double)]
@@ -529,6 +534,47 @@ def codec_dq(f, abs_eb, sub=(1, 1, 1)):
                              hist.ctypes.data_as(ctypes.c_void_p), ctypes.byref(info))
     _check(rc, "pmsz_codec_dq")
     return g0, hist, {"compressed_bytes": info[0], "escapes": info[1], "verbatim": info[2]}
+
+
+def codec_dq_compress(f, abs_eb, sub=(1, 1, 1)):
+    """The codec's compressed stream on the device (pmsz_codec_dq_compress) for a
+    CUDA tensor f.  Returns (g0 tensor, stream as a uint8 CUDA tensor, info dict
+    with compressed_bytes (the entropy estimate), escapes, verbatim)."""
+    import torch
+    p = make_params(f.shape, f.dtype, abs_eb, sub=sub)
+    bound = ctypes.c_int64(0)
+    _check(lib().pmsz_codec_dq_stream_bound(ctypes.byref(p), ctypes.byref(bound)),
+           "pmsz_codec_dq_stream_bound")
+    g0 = torch.empty_like(f)
+    stream = torch.empty(bound.value, dtype=torch.uint8, device=f.device)
+    nbytes = ctypes.c_int64(0)
+    info = (ctypes.c_int64 * 3)()
+    torch.cuda.current_stream().synchronize()
+    rc = lib().pmsz_codec_dq_compress(ctypes.byref(p), ctypes.c_void_p(f.contiguous().data_ptr()),
+                                      ctypes.c_void_p(g0.data_ptr()),
+                                      ctypes.c_void_p(stream.data_ptr()), bound.value,
+                                      ctypes.byref(nbytes), ctypes.byref(info))
+    _check(rc, "pmsz_codec_dq_compress")
+    return g0, stream[:nbytes.value], {"compressed_bytes": info[0], "escapes": info[1],
+                                       "verbatim": info[2]}
+
+
+def codec_dq_decompress(stream):
+    """g0 from a compressed stream held in a uint8 CUDA tensor
+    (pmsz_codec_dq_decompress)."""
+    import torch
+    stream = stream.contiguous()
+    p = Params()
+    rc = lib().pmsz_codec_dq_stream_params(ctypes.c_void_p(stream.data_ptr()), stream.numel(),
+                                           ctypes.byref(p))
+    _check(rc, "pmsz_codec_dq_stream_params")
+    dt = torch.float32 if p.dtype == 0 else torch.float64
+    g0 = torch.empty((p.nz, p.ny, p.nx), dtype=dt, device=stream.device)
+    torch.cuda.current_stream().synchronize()
+    rc = lib().pmsz_codec_dq_decompress(ctypes.c_void_p(stream.data_ptr()), stream.numel(),
+                                        ctypes.c_void_p(g0.data_ptr()))
+    _check(rc, "pmsz_codec_dq_decompress")
+    return g0
 
 
 def time_first_pass(f, g, abs_eb, iters=5, **kw):

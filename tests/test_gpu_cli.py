@@ -106,7 +106,13 @@ def test_cli_gpu_codec(tmp_path):
     assert rc == 0
     assert np.fromfile(out, np.float64).tobytes() == go.tobytes()
     m = re.search(r"compression time: \S+ s \(dq, (\d+) bytes\)", r.stdout)
-    assert m and int(m.group(1)) > 0
+    # the size is the real stream's: the oracle's stream of the same field
+    _, ostream = O.dq_compress(f, eb, (2, 1, 1))
+    assert m and int(m.group(1)) == ostream.size
+    assert "decoded back exactly" in r.stdout
+    assert f"compressed size: {ostream.size} bytes (stream)" in r.stdout
+    cr = float(re.search(r"original CR is: (\S+)", r.stdout).group(1))
+    assert abs(cr - f.nbytes / ostream.size) < 1e-9 * cr
 
 
 def test_cli_lazy_after_parent_cooperative_tail(tmp_path):
