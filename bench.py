@@ -51,26 +51,123 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(f, g0, eb, edge):
-    """The oracle (the reference algorithm restated on the host cores) on a
-    bounded sample: the first edge^3 block of the same field."""
+def cpu_info():
+    """The host's CPU model and the cores this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "affinity_cores": len(os.sched_getaffinity(0))}
+
+
+def _omp_threads(n):
+    """Sets the OpenMP thread count of the oracle (libgomp) for later calls."""
+    import ctypes
+    try:
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+    except OSError:
+        pass
+
+
+def cpu_baseline(f, g0, eb, single_edge=128):
+    """The oracle (the reference algorithm restated on the host cores) on the
+    whole field with every core the process may use, and on one core over a
+    bounded sample (the first single_edge^3 block of the same field)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     pyoracle.lib()
-    fs = np.ascontiguousarray(f[:edge, :edge, :edge])
-    gs = np.ascontiguousarray(g0[:edge, :edge, :edge])
-    t0 = time.perf_counter()
-    _, _, st, _, rc = pyoracle.correct(fs, gs, eb)
-    dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    _omp_threads(cores)
+    t0 = time.perf_counter()
+    _, _, st, _, rc = pyoracle.correct(f, g0, eb)
+    dt = time.perf_counter() - t0
+    e = min(single_edge, *f.shape)
+    fs = np.ascontiguousarray(f[:e, :e, :e])
+    gs = np.ascontiguousarray(g0[:e, :e, :e])
+    _omp_threads(1)
+    t1 = time.perf_counter()
+    _, _, st1, _, rc1 = pyoracle.correct(fs, gs, eb)
+    dt1 = time.perf_counter() - t1
+    _omp_threads(cores)
     return {
-        "value": fs.size / dt / 1e9,
+        "value": f.size / dt / 1e9,
         "unit": "GVoxels/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"first {edge}^3 block of the same field and eb, oracle correction, "
-                  f"{dt:.2f} s, {st['edit_sweeps']} sweeps, rc={rc}",
+        "sample": f"the whole {'x'.join(map(str, f.shape[::-1]))} field and eb, oracle "
+                  f"correction on {cores} threads, {dt:.2f} s, {st['edit_sweeps']} sweeps, "
+                  f"rc={rc}",
+        "single_core": {"value": fs.size / dt1 / 1e9, "unit": "GVoxels/s", "cores": 1,
+                        "sample": f"first {e}^3 block, {dt1:.2f} s, {st1['edit_sweeps']} "
+                                  f"sweeps, rc={rc1}"},
+        **cpu_info(),
     }
+
+
+def c1_cpu():
+    """C1 (64^3 f32) on the oracle, every core and one core, with the GPU beside."""
+    import torch
+    from pmsz_anonymous_amd import engine, workloads
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    f, g0, eb = workloads.c1_case()
+    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    out = {"workload": "64^3 f32 Perlin (seed 0, freq 4, 4 octaves), SZ-like, eb 1e-3 x range"}
+    for name, n in (("all_cores", cores), ("one_core", 1)):
+        _omp_threads(n)
+        ts = []
+        for _ in range(3):
+            t = time.perf_counter()
+            go, _, _, _, _ = pyoracle.correct(f, g0, eb)
+            ts.append(time.perf_counter() - t)
+        out[name] = {"ms": float(np.median(ts)) * 1e3, "cores": n,
+                     "gvoxels_per_s": f.size / float(np.median(ts)) / 1e9}
+    _omp_threads(cores)
+    tf, tg0 = torch.from_numpy(f).cuda(), torch.from_numpy(g0).cuda()
+    engine.correct(tf, tg0.clone(), eb)
+    ts = []
+    for _ in range(10):
+        g = tg0.clone()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        engine.correct(tf, g, eb)
+        ts.append(time.perf_counter() - t)
+    out["gpu"] = {"ms": float(np.median(ts)) * 1e3,
+                  "gvoxels_per_s": f.size / float(np.median(ts)) / 1e9,
+                  "identical_to_oracle": g.cpu().numpy().tobytes() == go.tobytes()}
+    return out
+
+
+def f64_single(steps=5):
+    """The reference's own value type at the one-rank 512^3 shape: the same
+    Perlin field in float64, SZ-like codec, eb = 1e-3 x range."""
+    import torch
+    from pmsz_anonymous_amd import engine, fieldio
+    f, g0, eb = fieldio.perlin_case((512, 512, 512), 1, 8.0, 5, 1e-3, "sz3", np.float64)
+    tf, tg0 = torch.from_numpy(f).cuda(), torch.from_numpy(g0).cuda()
+    del f, g0
+    engine.correct(tf, tg0.clone(), eb)
+    copies = [tg0.clone() for _ in range(steps)]
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for g in copies:
+        _, st = engine.correct(tf, g, eb)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) / steps * 1e3
+    ok = torch.equal(engine.codes(tf), engine.codes(copies[-1]))
+    det = tf.numel() * (8 + 1)
+    return {"workload": "512^3 f64 Perlin (seed 1, freq 8, 5 octaves), SZ-like, eb 1e-3 x range",
+            "ms_per_step": ms, "value": tf.numel() / ms / 1e6, "unit": "GVoxels/s",
+            "edit_sweeps": st["edit_sweeps"], "topology_preserved": bool(ok),
+            "detection1": {"kernel_ms": st["t_dense_s"] * 1e3, "bytes_per_launch": det,
+                           "achieved": det / st["t_dense_s"] / 1e9,
+                           "frac": det / st["t_dense_s"] / 1e9 / HBM_PEAK_GBPS}}
 
 
 def pmc_summary(nvox, key):
@@ -398,7 +495,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=512)
-    ap.add_argument("--cpu-edge", type=int, default=256)
+    ap.add_argument("--cpu-edge", type=int, default=128,
+                    help="edge of the one-core oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-field batched run")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
@@ -412,6 +510,8 @@ def main():
                     help="skip the naive-MSz baseline run on the same field")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuned repeat of the headline workload")
+    ap.add_argument("--no-f64", action="store_true",
+                    help="skip the 512^3 float64 single-field run")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the multi-GPU path is a design "
@@ -555,8 +655,13 @@ def main():
     if not args.no_subdomains:
         del copies
         res["subdomains"] = subdomains_lazy_vs_sync()
+    if not args.no_f64:
+        res["f64"] = f64_single()
+        log("f64 single field done")
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(f, g0, eb, min(args.cpu_edge, n))
+        res["c1"] = c1_cpu()
+        log("cpu baseline done")
     print(json.dumps(res), flush=True)
 
 
