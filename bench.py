@@ -41,6 +41,10 @@ LABEL_BYTES_PER_VOXEL = 4 + 1 + 1
 # detection 1, each scaled by what the counter reads for a device copy of known
 # size.
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_columns_detect_512cube_f32_pmc.md")
+# Committed rocprofv3 --kernel-trace summary of one correction of the same
+# workload: "sparse_us" is the time of its apply, work-list detection and tail
+# launches.
+KSTATS_SUMMARY = os.path.join(ROOT, "profiles", "correct_512cube_f32_kernel_stats.md")
 # VALU issue peak: one wave64 VALU instruction per CU and clock, 256 CUs at the
 # 2.4 GHz peak clock (G wave instructions/s).  The same summary holds the
 # kernel's SQ_INSTS_VALU per launch ("valu_instructions_per_launch").
@@ -170,19 +174,19 @@ def f64_single(steps=5):
                            "frac": det / st["t_dense_s"] / 1e9 / HBM_PEAK_GBPS}}
 
 
-def pmc_summary(nvox, key):
-    """One per-launch figure of detection 1 from the committed --pmc summary
-    (None when the summary is absent or was taken on another size)."""
+def pmc_summary(nvox, key, path=PMC_SUMMARY):
+    """One figure from a committed profile summary's "- key = value" lines (None
+    when the summary is absent or was taken on another size)."""
     try:
-        with open(PMC_SUMMARY) as fh:
+        with open(path) as fh:
             kv = {}
             for line in fh:
                 if "=" in line and line.lstrip().startswith("- "):
                     k, v = line.lstrip()[2:].split("=", 1)
                     kv[k.strip()] = v.strip().split()[0]
-        if int(kv.get("voxels", -1)) != nvox:
+        if path == PMC_SUMMARY and int(kv.get("voxels", -1)) != nvox:
             return None
-        return int(kv[key])
+        return float(kv[key])
     except (OSError, KeyError, ValueError):
         return None
 
@@ -236,6 +240,21 @@ def batched_vs_plain(reps=3):
         same = same and all(torch.equal(a, b) for a, b in zip(gt, gb))
     engine.set_tuning({})
     tt = float(np.median(tuned_t))
+    # the same batch replayed as captured HIP graphs (one graph per chain of 8
+    # sweeps, captured on the first run of this layout)
+    engine.set_tuning({"graphs": 1})
+    engine.correct_batch(fs, [g.clone() for g in g0s], ebs)
+    graph_t = []
+    for _ in range(reps):
+        gg = [g.clone() for g in g0s]
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        engine.correct_batch(fs, gg, ebs)
+        torch.cuda.synchronize()
+        graph_t.append(time.perf_counter() - t)
+        same = same and all(torch.equal(a, b) for a, b in zip(gg, gb))
+    engine.set_tuning({})
+    tg = float(np.median(graph_t))
     return {
         "workload": "64 f32 Perlin fields, edges log-uniform [32,256] per axis (rng 0), seeds "
                     "100-163, eb log-uniform [1e-4,1e-2] x range, SZ-like / ZFP by seed parity",
@@ -247,6 +266,8 @@ def batched_vs_plain(reps=3):
         "plain_ms": tp * 1e3,
         "autotuned": {"tuning": tuning, "batched_ms": tt * 1e3,
                       "batched_gvox_s": nvox / tt / 1e9, "speedup_vs_plain": tp / tt},
+        "graph_replay": {"batched_ms": tg * 1e3, "batched_gvox_s": nvox / tg / 1e9,
+                         "speedup_vs_plain": tp / tg},
         "identical": bool(same),
         "sweeps_max": int(max(s["edit_sweeps"] for s in sts)),
         "codecs": {c: sum(1 for *_, cc in cases if cc == c) for c in ("sz3", "zfp")},
@@ -558,6 +579,8 @@ def main():
     label_gbs = nvox * LABEL_BYTES_PER_VOXEL / label_s / 1e9
     traffic = pmc_summary(nvox, "traffic_bytes_per_launch")
     valu = pmc_summary(nvox, "valu_instructions_per_launch")
+    traffic = None if traffic is None else int(traffic)
+    valu = None if valu is None else int(valu)
     # full check of one result against the original's codes (topology preserved)
     ok = torch.equal(engine.codes(tf), engine.codes(copies[-1]))
     res = {
@@ -589,6 +612,9 @@ def main():
         "roofline": {
             "kernel": "k_columns<float,false,true> (full-volume detection sweep, detection 1)",
             "bound": "hbm",
+            # the HBM roof is the reference line; what holds this kernel back is
+            # VALU issue (see "valu")
+            "limited_by": "valu_issue",
             "achieved": achieved,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
@@ -619,10 +645,30 @@ def main():
     detections = s0["edit_sweeps"] + 1
     e2e_bytes = nvox * (s_bytes + 1) * (1 + detections) + s0["total_targets"] * (3 * s_bytes + 2)
     e2e = e2e_bytes / (elapsed / args.steps) / 1e9
+    ctraffic = pmc_summary(nvox, "correction_traffic_bytes")
     res["end_to_end"] = {
         "definition": "(N*(s+1)*(1+detections) + edits*(3s+2)) / time per correction",
         "detections": detections, "algorithmic_bytes": e2e_bytes,
-        "achieved": e2e, "unit": "GB/s", "frac": e2e / HBM_PEAK_GBPS}
+        "achieved": e2e, "unit": "GB/s", "frac": e2e / HBM_PEAK_GBPS,
+        # what the kernels of one correction move (counters, committed profile)
+        "traffic_bytes": ctraffic,
+        "traffic_gbs": None if ctraffic is None else ctraffic / (elapsed / args.steps) / 1e9,
+        "traffic_frac": None if ctraffic is None else
+        ctraffic / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBPS}
+    # the sparse phase: every apply, work-list detection and tail launch; its
+    # algorithmic bytes are 3s+2 per edit (read f, g, dc; write g, dc) and s+1 per
+    # re-examined voxel (read g and the code)
+    sp_us = pmc_summary(nvox, "sparse_us", KSTATS_SUMMARY)
+    sp_traffic = pmc_summary(nvox, "sparse_traffic_bytes")
+    sp_alg = s0["total_targets"] * (3 * s_bytes + 2) + s0["sparse_voxels"] * (s_bytes + 1)
+    res["roofline"]["sparse"] = {
+        "kernels": "k_apply + k_detect_list + k_tail of one correction",
+        "algorithmic_bytes": sp_alg,
+        "kernel_us": sp_us, "time_source": os.path.relpath(KSTATS_SUMMARY, ROOT),
+        "achieved": None if not sp_us else sp_alg / sp_us / 1e3,
+        "frac": None if not sp_us else sp_alg / sp_us / 1e3 / HBM_PEAK_GBPS,
+        "traffic": sp_traffic, "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT),
+        "traffic_over_algorithmic": None if not sp_traffic else sp_traffic / sp_alg}
     if not args.no_edits:
         res["edit_channel"] = edit_channel(tf, tg0, copies[-1], eb)
         log("edit channel done")
