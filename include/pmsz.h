@@ -201,7 +201,7 @@ typedef struct {
                         as a captured HIP graph, kept by the context per job layout (0) */
   int stream_div;    /* one field: the targets of a full-volume detection with at least
                         voxels / stream_div violations are edited by one pass over the target
-                        byte map instead of a target list (16; 0: never) */
+                        byte map instead of a target list (32; 0: never) */
 } pmsz_tuning;
 void pmsz_default_tuning(pmsz_tuning *t);
 /* ctx NULL: the default context. */

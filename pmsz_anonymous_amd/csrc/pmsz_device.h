@@ -122,6 +122,9 @@ struct Counters {
 };
 static_assert(sizeof(Counters) == 320, "Counters layout");
 
+// Slot 3 of each stripe: the targets of a full-volume detection that the
+// streamed apply edited (no list counted them); not a violation counter.
+constexpr int kCntStreamed = 3;
 __device__ __forceinline__ void cnt_add(Counters &c, int q, uint32_t v) {
   if (v) atomicAdd(&c.cs[blockIdx.x % kStripes][q], v);
 }

@@ -54,6 +54,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/pmsz.h"
@@ -499,90 +500,211 @@ __device__ __forceinline__ int apply_target(uint32_t t, const Dom &d0, const Dom
 
 // ---------------------------------------------------------------------------
 // Edits of the targets of a full-volume detection, streamed from the target
-// byte map (one domain): a thread takes 16 cells at a time, loads their tags,
-// and where any cell carries the sweep's tag loads those cells' f, g and dc as
-// vectors (by halves of 8 cells), edits the tagged cells and stores the vectors
-// back.  The 16 cells belong to that thread
-// alone, so the stores race with nothing.  Counts the targets (for the
-// statistics) into cj.ntargets.
+// byte map (one domain).
+// * A lane takes 4 consecutive cells (a lane unit: one tag word, one delta-
+//   counter word, 4 values of f and g), four neighbouring lanes a 16-cell group,
+//   so every load of a wave is contiguous.
+// * A thread takes 8 units per step; their tag words are loaded a step ahead.
+// * Their data come in rounds of two units.  A group holding any tagged cell is
+//   loaded and stored whole by its four lanes, so its lines are written whole;
+//   other groups cost their tag word only, and a round in which no group of the
+//   wave has a target costs a few scalar instructions.
+// * Within a round the tagged cells are queued in LDS (values, counters and the
+//   queue of cell slots) and edited one per lane, so the edit's instructions run
+//   once per 64 targets rather than once per cell slot of every lane.
+// Counts the targets (for the statistics) into cj's kCntStreamed slot.
+constexpr int kStreamRC = 8;  // cells per lane and round (two lane units)
+template <typename T>
+struct StreamLds {
+  T f[kApplyThreads / 64][64 * kStreamRC];  // [wave][lane * kStreamRC + slot]
+  T g[kApplyThreads / 64][64 * kStreamRC];
+  uint8_t c[kApplyThreads / 64][64 * kStreamRC];
+  uint16_t q[kApplyThreads / 64][64 * kStreamRC];  // queued cell slots
+  uint32_t red[3][kApplyThreads / 64];
+};
+
+__device__ __forceinline__ uint32_t quad_or(uint32_t v) {
+  // OR over the 4 lanes of a quad (DPP quad_perm [1,0,3,2], then [2,3,0,1])
+  v |= (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xb1, 0xf, 0xf, false);
+  v |= (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4e, 0xf, 0xf, false);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t b) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+}
+
 template <typename T>
 __device__ __forceinline__ void apply_stream(const Dom &d0, const uint8_t *__restrict__ tbyte,
                                              uint32_t ngroups, uint32_t tag, Counters &cj,
-                                             RunState *run) {
-  constexpr int kVec = 8 * (int)sizeof(T) / 16;  // uint4 vectors per 8 cells of T
+                                             RunState *run, StreamLds<T> &sl) {
+  constexpr int kVec = 4 * (int)sizeof(T) / 16;  // 16-byte vectors per 4 cells of T
+  constexpr int kTU = 8;                          // lane units per thread and step
+  // clang vector types (first-class values, unlike uint4), so a round's data
+  // stay in registers
+  using v4u = uint32_t __attribute__((ext_vector_type(4)));
   T *g = (T *)d0.g;
   const T *f = (const T *)d0.f;
   uint8_t *dc = d0.dc;
   const uint64_t n = (uint64_t)d0.d.nx * (uint64_t)d0.d.ny * (uint64_t)d0.d.nz;
-  const uint64_t nfull = n / 16;  // groups whose 16 cells all lie in the field
+  const uint32_t nunits = ngroups * 4u;               // tag words of the map
+  const uint32_t nfull = (uint32_t)(n / 16u) * 4u;    // units of groups wholly in the field
   const uint32_t rep = tag * 0x01010101u;
-  const uint4 *t16 = reinterpret_cast<const uint4 *>(tbyte);
+  const uint32_t *t32 = reinterpret_cast<const uint32_t *>(tbyte);
+  uint32_t *dc32 = reinterpret_cast<uint32_t *>(dc);
+  const v4u *f16 = reinterpret_cast<const v4u *>(f);
+  v4u *g16 = reinterpret_cast<v4u *>(g);
+  const uint32_t lane = lane_id(), w = threadIdx.x / 64;
+  T *sf = sl.f[w];
+  T *sg = sl.g[w];
+  uint8_t *sc = sl.c[w];
+  uint16_t *sq = sl.q[w];
+  v4u *sf16 = reinterpret_cast<v4u *>(sf + lane * kStreamRC);
+  v4u *sg16 = reinterpret_cast<v4u *>(sg + lane * kStreamRC);
+  uint2 *sc8 = reinterpret_cast<uint2 *>(sc + lane * kStreamRC);
   uint32_t touched = 0, clamped = 0, count = 0;
-  const uint32_t stride = gridDim.x * kApplyThreads;
-  // the next group's tags are loaded while this group is edited
-  uint32_t gi = blockIdx.x * kApplyThreads + threadIdx.x;
-  uint4 tnext = gi < ngroups ? t16[gi] : make_uint4(0, 0, 0, 0);
-  for (; gi < ngroups; gi += stride) {
-    const uint4 tv = tnext;
-    if (gi + stride < ngroups) tnext = t16[gi + stride];
-    const uint32_t m = byte_bits(tag_bytes(tv.x, rep)) | (byte_bits(tag_bytes(tv.y, rep)) << 4) |
-                       (byte_bits(tag_bytes(tv.z, rep)) << 8) |
-                       (byte_bits(tag_bytes(tv.w, rep)) << 12);
-    if (!m) continue;
-    count += __popc(m);
-    const uint64_t c0 = (uint64_t)gi * 16u;
-    if (gi < nfull) {
-      const uint4 cv = reinterpret_cast<const uint4 *>(dc)[gi];
-      uint32_t cw[4] = {cv.x, cv.y, cv.z, cv.w};
-      // both halves are loaded and stored even without targets: the group's
-      // lines are then written whole
+  const uint32_t per_block = (uint32_t)kTU * kApplyThreads;
+  const uint32_t stride = gridDim.x * per_block;
+  uint32_t base = blockIdx.x * per_block;  // uniform in the block
+  uint32_t tw[kTU];
 #pragma unroll
-      for (int hf = 0; hf < 2; hf++) {
-        const uint32_t mh = (m >> (8 * hf)) & 0xffu;
-        union {
-          uint4 v[kVec];
-          T x[8];
-        } fv, gv;
-        const uint4 *fp = reinterpret_cast<const uint4 *>(f + c0 + 8 * hf);
-        uint4 *gp = reinterpret_cast<uint4 *>(g + c0 + 8 * hf);
+  for (int q = 0; q < kTU; q++) {
+    const uint32_t u = base + (uint32_t)q * kApplyThreads + threadIdx.x;
+    tw[q] = u < nunits ? t32[u] : 0u;
+  }
+  for (; base < nunits; base += stride) {
+    // the step's tagged cells, 4 bits per unit, and the same ORed over each quad
+    uint32_t mp = 0;
+#pragma unroll
+    for (int q = 0; q < kTU; q++) mp |= byte_bits(tag_bytes(tw[q], rep)) << (4 * q);
+#pragma unroll
+    for (int q = 0; q < kTU; q++) {  // the next step's tag words
+      const uint32_t u = base + stride + (uint32_t)q * kApplyThreads + threadIdx.x;
+      tw[q] = u < nunits ? t32[u] : 0u;
+    }
+    const uint32_t gp = quad_or(mp);
+    for (int r = 0; r < kTU; r += 2) {
+      const uint32_t gr = (gp >> (4 * r)) & 0xffu;  // this round's two units, by group
+      if (__ballot(gr != 0) == 0) continue;
+      const uint32_t u0 = base + (uint32_t)r * kApplyThreads + threadIdx.x;
+      const uint32_t u1 = u0 + kApplyThreads;
+      const bool full0 = u0 < nfull, full1 = u1 < nfull;
+      const bool ld0 = (gr & 15u) && full0, ld1 = (gr >> 4) && full1;
+      uint32_t mr = (mp >> (4 * r)) & 0xffu;  // this lane's tagged cells
+      // the field's last, partial group: cell by cell (tags lie on cells only)
+      const uint32_t mpart = (full0 ? 0u : mr & 15u) | (full1 ? 0u : mr & 0xf0u);
+      mr &= ~mpart;
+      count += __popc(mr | mpart);
+      for (uint32_t mm = mpart; mm; mm &= mm - 1) {
+        const uint32_t k = (uint32_t)(__ffs(mm) - 1);
+        apply_one<T>(g, f, dc, (k < 4 ? u0 : u1) * 4u + (k & 3u), d0.ep, touched, clamped);
+      }
+      v4u fa[kVec], fb[kVec], ga[kVec], gb[kVec];
+      uint32_t ca = 0, cb = 0;
+#pragma unroll
+      for (int k = 0; k < kVec; k++) {
+        fa[k] = fb[k] = ga[k] = gb[k] = v4u{0u, 0u, 0u, 0u};
+        if (ld0) {
+          fa[k] = f16[(size_t)u0 * kVec + k];
+          ga[k] = g16[(size_t)u0 * kVec + k];
+        }
+        if (ld1) {
+          fb[k] = f16[(size_t)u1 * kVec + k];
+          gb[k] = g16[(size_t)u1 * kVec + k];
+        }
+      }
+      if (ld0) ca = dc32[u0];
+      if (ld1) cb = dc32[u1];
+      // queue the tagged cells: slot k of lane L is entry L * 8 + k
+      if (mr) {
 #pragma unroll
         for (int k = 0; k < kVec; k++) {
-          fv.v[k] = fp[k];
-          gv.v[k] = gp[k];
+          sf16[k] = fa[k];
+          sf16[kVec + k] = fb[k];
+          sg16[k] = ga[k];
+          sg16[kVec + k] = gb[k];
         }
+        *sc8 = make_uint2(ca, cb);
+      }
+      uint32_t nq = 0;
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-          if ((mh >> b) & 1u) {
-            const int w = 2 * hf + (b >> 2);
-            const uint32_t sh = 8u * (uint32_t)(b & 3);
-            const uint8_t c0b = (uint8_t)(cw[w] >> sh);
-            uint8_t c = c0b;
-            edit_value<T>(fv.x[b], gv.x[b], c, d0.ep);
-            cw[w] = (cw[w] & ~(0xffu << sh)) | ((uint32_t)c << sh);
-            touched += c0b == 0;
-            clamped += (c == d0.ep.q && c0b != d0.ep.q);
-          }
+      for (int k = 0; k < kStreamRC; k++) {
+        const bool on = (mr >> k) & 1u;
+        const uint64_t bk = __ballot(on);
+        if (on) sq[nq + lanes_below(bk)] = (uint16_t)(lane * kStreamRC + (uint32_t)k);
+        nq += (uint32_t)__popcll(bk);
+      }
+      wave_lds_sync();
+      for (uint32_t e = lane; e < nq; e += 64) {
+        const uint32_t idx = sq[e];
+        T gv = sg[idx];
+        const uint8_t c0 = sc[idx];
+        uint8_t c = c0;
+        edit_value<T>(sf[idx], gv, c, d0.ep);
+        sg[idx] = gv;
+        sc[idx] = c;
+        touched += c0 == 0;
+        clamped += (c == d0.ep.q && c0 != d0.ep.q);
+      }
+      wave_lds_sync();
+      if (mr) {
+#pragma unroll
+        for (int k = 0; k < kVec; k++) {
+          ga[k] = sg16[k];
+          gb[k] = sg16[kVec + k];
         }
+        const uint2 cc = *sc8;
+        ca = cc.x;
+        cb = cc.y;
+      }
+      if (ld0) {
 #pragma unroll
-        for (int k = 0; k < kVec; k++) gp[k] = gv.v[k];
+        for (int k = 0; k < kVec; k++) g16[(size_t)u0 * kVec + k] = ga[k];
+        dc32[u0] = ca;
       }
-      reinterpret_cast<uint4 *>(dc)[gi] = make_uint4(cw[0], cw[1], cw[2], cw[3]);
-    } else {
-      // the field's last, partial group: cell by cell (tags lie on cells only)
-      for (uint32_t mm = m; mm; mm &= mm - 1) {
-        const uint32_t t = (uint32_t)(c0 + (uint32_t)(__ffs(mm) - 1));
-        apply_one<T>(g, f, dc, t, d0.ep, touched, clamped);
+      if (ld1) {
+#pragma unroll
+        for (int k = 0; k < kVec; k++) g16[(size_t)u1 * kVec + k] = gb[k];
+        dc32[u1] = cb;
       }
+      wave_lds_sync();  // the slots and the queue are rewritten by the next round
     }
   }
+  // one set of atomics per block: the target count goes to the counters' fourth
+  // stripe slot (same-address atomics serialise; one per wave cost ~80 us)
   const uint32_t ts = wave_sum(touched), cs = wave_sum(clamped), ns = wave_sum(count);
-  if (lane_id() == 0) {
-    acc_add(run, kAccTouched, ts);
-    acc_add(run, kAccClamped, cs);
-    acc_add(run, kAccTargets, ns);
-    if (ns) atomicAdd(&cj.ntargets, ns);
+  if (lane == 0) {
+    sl.red[0][w] = ts;
+    sl.red[1][w] = cs;
+    sl.red[2][w] = ns;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t a = 0, b = 0, c = 0;
+#pragma unroll
+    for (int k = 0; k < kApplyThreads / 64; k++) {
+      a += sl.red[0][k];
+      b += sl.red[1][k];
+      c += sl.red[2][k];
+    }
+    acc_add(run, kAccTouched, a);
+    acc_add(run, kAccClamped, b);
+    acc_add(run, kAccTargets, c);
+    cnt_add(cj, kCntStreamed, c);
   }
 }
+
+// LDS of the apply kernel: the list staging or (one domain) the streamed
+// apply's queues.
+template <typename T>
+union ApplyLds {
+  StageLds<kApplyThreads, kApplyThreads * kCand> st;
+  StreamLds<T> sl;
+};
+struct ApplyLdsMulti {
+  StageLds<kApplyThreads, kApplyThreads * kCand> st;
+};
 
 // ---------------------------------------------------------------------------
 // Apply of sweep j (grid-stride over its targets), queueing the next work list
@@ -619,10 +741,13 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
   }
   if (mode == kModeTail) return;
   if (!kMulti && mode == kModeDone && (done == 1 || s.mode == PMSZ_SYNC)) return;
-  if (stream) {
-    if (lead && mode != kModeDone) zero_counters(&ring[(j + 1) % kCtrRing]);
-    apply_stream<T>(d0, tbyte, ngroups, detect_tag(j), cj, run);
-    return;
+  __shared__ std::conditional_t<kMulti, ApplyLdsMulti, ApplyLds<T>> lds;
+  if constexpr (!kMulti) {
+    if (stream) {
+      if (lead && mode != kModeDone) zero_counters(&ring[(j + 1) % kCtrRing]);
+      apply_stream<T>(d0, tbyte, ngroups, detect_tag(j), cj, run, lds.sl);
+      return;
+    }
   }
   const uint32_t nt = cj.ntargets;
   if (kMulti && mode == kModeDone && nt == 0) return;
@@ -634,7 +759,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
   // tmark; every set bit is a target of this sweep, so their words are cleared with
   // plain stores
   const bool clear = (!kMulti || s.append) && pm == kModeSparse;
-  __shared__ StageLds<kApplyThreads, kApplyThreads * kCand> st;
+  auto &st = lds.st;
   if (!kMulti && queue) {
     // one domain, work-list detection next: the edit, then the neighbourhood's
     // claims in wmark, listed at once (staged per block)
@@ -1415,7 +1540,7 @@ pmsz_tuning default_tuning() {
   t.solo_targets = 64;
   t.zchunk = kColZ;
   t.graphs = 0;
-  t.stream_div = 16;
+  t.stream_div = 32;
   return t;
 }
 
@@ -1705,8 +1830,10 @@ int chain_graph(Ctx &c, const std::vector<uint64_t> &key, uint32_t j0, F &&enque
 
 template <typename T, bool kMulti>
 int drive(Ctx &c, Job &job, Outcome &out) {
-  // the streamed apply is for one domain whose f and g take 16-byte vector accesses
-  if (kMulti || (((uintptr_t)job.doms[0].f | (uintptr_t)job.doms[0].g) & 15u)) job.s.stream_min = 0;
+  // the streamed apply is for one domain whose f, g and dc take 16-byte vector accesses
+  if (kMulti ||
+      (((uintptr_t)job.doms[0].f | (uintptr_t)job.doms[0].g | (uintptr_t)job.doms[0].dc) & 15u))
+    job.s.stream_min = 0;
   // work lists from claims, except where ghost syncs mark the bitmaps too
   job.s.append = job.subdomains ? 0u : 1u;
   const uint32_t ndom = (uint32_t)job.doms.size();
@@ -2089,7 +2216,7 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
     for (int64_t k = 0; k < st->trace_n; k++) {
       const Counters &ck = c.h_ring[(k + 1) % kCtrRing];
       for (int q = 0; q < 3; q++) st->trace_cnt[k][q] = cnt_host(ck, q);
-      st->trace_targets[k] = ck.ntargets;
+      st->trace_targets[k] = ck.ntargets + cnt_host(ck, kCntStreamed);
     }
   }
   st->total_targets = (int64_t)acc_host(r, kAccTargets);
@@ -3519,8 +3646,8 @@ int pmsz_autotune(pmsz_ctx *ctx, int n, const pmsz_params *params, const void *c
   if (rc == PMSZ_OK) sweep(&pmsz_tuning::tail_blocks, tb, 4);
   if (rc == PMSZ_OK) sweep(&pmsz_tuning::solo_targets, so, 4);
   const bool multi = n > 1 || params[0].sub[0] * params[0].sub[1] * params[0].sub[2] != 1;
-  const int sd[] = {16, 8, 32, 0};  // the streamed apply: one-domain runs only
-  if (rc == PMSZ_OK && !multi) sweep(&pmsz_tuning::stream_div, sd, 4);
+  const int sd[] = {32, 16, 64, 8, 0};  // the streamed apply: one-domain runs only
+  if (rc == PMSZ_OK && !multi) sweep(&pmsz_tuning::stream_div, sd, 5);
   const int gr[] = {0, 1};  // captured chains: multi-domain runs only
   if (rc == PMSZ_OK && multi) sweep(&pmsz_tuning::graphs, gr, 2);
   (void)hipFree(scratch);

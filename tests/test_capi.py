@@ -57,7 +57,7 @@ def test_tuning_defaults_and_validation():
     L.pmsz_default_tuning(ctypes.byref(t))
     assert t.as_dict() == {"dense_div": 32, "tail_targets": 4096, "tail_blocks": 32,
                            "solo_targets": 64, "zchunk": 64, "graphs": 0,
-                           "stream_div": 16}
+                           "stream_div": 32}
     for bad in ({"zchunk": 48}, {"zchunk": 128}, {"dense_div": 0}, {"tail_blocks": 0},
                 {"tail_blocks": 257}, {"tail_targets": -1}, {"graphs": 2}, {"stream_div": -1},
                 {"solo_targets": -1}):

@@ -20,6 +20,7 @@ TUNINGS = [
     {"solo_targets": 1 << 20, "tail_blocks": 4},  # ... or from its first sweep
     {"stream_div": 0},            # targets of full-volume detections always listed
     {"stream_div": 1 << 24},      # ... streamed from the byte map whenever any violation
+    {"stream_div": 64, "dense_div": 8},
     {"graphs": 1},                # multi-domain runs replay captured chains
     {"graphs": 1, "zchunk": 32, "tail_targets": 4096},
 ]
