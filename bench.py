@@ -42,8 +42,8 @@ LABEL_BYTES_PER_VOXEL = 4 + 1 + 1
 # size.
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_columns_detect_512cube_f32_pmc.md")
 # Committed rocprofv3 --kernel-trace summary of one correction of the same
-# workload: "sparse_us" is the time of its apply, work-list detection and tail
-# launches.
+# workload: "apply_list_us" is the time of its apply, work-list detection and
+# tail launches.
 KSTATS_SUMMARY = os.path.join(ROOT, "profiles", "correct_512cube_f32_kernel_stats.md")
 # VALU issue peak: one wave64 VALU instruction per CU and clock, 256 CUs at the
 # 2.4 GHz peak clock (G wave instructions/s).  The same summary holds the
@@ -655,13 +655,14 @@ def main():
         "traffic_gbs": None if ctraffic is None else ctraffic / (elapsed / args.steps) / 1e9,
         "traffic_frac": None if ctraffic is None else
         ctraffic / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBPS}
-    # the sparse phase: every apply, work-list detection and tail launch; its
-    # algorithmic bytes are 3s+2 per edit (read f, g, dc; write g, dc) and s+1 per
-    # re-examined voxel (read g and the code)
-    sp_us = pmc_summary(nvox, "sparse_us", KSTATS_SUMMARY)
-    sp_traffic = pmc_summary(nvox, "sparse_traffic_bytes")
+    # everything but the full-volume passes: every apply (streamed or listed),
+    # work-list detection and tail launch; its algorithmic bytes are 3s+2 per edit
+    # (read f, g, dc; write g, dc) and s+1 per voxel a work list re-examines (read
+    # g and the code)
+    sp_us = pmc_summary(nvox, "apply_list_us", KSTATS_SUMMARY)
+    sp_traffic = pmc_summary(nvox, "apply_list_traffic_bytes")
     sp_alg = s0["total_targets"] * (3 * s_bytes + 2) + s0["sparse_voxels"] * (s_bytes + 1)
-    res["roofline"]["sparse"] = {
+    res["roofline"]["applies_and_lists"] = {
         "kernels": "k_apply + k_detect_list + k_tail of one correction",
         "algorithmic_bytes": sp_alg,
         "kernel_us": sp_us, "time_source": os.path.relpath(KSTATS_SUMMARY, ROOT),
