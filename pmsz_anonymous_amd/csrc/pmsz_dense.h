@@ -317,6 +317,17 @@ __host__ __device__ inline uint32_t expect_entry(uint32_t c) {
   return a | (d << 16);
 }
 
+// Wave masks of compares (one v_cmp into a lane mask, no select-and-compare):
+// integer predicates of __builtin_amdgcn_uicmp, and the lanes whose value is NaN
+// or the sentinel (unordered-or-equal, v_cmp_nlg).
+constexpr int kICmpEq = 32, kICmpNe = 33;
+__device__ __forceinline__ uint64_t bad_mask(float v) {
+  return __builtin_amdgcn_fcmpf(v, sentinel<float>(), 9);
+}
+__device__ __forceinline__ uint64_t bad_mask(double v) {
+  return __builtin_amdgcn_fcmp(v, sentinel<double>(), 9);
+}
+
 // Loads through a buffer resource whose base is a plane's first byte: the
 // in-plane byte offset is the instruction's 32-bit VGPR offset, so a load
 // costs no address arithmetic.
@@ -330,19 +341,6 @@ __device__ __forceinline__ void plane_load(__amdgpu_buffer_rsrc_t rs, uint32_t o
 __device__ __forceinline__ void plane_load(__amdgpu_buffer_rsrc_t rs, uint32_t off, double &v) {
   const auto w = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
   v = __hiloint2double((int)w[1], (int)w[0]);
-}
-
-// Candidate offset of rank r in a block of extent d.
-__device__ __forceinline__ int64_t cand_off(int r, const Grid3 &d) {
-  constexpr uint64_t kXY = [] {
-    uint64_t t = 0;
-    for (int q = 0; q < kCand; q++)
-      t |= (uint64_t)((rank_dx(q) + 1) | ((rank_dy(q) + 1) << 2)) << (4 * q);
-    return t;
-  }();
-  const uint32_t f = (uint32_t)(kXY >> (4 * r)) & 15u;
-  const int dz = (r >= 4 ? 1 : 0) + (r >= 11 ? 1 : 0) - 1;
-  return (int64_t)((int)(f & 3u) - 1) + d.nx * (int64_t)((int)(f >> 2) - 1) + d.sxy * dz;
 }
 
 // The full-volume kernel: with kLabel the original labeling of f (codes; the
@@ -366,6 +364,8 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
   const double kNaN = __builtin_nan("");
   __shared__ uint32_t s_exp[kDetect && kF32 ? 256 : 1];
   __shared__ uint2 s_q[kDetect ? kColWaves * kQCap : 1];
+  // per wave: the cell offsets of the 15 candidates in the current unit's domain
+  __shared__ int32_t s_off[kDetect ? kColWaves * 16 : 1];
   if (kDetect && kF32) {
     s_exp[threadIdx.x] = expect_entry(threadIdx.x);
     __syncthreads();
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
   const uint32_t u_end = u_begin + chunk < nunits ? u_begin + chunk : nunits;
   Counters *out = kDetect ? &ring[jslot] : nullptr;
   uint32_t lc[3] = {0, 0, 0};
-  uint32_t lerr = 0;
+  uint64_t lerr = 0;  // lanes that loaded a NaN or the sentinel (a wave mask)
   const uint32_t qbase = wave * (uint32_t)kQCap;
 
   for (uint32_t unit = u_begin + (blockIdx.x / 8) * kColWaves + wave; unit < u_end;
@@ -442,6 +442,10 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
       vis[h] = owned && y < D.hi[1];
       coff[h] = vis[h] ? (uint32_t)x + nx * (uint32_t)y : 0u;
     }
+    // the fast test's mask of the tag bits: 511 where the lane has the voxel row
+    uint32_t vmask[kColRows];
+#pragma unroll
+    for (int h = 0; h < kColRows; h++) vmask[h] = vis[h] ? 511u : 0u;
     // raw values of plane pz into nv; the plane's validity is uniform over the
     // wave, so an out-of-domain plane is a branch, not a select
     auto fetch = [&](int pz, T (&nv)[kColH]) {
@@ -458,16 +462,27 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
     };
     // the value's validity: a padded block's sentinel cells carry no value; in
     // a field, NaN or the sentinel is an input error
-    auto valid = [&](bool ok, T v) -> bool {
+    // (pin: the plane lies in the domain, uniform).  A lane outside the domain
+    // loaded the plane's first cell (offset 0), a cell of the domain, so the
+    // input check needs no lane mask: whatever it flags is a NaN or sentinel
+    // of the input.
+    auto valid = [&](bool ok, bool pin, T v) -> bool {
       if (padded) return ok && v != sentinel<T>();
-      if (kCheckV && ok && (!(v == v) || v == sentinel<T>())) lerr |= 1u;
+      if (kCheckV && pin) lerr |= bad_mask(v);
       return ok;
     };
 
     // the violation queue of this wave: entries [qh, qt), resolved per unit
     uint32_t uc[3] = {0, 0, 0};  // counters of this unit
     uint32_t qh = 0, qt = 0;
-    const int64_t cbase = (int64_t)D.cbase;
+    // target bytes are stored through a buffer resource based at the domain's
+    // first cell, with the target's 32-bit cell index (below 2^32 - 1) as offset;
+    // the candidates' offsets come from this wave's table (written here, read
+    // after the wave_lds_sync that precedes every resolve; the previous unit's
+    // queue was drained before)
+    const auto trs = plane_rsrc(tbyte + D.cbase);
+    const uint32_t obase = wave * 16u;
+    if (kDetect && lane < (uint32_t)kCand) s_off[obase + lane] = (int32_t)rank_off((int)lane, D.d);
     auto resolve = [&](uint32_t count) {  // the `count` oldest entries (count <= 64)
       if (lane < count) {
         const uint2 q = s_q[qbase + ((qh + lane) & (uint32_t)(kQCap - 1))];
@@ -486,18 +501,20 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
         bool v1, v2;
         detect_pair(oa, od, da, dd, uc, t1, v1, t2, v2);
         if (emit) {  // a target is a cell of the block
-          const int64_t c0 = cbase + (int64_t)q.x;
-          if (v1) tbyte[c0 + cand_off(t1, D.d)] = (uint8_t)tag;
-          if (v2) tbyte[c0 + cand_off(t2, D.d)] = (uint8_t)tag;
+          if (v1)
+            __builtin_amdgcn_raw_buffer_store_b8((uint8_t)tag, trs,
+                                                 q.x + (uint32_t)s_off[obase + (uint32_t)t1], 0, 0);
+          if (v2)
+            __builtin_amdgcn_raw_buffer_store_b8((uint8_t)tag, trs,
+                                                 q.x + (uint32_t)s_off[obase + (uint32_t)t2], 0, 0);
         }
       }
       qh += count;
     };
-    // append the lanes of wave mask bm (mine: this lane's bit); `info` is
-    // evaluated only if bm is not 0
+    // append the lanes of wave mask bm; `info` is evaluated only if bm is not 0
     auto enqueue = [&](uint64_t bm, int h, int e, auto info) {
       if (bm == 0) return;  // uniform
-      if ((bm >> lane) & 1u) {
+      if (__builtin_amdgcn_inverse_ballot_w64(bm)) {
         const uint32_t pos = qt + __builtin_amdgcn_mbcnt_hi(
                                       (uint32_t)(bm >> 32),
                                       __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
@@ -544,7 +561,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
         double a2x = 0, a2n = 0, a1x = 0, a1n = 0, c1x = 0, c1n = 0, k1 = 0, r1 = 0;
 #pragma unroll
         for (int k = 0; k < kColH; k++) {
-          const bool ok = valid(pin && rok[k], rv[k]);
+          const bool ok = valid(pin && rok[k], pin, rv[k]);
           const float cv = rv[k] + 0.0f;  // -0 -> +0: the reference compares -0 == +0
           const uint32_t b = __float_as_uint(cv);
           const uint32_t ord = b ^ ((uint32_t)((int32_t)b >> 31) | 0x80000000u);
@@ -573,7 +590,9 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
                 const uint32_t ex = s_exp[c] + ts * 0x10001u;
                 const uint32_t ra = (uint32_t)__double2loint(wmx) - (ex & 0xffffu);
                 const uint32_t rd = (uint32_t)__double2loint(wmn) - (ex >> 16);
-                const uint64_t bm = __ballot(((ra | rd) & 511u) != 0u && vis[h - 1]);
+                // lanes whose winners differ: one compare into a wave mask
+                const uint64_t bm =
+                    __builtin_amdgcn_uicmp((ra | rd) & vmask[h - 1], 0u, kICmpNe);
                 // info: the code byte, then the low bytes of ra and rd
                 enqueue(bm, h - 1, e, [&] {
                   return c | (__builtin_amdgcn_perm(rd, ra, 0x0c0c0400u) << 8);
@@ -634,7 +653,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
           for (int h = 0; h < kColH; h++) cp.k[h] = cp.l[h] = cp.r[h] = kNaN;
         } else {
 #pragma unroll
-          for (int h = 0; h < kColH; h++) cp.k[h] = valid(rok[h], nv[h]) ? (double)nv[h] : kNaN;
+          for (int h = 0; h < kColH; h++) cp.k[h] = valid(rok[h], true, nv[h]) ? (double)nv[h] : kNaN;
           shift_plane(cp);
         }
         if (kDetect) {
@@ -660,7 +679,9 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
               const uint32_t c = ccode[h - 1];
               const uint32_t w = (uint32_t)da | ((uint32_t)dd << 4);
               const uint64_t bm =
-                  __ballot((w != c || c == (uint32_t)(kSelf | (kSelf << 4))) && vis[h - 1]);
+                  (__builtin_amdgcn_uicmp(w, c, kICmpNe) |
+                   __builtin_amdgcn_uicmp(c, (uint32_t)(kSelf | (kSelf << 4)), kICmpEq)) &
+                  __builtin_amdgcn_ballot_w64(vis[h - 1]);
               enqueue(bm, h - 1, e, [&] { return c | (w << 8); });
             }
           }
@@ -689,8 +710,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
       for (int c = 0; c < 3; c++) lc[c] += uc[c];
     }
   }
-  const uint64_t em = __ballot(lerr != 0);
-  if (em && lane == 0) atomicOr(&run->err, 1u);
+  if (lerr && lane == 0) atomicOr(&run->err, 1u);
   if (!kDetect) return;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
