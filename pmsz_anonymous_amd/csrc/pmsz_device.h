@@ -174,6 +174,7 @@ struct Sched {
                             // least this many violations are edited by one pass over the
                             // target byte map instead of a list
   uint32_t append;          // work-list sweeps list claimed bits (else bitmaps are compacted)
+  uint32_t tag0;            // detection jd marks its targets with detect_tag(jd + tag0)
 };
 
 constexpr uint32_t kErrInput = 1u, kErrBarrier = 2u;

@@ -745,7 +745,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
   if constexpr (!kMulti) {
     if (stream) {
       if (lead && mode != kModeDone) zero_counters(&ring[(j + 1) % kCtrRing]);
-      apply_stream<T>(d0, tbyte, ngroups, detect_tag(j), cj, run, lds.sl);
+      apply_stream<T>(d0, tbyte, ngroups, detect_tag(j + s.tag0), cj, run, lds.sl);
       return;
     }
   }
@@ -1141,6 +1141,34 @@ __global__ void k_reset_phase(Counters *ring, RunState *run, uint32_t m) {
   zero_counters(&ring[(m + 1) % kCtrRing]);
   run->done = 0;
   run->tail_j = 0;
+}
+
+// Start of a run: clears both mark bitmaps (`words` 32-bit words each), the
+// counter ring, the run state and the tail's barrier, marks detection 1 as a
+// running dense sweep (slot 0's mode, read by the first apply), and clears the
+// first tgroups 16-byte groups of the target byte map -- one launch instead of a
+// memset per buffer.  Every pointer is a hipMalloc base (16-byte aligned).
+__global__ void k_init_run(uint32_t *__restrict__ tmark, uint32_t *__restrict__ wmark,
+                           uint64_t words, Counters *__restrict__ ring, RunState *__restrict__ run,
+                           uint32_t *__restrict__ bar, uint4 *__restrict__ tb, uint64_t tgroups) {
+  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t nt = (uint64_t)gridDim.x * blockDim.x;
+  const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+  const uint64_t w4 = words / 4;
+  for (uint64_t i = t0; i < w4; i += nt) {
+    reinterpret_cast<uint4 *>(tmark)[i] = z;
+    reinterpret_cast<uint4 *>(wmark)[i] = z;
+  }
+  for (uint64_t i = 4 * w4 + t0; i < words; i += nt) tmark[i] = wmark[i] = 0u;
+  for (uint64_t i = t0; i < tgroups; i += nt) tb[i] = z;
+  constexpr uint64_t kRingWords = sizeof(Counters) * kCtrRing / 4;
+  constexpr uint64_t kModeWord = offsetof(Counters, mode) / 4;
+  static_assert(sizeof(Counters) % 4 == 0 && sizeof(RunState) % 4 == 0, "word-sized state");
+  for (uint64_t i = t0; i < kRingWords; i += nt)
+    reinterpret_cast<uint32_t *>(ring)[i] = i == kModeWord ? (uint32_t)kModeDense : 0u;
+  for (uint64_t i = t0; i < sizeof(RunState) / 4; i += nt)
+    reinterpret_cast<uint32_t *>(run)[i] = 0u;
+  for (uint64_t i = t0; i < kBarBytes / 4; i += nt) bar[i] = 0u;
 }
 
 // Zero fill of n16 16-byte groups (the target byte map when the tags start over;
@@ -1577,6 +1605,11 @@ struct Ctx {
   // the runtime refused a cooperative launch of the tail in this context: its
   // tails run as one workgroup from then on
   bool coop_refused = false;
+  // Tags of the target byte map go on from run to run: since the map was last
+  // cleared whole, it holds no tag from tag_next up to 255 (so a run can start
+  // at tag_next without clearing).  0: its contents are unknown (new memory, or
+  // scratch of another kernel) and the next run clears it whole.
+  uint32_t tag_next = 0;
 };
 
 // Records a refused cooperative launch: the context falls back to one-workgroup
@@ -1621,6 +1654,7 @@ void ctx_release(Ctx &c) {
   c.h_run = nullptr;
   c.h_ds = nullptr;
   c.h_ds_cap = 0;
+  c.tag_next = 0;
   for (hipEvent_t *e : {&c.ev0, &c.ev1, &c.ev2, &c.ev3, &c.ev4}) {
     if (*e) (void)hipEventDestroy(*e);
     *e = nullptr;
@@ -1861,8 +1895,6 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   const unsigned g_tail = (unsigned)std::max(
       1, std::min(std::min(kTailBlocks, c.tune.tail_blocks), c.cus * occ_tail / (kMaxContexts + 1)));
   const unsigned g_compact = compact_blocks(c, (int64_t)job.groups);
-  const unsigned g_fill =
-      (unsigned)std::max<int64_t>(1, std::min<int64_t>(((int64_t)job.groups + 255) / 256, 2048));
   const unsigned g_bits = (unsigned)std::max<int64_t>(
       1, std::min<int64_t>(((int64_t)job.wdense + kBitsThreads * kBitsPT - 1) /
                                (kBitsThreads * kBitsPT),
@@ -1877,17 +1909,26 @@ int drive(Ctx &c, Job &job, Outcome &out) {
                                                          (uint32_t)c.cus * 4)),
       ndom);
 
+  // Target tags go on from the previous run of this context: detection jd uses
+  // detect_tag(jd + tag0), and the map holds none of this run's tags until they
+  // start over, when the whole map is cleared (the map may be larger than this
+  // run's cells: a later, larger run reads the rest).  tag0 = 0 clears it now.
+  const uint64_t map_groups = c.tbyte.cap / 16;
+  const uint32_t tag0 = c.tag_next ? (c.tag_next - 1u) % 255u : 0u;
+  job.s.tag0 = tag0;
+  c.tag_next = 0;  // unknown until the run has ended
+  uint32_t max_jd = 1;  // the last detection enqueued
+  const uint64_t init_work = std::max<uint64_t>(job.words / 4, tag0 == 0 ? map_groups : 0);
+  const unsigned g_init =
+      (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((init_work + 255) / 256, 4096));
+  const unsigned g_fill =
+      (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((map_groups + 255) / 256, 2048));
   PMSZ_HIP(hipEventRecord(c.ev0, c.stream));
-  PMSZ_HIP(hipMemsetAsync(c.tmark.p, 0, sizeof(uint32_t) * job.words, c.stream));
-  PMSZ_HIP(hipMemsetAsync(c.wmark.p, 0, sizeof(uint32_t) * job.words, c.stream));
-  PMSZ_HIP(hipMemsetAsync(c.tbyte.p, 0, 16 * job.groups, c.stream));
-  PMSZ_HIP(hipMemsetAsync(c.ring, 0, sizeof(Counters) * kCtrRing, c.stream));
-  PMSZ_HIP(hipMemsetAsync(c.run, 0, sizeof(RunState), c.stream));
-  PMSZ_HIP(hipMemsetAsync(c.bar, 0, kBarBytes, c.stream));
-  // apply(1) reads slot 0's mode: mark detection 1 as a running sweep
-  static const uint32_t kRunning = kModeDense;
-  PMSZ_HIP(hipMemcpyAsync(&c.ring[0].mode, &kRunning, sizeof(uint32_t), hipMemcpyHostToDevice,
-                          c.stream));
+  // clears the marks, counters, run state and barrier (and the map when tag0 = 0);
+  // apply(1) reads slot 0's mode, which it sets to a running dense sweep
+  k_init_run<<<g_init, 256, 0, c.stream>>>(c.tmark.p, c.wmark.p, job.words, c.ring, c.run,
+                                           c.bar, (uint4 *)c.tbyte.p,
+                                           tag0 == 0 ? map_groups : 0);
   // the original labeling (codes, delta counters cleared), then detection 1 as
   // a dense detection sweep (slot 0 holds mode Dense)
   PMSZ_HIP(hipEventRecord(c.ev2, c.stream));
@@ -1897,16 +1938,16 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   // detection 1 also rejects NaN / sentinel values of g (padded blocks: the scatter did)
   if (job.subdomains)
     k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
-        d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, detect_tag(1));
+        d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, detect_tag(1 + tag0));
   else
     k_columns<T, false, true, kMulti, true><<<g_dense1, kColThreads, 0, c.stream>>>(
-        d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, detect_tag(1));
+        d0, doms, ndom, ds, c.tbyte.p, c.ring, 0, 1, 1u, c.run, detect_tag(1 + tag0));
   PMSZ_HIP(hipGetLastError());
   PMSZ_HIP(hipEventRecord(c.ev4, c.stream));
   if (kMulti) {
     k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(c.tbyte.p, c.tlist.p, c.ring, 0, 1,
                                                            true, (uint32_t)job.groups, doms,
-                                                           ndom, detect_tag(1), 0u);
+                                                           ndom, detect_tag(1 + tag0), 0u);
     k_domains<<<1, kMaxDomains, 0, c.stream>>>(ds, ndom, c.ring, 0, 0, 1, kOpDetect, job.s, 1);
   }
 
@@ -1927,7 +1968,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
           // after a full-volume detection j: its target list
           k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
               c.tbyte.p, c.tlist.p, c.ring, (j - 1) % kCtrRing, j % kCtrRing, false,
-              (uint32_t)job.groups, doms, ndom, detect_tag(j), job.s.stream_min);
+              (uint32_t)job.groups, doms, ndom, detect_tag(j + tag0), job.s.stream_min);
         }
         k_apply<T, kMulti><<<g_apply, kApplyThreads, 0, c.stream>>>(
             d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j,
@@ -1945,16 +1986,16 @@ int drive(Ctx &c, Job &job, Outcome &out) {
             c.wmark.p, doms, ndom, (uint32_t)job.wdense, c.wlist.p, &c.ring[j % kCtrRing].nwork,
             &c.ring[j % kCtrRing], kModeSparse);
         // the tags start over: no mark of an earlier sweep may carry detection j+1's
-        if (detect_tag(j + 1) == 1u)
-          k_fill16<<<g_fill, 256, 0, c.stream>>>((uint4 *)c.tbyte.p, job.groups);
+        if (detect_tag(j + 1 + tag0) == 1u)
+          k_fill16<<<g_fill, 256, 0, c.stream>>>((uint4 *)c.tbyte.p, map_groups);
         if (!sparse_chain)
           k_columns<T, false, true, kMulti><<<g_dense, kColThreads, 0, c.stream>>>(
               d0, doms, ndom, ds, c.tbyte.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing,
-              (j + 1) & 1u, c.run, detect_tag(j + 1));
+              (j + 1) & 1u, c.run, detect_tag(j + 1 + tag0));
         if (kMulti && !sparse_chain)
           k_compact<<<g_compact, kCompactThreads, 0, c.stream>>>(
               c.tbyte.p, c.tlist.p, c.ring, j % kCtrRing, (j + 1) % kCtrRing, false,
-              (uint32_t)job.groups, doms, ndom, detect_tag(j + 1), job.s.stream_min);
+              (uint32_t)job.groups, doms, ndom, detect_tag(j + 1 + tag0), job.s.stream_min);
         k_detect_list<T, kMulti><<<g_list, kApplyThreads, 0, c.stream>>>(
             d0, doms, job.shift, ds, c.tmark.p, c.tlist.p, c.wmark.p, c.wlist.p, c.ring, c.run, j,
             job.s.append);
@@ -1979,7 +2020,8 @@ int drive(Ctx &c, Job &job, Outcome &out) {
           (uint64_t)(uintptr_t)ds, (uint64_t)(uintptr_t)c.tbyte.p, (uint64_t)(uintptr_t)c.tmark.p,
           (uint64_t)(uintptr_t)c.tlist.p, (uint64_t)(uintptr_t)c.wmark.p,
           (uint64_t)(uintptr_t)c.wlist.p, (uint64_t)(uintptr_t)c.ring, (uint64_t)(uintptr_t)c.run,
-          (uint64_t)job.s.mode, job.s.max_sweeps, job.s.dense_min_work, job.s.tail_ok,
+          (uint64_t)job.s.mode, job.s.max_sweeps, job.s.dense_min_work, job.s.tail_ok, tag0,
+          map_groups,
           job.s.tail_targets, job.s.solo_targets, job.s.dense_ok, job.s.stream_min, job.s.append,
           sparse_chain ? 1u : 0u, chain,
           g_compact,
@@ -1992,6 +2034,7 @@ int drive(Ctx &c, Job &job, Outcome &out) {
       int rc = enqueue_chain(j0);
       if (rc) return rc;
     }
+    if (!sparse_chain) max_jd = std::max(max_jd, j0 + chain);
     if (job.s.tail_ok) {
       Dom a_d0 = d0;
       const Dom *a_doms = doms;
@@ -2096,6 +2139,8 @@ int drive(Ctx &c, Job &job, Outcome &out) {
   PMSZ_HIP(hipEventElapsedTime(&out.ms_total, c.ev0, c.ev1));
   PMSZ_HIP(hipEventElapsedTime(&out.ms_label, c.ev2, c.ev3));
   PMSZ_HIP(hipEventElapsedTime(&out.ms_first, c.ev3, c.ev4));
+  // the next run starts after the last tag this one may have written
+  c.tag_next = detect_tag(max_jd + tag0) + 1u;
   return PMSZ_OK;
 }
 
@@ -2124,7 +2169,10 @@ int prepare(Ctx &c, Job &job) {
   job.groups = (cells + 15) / 16;
   job.list_cap = cells;
   int rc;
-  if ((rc = c.tbyte.ensure(16 * job.groups)) || (rc = c.tmark.ensure(job.words)) ||
+  const size_t map_before = c.tbyte.cap;
+  if ((rc = c.tbyte.ensure(16 * job.groups))) return rc;
+  if (c.tbyte.cap != map_before) c.tag_next = 0;  // new memory: contents unknown
+  if ((rc = c.tmark.ensure(job.words)) ||
       (rc = c.wmark.ensure(job.words)) ||
       (rc = c.tlist.ensure(job.list_cap)) || (rc = c.wlist.ensure(job.list_cap)))
     return rc;
@@ -2161,6 +2209,7 @@ Sched make_sched(const Ctx &c, const pmsz_params *p, uint64_t work_voxels, bool 
   s.solo_targets = (uint32_t)c.tune.solo_targets;
   s.dense_ok = 1;
   s.append = 1;
+  s.tag0 = 0;
   s.stream_min = c.tune.stream_div
                      ? (uint32_t)std::min<uint64_t>(
                            std::max<uint64_t>(work_voxels / (uint64_t)c.tune.stream_div, 1),
@@ -2479,6 +2528,7 @@ int time_first_pass(Ctx &c, const pmsz_params *p, const T *f, const T *g, int it
   const unsigned gd = col_blocks(c, dom_units(job.doms[0]), occ_d);
   double tl = 0, td = 0;
   static const uint32_t kRunning = kModeDense;
+  c.tag_next = 0;  // the passes below leave tag-1 marks in the map
   for (int it = 0; it < iters; it++) {
     PMSZ_HIP(hipMemsetAsync(c.tbyte.p, 0, 16 * job.groups, c.stream));
     PMSZ_HIP(hipMemsetAsync(c.ring, 0, sizeof(Counters) * 2, c.stream));
@@ -2585,6 +2635,7 @@ int correct_naive(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *d_dc,
       (rc = c.tbyte.ensure(nb + 256 + sbytes + 256)) || (rc = c.tmark.ensure(n)) ||
       (rc = c.wmark.ensure(n)) || (rc = c.tlist.ensure(n)) || (rc = c.wlist.ensure(n)))
     return rc;
+  c.tag_next = 0;  // the map's memory is this mode's scratch
   uint8_t *cf = c.code.p, *cg = c.snap.p, *dc = d_dc ? d_dc : c.dc.p;
   uint8_t *mark = c.tbyte.p;
   unsigned *cnt = (unsigned *)(c.tbyte.p + nb);
@@ -2755,6 +2806,7 @@ int dq_quant_hist(Ctx &c, const pmsz_params *p, const T *f, T *g0,
   const size_t hbytes = (size_t)nb * kDqSlots * sizeof(unsigned long long);
   int rc;
   if ((rc = c.tmark.ensure((size_t)n)) || (rc = c.tbyte.ensure(o_hist + hbytes))) return rc;
+  c.tag_next = 0;  // the map's memory is the codec's scratch
   DqBlock *d_bl = (DqBlock *)c.tbyte.p;
   unsigned long long *d_hist = (unsigned long long *)(c.tbyte.p + o_hist);
   int32_t *kq = (int32_t *)c.tmark.p;
@@ -3339,6 +3391,7 @@ int encode_edits(Ctx &c, const pmsz_params *p, const T *f, const T *g0, const T 
   // the scratch lives in the context's byte map
   int rc = c.tbyte.ensure(enc_scratch(nb, ntiles).total);
   if (rc) return rc;
+  c.tag_next = 0;  // the map's memory is the encoder's scratch
   std::vector<unsigned long long> h(nb + 2);
   uint32_t herr = 0;
   rc = encode_edits_on<T>(c.stream, c.tbyte.p, p, bl.data(), nb, ntiles, nbits, f, g0, g, packed,
@@ -3487,6 +3540,7 @@ int pmsz_ctx_trim(pmsz_ctx *ctx) {
     (void)hipFree(b->p);
     *b = Buf<uint8_t>();
   }
+  c.tag_next = 0;
   for (Buf<uint32_t> *b : {&c.tmark, &c.wmark, &c.tlist, &c.wlist}) {
     (void)hipFree(b->p);
     *b = Buf<uint32_t>();
