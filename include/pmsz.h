@@ -347,6 +347,16 @@ int pmsz_codec_dq_decompress(const uint8_t *d_stream, int64_t bytes, void *d_g0)
 int pmsz_time_first_pass(const pmsz_params *p, const void *d_f, const void *d_g, int iters,
                          double *avg_seconds);
 
+/* Test hook (no counterpart in the reference): the step budgets the edits of one
+ * field store in the high nibble of a delta counter.  For each i < n (device
+ * arrays; dtype PMSZ_F32 or PMSZ_F64 values f[i], g[i], counters d[i] < 16),
+ * d_out[i] = the largest B <= q - 1 such that every edit made while the counter
+ * is below B is provably the plain step g - abs_eb/q within the bound (clamp_mode
+ * as in pmsz_params), or 0.  Needs 2 <= q <= 15. */
+int pmsz_test_step_budget(int dtype, int64_t n, const void *d_f, const void *d_g,
+                          const uint8_t *d_d, double abs_eb, int q, int clamp_mode,
+                          uint8_t *d_out);
+
 const char *pmsz_strerror(int code);
 /* Library version string. */
 const char *pmsz_version(void);

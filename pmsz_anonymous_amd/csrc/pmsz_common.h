@@ -70,6 +70,7 @@ struct EditParams {
   double delta_d; // eb / q in double (delta = bound / q, pMSz.cu:2814)
   int q;
   int clamp_mode; // 0 reference, 1 bound-safe
+  int budgets;    // 1: the high nibble of a delta counter holds a step budget (q <= 15)
 };
 
 // Clamp value of the edit primitive (pMSz.cu:636).  Bound-safe mode keeps that

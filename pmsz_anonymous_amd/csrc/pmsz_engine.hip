@@ -242,18 +242,121 @@ __global__ __launch_bounds__(kMaxDomains) void k_domains(DomState *__restrict__ 
 // ---------------------------------------------------------------------------
 // Shared pieces of the apply and work-list phases.
 
-// One edit (applyDeltaBuffer_local) on local cell t, counting first edits / clamps.
+// Step budgets, for one field with q <= 15 (EditParams::budgets; the ghost syncs
+// of subdomain runs and the counter scans of batches read plain counters).  A
+// delta counter's high nibble then holds a budget B: the low nibble d counts the
+// edits, and while d < B the next edit is the plain step g - delta, which was
+// proved to honour the bound when B was set, so it needs no f.  Every edit that
+// takes the exact path (it reads f) sets the cell's budget; counters start at 0
+// (no budget).  Counters are handed out with the nibble cleared.
+__host__ __device__ inline bool budgets_on(const EditParams &ep) { return ep.budgets != 0; }
+
+// The budget of a cell with counter d < q, value gv and original fv: the largest
+// B <= q - 1 such that every edit made while the counter is below B is the plain
+// step and honours the bound in both clamp modes.  With h_0 = gv and
+// h_j = fl(h_{j-1} - delta), each h_j lies within j * (delta + M*u) below gv (u:
+// half an ulp relative to M >= |h_j|, plus the smallest subnormal), and no higher
+// than the first step.  So the steps land in [f - eb', f + eb'] with eb' =
+// eb (1 - 2^-20), which lies below T's rounded bound and its predecessor (the
+// reference mode's test) and below eb (the bound-safe test).  A margin covers the
+// double arithmetic here.  Returns 0 when nothing can be proved.
+template <typename T>
+__device__ __forceinline__ uint32_t step_budget(T fv, T gv, uint32_t d, const EditParams &ep) {
+  const uint32_t qm1 = (uint32_t)ep.q - 1u;
+  if (d >= qm1) return 0;  // the next edit clamps
+  const double dl = (double)(T)ep.delta_d;
+  const double f = (double)fv, g = (double)gv;
+  const double u = sizeof(T) == 4 ? 0x1p-24 : 0x1p-53;
+  const double tiny = sizeof(T) == 4 ? 0x1p-149 : 0x1p-1074;
+  const double qd = (double)ep.q * dl;
+  const double err = (fabs(g) + qd) * u + tiny;                   // rounding of one step
+  const double marg = (fabs(g) + fabs(f) + ep.eb + qd) * 0x1p-48;  // of this arithmetic
+  const double ebp = ep.eb * (1.0 - 0x1p-20);
+  // every step lowers g, and the first one lands at or below f + eb'
+  if (!(dl > 4.0 * err) || !(g - f - dl + err + marg <= ebp)) return 0;
+  const double room = g - f + ebp - marg;  // how far g may go down
+  if (!(room > 0.0)) return 0;
+  // v_rcp_f64 is accurate to about 2^-23 (2^29 ulp); the quotient gives up 2^-20.
+  // A subnormal step gets no budget (the reciprocal may flush it).
+  if (!(dl + err >= 0x1p-1022)) return 0;
+  const double k = floor(room * __builtin_amdgcn_rcp(dl + err) * (1.0 - 0x1p-20));
+  const uint32_t b = k >= (double)qm1 ? qm1 : (uint32_t)k;
+  return d + b < qm1 ? d + b : qm1;
+}
+
+// The same bound for float fields in float arithmetic (half the registers of the
+// double version): every float operation here rounds by at most 2^-24 of the
+// magnitudes involved, which the margin of 2^-20 of them covers, and the quotient
+// gives up 2^-16 (v_rcp_f32 is exact to 1 ulp).  eb' is formed in double and
+// rounded to float, which keeps it below eb (1 - 2^-20).
+template <>
+__device__ __forceinline__ uint32_t step_budget<float>(float fv, float gv, uint32_t d,
+                                                       const EditParams &ep) {
+  const uint32_t qm1 = (uint32_t)ep.q - 1u;
+  if (d >= qm1) return 0;
+  const float dl = (float)ep.delta_d;  // the step the edits take
+  const float ebp = (float)(ep.eb * (1.0 - 0x1p-19));  // at most eb (1 - 2^-20)
+  const float qd = (float)ep.q * dl;
+  const float err = 2.0f * ((fabsf(gv) + qd) * 0x1p-24f) + 0x1p-149f;
+  const float marg = (fabsf(gv) + fabsf(fv) + ebp + qd) * 0x1p-20f;
+  if (!(dl > 4.0f * err) || !(gv - fv - dl + err + marg <= ebp)) return 0;
+  const float room = gv - fv + ebp - marg;
+  if (!(room > 0.0f)) return 0;
+  if (!(dl + err >= 0x1p-126f)) return 0;  // subnormal: the reciprocal may flush it
+  const float k = floorf(room * __builtin_amdgcn_rcpf(dl + err) * (1.0f - 0x1p-16f));
+  const uint32_t b = k >= (float)qm1 ? qm1 : (uint32_t)k;
+  return d + b < qm1 ? d + b : qm1;
+}
+
+// The budgets of n cells (the test hook pmsz_test_step_budget).
+template <typename T>
+__global__ void k_test_step_budget(int64_t n, const T *f, const T *g, const uint8_t *d,
+                                   EditParams ep, uint8_t *out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (uint8_t)step_budget<T>(f[i], g[i], d[i], ep);
+}
+
+// One edit (applyDeltaBuffer_local) of a cell with counter byte c0 and value gv:
+// within its budget the plain step, otherwise the exact edit with fv() (the only
+// read of f) and, with kSet, the cell's new budget (without, the old one stays; it
+// is spent).  Budgets are set where the cells are edited again later: the edits
+// of full-volume detections' targets.  Returns the new counter byte; counts first
+// edits and clamps.
+template <typename T, bool kSet, typename FV>
+__device__ __forceinline__ uint8_t edit_cell(T &gv, uint8_t c0, FV fv, const EditParams &ep,
+                                             uint32_t &touched, uint32_t &clamped) {
+  const bool bon = budgets_on(ep);
+  const uint32_t d = bon ? c0 & 15u : c0;
+  uint32_t bud = bon ? (uint32_t)(c0 >> 4) : 0u;
+  uint8_t c = (uint8_t)d;
+  if (d < bud) {
+    gv = gv - (T)ep.delta_d;
+    c = (uint8_t)(d + 1u);
+  } else {
+    const T f = fv();
+    edit_value<T>(f, gv, c, ep);
+    if (kSet && bon) bud = step_budget<T>(f, gv, c, ep);
+  }
+  touched += d == 0;
+  clamped += (c == ep.q && d != (uint32_t)ep.q);
+  return bon ? (uint8_t)(c | (bud << 4)) : c;
+}
+
+// One edit on local cell t, where edits are latency-bound (work lists, the tail,
+// batches): f is loaded together with g and the counter, so that an edit outside
+// its budget waits for no second load; one within its budget ignores it.
 template <typename T>
 __device__ __forceinline__ void apply_one(T *g, const T *f, uint8_t *dc, uint32_t t,
                                           const EditParams &ep, uint32_t &touched,
                                           uint32_t &clamped) {
   T gv = g[t];
-  uint8_t c0 = dc[t], c = c0;
-  edit_value<T>(f[t], gv, c, ep);
+  const uint8_t c0 = dc[t];
+  const T fv = f[t];
+  asm volatile("" ::"v"(fv));  // the load stays here, not in the exact path
+  const uint8_t c = edit_cell<T, false>(gv, c0, [&] { return fv; }, ep, touched, clamped);
   g[t] = gv;
   dc[t] = c;
-  touched += c0 == 0;
-  clamped += (c == ep.q && c0 != ep.q);
 }
 
 // Claims the classified voxels of the closed stencil neighbourhood of local cell
@@ -639,13 +742,9 @@ __device__ __forceinline__ void apply_stream(const Dom &d0, const uint8_t *__res
       for (uint32_t e = lane; e < nq; e += 64) {
         const uint32_t idx = sq[e];
         T gv = sg[idx];
-        const uint8_t c0 = sc[idx];
-        uint8_t c = c0;
-        edit_value<T>(sf[idx], gv, c, d0.ep);
+        sc[idx] =
+            edit_cell<T, true>(gv, sc[idx], [&] { return sf[idx]; }, d0.ep, touched, clamped);
         sg[idx] = gv;
-        sc[idx] = c;
-        touched += c0 == 0;
-        clamped += (c == d0.ep.q && c0 != d0.ep.q);
       }
       wave_lds_sync();
       if (mr) {
@@ -836,20 +935,25 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
 #pragma unroll
       for (int q = 0; q < kU; q++) {
         if (t[q] != 0xffffffffu) {
-          fv[q] = f[t[q]];
           gv[q] = g[t[q]];
           cv[q] = dc[t[q]];
         }
       }
+      // f only where the edit is not within a step budget
+      const bool bon = budgets_on(d0.ep);
+#pragma unroll
+      for (int q = 0; q < kU; q++) {
+        fv[q] = (T)0;
+        if (t[q] != 0xffffffffu && !(bon && (cv[q] & 15u) < (uint32_t)(cv[q] >> 4)))
+          fv[q] = f[t[q]];
+      }
 #pragma unroll
       for (int q = 0; q < kU; q++) {
         if (t[q] != 0xffffffffu) {
-          const uint8_t c0 = cv[q];
-          edit_value<T>(fv[q], gv[q], cv[q], d0.ep);
+          const uint8_t c =
+              edit_cell<T, true>(gv[q], cv[q], [&] { return fv[q]; }, d0.ep, touched, clamped);
           g[t[q]] = gv[q];
-          dc[t[q]] = cv[q];
-          touched += c0 == 0;
-          clamped += (cv[q] == d0.ep.q && c0 != d0.ep.q);
+          dc[t[q]] = c;
         }
       }
     }
@@ -1169,6 +1273,27 @@ __global__ void k_init_run(uint32_t *__restrict__ tmark, uint32_t *__restrict__ 
   for (uint64_t i = t0; i < sizeof(RunState) / 4; i += nt)
     reinterpret_cast<uint32_t *>(run)[i] = 0u;
   for (uint64_t i = t0; i < kBarBytes / 4; i += nt) bar[i] = 0u;
+}
+
+// The caller's copy of n delta counters: the low nibble of each (the counters
+// without their step budgets).  src is a hipMalloc base; dst may have any alignment.
+__global__ void k_dc_out(uint8_t *__restrict__ dst, const uint8_t *__restrict__ src, uint64_t n) {
+  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t nt = (uint64_t)gridDim.x * blockDim.x;
+  if (((uintptr_t)dst & 15u) == 0) {
+    const uint64_t n16 = n / 16;
+    for (uint64_t i = t0; i < n16; i += nt) {
+      uint4 v = reinterpret_cast<const uint4 *>(src)[i];
+      v.x &= 0x0f0f0f0fu;
+      v.y &= 0x0f0f0f0fu;
+      v.z &= 0x0f0f0f0fu;
+      v.w &= 0x0f0f0f0fu;
+      reinterpret_cast<uint4 *>(dst)[i] = v;
+    }
+    for (uint64_t i = 16 * n16 + t0; i < n; i += nt) dst[i] = src[i] & 15u;
+  } else {
+    for (uint64_t i = t0; i < n; i += nt) dst[i] = src[i] & 15u;
+  }
 }
 
 // Zero fill of n16 16-byte groups (the target byte map when the tags start over;
@@ -1758,6 +1883,7 @@ EditParams edit_params(const pmsz_params *p) {
   ep.delta_d = p->abs_eb / p->q;
   ep.q = p->q;
   ep.clamp_mode = p->clamp_mode;
+  ep.budgets = 0;
   return ep;
 }
 
@@ -2241,14 +2367,23 @@ int correct_single(Ctx &c, const pmsz_params *p, const T *f, T *g, uint8_t *dc_o
   if ((rc = c.code.ensure(n)) || (rc = c.dc.ensure(n))) return rc;
   Job job;
   job.doms.push_back(field_dom(p, f, g, c.code.p, c.dc.p));
+  job.doms[0].ep.budgets = p->q <= 15 ? 1 : 0;
   job.s = make_sched(c, p, (uint64_t)n, true);
   if ((rc = prepare(c, job))) return rc;
   std::memset(st, 0, sizeof(*st));
   Outcome o;
   rc = drive<T, false>(c, job, o);
   if (rc) return rc;
-  if (dc_out)
-    PMSZ_HIP(hipMemcpyAsync(dc_out, c.dc.p, (size_t)n, hipMemcpyDeviceToDevice, c.stream));
+  if (dc_out) {
+    if (budgets_on(job.doms[0].ep)) {  // the counters without their step budgets
+      const unsigned gb = (unsigned)std::max<int64_t>(
+          1, std::min<int64_t>((n / 16 + 255) / 256, 4096));
+      k_dc_out<<<gb, 256, 0, c.stream>>>(dc_out, c.dc.p, (uint64_t)n);
+      PMSZ_HIP(hipGetLastError());
+    } else {
+      PMSZ_HIP(hipMemcpyAsync(dc_out, c.dc.p, (size_t)n, hipMemcpyDeviceToDevice, c.stream));
+    }
+  }
   PMSZ_HIP(hipStreamSynchronize(c.stream));
   st->t_total_s = o.ms_total * 1e-3;
   st->t_dense_s = o.ms_first * 1e-3;
@@ -3725,6 +3860,36 @@ int pmsz_time_first_pass(const pmsz_params *p, const void *d_f, const void *d_g,
                                   avg_seconds);
   return time_first_pass<double>(c, p, (const double *)d_f, (const double *)d_g, iters,
                                  avg_seconds);
+}
+
+int pmsz_test_step_budget(int dtype, int64_t n, const void *d_f, const void *d_g,
+                          const uint8_t *d_d, double abs_eb, int q, int clamp_mode,
+                          uint8_t *d_out) {
+  if (n < 0 || (n > 0 && (!d_f || !d_g || !d_d || !d_out)) || q < 2 || q > 15 ||
+      !(abs_eb > 0) || (dtype != PMSZ_F32 && dtype != PMSZ_F64) ||
+      (clamp_mode != PMSZ_CLAMP_REFERENCE && clamp_mode != PMSZ_CLAMP_BOUND_SAFE))
+    return PMSZ_EINVAL;
+  if (n == 0) return PMSZ_OK;
+  Ctx &c = default_ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  int rc = ctx_device(c);
+  if (rc) return rc;
+  EditParams ep;
+  ep.eb = abs_eb;
+  ep.delta_d = abs_eb / q;
+  ep.q = q;
+  ep.clamp_mode = clamp_mode;
+  ep.budgets = 1;
+  const unsigned gb = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+  if (dtype == PMSZ_F32)
+    k_test_step_budget<float><<<gb, 256, 0, c.stream>>>(n, (const float *)d_f,
+                                                        (const float *)d_g, d_d, ep, d_out);
+  else
+    k_test_step_budget<double><<<gb, 256, 0, c.stream>>>(n, (const double *)d_f,
+                                                         (const double *)d_g, d_d, ep, d_out);
+  PMSZ_HIP(hipGetLastError());
+  PMSZ_HIP(hipStreamSynchronize(c.stream));
+  return PMSZ_OK;
 }
 
 int pmsz_correct_host(const pmsz_params *p, const void *h_f, void *h_g, uint8_t *h_dc,

@@ -32,7 +32,7 @@ EXPORTED = [
     "pmsz_codes", "pmsz_labels", "pmsz_edit_symbols", "pmsz_edit_packed_bytes",
     "pmsz_encode_edits", "pmsz_codec_dq", "pmsz_time_first_pass", "pmsz_strerror",
     "pmsz_version", "pmsz_default_tuning", "pmsz_ctx_set_tuning", "pmsz_ctx_get_tuning",
-    "pmsz_autotune",
+    "pmsz_autotune", "pmsz_test_step_budget",
 ]
 
 
@@ -200,6 +200,7 @@ def lib():
         L.pmsz_encode_edits_noalloc.argtypes = [P, vp, vp, vp, vp, vp, ctypes.POINTER(EditBlock),
                                                 ctypes.POINTER(EditInfo), vp]
         L.pmsz_time_first_pass.argtypes = [P, vp, vp, i32, ctypes.POINTER(ctypes.c_double)]
+        L.pmsz_test_step_budget.argtypes = [i32, i64, vp, vp, vp, ctypes.c_double, i32, i32, vp]
         L.pmsz_strerror.argtypes = [i32]
         L.pmsz_strerror.restype = ctypes.c_char_p
         L.pmsz_version.argtypes = []
@@ -588,6 +589,19 @@ def time_first_pass(f, g, abs_eb, iters=5, **kw):
                                     ctypes.c_void_p(g.data_ptr()), int(iters), out)
     _check(rc, "pmsz_time_first_pass")
     return out[0], out[1]
+
+
+def test_step_budget(f, g, d, abs_eb, q, clamp_mode=CLAMP_BOUND_SAFE):
+    """Test hook: the step budgets (pmsz_test_step_budget) of CUDA tensors f, g
+    (float32 or float64) and uint8 counters d; returns a uint8 tensor."""
+    import torch
+    out = torch.empty_like(d)
+    rc = lib().pmsz_test_step_budget(_dtype_code(f.dtype), f.numel(),
+                                     ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(g.data_ptr()),
+                                     ctypes.c_void_p(d.data_ptr()), float(abs_eb), int(q),
+                                     int(clamp_mode), ctypes.c_void_p(out.data_ptr()))
+    _check(rc, "pmsz_test_step_budget")
+    return out
 
 
 def version():
