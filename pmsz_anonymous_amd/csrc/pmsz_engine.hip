@@ -317,6 +317,18 @@ __global__ void k_test_step_budget(int64_t n, const T *f, const T *g, const uint
     out[i] = (uint8_t)step_budget<T>(f[i], g[i], d[i], ep);
 }
 
+// Whether any of the cells picked by the 4-bit mask m, whose counter bytes are
+// the bytes of cw, is not within its step budget (takes the exact edit).
+__device__ __forceinline__ bool exact_any(uint32_t cw, uint32_t m) {
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t b = (cw >> (8 * k)) & 0xffu;
+    any |= ((m >> k) & 1u) && (b & 15u) >= (b >> 4);
+  }
+  return any;
+}
+
 // One edit (applyDeltaBuffer_local) of a cell with counter byte c0 and value gv:
 // within its budget the plain step, otherwise the exact edit with fv() (the only
 // read of f) and, with kSet, the cell's new budget (without, the old one stays; it
@@ -640,8 +652,8 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t b) {
 
 template <typename T>
 __device__ __forceinline__ void apply_stream(const Dom &d0, const uint8_t *__restrict__ tbyte,
-                                             uint32_t ngroups, uint32_t tag, Counters &cj,
-                                             RunState *run, StreamLds<T> &sl) {
+                                             uint32_t ngroups, uint32_t tag, bool live,
+                                             Counters &cj, RunState *run, StreamLds<T> &sl) {
   constexpr int kVec = 4 * (int)sizeof(T) / 16;  // 16-byte vectors per 4 cells of T
   constexpr int kTU = 8;                          // lane units per thread and step
   // clang vector types (first-class values, unlike uint4), so a round's data
@@ -708,17 +720,20 @@ __device__ __forceinline__ void apply_stream(const Dom &d0, const uint8_t *__res
 #pragma unroll
       for (int k = 0; k < kVec; k++) {
         fa[k] = fb[k] = ga[k] = gb[k] = v4u{0u, 0u, 0u, 0u};
-        if (ld0) {
-          fa[k] = f16[(size_t)u0 * kVec + k];
-          ga[k] = g16[(size_t)u0 * kVec + k];
-        }
-        if (ld1) {
-          fb[k] = f16[(size_t)u1 * kVec + k];
-          gb[k] = g16[(size_t)u1 * kVec + k];
-        }
+        if (ld0) ga[k] = g16[(size_t)u0 * kVec + k];
+        if (ld1) gb[k] = g16[(size_t)u1 * kVec + k];
       }
       if (ld0) ca = dc32[u0];
       if (ld1) cb = dc32[u1];
+      // f only for this lane's own tagged cells that take the exact edit (before
+      // the first edits of a run no cell has a budget, so the tags decide alone)
+      const bool fna = (mr & 15u) && (!live || exact_any(ca, mr & 15u));
+      const bool fnb = (mr >> 4) && (!live || exact_any(cb, mr >> 4));
+#pragma unroll
+      for (int k = 0; k < kVec; k++) {
+        if (fna) fa[k] = f16[(size_t)u0 * kVec + k];
+        if (fnb) fb[k] = f16[(size_t)u1 * kVec + k];
+      }
       // queue the tagged cells: slot k of lane L is entry L * 8 + k
       if (mr) {
 #pragma unroll
@@ -844,7 +859,9 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
   if constexpr (!kMulti) {
     if (stream) {
       if (lead && mode != kModeDone) zero_counters(&ring[(j + 1) % kCtrRing]);
-      apply_stream<T>(d0, tbyte, ngroups, detect_tag(j + s.tag0), cj, run, lds.sl);
+      // budgets exist once a sweep has edited (j > 1)
+      apply_stream<T>(d0, tbyte, ngroups, detect_tag(j + s.tag0), budgets_on(d0.ep) && j > 1,
+                      cj, run, lds.sl);
       return;
     }
   }
