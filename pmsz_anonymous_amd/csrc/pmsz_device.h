@@ -290,29 +290,6 @@ __device__ __forceinline__ double stage_value(float v) {
 }
 __device__ __forceinline__ double stage_value(double v) { return v; }
 
-// Classification from staged values st[r] (r = rank): float32 keys.
-__device__ __forceinline__ void classify_keys(const double (&st)[kCand], bool interior,
-                                              uint32_t vm, int &asc, int &desc) {
-  double k[kCand];
-#pragma unroll
-  for (int r = 0; r < kCand; r++) k[r] = st[r] + (double)r;
-  if (!interior) {
-    // an invalid candidate stands in as the voxel itself, which never changes
-    // the maximum or minimum over the valid ones
-#pragma unroll
-    for (int r = 0; r < kCand; r++)
-      if (r != kSelf) k[r] = ((vm >> r) & 1u) ? k[r] : k[kSelf];
-  }
-  double kmax = k[0], kmin = k[0];
-#pragma unroll
-  for (int r = 1; r < kCand; r++) {
-    kmax = fmax(kmax, k[r]);
-    kmin = fmin(kmin, k[r]);
-  }
-  asc = __double2loint(kmax) & 15;
-  desc = __double2loint(kmin) & 15;
-}
-
 // Detection at one voxel from its original codes (oa, od) and the current
 // steepest candidates (da, dd): iscriticle data_type=0, pMSz.cu:786-898.
 // Returns the number of target ranks written to t[], counters in c[].

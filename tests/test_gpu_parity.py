@@ -126,6 +126,55 @@ def test_random_shapes_parity(dev, dtype, mode, shape, eb_rel):
     _compare(f, g0, eb, dev, mode=mode)
 
 
+def _near_ties(shape, seed, step):
+    """float64 values on a grid of `step`, each moved by -3..3 ulps (one in six
+    left exact), with signed zeros: neighbourhoods full of candidates that agree
+    in every bit but the last few, and of exact ties."""
+    from pmsz_anonymous_amd import fieldio
+    rng = np.random.default_rng(seed)
+    base = np.round(fieldio.perlin(shape, seed, 5, 3, dtype=np.float64) / step) * step
+    k = rng.integers(-3, 4, size=shape)
+    k[rng.random(shape) < 1 / 6] = 0
+    v = base.copy()
+    for _ in range(3):
+        up, dn = k > 0, k < 0
+        v[up] = np.nextafter(v[up], np.inf)
+        v[dn] = np.nextafter(v[dn], -np.inf)
+        k = k - np.sign(k)
+    zero = base == 0
+    v[zero & (rng.random(shape) < 0.5)] = -0.0
+    return v
+
+
+def test_codes_f64_near_ties(dev):
+    # float64 candidates that differ in the last few bits, exact ties and signed
+    # zeros: the (value, index) order holds to the last bit (such values are what
+    # a reconstructed field holds: one in ten voxels of the 512^3 float64 case
+    # has two candidates agreeing in their top 43 order bits)
+    from pmsz_anonymous_amd import engine
+    for shape, step in (((37, 66, 131), 0.05), ((20, 70, 9), 0.25)):
+        f = _near_ties(shape, sum(shape), step)
+        got = engine.codes(torch.from_numpy(f).to(dev)).cpu().numpy()
+        assert np.array_equal(got, R.dir_codes_to_rank(O.codes(f))), shape
+
+
+@pytest.mark.parametrize("mode", [O.LAZY, O.SYNC])
+def test_f64_near_ties_parity(dev, mode):
+    # g within the bound on a grid of eb/4, half of it moved down by one ulp:
+    # detections meet neighbours equal up to the last bit (the loop does not
+    # settle within 40 sweeps; both sides stop at the cap with the same state)
+    f = _near_ties((40, 45, 50), 3, 0.02)
+    eb = 0.01
+    rng = np.random.default_rng(5)
+    g = np.round((f + rng.uniform(-eb, eb, f.shape)) / (eb / 4)) * (eb / 4)
+    g = np.clip(g, f - eb, f + eb)
+    near = rng.random(f.shape) < 0.5
+    g[near] = np.nextafter(g[near], -np.inf)
+    g = np.maximum(g, f - eb)
+    go, sto = _compare(f, g, eb, dev, mode=mode, max_sweeps=40)
+    assert sto["edit_sweeps"] > 2
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("q", [1, 2])
 def test_clamp_near_zero_parity(dev, dtype, q):
