@@ -596,7 +596,10 @@ __device__ __forceinline__ int apply_target(uint32_t t, const Dom &d0, const Dom
                                             uint32_t &touched, uint32_t &clamped,
                                             uint32_t (&u)[kCand]) {
   uint32_t local;
-  const Dom &D = dom_of<kMulti>(t, d0, doms, shift, dk, local);
+  // the descriptor by value: its fields are loaded once, before the edit's stores
+  // (a reference into the domain table would be reloaded after them, and a load
+  // issued after a store waits for it)
+  const Dom D = dom_of<kMulti>(t, d0, doms, shift, dk, local);
   apply_one<T>((T *)D.g, (const T *)D.f, D.dc, local, D.ep, touched, clamped);
   // every set bit of tmark is a target of this sweep and is applied in this phase,
   // so the word can be cleared with a plain store (tmark is null when the targets
@@ -910,7 +913,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
         apply_target<T, kMulti>(t, d0, doms, shift, ds, clear ? tmark : nullptr, wmark, false,
                                 false, dk, touched, clamped, u);
         uint32_t local;
-        const Dom &D = dom_of<kMulti>(t, d0, doms, shift, dk, local);
+        const Dom D = dom_of<kMulti>(t, d0, doms, shift, dk, local);  // by value (above)
         claim_neighbourhood(local, D, wmark, cl);
         n = cl.count();
       }
@@ -1028,8 +1031,8 @@ __global__ __launch_bounds__(kApplyThreads) void k_detect_list(
       int n = 0;
       if (i < nw) {
         const uint32_t v = wlist[i];
-        wmark[v >> 5] = 0u;
         n = detect_listed<T, kMulti>(v, d0, doms, shift, ds, tmark, dk, has, vc, out);
+        wmark[v >> 5] = 0u;  // after the detection's loads, which would wait for it
       }
       if (kMulti) dom_add3(has, dk, vc, ds, par);
 #pragma unroll
@@ -1203,8 +1206,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
       int n = 0;
       if (i < nw) {
         const uint32_t v = wlist[i];
-        wmark[v >> 5] = 0u;  // every set bit is a listed voxel of this sweep
         n = detect_listed<T, kMulti>(v, d0, doms, shift, ds, tmark, dk, has, vc, out);
+        wmark[v >> 5] = 0u;  // every set bit is a listed voxel of this sweep
       }
       if (kMulti) dom_add3(has, dk, vc, ds, par);
 #pragma unroll
