@@ -356,34 +356,59 @@ __device__ __forceinline__ uint8_t edit_cell(T &gv, uint8_t c0, FV fv, const Edi
 }
 
 // One edit on local cell t, where edits are latency-bound (work lists, the tail,
-// batches): f is loaded together with g and the counter, so that an edit outside
-// its budget waits for no second load; one within its budget ignores it.
+// batches), in two parts: edit_load issues the cell's loads -- f together with g
+// and the counter, so that an edit outside its budget waits for no second load
+// (one within its budget ignores it) -- and edit_store edits and stores.  A
+// wave's vector memory operations complete in issue order, so work that does not
+// depend on the edit (the claims of the target's neighbourhood) is issued between
+// the two: it then waits neither for the cell's loads to be used nor behind the
+// edit's stores.
+template <typename T>
+struct CellLoad {
+  T gv, fv;
+  uint8_t c0;
+};
+template <typename T>
+__device__ __forceinline__ CellLoad<T> edit_load(const T *g, const T *f, const uint8_t *dc,
+                                                 uint32_t t) {
+  CellLoad<T> e;
+  e.gv = g[t];
+  e.c0 = dc[t];
+  e.fv = f[t];
+  return e;
+}
+template <typename T>
+__device__ __forceinline__ void edit_store(CellLoad<T> e, T *g, uint8_t *dc, uint32_t t,
+                                           const EditParams &ep, uint32_t &touched,
+                                           uint32_t &clamped) {
+  asm volatile("" ::"v"(e.fv));  // f arrives with g, not in the exact path
+  const uint8_t c = edit_cell<T, false>(e.gv, e.c0, [&] { return e.fv; }, ep, touched, clamped);
+  g[t] = e.gv;
+  dc[t] = c;
+}
 template <typename T>
 __device__ __forceinline__ void apply_one(T *g, const T *f, uint8_t *dc, uint32_t t,
                                           const EditParams &ep, uint32_t &touched,
                                           uint32_t &clamped) {
-  T gv = g[t];
-  const uint8_t c0 = dc[t];
-  const T fv = f[t];
-  asm volatile("" ::"v"(fv));  // the load stays here, not in the exact path
-  const uint8_t c = edit_cell<T, false>(gv, c0, [&] { return fv; }, ep, touched, clamped);
-  g[t] = gv;
-  dc[t] = c;
+  edit_store<T>(edit_load<T>(g, f, dc, t), g, dc, t, ep, touched, clamped);
 }
 
 // Claims the classified voxels of the closed stencil neighbourhood of local cell
-// t = (x, y, z) in wmark; returns their ids where the claim was new.
-__device__ __forceinline__ int queue_neighbourhood(uint32_t t, int64_t x, int64_t y, int64_t z,
-                                                   const Dom &D, uint32_t *wmark,
-                                                   uint32_t (&u)[kCand]) {
+// t = (x, y, z) in wmark: queue_issue issues every claim (old: the atomics'
+// results), queue_collect returns the ids whose claim was new.
+__device__ __forceinline__ void queue_issue(uint32_t t, int64_t x, int64_t y, int64_t z,
+                                            const Dom &D, uint32_t *wmark,
+                                            uint32_t (&old)[kCand]) {
   const uint32_t vm = box_mask(x, y, z, D.lo, D.hi);
-  // issue all claims before looking at any result, so the atomics overlap
-  uint32_t old[kCand];
 #pragma unroll
   for (int r = 0; r < kCand; r++) {
     const uint32_t v = D.gbase + (uint32_t)((int64_t)t + rank_off(r, D.d));
     old[r] = ((vm >> r) & 1u) ? atomicOr(&wmark[v >> 5], 1u << (v & 31u)) : 0xffffffffu;
   }
+}
+__device__ __forceinline__ int queue_collect(uint32_t t, const Dom &D,
+                                             const uint32_t (&old)[kCand],
+                                             uint32_t (&u)[kCand]) {
   int n = 0;
 #pragma unroll
   for (int r = 0; r < kCand; r++) {
@@ -434,10 +459,11 @@ struct Claims {
     }
   }
 };
-__device__ __forceinline__ void claim_neighbourhood(uint32_t t, const Dom &D, uint32_t *wmark,
-                                                    Claims &c) {
-  int64_t x, y, z;
-  decompose(t, D.d, x, y, z);
+// claim_issue issues every row's claims (c.lo / c.hi: the bits asked for; ol /
+// oh: the atomics' results); claim_finish keeps the bits this call set.
+__device__ __forceinline__ void claim_issue(uint32_t t, int64_t x, int64_t y, int64_t z,
+                                            const Dom &D, uint32_t *wmark, Claims &c,
+                                            uint32_t (&ol)[7], uint32_t (&oh)[7]) {
   const uint32_t vm = box_mask(x, y, z, D.lo, D.hi);
   constexpr int kRowFirst[7] = {0, 2, 4, 6, 9, 11, 13};
   constexpr int kRowLen[7] = {2, 2, 2, 3, 2, 2, 2};
@@ -448,10 +474,18 @@ __device__ __forceinline__ void claim_neighbourhood(uint32_t t, const Dom &D, ui
     const uint32_t v0 = D.gbase + (uint32_t)((int64_t)t + rank_off(kRowFirst[k], D.d));
     const uint64_t m = (uint64_t)rb << (v0 & 31u);
     c.w[k] = v0 >> 5;
-    c.lo[k] = (uint32_t)m ? atomicOr(&wmark[c.w[k]], (uint32_t)m) : 0u;
-    c.hi[k] = (m >> 32) ? atomicOr(&wmark[c.w[k] + 1], (uint32_t)(m >> 32)) : 0u;
-    c.lo[k] = (uint32_t)m & ~c.lo[k];
-    c.hi[k] = (uint32_t)(m >> 32) & ~c.hi[k];
+    c.lo[k] = (uint32_t)m;
+    c.hi[k] = (uint32_t)(m >> 32);
+    ol[k] = c.lo[k] ? atomicOr(&wmark[c.w[k]], c.lo[k]) : 0u;
+    oh[k] = c.hi[k] ? atomicOr(&wmark[c.w[k] + 1], c.hi[k]) : 0u;
+  }
+}
+__device__ __forceinline__ void claim_finish(Claims &c, const uint32_t (&ol)[7],
+                                             const uint32_t (&oh)[7]) {
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    c.lo[k] &= ~ol[k];
+    c.hi[k] &= ~oh[k];
   }
 }
 
@@ -588,32 +622,46 @@ __device__ __forceinline__ void zero_counters(Counters *c) {
 }
 
 // Apply of one target id (shared by the apply and tail kernels): the edit, the
-// mark clear, the ghost-sync flag and the queueing of the neighbourhood.
+// mark clear, the ghost-sync flag and the claims of the neighbourhood -- queued
+// into u (queue: returns their count), marked without results (mark), or as row
+// claims into *cl.  One field: the claims go out between the cell's loads and the
+// edit's stores (edit_load).  Several domains: after the stores, because holding
+// the claims' results across the edit costs the batch kernels more occupancy
+// than the overlap gains (measured).
 template <typename T, bool kMulti>
 __device__ __forceinline__ int apply_target(uint32_t t, const Dom &d0, const Dom *doms,
                                             uint32_t shift, DomState *ds, uint32_t *tmark,
-                                            uint32_t *wmark, bool queue, bool mark, uint32_t &dk,
-                                            uint32_t &touched, uint32_t &clamped,
+                                            uint32_t *wmark, bool queue, bool mark, Claims *cl,
+                                            uint32_t &dk, uint32_t &touched, uint32_t &clamped,
                                             uint32_t (&u)[kCand]) {
   uint32_t local;
   // the descriptor by value: its fields are loaded once, before the edit's stores
   // (a reference into the domain table would be reloaded after them, and a load
   // issued after a store waits for it)
   const Dom D = dom_of<kMulti>(t, d0, doms, shift, dk, local);
-  apply_one<T>((T *)D.g, (const T *)D.f, D.dc, local, D.ep, touched, clamped);
+  const CellLoad<T> e = edit_load<T>((const T *)D.g, (const T *)D.f, D.dc, local);
+  int64_t x, y, z;
+  uint32_t old[kCand], ol[7], oh[7];
+  auto claims = [&] {
+    decompose(local, D.d, x, y, z);
+    if (mark) mark_neighbourhood(local, x, y, z, D, wmark);
+    if (queue) queue_issue(local, x, y, z, D, wmark, old);
+    if (cl) claim_issue(local, x, y, z, D, wmark, *cl, ol, oh);
+  };
+  if (!kMulti) claims();
+  edit_store<T>(e, (T *)D.g, D.dc, local, D.ep, touched, clamped);
   // every set bit of tmark is a target of this sweep and is applied in this phase,
   // so the word can be cleared with a plain store (tmark is null when the targets
   // come from a full-volume detection, which marks none)
   if (tmark) tmark[t >> 5] = 0u;
-  int64_t x, y, z;
-  decompose(local, D.d, x, y, z);
+  if (kMulti) claims();
   if (kMulti && D.padded) {
     // send_flag: an edit within two layers of the padded boundary (pMSz.cu:642)
     if (x <= 1 || x >= D.d.nx - 2 || y <= 1 || y >= D.d.ny - 2 || z <= 1 || z >= D.d.nz - 2)
       if (!(ld_agent(&ds[dk].flags) & kDomSend)) atomicOr(&ds[dk].flags, kDomSend);
   }
-  if (mark) mark_neighbourhood(local, x, y, z, D, wmark);
-  return queue ? queue_neighbourhood(local, x, y, z, D, wmark, u) : 0;
+  if (cl) claim_finish(*cl, ol, oh);
+  return queue ? queue_collect(local, D, old, u) : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -889,9 +937,15 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
       uint32_t n = 0;
       if (i < nt) {
         const uint32_t t = tlist[i];
-        apply_one<T>((T *)d0.g, (const T *)d0.f, d0.dc, t, d0.ep, touched, clamped);
+        // the claims go out between the cell's loads and the edit's stores
+        const CellLoad<T> e = edit_load<T>((const T *)d0.g, (const T *)d0.f, d0.dc, t);
+        int64_t x, y, z;
+        decompose(t, d0.d, x, y, z);
+        uint32_t ol[7], oh[7];
+        claim_issue(t, x, y, z, d0, wmark, cl, ol, oh);
+        edit_store<T>(e, (T *)d0.g, d0.dc, t, d0.ep, touched, clamped);
         if (clear) tmark[t >> 5] = 0u;
-        claim_neighbourhood(t, d0, wmark, cl);
+        claim_finish(cl, ol, oh);
         n = cl.count();
       }
       const uint32_t pos = stage_reserve(st, n, wlist, &cj.nwork);
@@ -909,12 +963,8 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
       Claims cl;
       const bool has = i < nt;
       if (has) {
-        const uint32_t t = tlist[i];
-        apply_target<T, kMulti>(t, d0, doms, shift, ds, clear ? tmark : nullptr, wmark, false,
-                                false, dk, touched, clamped, u);
-        uint32_t local;
-        const Dom D = dom_of<kMulti>(t, d0, doms, shift, dk, local);  // by value (above)
-        claim_neighbourhood(local, D, wmark, cl);
+        apply_target<T, kMulti>(tlist[i], d0, doms, shift, ds, clear ? tmark : nullptr, wmark,
+                                false, false, &cl, dk, touched, clamped, u);
         n = cl.count();
       }
       dom_add_targets(has, dk, ds);
@@ -932,7 +982,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
       const bool has = i < nt;
       if (has)
         apply_target<T, kMulti>(tlist[i], d0, doms, shift, ds, clear ? tmark : nullptr, wmark,
-                                false, queue, dk, touched, clamped, u);
+                                false, queue, nullptr, dk, touched, clamped, u);
       dom_add_targets(has, dk, ds);
     }
   } else {
@@ -1181,8 +1231,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(
       uint32_t dk = 0;
       const bool has = i < nt;
       if (has)
-        n = apply_target<T, kMulti>(tlist[i], d0, doms, shift, ds, tmark, wmark, queue, false, dk,
-                                    touched, clamped, u);
+        n = apply_target<T, kMulti>(tlist[i], d0, doms, shift, ds, tmark, wmark, queue, false,
+                                    nullptr, dk, touched, clamped, u);
       if (kMulti) dom_add_targets(has, dk, ds);
       if (queue) {
         uint32_t pos = block_reserve<kTailThreads>((uint32_t)n, &cj.nwork);
