@@ -85,27 +85,23 @@ struct Ext<double> {  // (value, rank) pairs
     mx = mn = v;
     rmx = rmn = r;
   }
-  // b comes later in rank order: it wins a tie for the maximum, loses it for
-  // the minimum; NaN (invalid) never wins against a valid value
-  __device__ __forceinline__ void put(double v, int r) {
-    const bool anan = mx != mx;
-    const bool tmx = v >= mx || anan;
-    mx = tmx ? v : mx;
-    rmx = tmx ? r : rmx;
-    const bool tmn = v < mn || anan;
-    mn = tmn ? v : mn;
-    rmn = tmn ? r : rmn;
+  // (b, rb) comes later in rank order: it wins a tie for the maximum and loses
+  // it for the minimum.  fmax / fmin skip a NaN (an invalid candidate) and keep
+  // the other operand, so the maximum's rank moves to rb iff b equals the new
+  // maximum, and the minimum's iff the new minimum differs from the old one (a
+  // NaN old value always differs; -0 equals +0, as the reference's compares have
+  // it).  Two v_max_f64 / v_min_f64 and two compare-selects per combine.
+  __device__ __forceinline__ void take(double b, int rb, double bn, int rbn) {
+    const double nmx = fmax(mx, b);
+    rmx = b == nmx ? rb : rmx;
+    mx = nmx;
+    const double nmn = fmin(mn, bn);
+    rmn = nmn != mn ? rbn : rmn;
+    mn = nmn;
   }
+  __device__ __forceinline__ void put(double v, int r) { take(v, r, v, r); }
   __device__ __forceinline__ void add(double v, int r) { put(v, r); }
-  __device__ __forceinline__ void merge(const Ext &o) {
-    const bool anan = mx != mx;
-    const bool tmx = o.mx >= mx || anan;
-    mx = tmx ? o.mx : mx;
-    rmx = tmx ? o.rmx : rmx;
-    const bool tmn = o.mn < mn || anan;
-    mn = tmn ? o.mn : mn;
-    rmn = tmn ? o.rmn : rmn;
-  }
+  __device__ __forceinline__ void merge(const Ext &o) { take(o.mx, o.rmx, o.mn, o.rmn); }
   __device__ __forceinline__ int asc() const { return rmx; }
   __device__ __forceinline__ int desc() const { return rmn; }
   static __device__ __forceinline__ double stage(double v) { return v; }
