@@ -55,27 +55,10 @@ __device__ __forceinline__ double from_right(double v) {
                           (int)from_right_u((uint32_t)__double2loint(v)));
 }
 
-// Running extremes over candidates visited in increasing rank order.
+// Running extremes over candidates visited in increasing rank order (the float64
+// passes; float32 takes the keys below).
 template <typename T>
 struct Ext;
-
-template <>
-struct Ext<float> {  // staged keys
-  double mx, mn;
-  __device__ __forceinline__ void first(double k, int r) { mx = mn = k + (double)r; }
-  __device__ __forceinline__ void add(double k, int r) {
-    const double v = k + (double)r;
-    mx = fmax(mx, v);
-    mn = fmin(mn, v);
-  }
-  __device__ __forceinline__ void merge(const Ext &o) {  // o holds higher ranks
-    mx = fmax(mx, o.mx);
-    mn = fmin(mn, o.mn);
-  }
-  __device__ __forceinline__ int asc() const { return __double2loint(mx) & 15; }
-  __device__ __forceinline__ int desc() const { return __double2loint(mn) & 15; }
-  static __device__ __forceinline__ double stage(float v) { return stage_value(v); }
-};
 
 template <>
 struct Ext<double> {  // (value, rank) pairs
@@ -104,7 +87,6 @@ struct Ext<double> {  // (value, rank) pairs
   __device__ __forceinline__ void merge(const Ext &o) { take(o.mx, o.rmx, o.mn, o.rmn); }
   __device__ __forceinline__ int asc() const { return rmx; }
   __device__ __forceinline__ int desc() const { return rmn; }
-  static __device__ __forceinline__ double stage(double v) { return v; }
 };
 
 // One plane's staged column of one field, with the x-1 / x+1 neighbours.

@@ -269,27 +269,6 @@ __device__ __forceinline__ void scan_masked(const T (&c)[kCand], uint32_t valid,
   desc = sr;
 }
 
-// Staged form of a field value in LDS (8 bytes either way).
-//
-// float32: the double 2^52 + 16 * ord(v), with ord the order-preserving map of
-// the float's bits onto uint32 (-0 canonicalised to +0, so -0 == +0 as the
-// reference's float compare has it).  Adding a candidate's rank (0..14) keeps
-// the value an exact integer below 2^53, and the numeric order of the sums is
-// exactly the (value, linear index) order of islarger_shared / isless_shared
-// (pMSz.cu:610-620): equal values are ordered by rank.  The steepest ascent /
-// descent candidate is then one v_max_f64 / v_min_f64 chain, and the winning
-// rank is the low 4 bits of the result.  (The sums come from arithmetic, so the
-// compiler needs no NaN canonicalisation before max / min.)
-//
-// float64: the value itself, classified by the rank scan.
-__device__ __forceinline__ double stage_value(float v) {
-  const float c = v + 0.0f;
-  const uint32_t b = __float_as_uint(c);
-  const uint32_t ord = b ^ ((uint32_t)((int32_t)b >> 31) | 0x80000000u);
-  return __hiloint2double((int)(0x43300000u | (ord >> 28)), (int)(ord << 4));
-}
-__device__ __forceinline__ double stage_value(double v) { return v; }
-
 // Detection at one voxel from its original codes (oa, od) and the current
 // steepest candidates (da, dd): iscriticle data_type=0, pMSz.cu:786-898.
 // Returns the number of target ranks written to t[], counters in c[].
