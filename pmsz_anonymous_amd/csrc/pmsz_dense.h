@@ -663,13 +663,14 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
               enqueue(bm, h - 1, e, [&] { return c | (w << 8); });
             }
           }
-          // partials of plane p for the voxels of planes p and p+1
-          Ext<T> mid, lo;
+          // partials of plane p for the voxels of planes p and p+1; the old lo is
+          // consumed before the new one is formed in its place, so the loop
+          // carries both without copies
+          Ext<T> mid;
           partial_mid<T>(cp, h, mid);
-          partial_lo<T>(cp, h, lo);
           part[h - 1] = lo_e[h - 1];
           part[h - 1].merge(mid);
-          lo_e[h - 1] = lo;
+          partial_lo<T>(cp, h, lo_e[h - 1]);
         }
       }
     }
