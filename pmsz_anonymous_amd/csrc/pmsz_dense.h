@@ -2,7 +2,9 @@
 // labeling of f, or one detection sweep over g, for every voxel.
 //
 // Layout of the work: a wave owns 62 consecutive x (lanes 1..62; lanes 0 and 63
-// carry the x-1 / x+1 halo), kColRows consecutive y, and walks z.  A lane loads
+// carry the x-1 / x+1 halo), kColRows consecutive y, and walks z -- or, where a
+// box's x extent would leave much of such a column empty, two half-columns of 30
+// x, one per half-wave, each with its own halo lanes (col_units).  A lane loads
 // its x column of kColRows+2 rows per plane from HBM (coalesced across the
 // wave), the x-1 / x+1 neighbours arrive through DPP wave shifts, and z is a
 // register pipeline.  No barriers; LDS holds only the detection's fast-test
@@ -32,7 +34,8 @@
 
 namespace pmsz {
 
-constexpr int kColOwned = 62;  // x owned by a wave (64 lanes minus the two halo lanes)
+constexpr int kColOwned = 62;   // x owned by a wave (64 lanes minus the two halo lanes)
+constexpr int kHalfOwned = 30;  // x owned by a half-wave (32 lanes minus its two halo lanes)
 constexpr int kColRows = 4;    // y rows per wave
 constexpr int kColH = kColRows + 2;
 constexpr int kColZ = 64;      // planes per work unit by default (a domain's zc: 16, 32 or 64)
@@ -250,12 +253,20 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(
 // map is cleared whenever the tags start over).
 __host__ __device__ inline uint32_t detect_tag(uint32_t jd) { return 1u + (jd - 1u) % 255u; }
 
-// Units (62 x 4 x zc column blocks) of one domain's box.
-__host__ inline void col_units(int64_t bx, int64_t by, int64_t bz, uint32_t &nux, uint32_t &nuy,
-                               uint32_t &nuz, int64_t zc = kColZ) {
-  nux = (uint32_t)((bx + kColOwned - 1) / kColOwned);
-  nuy = (uint32_t)((by + kColRows - 1) / kColRows);
-  nuz = (uint32_t)((bz + zc - 1) / zc);
+// Units of one domain's box: waves walking zc planes of columns of kColRows y.
+// A wave takes one column 62 x wide, or two half-columns 30 x wide (the
+// half-columns of a z chunk in (x, y) order, two to a wave) where that needs
+// fewer waves: boxes whose x extent leaves much of a 62-wide column empty.
+__host__ inline void col_units(Dom &D, int64_t bx, int64_t by, int64_t bz, int64_t zc) {
+  const uint32_t nyb = (uint32_t)((by + kColRows - 1) / kColRows);
+  const uint32_t nfull = (uint32_t)((bx + kColOwned - 1) / kColOwned);
+  const uint32_t nhalf = (uint32_t)((bx + kHalfOwned - 1) / kHalfOwned);
+  D.half = (nhalf * nyb + 1) / 2 < nfull * nyb ? 1u : 0u;
+  D.nhx = D.half ? nhalf : nfull;
+  D.nyb = nyb;
+  D.nux = D.half ? (nhalf * nyb + 1) / 2 : nfull * nyb;
+  D.nuy = 1;
+  D.nuz = (uint32_t)((bz + zc - 1) / zc);
 }
 
 
@@ -396,14 +407,22 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
     const uint32_t nx = (uint32_t)D.d.nx, ny = (uint32_t)D.d.ny, nz = (uint32_t)D.d.nz;
     const uint64_t sxy = (uint64_t)D.d.sxy;
     const uint32_t u = unit - D.unit_base;
-    const uint32_t uz = u / (D.nux * D.nuy), rem = u - uz * D.nux * D.nuy;
-    const uint32_t uy = rem / D.nux, ux = rem - uy * D.nux;
-    const int x = D.lo[0] + (int)(ux * kColOwned) - 1 + (int)lane;  // this lane's x
-    const int y0m1 = D.lo[1] + (int)(uy * kColRows) - 1;             // y of loaded row 0
+    const uint32_t uz = u / D.nux, w = u - uz * D.nux;  // (nuy = 1)
+    // this lane's column: the wave's, or with half-columns the one of its half
+    // (lanes 0 and 31 of a half carry its x-1 / x+1 halo; a DPP shift across
+    // the halves reaches only halo lanes, whose results are not used, and every
+    // voxel's candidates lie in its own half)
+    const bool half = D.half != 0;  // uniform
+    const uint32_t cidx = half ? 2u * w + (lane >> 5) : w, cl = half ? lane & 31u : lane;
+    const uint32_t cown = half ? (uint32_t)kHalfOwned : (uint32_t)kColOwned;
+    const bool cvalid = cidx < D.nhx * D.nyb;
+    const uint32_t cy = cidx / D.nhx, cx = cidx - cy * D.nhx;
+    const int x = D.lo[0] + (int)(cx * cown) - 1 + (int)cl;  // this lane's x
+    const int y0m1 = D.lo[1] + (int)(cy * kColRows) - 1;     // y of loaded row 0
     const int zs = D.lo[2] + (int)(uz * D.zc);
     const int ze = zs + (int)D.zc < D.hi[2] ? zs + (int)D.zc : D.hi[2];
-    const bool xok = x >= 0 && x < (int)nx;
-    const bool owned = lane >= 1 && lane <= (uint32_t)kColOwned && x < D.hi[0];
+    const bool xok = cvalid && x >= 0 && x < (int)nx;
+    const bool owned = cvalid && cl >= 1 && cl <= cown && x < D.hi[0];
     // in-plane offsets of the loaded rows (bytes of T) and of the voxel rows
     // (cells; 0 where the lane has no such cell), fixed for the unit
     uint32_t roff[kColH], coff[kColRows];

@@ -61,7 +61,9 @@ struct Dom {
   int32_t off[3];     // global coordinate of local cell lo (subdomains)
   int32_t cc[3];      // coordinates in the subdomain grid
   uint32_t unit_base; // first column unit of this domain
-  uint32_t nux, nuy, nuz;  // column units of the box
+  uint32_t nux, nuy, nuz;  // column units of the box (waves per z chunk, 1, z chunks)
+  uint32_t nhx, nyb;       // columns across x and y blocks of a z chunk
+  uint32_t half;           // 1: half-wave columns (30 x), 0: wave columns (62 x)
   uint32_t zc;             // planes per column unit (16, 32 or 64)
   uint32_t shell_n;   // cells of the 2-layer shell (padded blocks)
   uint32_t padded;    // 1: cells holding the sentinel are missing neighbours

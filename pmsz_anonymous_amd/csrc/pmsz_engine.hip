@@ -1970,7 +1970,7 @@ Dom field_dom(const pmsz_params *p, const void *f, void *g, uint8_t *code, uint8
   D.hi[1] = (int32_t)p->ny;
   D.hi[2] = (int32_t)p->nz;
   D.zc = (uint32_t)kColZ;
-  col_units(p->nx, p->ny, p->nz, D.nux, D.nuy, D.nuz);
+  col_units(D, p->nx, p->ny, p->nz, kColZ);
   D.ep = edit_params(p);
   return D;
 }
@@ -2346,8 +2346,7 @@ int prepare(Ctx &c, Job &job) {
   uint32_t ubase = 0;
   for (Dom &D : job.doms) {
     D.zc = (uint32_t)c.tune.zchunk;
-    col_units(D.hi[0] - D.lo[0], D.hi[1] - D.lo[1], D.hi[2] - D.lo[2], D.nux, D.nuy, D.nuz,
-              D.zc);
+    col_units(D, D.hi[0] - D.lo[0], D.hi[1] - D.lo[1], D.hi[2] - D.lo[2], D.zc);
     D.unit_base = ubase;
     ubase += D.nux * D.nuy * D.nuz;
   }
@@ -2527,7 +2526,7 @@ int correct_subdomains(Ctx &c, const pmsz_params *p, const T *F, T *G, uint8_t *
         D.cc[1] = cy;
         D.cc[2] = cz;
         D.zc = (uint32_t)kColZ;
-        col_units(l[0], l[1], l[2], D.nux, D.nuy, D.nuz);
+        col_units(D, l[0], l[1], l[2], kColZ);
         D.shell_n = shell_count((uint32_t)D.d.nx, (uint32_t)D.d.ny, (uint32_t)D.d.nz);
         D.padded = 1;
         D.ep = edit_params(p);
