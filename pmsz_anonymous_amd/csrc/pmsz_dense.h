@@ -7,8 +7,8 @@
 // x, one per half-wave, each with its own halo lanes (col_units).  A lane loads
 // its x column of kColRows+2 rows per plane from HBM (coalesced across the
 // wave), the x-1 / x+1 neighbours arrive through DPP wave shifts, and z is a
-// register pipeline.  No barriers; LDS holds only the detection's fast-test
-// table and each wave's violation queue.
+// register pipeline.  No barriers; LDS holds only a table (the detection's fast
+// test, the float32 labeling's code bytes) and each wave's violation queue.
 //
 // The 15 candidates of voxel (x, y, z) -- itself and its 14 stencil neighbours,
 // ranked by linear index (pmsz_common.h) -- split by plane: ranks 0..3 lie in
@@ -139,16 +139,12 @@ __device__ __forceinline__ void partial_hi(const ColPlane &c, int h, Ext<T> &e) 
 // at most 2), so the numeric order of the keys is the (value, index) order of
 // islarger_shared / isless_shared (pMSz.cu:610-620) with no per-candidate rank.
 // The winner's tag minus the voxel's own is one of -8, -7, -5, -4, ..., 5, 7, 8
-// in rank order, which decodes to the rank with two compares.
+// in rank order, which decodes to the rank with two compares (diff_rank).
 __device__ __forceinline__ double key32(float v, uint32_t t) {
   const float c = v + 0.0f;  // -0 -> +0: the reference compares -0 == +0
   const uint32_t b = __float_as_uint(c);
   const uint32_t ord = b ^ ((uint32_t)((int32_t)b >> 31) | 0x80000000u);
   return __hiloint2double((int)(0x43300000u | (ord >> 23)), (int)((ord << 9) | t));
-}
-__device__ __forceinline__ int key_rank(double w, uint32_t tself) {
-  const int d = (int)((uint32_t)__double2loint(w) & 511u) - (int)tself + 8;
-  return d - (d > 2 ? 1 : 0) - (d > 14 ? 1 : 0);
 }
 
 // max / min of two keys as single v_max_f64 / v_min_f64.  Keys are exact
@@ -355,8 +351,17 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
   __shared__ uint2 s_q[kDetect ? kColWaves * kQCap : 1];
   // per wave: the cell offsets of the 15 candidates in the current unit's domain
   __shared__ int32_t s_off[kDetect ? kColWaves * 16 : 1];
+  // float32 labeling: the code byte of the winners' tag differences (+ 8, 5 bits
+  // each, ascent low), one table read per voxel row instead of two rank decodes
+  __shared__ uint8_t s_code[kLabel && kF32 ? 1024 : 1];
   if (kDetect && kF32) {
     s_exp[threadIdx.x] = expect_entry(threadIdx.x);
+    __syncthreads();
+  }
+  if (kLabel && kF32) {
+    for (uint32_t k = threadIdx.x; k < 1024u; k += kColThreads)
+      s_code[k] = (uint8_t)(diff_rank((int)min(k & 31u, 16u)) |
+                            (diff_rank((int)min(k >> 5, 16u)) << 4));
     __syncthreads();
   }
   const uint32_t lane = lane_id();
@@ -525,11 +530,14 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
         wave_lds_sync();
       }
     };
-    auto label = [&](int h, int e, int oa, int od) {
+    // code and counter bytes go through buffer resources of plane e, with the
+    // voxel row's in-plane offset (no 64-bit address arithmetic per row)
+    auto label = [&](int h, int e, uint32_t cbyte) {
       if (!vis[h]) return;
-      const uint64_t i = (uint64_t)coff[h] + (uint64_t)e * sxy;
-      code[i] = (uint8_t)(oa | (od << 4));
-      if (dc) dc[i] = 0;  // the run's delta counters start at zero
+      const uint64_t pb = (uint64_t)e * sxy;
+      __builtin_amdgcn_raw_buffer_store_b8((uint8_t)cbyte, plane_rsrc(code + pb), coff[h], 0, 0);
+      if (dc)  // the run's delta counters start at zero
+        __builtin_amdgcn_raw_buffer_store_b8((uint8_t)0, plane_rsrc(dc + pb), coff[h], 0, 0);
     };
 
     if constexpr (kF32) {
@@ -581,7 +589,12 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
               const double wmn = kmin(pt_mn[h - 1], dn);
               const uint32_t ts = tself0 + 3u * (uint32_t)h;
               if (kLabel) {
-                label(h - 1, e, key_rank(wmx, ts), key_rank(wmn, ts));
+                // a winner's tag difference + 8 lies in 0..16 and the tags sit
+                // below the key's ordinal bits: (low word + 8 - ts) mod 32 is it
+                const uint32_t k8 = 8u - ts;
+                const uint32_t ia = ((uint32_t)__double2loint(wmx) + k8) & 31u;
+                const uint32_t id = ((uint32_t)__double2loint(wmn) + k8) & 31u;
+                label(h - 1, e, s_code[ia | (id << 5)]);
               } else {
                 const uint32_t c = cc[h - 1];
                 const uint32_t ex = s_exp[c] + ts * 0x10001u;
@@ -671,7 +684,7 @@ __global__ __launch_bounds__(kColThreads) void k_columns(
             part[h - 1].merge(hi);
             const int da = part[h - 1].asc(), dd = part[h - 1].desc();
             if (kLabel) {
-              label(h - 1, e, da, dd);
+              label(h - 1, e, (uint32_t)da | ((uint32_t)dd << 4));
             } else {
               const uint32_t c = ccode[h - 1];
               const uint32_t w = (uint32_t)da | ((uint32_t)dd << 4);

@@ -678,6 +678,8 @@ __device__ __forceinline__ int apply_target(uint32_t t, const Dom &d0, const Dom
 // * Within a round the tagged cells are queued in LDS (values, counters and the
 //   queue of cell slots) and edited one per lane, so the edit's instructions run
 //   once per 64 targets rather than once per cell slot of every lane.
+// * In the run's first sweep (zero_dc) every delta counter is zero, cleared by
+//   the labeling, so the counters are stored but not loaded.
 // Counts the targets (for the statistics) into cj's kCntStreamed slot.
 constexpr int kStreamRC = 8;  // cells per lane and round (two lane units)
 template <typename T>
@@ -704,7 +706,8 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t b) {
 template <typename T>
 __device__ __forceinline__ void apply_stream(const Dom &d0, const uint8_t *__restrict__ tbyte,
                                              uint32_t ngroups, uint32_t tag, bool live,
-                                             Counters &cj, RunState *run, StreamLds<T> &sl) {
+                                             bool zero_dc, Counters &cj, RunState *run,
+                                             StreamLds<T> &sl) {
   constexpr int kVec = 4 * (int)sizeof(T) / 16;  // 16-byte vectors per 4 cells of T
   constexpr int kTU = 8;                          // lane units per thread and step
   // clang vector types (first-class values, unlike uint4), so a round's data
@@ -774,8 +777,9 @@ __device__ __forceinline__ void apply_stream(const Dom &d0, const uint8_t *__res
         if (ld0) ga[k] = g16[(size_t)u0 * kVec + k];
         if (ld1) gb[k] = g16[(size_t)u1 * kVec + k];
       }
-      if (ld0) ca = dc32[u0];
-      if (ld1) cb = dc32[u1];
+      // (the run's first sweep: the labeling has just cleared every counter)
+      if (ld0 && !zero_dc) ca = dc32[u0];
+      if (ld1 && !zero_dc) cb = dc32[u1];
       // f only for this lane's own tagged cells that take the exact edit (before
       // the first edits of a run no cell has a budget, so the tags decide alone)
       const bool fna = (mr & 15u) && (!live || exact_any(ca, mr & 15u));
@@ -912,7 +916,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_apply(
       if (lead && mode != kModeDone) zero_counters(&ring[(j + 1) % kCtrRing]);
       // budgets exist once a sweep has edited (j > 1)
       apply_stream<T>(d0, tbyte, ngroups, detect_tag(j + s.tag0), budgets_on(d0.ep) && j > 1,
-                      cj, run, lds.sl);
+                      j == 1, cj, run, lds.sl);
       return;
     }
   }
