@@ -334,17 +334,20 @@ __device__ __forceinline__ void plane_load(__amdgpu_buffer_rsrc_t rs, uint32_t o
 // of g, in the first detection).  kMulti: the units of all domains of the run,
 // domain state in ds (par = parity of the detection).  Targets are marked in
 // the byte map tbyte with the detection's tag.
-// Four waves per SIMD (at most 128 VGPRs) are asked for where that was measured
-// faster: the float32 multi-domain detections (135 VGPRs otherwise; with a few
-// spilled registers the 64-field batch takes 3-5 % less time) and the float64
-// labeling (140 otherwise; no spills).  The float64 detections spill 96 bytes at
-// 128 VGPRs and are slower, so they keep three waves.
-constexpr bool cols_four_waves(bool f32, bool label, bool detect, bool multi) {
-  return (detect && multi && f32) || (label && !f32);
+// Waves per SIMD asked for where that was measured faster (1: no request):
+// * four (at most 128 VGPRs) for the float32 multi-domain detections (135 VGPRs
+//   otherwise; with a few spilled registers the 64-field batch takes 3-5 % less
+//   time) and the float64 labeling (140 otherwise; no spills);
+// * five (at most 102) for the float32 labeling (114 otherwise; 0.277 -> 0.261 ms
+//   at 512^3, two registers spilled in the one-field kernel).
+// The float64 detections spill 96 bytes at 128 VGPRs and are slower, so they keep
+// three waves.
+constexpr int cols_waves(bool f32, bool label, bool detect, bool multi) {
+  return label ? (f32 ? 5 : 4) : (detect && multi && f32) ? 4 : 1;
 }
 template <typename T, bool kLabel, bool kDetect, bool kMulti, bool kCheck = false>
 __global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(
-    cols_four_waves(sizeof(T) == 4, kLabel, kDetect, kMulti) ? 4 : 1))) void k_columns(
+    cols_waves(sizeof(T) == 4, kLabel, kDetect, kMulti)))) void k_columns(
     Dom d0, const Dom *__restrict__ doms, uint32_t ndom, DomState *__restrict__ ds,
     uint8_t *__restrict__ tbyte, Counters *__restrict__ ring, uint32_t jslot_prev,
     uint32_t jslot, uint32_t par, RunState *run, uint32_t tag) {
